@@ -1,0 +1,139 @@
+/*
+ * spimrender.h -- C ABI of libspimrender.so, the MI355X (gfx950) volume ray caster
+ * behind spimagine's VolumeRenderer.
+ *
+ * Every entry point returns an int status (SPR_OK == 0, negative on error); no C++
+ * exception crosses the boundary.  spr_last_error(ctx) returns the message of the last
+ * failing call on that context (spr_last_error(NULL): last failure of spr_create or of a
+ * call that received a NULL context).
+ *
+ * A context owns one HIP device, one HIP stream (default priority), the resident volume
+ * and the device-side output buffers.  A context is used by one host thread at a time.
+ *
+ * Reference interfaces replaced (paths relative to the spimagine checkout):
+ *   VolumeRenderer.__init__ / resize / reset_buffer   spimagine/volumerender/volumerender.py:72-135,181-197
+ *   VolumeRenderer.set_data / update_data             spimagine/volumerender/volumerender.py:235-294
+ *   VolumeRenderer.update_matrices + setters          spimagine/volumerender/volumerender.py:214-233,296-325
+ *   VolumeRenderer.rebuild_program (interpolation)    spimagine/volumerender/volumerender.py:137-165
+ *   VolumeRenderer.render / _render_max_project       spimagine/volumerender/volumerender.py:327-390,508-547
+ *   VolumeRenderer._render_isosurface                 spimagine/volumerender/volumerender.py:446-506
+ *   GLWidget._get_min_max (GPU min/max of the volume) spimagine/gui/glwidget.py:328-344
+ */
+#ifndef SPIMRENDER_H
+#define SPIMRENDER_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPR_ABI_VERSION 1
+
+/* status codes */
+#define SPR_OK 0
+#define SPR_ERR_INVALID_ARG (-1)
+#define SPR_ERR_NO_VOLUME (-2)
+#define SPR_ERR_UNSUPPORTED_DTYPE (-3)
+#define SPR_ERR_OUT_OF_MEMORY (-4)
+#define SPR_ERR_HIP (-5)
+#define SPR_ERR_NO_DEVICE (-6)
+
+/* voxel types: VolumeRenderer.dtypes = [float32, uint16, uint8] (volumerender.py:68) */
+#define SPR_DTYPE_F32 0
+#define SPR_DTYPE_U16 1
+#define SPR_DTYPE_U8 2
+
+/* sampler filter: interpolation_defines "linear" / "nearest" (volumerender.py:69-70) */
+#define SPR_FILTER_LINEAR 0
+#define SPR_FILTER_NEAREST 1
+
+/* render(method=...) (volumerender.py:543-547) */
+#define SPR_METHOD_MAX_PROJECT 0
+#define SPR_METHOD_ISO_SURFACE 1
+
+typedef struct spr_ctx spr_ctx;
+
+/* Everything VolumeRenderer passes to its kernels besides the volume
+   (volumerender.py:366-386, 451-497).  inv_p / inv_m are the row-major float32
+   inverses inv(projection) and inv(modelView . mScale) computed by update_matrices
+   (volumerender.py:310-316).  box = [x0,x1,y0,y1,z0,z1] (set_box_boundaries).
+   max_steps is the config value max_steps (config.py:27, default 200), a runtime
+   argument here instead of a compile-time define. */
+typedef struct spr_params {
+    float inv_p[16];
+    float inv_m[16];
+    float box[6];
+    float min_val;      /* set_min_val */
+    float max_val;      /* set_max_val; iso level is max_val/2 (volumerender.py:463) */
+    float gamma;        /* set_gamma */
+    float alpha_pow;    /* set_alpha_pow; 0 = plain MIP */
+    float occ_strength; /* set_occ_strength */
+    int occ_radius;     /* set_occ_radius */
+    int occ_n_points;   /* set_occ_n_points */
+    int max_steps;
+    int filter;         /* SPR_FILTER_* */
+} spr_params;
+
+/* ABI version (SPR_ABI_VERSION of the built library). */
+int spr_abi_version(void);
+
+/* Number of visible HIP devices (0 without a GPU). */
+int spr_device_count(int *count);
+
+/* Free and total bytes of device memory (the reference sizes its volume budget as
+   0.7 x MAX_MEM_ALLOC_SIZE, volumerender.py:102). */
+int spr_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
+
+/* VolumeRenderer.__init__ (volumerender.py:72-135): bind a device, create the stream. */
+int spr_create(int device, spr_ctx **ctx);
+int spr_destroy(spr_ctx *ctx);
+const char *spr_last_error(const spr_ctx *ctx);
+
+/* set_data / update_data (volumerender.py:235-294): copy a (nz,ny,nx) C-order volume,
+   x fastest, from host memory into the context's device layout.  The caller keeps
+   ownership of `host`.  dtype is SPR_DTYPE_*. */
+int spr_set_volume(spr_ctx *ctx, const void *host, int dtype, int nx, int ny, int nz);
+
+/* The same from a device pointer on the context's device (device-to-device copy). */
+int spr_set_volume_device(spr_ctx *ctx, const void *dev, int dtype, int nx, int ny, int nz);
+
+/* GPU min/max of the resident volume (glwidget.py:328-344 _get_min_max). */
+int spr_volume_minmax(spr_ctx *ctx, float *min_out, float *max_out);
+
+/* update_matrices + scalar setters (volumerender.py:214-233, 296-316). */
+int spr_set_params(spr_ctx *ctx, const spr_params *params);
+
+/* render(method, numParts, currentPart) (volumerender.py:508-547) followed by the
+   blocking .get() copies (:388-390, :502-506).  The context keeps w x h device output
+   buffers between calls (reallocated, zero-filled, when w or h changes), so parts > 0
+   of a multipass render merge into the previous contents with fmax
+   (volume_kernel.cl:172-183).  max_project writes out/alpha and leaves the depth buffer
+   untouched (volume_kernel.cl never writes d_depth_output); iso_surface writes all five.
+   Any host pointer may be NULL to skip that copy.  normals is (h,w,3). */
+int spr_render(spr_ctx *ctx, int method, int num_parts, int current_part, int w, int h,
+               float *out, float *alpha, float *depth, float *normals, float *occlusion);
+
+/* The same render into caller-owned device buffers, enqueued on `stream` (a hipStream_t;
+   NULL = the context's stream) without host synchronisation.  depth, normals and
+   occlusion may be NULL for SPR_METHOD_MAX_PROJECT; iso_surface needs all of them. */
+int spr_render_device(spr_ctx *ctx, int method, int num_parts, int current_part, int w, int h,
+                      float *d_out, float *d_alpha, float *d_depth, float *d_normals,
+                      float *d_occlusion, void *stream);
+
+/* Wait for all work on the context's stream. */
+int spr_synchronize(spr_ctx *ctx);
+
+/* The context's hipStream_t (for callers that order their own work after a render). */
+int spr_get_stream(spr_ctx *ctx, void **stream);
+
+/* Number of reference samples a max_project frame takes with the current volume and
+   params: (#rays hitting the box) x ((S/numParts)/16 + 1) x 16.  Used to report
+   Gsamples/s; computed on the device with the kernels' own ray set-up. */
+int spr_count_samples(spr_ctx *ctx, int num_parts, int w, int h, double *samples);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPIMRENDER_H */

@@ -1,0 +1,207 @@
+"""ctypes binding of libspimrender.so (C ABI declared in include/spimrender.h).
+
+The library is the only compute path: when it is missing or no HIP device is visible,
+the calls raise -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspimrender.so")
+
+SPR_OK = 0
+SPR_ERR_INVALID_ARG = -1
+SPR_ERR_NO_VOLUME = -2
+SPR_ERR_UNSUPPORTED_DTYPE = -3
+SPR_ERR_OUT_OF_MEMORY = -4
+SPR_ERR_HIP = -5
+SPR_ERR_NO_DEVICE = -6
+
+DTYPE_F32, DTYPE_U16, DTYPE_U8 = 0, 1, 2
+DTYPE_CODES = {np.dtype(np.float32): DTYPE_F32, np.dtype(np.uint16): DTYPE_U16,
+               np.dtype(np.uint8): DTYPE_U8}
+FILTER_LINEAR, FILTER_NEAREST = 0, 1
+METHOD_MAX_PROJECT, METHOD_ISO_SURFACE = 0, 1
+
+# every symbol include/spimrender.h declares
+EXPORTS = (
+    "spr_abi_version", "spr_device_count", "spr_device_memory", "spr_create", "spr_destroy",
+    "spr_last_error", "spr_set_volume", "spr_set_volume_device", "spr_volume_minmax",
+    "spr_set_params", "spr_render", "spr_render_device", "spr_synchronize", "spr_get_stream",
+    "spr_count_samples",
+)
+
+
+class SprParams(ctypes.Structure):
+    """struct spr_params (include/spimrender.h)."""
+    _fields_ = [
+        ("inv_p", ctypes.c_float * 16),
+        ("inv_m", ctypes.c_float * 16),
+        ("box", ctypes.c_float * 6),
+        ("min_val", ctypes.c_float),
+        ("max_val", ctypes.c_float),
+        ("gamma", ctypes.c_float),
+        ("alpha_pow", ctypes.c_float),
+        ("occ_strength", ctypes.c_float),
+        ("occ_radius", ctypes.c_int),
+        ("occ_n_points", ctypes.c_int),
+        ("max_steps", ctypes.c_int),
+        ("filter", ctypes.c_int),
+    ]
+
+
+class SprError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("spimrender error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libspimrender.so and declare argument types (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("%s is not built; run `python -m spimagine_amd.build`" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    c_int, c_void_p, c_float = ctypes.c_int, ctypes.c_void_p, ctypes.c_float
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.spr_abi_version.argtypes = []
+    L.spr_abi_version.restype = c_int
+    L.spr_device_count.argtypes = [ctypes.POINTER(c_int)]
+    L.spr_device_memory.argtypes = [c_int, ctypes.POINTER(ctypes.c_size_t),
+                                    ctypes.POINTER(ctypes.c_size_t)]
+    L.spr_create.argtypes = [c_int, ctypes.POINTER(c_void_p)]
+    L.spr_destroy.argtypes = [c_void_p]
+    L.spr_last_error.argtypes = [c_void_p]
+    L.spr_last_error.restype = ctypes.c_char_p
+    L.spr_set_volume.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int]
+    L.spr_set_volume_device.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int]
+    L.spr_volume_minmax.argtypes = [c_void_p, fp, fp]
+    L.spr_set_params.argtypes = [c_void_p, ctypes.POINTER(SprParams)]
+    L.spr_render.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                             c_void_p, c_void_p, c_void_p]
+    L.spr_render_device.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p,
+                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.spr_synchronize.argtypes = [c_void_p]
+    L.spr_get_stream.argtypes = [c_void_p, ctypes.POINTER(c_void_p)]
+    L.spr_count_samples.argtypes = [c_void_p, c_int, c_int, c_int, ctypes.POINTER(ctypes.c_double)]
+    _lib = L
+    return L
+
+
+def check(rc, ctx=None):
+    if rc != SPR_OK:
+        msg = load().spr_last_error(ctx)
+        raise SprError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(load().spr_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def device_memory(device=0):
+    free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    check(load().spr_device_memory(device, ctypes.byref(free), ctypes.byref(total)))
+    return free.value, total.value
+
+
+def ptr(a):
+    """Host pointer of a C-contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+class Context:
+    """Owns one spr_ctx (one device, one stream, a resident volume, output buffers)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.spr_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.spr_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        return check(rc, self.handle)
+
+    def set_volume(self, data):
+        data = np.ascontiguousarray(data)
+        if data.dtype not in DTYPE_CODES:
+            raise NotImplementedError("unsupported voxel type %s" % data.dtype)
+        nz, ny, nx = data.shape
+        self._check(self.lib.spr_set_volume(self.handle, data.ctypes.data, DTYPE_CODES[data.dtype],
+                                            nx, ny, nz))
+
+    def set_volume_device(self, dev_ptr, dtype, shape):
+        nz, ny, nx = shape
+        self._check(self.lib.spr_set_volume_device(self.handle, ctypes.c_void_p(dev_ptr),
+                                                   DTYPE_CODES[np.dtype(dtype)], nx, ny, nz))
+
+    def volume_minmax(self):
+        mn, mx = ctypes.c_float(), ctypes.c_float()
+        self._check(self.lib.spr_volume_minmax(self.handle, ctypes.byref(mn), ctypes.byref(mx)))
+        return mn.value, mx.value
+
+    def set_params(self, inv_p, inv_m, box, min_val, max_val, gamma, alpha_pow, occ_strength,
+                   occ_radius, occ_n_points, max_steps, filter):
+        p = SprParams()
+        p.inv_p[:] = [float(v) for v in np.asarray(inv_p, np.float32).ravel()]
+        p.inv_m[:] = [float(v) for v in np.asarray(inv_m, np.float32).ravel()]
+        p.box[:] = [float(v) for v in np.asarray(box, np.float32).ravel()]
+        p.min_val = float(np.float32(min_val))
+        p.max_val = float(np.float32(max_val))
+        p.gamma = float(np.float32(gamma))
+        p.alpha_pow = float(np.float32(alpha_pow))
+        p.occ_strength = float(np.float32(occ_strength))
+        p.occ_radius = int(occ_radius)
+        p.occ_n_points = int(occ_n_points)
+        p.max_steps = int(max_steps)
+        p.filter = int(filter)
+        self._check(self.lib.spr_set_params(self.handle, ctypes.byref(p)))
+
+    def render(self, method, num_parts, current_part, w, h, out=None, alpha=None, depth=None,
+               normals=None, occlusion=None):
+        self._check(self.lib.spr_render(self.handle, method, num_parts, current_part, w, h,
+                                        ptr(out), ptr(alpha), ptr(depth), ptr(normals),
+                                        ptr(occlusion)))
+
+    def render_device(self, method, num_parts, current_part, w, h, d_out, d_alpha, d_depth=None,
+                      d_normals=None, d_occlusion=None, stream=None):
+        self._check(self.lib.spr_render_device(self.handle, method, num_parts, current_part, w, h,
+                                               d_out, d_alpha, d_depth, d_normals, d_occlusion,
+                                               stream))
+
+    def synchronize(self):
+        self._check(self.lib.spr_synchronize(self.handle))
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        self._check(self.lib.spr_get_stream(self.handle, ctypes.byref(s)))
+        return s.value
+
+    def count_samples(self, num_parts, w, h):
+        n = ctypes.c_double()
+        self._check(self.lib.spr_count_samples(self.handle, num_parts, w, h, ctypes.byref(n)))
+        return int(n.value)

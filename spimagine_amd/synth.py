@@ -1,0 +1,139 @@
+"""Seeded synthetic microscopy volumes (Gaussian blobs + camera noise) and camera poses.
+
+These are the workloads the renderer is measured and checked on:
+  parity   128^3 float32, 40 blobs, sigma U[2,6] per axis, amplitude U[0.3,1],
+           +0.05 offset, + N(0, 0.02^2) noise -> 256^2, max_steps 224
+  iso      512^3 uint16, 2000 blobs, sigma U[3,10], amplitude U[500,3000], background
+           100, Poisson shot noise + N(0, 5^2) read noise, clipped -> 1024^2, max_steps 896
+  headline 1024x1024x512 (x,y,z) uint16, 8000 blobs, same recipe -> 1024^2,
+           max_steps 1536 (<= 1 voxel per step)
+Blob parameters come from numpy.random.default_rng(seed); each blob is an axis-aligned
+anisotropic Gaussian evaluated within +-3 sigma of its centre.  ``blob_volume`` builds the
+volume with numpy on the host; ``blob_volume_device`` builds the same blobs with torch on
+a GPU (the noise then comes from torch's generator, seeded the same way), which keeps the
+1 GiB headline volume's generation to seconds.
+"""
+import math
+
+import numpy as np
+
+from .utils.transform_matrices import mat4_perspective, mat4_rotation, mat4_translate
+
+CONFIGS = {
+    "parity": dict(shape=(128, 128, 128), n_blobs=40, sigma=(2., 6.), amp=(.3, 1.),
+                   background=.05, poisson=False, read_noise=.02, dtype="float32",
+                   out=(256, 256), max_steps=224),
+    "iso": dict(shape=(512, 512, 512), n_blobs=2000, sigma=(3., 10.), amp=(500., 3000.),
+                background=100., poisson=True, read_noise=5., dtype="uint16",
+                out=(1024, 1024), max_steps=896),
+    "headline": dict(shape=(512, 1024, 1024), n_blobs=8000, sigma=(3., 10.), amp=(500., 3000.),
+                     background=100., poisson=True, read_noise=5., dtype="uint16",
+                     out=(1024, 1024), max_steps=1536),
+}
+
+
+def steps_for_shape(shape):
+    """max_steps = 16 * ceil(|(Nx,Ny,Nz)| / 16): every ray steps at most one voxel."""
+    return 16 * int(math.ceil(math.sqrt(sum(n * n for n in shape)) / 16.))
+
+
+def blob_params(shape, n_blobs, sigma, amp, seed=0):
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0, 1, (n_blobs, 3)) * np.array(shape, np.float64)  # (z, y, x)
+    sigmas = rng.uniform(sigma[0], sigma[1], (n_blobs, 3))
+    amps = rng.uniform(amp[0], amp[1], n_blobs)
+    return rng, centres, sigmas, amps
+
+
+def _axis_profile(c, s, n):
+    lo = max(0, int(math.floor(c - 3 * s)))
+    hi = min(n, int(math.ceil(c + 3 * s)) + 1)
+    x = np.arange(lo, hi, dtype=np.float64)
+    return lo, hi, np.exp(-.5 * ((x - c) / s) ** 2)
+
+
+def _finish(vol, rng_noise, cfg, dtype):
+    if cfg["poisson"]:
+        vol = rng_noise.poisson(vol).astype(np.float32)
+    if cfg["read_noise"]:
+        vol = vol + rng_noise.normal(0, cfg["read_noise"], vol.shape).astype(np.float32)
+    if np.dtype(dtype) == np.uint16:
+        return np.clip(np.rint(vol), 0, 65535).astype(np.uint16)
+    if np.dtype(dtype) == np.uint8:
+        return np.clip(np.rint(vol), 0, 255).astype(np.uint8)
+    return vol.astype(np.float32)
+
+
+def blob_volume(shape, n_blobs, sigma, amp, background, poisson, read_noise, dtype, seed=0, **_):
+    """Host (numpy) blob volume of numpy shape (Nz, Ny, Nx)."""
+    rng, centres, sigmas, amps = blob_params(shape, n_blobs, sigma, amp, seed)
+    vol = np.full(shape, background, np.float32)
+    for (cz, cy, cx), (sz, sy, sx), a in zip(centres, sigmas, amps):
+        z0, z1, gz = _axis_profile(cz, sz, shape[0])
+        y0, y1, gy = _axis_profile(cy, sy, shape[1])
+        x0, x1, gx = _axis_profile(cx, sx, shape[2])
+        vol[z0:z1, y0:y1, x0:x1] += (a * gz[:, None, None] * gy[None, :, None]
+                                     * gx[None, None, :]).astype(np.float32)
+    cfg = dict(poisson=poisson, read_noise=read_noise)
+    return _finish(vol, rng, cfg, dtype)
+
+
+def blob_volume_device(shape, n_blobs, sigma, amp, background, poisson, read_noise, dtype,
+                       seed=0, device="cuda", **_):
+    """The same blobs built on a GPU with torch; returns a device tensor."""
+    import torch
+    _, centres, sigmas, amps = blob_params(shape, n_blobs, sigma, amp, seed)
+    vol = torch.full(shape, float(background), dtype=torch.float32, device=device)
+    for (cz, cy, cx), (sz, sy, sx), a in zip(centres, sigmas, amps):
+        z0, z1, gz = _axis_profile(cz, sz, shape[0])
+        y0, y1, gy = _axis_profile(cy, sy, shape[1])
+        x0, x1, gx = _axis_profile(cx, sx, shape[2])
+        prof = torch.from_numpy(np.concatenate([a * gz, gy, gx]).astype(np.float32)).to(device)
+        tz, ty, tx = torch.split(prof, [len(gz), len(gy), len(gx)])
+        vol[z0:z1, y0:y1, x0:x1] += tz[:, None, None] * ty[None, :, None] * tx[None, None, :]
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed) + 1)
+    if poisson:
+        vol = torch.poisson(vol, generator=gen)
+    if read_noise:
+        vol += float(read_noise) * torch.randn(vol.shape, generator=gen, device=device)
+    dt = np.dtype(dtype)
+    if dt == np.uint16:
+        # torch has no general uint16 arithmetic: return the uint16 bit pattern as int16
+        # (view the host copy with .view(np.uint16))
+        v = torch.clamp(torch.round(vol), 0, 65535).to(torch.int32)
+        return torch.where(v >= 32768, v - 65536, v).to(torch.int16)
+    if dt == np.uint8:
+        return torch.clamp(torch.round(vol), 0, 255).to(torch.uint8)
+    return vol
+
+
+def device_volume_to_numpy(t, dtype):
+    """Host copy of a volume from blob_volume_device, as the numpy dtype it encodes."""
+    return t.cpu().numpy().view(np.dtype(dtype))
+
+
+def config_volume(name, seed=0, device=None):
+    cfg = CONFIGS[name]
+    if device is None:
+        return blob_volume(seed=seed, **cfg)
+    return blob_volume_device(seed=seed, device=device, **cfg)
+
+
+def headline_camera():
+    """P = perspective(45, 1, 0.1, 10), M = translate(0,0,-4) . rotation(pi/6 about y)."""
+    P = mat4_perspective(45, 1., .1, 10)
+    M = np.dot(mat4_translate(0, 0, -4), mat4_rotation(np.pi / 6, 0, 1, 0))
+    return M, P
+
+
+def random_poses(n, seed=0, distance=4.):
+    """n seeded poses: uniform axis on the sphere, angle U[0, 2pi), at the headline distance."""
+    rng = np.random.default_rng(seed)
+    poses = []
+    for _ in range(n):
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        ang = rng.uniform(0, 2 * np.pi)
+        poses.append(np.dot(mat4_translate(0, 0, -distance), mat4_rotation(ang, *axis)))
+    return poses

@@ -1,0 +1,107 @@
+"""Host-side logic that needs no GPU: pose matrices, config parsing, the matrix
+precision path of update_matrices and the synthetic workload generator."""
+import numpy as np
+import pytest
+
+from spimagine_amd import config, synth
+from spimagine_amd.utils import (Quaternion, mat4_frustrum, mat4_identity, mat4_lookat,
+                                 mat4_ortho, mat4_perspective, mat4_rotation, mat4_scale,
+                                 mat4_translate, quaternion_slerp)
+from spimagine_amd.volumerender.volumerender import render_matrices, stack_scale_mat
+
+
+def test_matrix_dtypes_are_the_reference_interface():
+    # transform_matrices.py:21-118: float32 projections/scales, float64 rigid motions
+    assert mat4_scale(1, 2, 3).dtype == np.float32
+    assert mat4_perspective().dtype == np.float32
+    assert mat4_ortho().dtype == np.float32
+    assert mat4_frustrum(-1, 1, -1, 1, 1, 2).dtype == np.float32
+    assert mat4_identity().dtype == np.float64
+    assert mat4_translate(1, 2, 3).dtype == np.float64
+    assert mat4_rotation(.3, 0, 1, 0).dtype == np.float64
+
+
+def test_perspective_maps_near_far_planes_to_ndc():
+    P = mat4_perspective(60, 1.5, .5, 20).astype(np.float64)
+    for z, ndc in ((-.5, -1.), (-20., 1.)):
+        p = P @ [0, 0, z, 1]
+        assert abs(p[2] / p[3] - ndc) < 1e-6
+    f = 1 / np.tan(np.deg2rad(30))
+    assert abs(P[1, 1] - f) < 1e-6 and abs(P[0, 0] - f / 1.5) < 1e-6
+
+
+def test_ortho_and_frustum_closed_forms():
+    O = mat4_ortho(-2, 2, -1, 3, -1.5, 1.5).astype(np.float64)
+    for x, y, z in ((-2, -1, 1.5), (2, 3, -1.5)):
+        p = O @ [x, y, z, 1]
+        assert np.allclose(p[:3], [np.sign(x), np.sign(y - 1), -np.sign(z)], atol=1e-6)
+    F = mat4_frustrum(-1, 1, -1, 1, 1, 10).astype(np.float64)
+    assert np.allclose(F, mat4_perspective(90, 1, 1, 10), atol=1e-6)
+
+
+def test_rotation_is_orthonormal_and_right_handed():
+    R = mat4_rotation(np.pi / 2, 0, 0, 1)
+    assert np.allclose(R[:3, :3] @ [1, 0, 0], [0, 1, 0], atol=1e-7)
+    rng = np.random.default_rng(0)
+    for _ in range(5):
+        R = mat4_rotation(rng.uniform(0, 6), *rng.normal(size=3))[:3, :3]
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-6)
+        assert abs(np.linalg.det(R) - 1) < 1e-6
+
+
+def test_quaternion_algebra_and_slerp():
+    q = Quaternion(np.cos(.3), 0, np.sin(.3), 0)
+    assert np.allclose((q * q.conj()).data, [1, 0, 0, 0])
+    assert np.allclose((q * q).toRotation3(), mat4_rotation(1.2, 0, 1, 0)[:3, :3])
+    a, b = Quaternion(1, 0, 0, 0), Quaternion(np.cos(.5), np.sin(.5), 0, 0)
+    mid = quaternion_slerp(a, b, .5)
+    assert np.allclose(mid.data, [np.cos(.25), np.sin(.25), 0, 0])
+
+
+def test_lookat_points_camera_down_minus_z():
+    M = mat4_lookat([0, 0, 10], [0, 0, 0], [0, 1, 0])
+    assert np.allclose(M @ [0, 0, 0, 1], [0, 0, -10, 1])
+
+
+def test_stack_scale_and_inversion_precision():
+    # volumerender.py:318-325: mScale = diag(dx Nx, dy Ny, dz Nz) / max(d N)
+    S = stack_scale_mat((64, 32, 16), np.array([1., 1., 4.]))
+    assert np.allclose(np.diag(S), [1, .5, 1, 1])
+    # modelView float64 (translate) -> invM computed in float64; projection float32
+    M = mat4_translate(0, 0, -4)
+    P = mat4_perspective()
+    invP, invM = render_matrices(M, P, (10, 10, 10))
+    assert invP.dtype == np.float32 and invM.dtype == np.float32
+    assert np.allclose(invM.reshape(4, 4) @ [0, 0, -4, 1], [0, 0, 0, 1])
+    from scipy.linalg import inv
+    assert np.array_equal(invP, inv(P).flatten().astype(np.float32))
+
+
+def test_config_file_parsing(tmp_path):
+    # spimagine/config/myconfigparser.py: section-less key = value lines
+    p = tmp_path / "spimagine"
+    p.write_text("max_steps = 1536\ninterpolation = nearest\n")
+    assert config.get_param("max_steps", int, str(p)) == 1536
+    assert config.get_param("interpolation", str, str(p)) == "nearest"
+    assert config.get_param("id_device", int, str(p)) == 0
+    assert config.get_param("max_steps", int, str(tmp_path / "missing")) == 200
+    p.write_text("max_steps = notanumber\n")
+    assert config.get_param("max_steps", int, str(p)) == 200
+
+
+def test_steps_for_shape():
+    assert synth.steps_for_shape((512, 1024, 1024)) == 1536
+    assert synth.steps_for_shape((128, 128, 128)) == 224
+    assert synth.steps_for_shape((512, 512, 512)) == 896
+
+
+def test_blob_volume_recipe():
+    vol = synth.blob_volume((32, 40, 48), n_blobs=30, sigma=(2., 4.), amp=(500., 3000.),
+                            background=100., poisson=True, read_noise=5., dtype="uint16", seed=3)
+    assert vol.dtype == np.uint16 and vol.shape == (32, 40, 48)
+    assert 70 < np.percentile(vol, 5) < 120 and vol.max() > 600
+    again = synth.blob_volume((32, 40, 48), n_blobs=30, sigma=(2., 4.), amp=(500., 3000.),
+                              background=100., poisson=True, read_noise=5., dtype="uint16", seed=3)
+    assert np.array_equal(vol, again)
+    f = synth.config_volume("parity")
+    assert f.dtype == np.float32 and f.shape == (128, 128, 128)
