@@ -46,6 +46,11 @@ def main():
     ap.add_argument("--latency-frames", type=int, default=104)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="launch tuning passed to spr_set_tuning, e.g. tile_map=1")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run a plain device copy (torch.neg on int16) of this many MiB, "
+                         "the reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -69,6 +74,11 @@ def main():
     log("bench: volume ready in %.1f s" % (time.time() - t))
 
     ctx = _lib.Context(0)
+    tuning = {}
+    for kv in args.tune:
+        k, v = kv.split("=")
+        ctx.set_tuning(k, int(v))
+        tuning[k] = int(v)
     ctx.set_volume_device(vt.data_ptr(), np.uint16, cfg["shape"])
     lo, hi = ctx.volume_minmax()
     M, P = synth.headline_camera()
@@ -81,8 +91,13 @@ def main():
     set_pose(invP, invM)
     out = torch.empty((H, W), dtype=torch.float32, device="cuda")
     alpha = torch.empty_like(out)
-    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    # a dedicated (default-priority) stream: the frames and the events that time them are
+    # enqueued on it (torch's default stream has handle 0, which the C ABI reads as
+    # "the context's own stream")
+    stream = torch.cuda.Stream()
     sptr = stream.cuda_stream
+    assert sptr
     n_samp = ctx.count_samples(1, W, H)
     per_ray = (S // 16 + 1) * 16
 
@@ -127,9 +142,19 @@ def main():
         frame()
         b.record(stream)
     torch.cuda.synchronize()
-    lat = [a.elapsed_time(b) for a, b in lat_ev]
-    pose_gsps = float(np.mean([samples_pose[i % 8] / (lat[i] * 1e6) for i in range(len(lat))]))
+    lat = [a.elapsed_time(b) for a, b in lat_ev] or [float("nan")]
+    pose_gsps = float(np.mean([samples_pose[i % 8] / (lat[i] * 1e6) for i in range(len(lat_ev))]
+                              or [float("nan")]))
     set_pose(invP, invM)
+
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib << 19  # int16 elements
+        src = torch.ones(n, dtype=torch.int16, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     b_alg = n_samp * 8 * 2 + W * H * 4 * 2
     achieved = b_alg / (kernel_ms * 1e-3) / 1e9
@@ -191,10 +216,11 @@ def main():
                                "read noise) -> 1024x1024 float32, max_steps 1536, perspective "
                                "45 deg, translate(0,0,-4).rotY(30 deg), linear sampler",
                    "voxel": "u16", "frame": [W, H], "max_steps": S,
-                   "samples_per_frame": int(n_samp), "samples_per_hit_ray": per_ray},
+                   "samples_per_frame": int(n_samp), "samples_per_hit_ray": per_ray,
+                   "tuning": {k: ctx.get_tuning(k) for k in _lib.TUNING_KEYS}},
         "frames_per_s": round(1e3 / ms_per_step, 2),
         "latency_ms": {"p50": round(pct(lat, 50), 4), "p99": round(pct(lat, 99), 4),
-                       "frames": len(lat), "poses": 8,
+                       "frames": len(lat_ev), "poses": 8,
                        "gsamples_per_s": round(pose_gsps, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -52,6 +52,23 @@ extern "C" {
 #define SPR_METHOD_MAX_PROJECT 0
 #define SPR_METHOD_ISO_SURFACE 1
 
+/* device layout of the resident volume (no counterpart in the reference, whose OpenCL
+   runtime owns the image layout):
+   PADDED  a (nz+2, ny+2, nx+2) copy with replicated edges (clamp-to-edge without
+           per-sample clamping), addressed with 32-bit offsets: up to 4 GiB padded
+   LINEAR  the (nz, ny, nx) volume as uploaded, any size up to 2^32-1 voxels
+   AUTO    PADDED when it fits, else LINEAR (default) */
+#define SPR_LAYOUT_AUTO 0
+#define SPR_LAYOUT_LINEAR 1
+#define SPR_LAYOUT_PADDED 2
+
+/* launch tuning keys (spr_set_tuning; no counterpart in the reference).  Results do not
+   depend on them.
+   SPR_TUNE_TILE_MAP  how 16x16 pixel tiles are dealt to workgroups and so to the 8 XCDs:
+                      0 = tile rows interleaved over XCDs (default), 1 = one contiguous
+                      band per XCD, 2 = row-major */
+#define SPR_TUNE_TILE_MAP 0
+
 typedef struct spr_ctx spr_ctx;
 
 /* Everything VolumeRenderer passes to its kernels besides the volume
@@ -90,12 +107,23 @@ int spr_create(int device, spr_ctx **ctx);
 int spr_destroy(spr_ctx *ctx);
 const char *spr_last_error(const spr_ctx *ctx);
 
+/* Layout (SPR_LAYOUT_*) used by the next spr_set_volume*; spr_get_layout reports the
+   resident volume's.  Both layouts give identical render results. */
+int spr_set_layout(spr_ctx *ctx, int layout);
+int spr_get_layout(spr_ctx *ctx, int *layout);
+
+/* Set / read a launch tuning value (SPR_TUNE_*). */
+int spr_set_tuning(spr_ctx *ctx, int key, int value);
+int spr_get_tuning(spr_ctx *ctx, int key, int *value);
+
 /* set_data / update_data (volumerender.py:235-294): copy a (nz,ny,nx) C-order volume,
    x fastest, from host memory into the context's device layout.  The caller keeps
    ownership of `host`.  dtype is SPR_DTYPE_*. */
 int spr_set_volume(spr_ctx *ctx, const void *host, int dtype, int nx, int ny, int nz);
 
-/* The same from a device pointer on the context's device (device-to-device copy). */
+/* The same from a device pointer on the context's device (device-to-device copy).  Waits
+   for all work already submitted to the device (any stream) before copying, so a source
+   still being produced on another stream is complete. */
 int spr_set_volume_device(spr_ctx *ctx, const void *dev, int dtype, int nx, int ny, int nz);
 
 /* GPU min/max of the resident volume (glwidget.py:328-344 _get_min_max). */

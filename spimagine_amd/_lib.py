@@ -24,11 +24,15 @@ DTYPE_CODES = {np.dtype(np.float32): DTYPE_F32, np.dtype(np.uint16): DTYPE_U16,
                np.dtype(np.uint8): DTYPE_U8}
 FILTER_LINEAR, FILTER_NEAREST = 0, 1
 METHOD_MAX_PROJECT, METHOD_ISO_SURFACE = 0, 1
+LAYOUT_AUTO, LAYOUT_LINEAR, LAYOUT_PADDED = 0, 1, 2
+TUNE_TILE_MAP = 0
+TUNING_KEYS = {"tile_map": TUNE_TILE_MAP}
 
 # every symbol include/spimrender.h declares
 EXPORTS = (
     "spr_abi_version", "spr_device_count", "spr_device_memory", "spr_create", "spr_destroy",
-    "spr_last_error", "spr_set_volume", "spr_set_volume_device", "spr_volume_minmax",
+    "spr_last_error", "spr_set_layout", "spr_get_layout", "spr_set_tuning", "spr_get_tuning",
+    "spr_set_volume", "spr_set_volume_device", "spr_volume_minmax",
     "spr_set_params", "spr_render", "spr_render_device", "spr_synchronize", "spr_get_stream",
     "spr_count_samples",
 )
@@ -68,6 +72,15 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError("%s is not built; run `python -m spimagine_amd.build`" % LIB_PATH)
+    # One HIP runtime per process: torch ships its own libamdhip64 (SONAME libamdhip64.so.7)
+    # and its extensions depend on it by file name, so a runtime this library pulled in
+    # from /opt/rocm first would be a second, separate runtime and torch would find no
+    # GPU.  Loading torch first makes this library's libamdhip64.so.7 dependency resolve
+    # to torch's already loaded runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     c_int, c_void_p, c_float = ctypes.c_int, ctypes.c_void_p, ctypes.c_float
     fp = ctypes.POINTER(ctypes.c_float)
@@ -80,6 +93,10 @@ def load():
     L.spr_destroy.argtypes = [c_void_p]
     L.spr_last_error.argtypes = [c_void_p]
     L.spr_last_error.restype = ctypes.c_char_p
+    L.spr_set_layout.argtypes = [c_void_p, c_int]
+    L.spr_get_layout.argtypes = [c_void_p, ctypes.POINTER(c_int)]
+    L.spr_set_tuning.argtypes = [c_void_p, c_int, c_int]
+    L.spr_get_tuning.argtypes = [c_void_p, c_int, ctypes.POINTER(c_int)]
     L.spr_set_volume.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int]
     L.spr_set_volume_device.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int]
     L.spr_volume_minmax.argtypes = [c_void_p, fp, fp]
@@ -145,6 +162,26 @@ class Context:
 
     def _check(self, rc):
         return check(rc, self.handle)
+
+    def set_layout(self, layout):
+        """Layout of the next upload (LAYOUT_AUTO / LAYOUT_LINEAR / LAYOUT_PADDED)."""
+        self._check(self.lib.spr_set_layout(self.handle, int(layout)))
+
+    def layout(self):
+        v = ctypes.c_int()
+        self._check(self.lib.spr_get_layout(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def set_tuning(self, key, value):
+        """Launch tuning (TUNING_KEYS name or SPR_TUNE_* code); results do not change."""
+        code = TUNING_KEYS[key] if isinstance(key, str) else int(key)
+        self._check(self.lib.spr_set_tuning(self.handle, code, int(value)))
+
+    def get_tuning(self, key):
+        code = TUNING_KEYS[key] if isinstance(key, str) else int(key)
+        v = ctypes.c_int()
+        self._check(self.lib.spr_get_tuning(self.handle, code, ctypes.byref(v)))
+        return v.value
 
     def set_volume(self, data):
         data = np.ascontiguousarray(data)

@@ -23,23 +23,45 @@ struct RayParams {
     int nouter;        // max_project: (S / numParts) / 16
     int current_part;  // max_project: multipass part
     uint32_t W, H;
-    uint32_t ntx, ntiles;  // 16x16 tiles per row, total tiles
+    uint32_t ntx, nty, ntiles;  // 16x16 tiles per row, tile rows, total tiles
+    uint32_t tile_map;          // kTileMap* below
 };
 
-// The dispatcher hands consecutive workgroups to the 8 XCDs round-robin.  Remap the
-// workgroup id so XCD k renders one contiguous run of tiles (a horizontal band of the
-// frame): its rays then walk one slab of the volume and share that XCD's L2.
-__device__ __forceinline__ bool tile_pixel(uint32_t ntx, uint32_t ntiles, uint32_t W, uint32_t H,
-                                           uint32_t* x, uint32_t* y) {
-    uint32_t b = blockIdx.x;
-    uint32_t q = ntiles >> 3, r = ntiles & 7u;
-    uint32_t xcd = b & 7u, k = b >> 3;
-    uint32_t t = xcd * q + min(xcd, r) + k;
-    uint32_t tx = t % ntx, ty = t / ntx;
-    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+// Workgroup -> 16x16 tile assignment.  The dispatcher hands consecutive workgroups to
+// the 8 XCDs round-robin (workgroup b runs on XCD b % 8), each XCD with its own L2.
+//   kTileMapInterleaved  XCD x renders tile rows x, x+8, x+16, ...: whole rows of tiles
+//                        (neighbouring beams through the volume) share one L2, and every
+//                        XCD gets a similar share of the frame's hit rays.  The grid is
+//                        8 * ceil(nty/8) * ntx workgroups; the surplus ones exit at once.
+//   kTileMapBands        XCD x renders one contiguous horizontal band of the frame.
+//   kTileMapLinear       tile = workgroup id (row-major).
+constexpr uint32_t kTileMapInterleaved = 0, kTileMapBands = 1, kTileMapLinear = 2;
+
+__host__ __device__ inline uint32_t tile_grid(uint32_t tile_map, uint32_t ntx, uint32_t nty) {
+    return tile_map == kTileMapInterleaved ? 8u * ((nty + 7u) / 8u) * ntx : ntx * nty;
+}
+
+__device__ __forceinline__ bool tile_pixel(const RayParams& p, uint32_t* x, uint32_t* y) {
+    const uint32_t b = blockIdx.x;
+    uint32_t tx, ty;
+    if (p.tile_map == kTileMapInterleaved) {
+        const uint32_t xcd = b & 7u, k = b >> 3;
+        ty = xcd + 8u * (k / p.ntx);
+        tx = k % p.ntx;
+    } else if (p.tile_map == kTileMapBands) {
+        const uint32_t q = p.ntiles >> 3, r = p.ntiles & 7u;
+        const uint32_t xcd = b & 7u, k = b >> 3;
+        const uint32_t t = xcd * q + min(xcd, r) + k;
+        tx = t % p.ntx;
+        ty = t / p.ntx;
+    } else {
+        tx = b % p.ntx;
+        ty = b / p.ntx;
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     *x = tx * 16u + (wave & 1u) * 8u + (lane & 7u);
     *y = ty * 16u + (wave >> 1) * 8u + (lane >> 3);
-    return *x < W && *y < H;
+    return ty < p.nty && *x < p.W && *y < p.H;
 }
 
 __device__ __forceinline__ float apply_gamma(float c, float gamma) {
@@ -49,13 +71,13 @@ __device__ __forceinline__ float apply_gamma(float c, float gamma) {
 }
 
 // volume_kernel.cl:20-185 (float) and :190-345 (short, also used for uint8)
-template <typename T, int FILTER>
-__global__ __launch_bounds__(256) void k_max_project(LinearVolume<T> vol, RayParams p,
+template <class Vol, int FILTER>
+__global__ __launch_bounds__(256) void k_max_project(Vol vol, RayParams p,
                                                      float* __restrict__ out,
                                                      float* __restrict__ alpha) {
-    constexpr bool kShort = !std::is_same<T, float>::value;
+    constexpr bool kShort = !std::is_same<typename Vol::value_type, float>::value;
     uint32_t x, y;
-    if (!tile_pixel(p.ntx, p.ntiles, p.W, p.H, &x, &y)) return;
+    if (!tile_pixel(p, &x, &y)) return;
     const uint32_t o = x + p.W * y;
     F4 orig, direc;
     ray_setup(p.invP, p.invM, x, y, p.W, p.H, &orig, &direc);
@@ -117,14 +139,14 @@ __global__ __launch_bounds__(256) void k_max_project(LinearVolume<T> vol, RayPar
 }
 
 // iso_kernel.cl:17-225
-template <typename T, int FILTER>
-__global__ __launch_bounds__(256) void k_iso_surface(LinearVolume<T> vol, RayParams p,
+template <class Vol, int FILTER>
+__global__ __launch_bounds__(256) void k_iso_surface(Vol vol, RayParams p,
                                                      float* __restrict__ out,
                                                      float* __restrict__ alpha,
                                                      float* __restrict__ depth,
                                                      float* __restrict__ normals) {
     uint32_t x, y;
-    if (!tile_pixel(p.ntx, p.ntiles, p.W, p.H, &x, &y)) return;
+    if (!tile_pixel(p, &x, &y)) return;
     const uint32_t o = x + p.W * y;
     F4 orig, direc;
     ray_setup(p.invP, p.invM, x, y, p.W, p.H, &orig, &direc);
@@ -306,7 +328,7 @@ __global__ __launch_bounds__(256) void k_shading(float* __restrict__ out, RayPar
 // rays of a w x h frame whose box test hits (for the reference sample count)
 __global__ __launch_bounds__(256) void k_count_hits(RayParams p, unsigned long long* __restrict__ count) {
     uint32_t x, y;
-    bool inside = tile_pixel(p.ntx, p.ntiles, p.W, p.H, &x, &y);
+    bool inside = tile_pixel(p, &x, &y);
     int hit = 0;
     if (inside) {
         F4 orig, direc;

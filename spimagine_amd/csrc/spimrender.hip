@@ -1,7 +1,7 @@
 // spimrender.hip -- libspimrender.so: C ABI (include/spimrender.h) over the gfx950
 // ray-casting kernels.  Host side of spimagine's VolumeRenderer
-// (spimagine/volumerender/volumerender.py): volume upload, parameter state, kernel
-// dispatch in the reference's launch order, and the device-to-host output copies.
+// (spimagine/volumerender/volumerender.py): volume upload and layout, parameter state,
+// kernel dispatch in the reference's launch order, and the device-to-host output copies.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -21,9 +21,13 @@ struct spr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    // resident volume, (nz, ny, nx) C order, x fastest
-    void* d_vol = nullptr;
+    // resident volume: (nz, ny, nx) C order, x fastest, in one of two layouts
+    int layout_req = SPR_LAYOUT_AUTO;  // requested for the next upload
+    int layout = SPR_LAYOUT_AUTO;      // layout of the resident volume
+    void* d_vol = nullptr;             // SPR_LAYOUT_LINEAR: the volume as uploaded
     size_t vol_bytes = 0;
+    void* d_pad = nullptr;             // SPR_LAYOUT_PADDED: (nz+2, ny+2, nx+2) apron copy
+    size_t pad_bytes = 0;
     int dtype = -1, nx = 0, ny = 0, nz = 0;
     spr_params params{};
     bool has_params = false;
@@ -36,6 +40,8 @@ struct spr_ctx {
     size_t scratch_px = 0;
     unsigned long long* d_count = nullptr;
     float* d_partial = nullptr;
+    // launch tuning (spr_set_tuning)
+    int tile_map = (int)kTileMapInterleaved;
 };
 
 namespace {
@@ -101,6 +107,15 @@ void free_scratch(spr_ctx* c) {
     c->scratch_px = 0;
 }
 
+void free_volume(spr_ctx* c) {
+    if (c->d_vol) (void)hipFree(c->d_vol);
+    if (c->d_pad) (void)hipFree(c->d_pad);
+    c->d_vol = c->d_pad = nullptr;
+    c->vol_bytes = c->pad_bytes = 0;
+}
+
+bool has_volume(const spr_ctx* c) { return c->d_vol || c->d_pad; }
+
 int ensure_outputs(spr_ctx* c, int w, int h) {
     if (c->w == w && c->h == h && c->d_out) return SPR_OK;
     free_outputs(c);
@@ -145,7 +160,9 @@ RayParams make_params(const spr_ctx* c, int w, int h) {
     p.W = (uint32_t)w;
     p.H = (uint32_t)h;
     p.ntx = (uint32_t)((w + 15) / 16);
-    p.ntiles = p.ntx * (uint32_t)((h + 15) / 16);
+    p.nty = (uint32_t)((h + 15) / 16);
+    p.ntiles = p.ntx * p.nty;
+    p.tile_map = (uint32_t)c->tile_map;
     return p;
 }
 
@@ -168,6 +185,8 @@ ConvWeights conv_weights(int nh, float k) {
     return cw;
 }
 
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
 template <typename T>
 LinearVolume<T> linear_volume(const spr_ctx* c) {
     LinearVolume<T> v;
@@ -181,35 +200,50 @@ LinearVolume<T> linear_volume(const spr_ctx* c) {
     return v;
 }
 
-template <typename T>
-hipError_t launch_mip(const spr_ctx* c, const RayParams& p, float* out, float* alpha,
-                      hipStream_t s) {
-    LinearVolume<T> v = linear_volume<T>(c);
-    dim3 grid(p.ntiles), block(256);
-    if (c->params.filter == SPR_FILTER_NEAREST)
-        hipLaunchKernelGGL((k_max_project<T, 1>), grid, block, 0, s, v, p, out, alpha);
-    else
-        hipLaunchKernelGGL((k_max_project<T, 0>), grid, block, 0, s, v, p, out, alpha);
-    return hipGetLastError();
+template <typename T, bool POW2>
+PaddedVolume<T, POW2> padded_volume(const spr_ctx* c) {
+    PaddedVolume<T, POW2> v;
+    v.base = c->d_pad;
+    v.nbytes = (uint32_t)c->pad_bytes;
+    const uint32_t PX = (uint32_t)c->nx + 2, PY = (uint32_t)c->ny + 2;
+    v.px_bytes = PX * (uint32_t)sizeof(T);
+    v.py = PY;
+    v.soff_y = v.px_bytes;
+    v.soff_z = v.px_bytes * PY;
+    v.soff_yz = v.soff_y + v.soff_z;
+    v.nx = c->nx;
+    v.ny = c->ny;
+    v.nz = c->nz;
+    v.fnx = (float)c->nx;
+    v.fny = (float)c->ny;
+    v.fnz = (float)c->nz;
+    v.mx = (float)(c->nx - 1);
+    v.my = (float)(c->ny - 1);
+    v.mz = (float)(c->nz - 1);
+    return v;
 }
 
-template <typename T>
-hipError_t launch_iso(const spr_ctx* c, const RayParams& p, float* out, float* alpha, float* depth,
-                      float* normals, hipStream_t s) {
-    LinearVolume<T> v = linear_volume<T>(c);
-    dim3 grid(p.ntiles), block(256);
-    if (c->params.filter == SPR_FILTER_NEAREST)
-        hipLaunchKernelGGL((k_iso_surface<T, 1>), grid, block, 0, s, v, p, out, alpha, depth,
-                           normals);
-    else
-        hipLaunchKernelGGL((k_iso_surface<T, 0>), grid, block, 0, s, v, p, out, alpha, depth,
-                           normals);
-    return hipGetLastError();
+// Calls f(volume accessor) with the accessor matching the resident volume's voxel type,
+// layout and (padded layout) whether every dimension is a power of two.
+template <typename T, typename F>
+hipError_t visit_typed(const spr_ctx* c, F&& f) {
+    if (c->layout == SPR_LAYOUT_LINEAR) return f(linear_volume<T>(c));
+    if (is_pow2(c->nx) && is_pow2(c->ny) && is_pow2(c->nz)) return f(padded_volume<T, true>(c));
+    return f(padded_volume<T, false>(c));
+}
+
+template <typename F>
+hipError_t visit_volume(const spr_ctx* c, F&& f) {
+    switch (c->dtype) {
+        case SPR_DTYPE_F32: return visit_typed<float>(c, f);
+        case SPR_DTYPE_U16: return visit_typed<uint16_t>(c, f);
+        default: return visit_typed<uint8_t>(c, f);
+    }
 }
 
 int check_ready(spr_ctx* c, int w, int h) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
-    if (!c->d_vol) return fail(c, SPR_ERR_NO_VOLUME, "no data provided, set_data(data) before");
+    if (!has_volume(c)) return fail(c, SPR_ERR_NO_VOLUME, "no data provided, set_data(data) before");
     if (!c->has_params) return fail(c, SPR_ERR_INVALID_ARG, "spr_set_params was not called");
     if (w <= 0 || h <= 0 || w > 32768 || h > 32768)
         return fail(c, SPR_ERR_INVALID_ARG, "invalid output size %d x %d", w, h);
@@ -223,7 +257,8 @@ int render_impl(spr_ctx* c, int method, int num_parts, int current_part, int w, 
     int rc = check_ready(c, w, h);
     if (rc) return rc;
     RayParams p = make_params(c, w, h);
-    hipError_t e = hipSuccess;
+    const dim3 grid(tile_grid(p.tile_map, p.ntx, p.nty)), block(256);
+    const bool nearest = c->params.filter == SPR_FILTER_NEAREST;
     if (method == SPR_METHOD_MAX_PROJECT) {
         if (num_parts < 1 || (c->params.max_steps / num_parts) / kLoopUnroll < 1)
             return fail(c, SPR_ERR_INVALID_ARG,
@@ -233,11 +268,14 @@ int render_impl(spr_ctx* c, int method, int num_parts, int current_part, int w, 
         if (!out || !alpha) return fail(c, SPR_ERR_INVALID_ARG, "out/alpha buffers are required");
         p.nouter = (c->params.max_steps / num_parts) / kLoopUnroll;
         p.current_part = current_part;
-        switch (c->dtype) {
-            case SPR_DTYPE_F32: e = launch_mip<float>(c, p, out, alpha, s); break;
-            case SPR_DTYPE_U16: e = launch_mip<uint16_t>(c, p, out, alpha, s); break;
-            default: e = launch_mip<uint8_t>(c, p, out, alpha, s); break;
-        }
+        hipError_t e = visit_volume(c, [&](auto v) {
+            using V = decltype(v);
+            if (nearest)
+                hipLaunchKernelGGL((k_max_project<V, 1>), grid, block, 0, s, v, p, out, alpha);
+            else
+                hipLaunchKernelGGL((k_max_project<V, 0>), grid, block, 0, s, v, p, out, alpha);
+            return hipGetLastError();
+        });
         HIPCALL(c, e);
         return SPR_OK;
     }
@@ -250,11 +288,16 @@ int render_impl(spr_ctx* c, int method, int num_parts, int current_part, int w, 
             return fail(c, SPR_ERR_INVALID_ARG, "invalid occlusion parameters");
         rc = ensure_scratch(c, (size_t)w * h);
         if (rc) return rc;
-        switch (c->dtype) {
-            case SPR_DTYPE_F32: e = launch_iso<float>(c, p, out, alpha, depth, normals, s); break;
-            case SPR_DTYPE_U16: e = launch_iso<uint16_t>(c, p, out, alpha, depth, normals, s); break;
-            default: e = launch_iso<uint8_t>(c, p, out, alpha, depth, normals, s); break;
-        }
+        hipError_t e = visit_volume(c, [&](auto v) {
+            using V = decltype(v);
+            if (nearest)
+                hipLaunchKernelGGL((k_iso_surface<V, 1>), grid, block, 0, s, v, p, out, alpha,
+                                   depth, normals);
+            else
+                hipLaunchKernelGGL((k_iso_surface<V, 0>), grid, block, 0, s, v, p, out, alpha,
+                                   depth, normals);
+            return hipGetLastError();
+        });
         HIPCALL(c, e);
         dim3 g2((w + 15) / 16, (h + 15) / 16), b2(256);
         const ConvWeights wv = conv_weights(7, 5.0f), ws = conv_weights(5, 10.0f);
@@ -277,6 +320,12 @@ int render_impl(spr_ctx* c, int method, int num_parts, int current_part, int w, 
     return SPR_OK;
 }
 
+template <typename T>
+void launch_pad(const void* src, void* dst, int nx, int ny, int nz, hipStream_t s) {
+    dim3 grid((nx + 2 + 255) / 256, ny + 2, nz + 2);
+    hipLaunchKernelGGL(k_pad_volume<T>, grid, dim3(256), 0, s, (const T*)src, (T*)dst, nx, ny, nz);
+}
+
 int set_volume_impl(spr_ctx* c, const void* src, bool src_on_device, int dtype, int nx, int ny,
                     int nz) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
@@ -284,30 +333,76 @@ int set_volume_impl(spr_ctx* c, const void* src, bool src_on_device, int dtype, 
     if (!es) return fail(c, SPR_ERR_UNSUPPORTED_DTYPE, "unsupported voxel type code %d", dtype);
     if (!src || nx <= 0 || ny <= 0 || nz <= 0)
         return fail(c, SPR_ERR_INVALID_ARG, "invalid volume (%d, %d, %d)", nz, ny, nx);
+    if (nx > 65000 || ny > 65000 || nz > 65000)
+        return fail(c, SPR_ERR_INVALID_ARG, "volume dimension above 65000");
     const size_t nvox = (size_t)nx * ny * nz;
     if (nvox > 0xFFFFFFFFull)
         return fail(c, SPR_ERR_INVALID_ARG, "volume of %zu voxels exceeds 2^32-1", nvox);
     DeviceGuard g(c->device);
     const size_t bytes = nvox * es;
-    if (c->d_vol && c->vol_bytes != bytes) {
-        HIPCALL(c, hipStreamSynchronize(c->stream));
+    const size_t pbytes = (size_t)(nx + 2) * (ny + 2) * (nz + 2) * es;
+    // the padded layout is addressed with 32-bit buffer offsets built from 24-bit products
+    const bool pad_fits = pbytes <= 0xFFFFFFFFull && (size_t)(nz + 2) * (ny + 2) < (1u << 24) &&
+                          (size_t)(nx + 2) * es < (1u << 24);
+    int layout = c->layout_req;
+    if (layout == SPR_LAYOUT_AUTO) layout = pad_fits ? SPR_LAYOUT_PADDED : SPR_LAYOUT_LINEAR;
+    if (layout == SPR_LAYOUT_PADDED && !pad_fits)
+        return fail(c, SPR_ERR_INVALID_ARG, "padded volume of %zu bytes exceeds 4 GiB", pbytes);
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    // a device source may still be in production on another stream (the context's stream
+    // does not synchronise with the null stream): wait for all prior work on the device
+    if (src_on_device) HIPCALL(c, hipDeviceSynchronize());
+
+    // linear copy: kept as the resident volume (LINEAR), or staged for padding (host source)
+    const bool need_linear = layout == SPR_LAYOUT_LINEAR || !src_on_device;
+    if (c->d_pad && (layout != SPR_LAYOUT_PADDED || c->pad_bytes != pbytes)) {
+        HIPCALL(c, hipFree(c->d_pad));
+        c->d_pad = nullptr;
+        c->pad_bytes = 0;
+    }
+    if (c->d_vol && (!need_linear || c->vol_bytes != bytes)) {
         HIPCALL(c, hipFree(c->d_vol));
         c->d_vol = nullptr;
         c->vol_bytes = 0;
     }
-    if (!c->d_vol) {
+    if (need_linear && !c->d_vol) {
         HIPCALL(c, hipMalloc(&c->d_vol, bytes));
         c->vol_bytes = bytes;
     }
-    HIPCALL(c, hipMemcpyAsync(c->d_vol, src, bytes,
-                              src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                              c->stream));
+    if (need_linear)
+        HIPCALL(c, hipMemcpyAsync(c->d_vol, src, bytes,
+                                  src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                  c->stream));
+    if (layout == SPR_LAYOUT_PADDED) {
+        if (!c->d_pad) {
+            HIPCALL(c, hipMalloc(&c->d_pad, pbytes));
+            c->pad_bytes = pbytes;
+        }
+        const void* from = need_linear ? c->d_vol : src;
+        switch (dtype) {
+            case SPR_DTYPE_F32: launch_pad<float>(from, c->d_pad, nx, ny, nz, c->stream); break;
+            case SPR_DTYPE_U16: launch_pad<uint16_t>(from, c->d_pad, nx, ny, nz, c->stream); break;
+            default: launch_pad<uint8_t>(from, c->d_pad, nx, ny, nz, c->stream); break;
+        }
+        HIPCALL(c, hipGetLastError());
+    }
     HIPCALL(c, hipStreamSynchronize(c->stream));
+    if (layout == SPR_LAYOUT_PADDED && c->d_vol) {  // staging copy no longer needed
+        HIPCALL(c, hipFree(c->d_vol));
+        c->d_vol = nullptr;
+        c->vol_bytes = 0;
+    }
+    c->layout = layout;
     c->dtype = dtype;
     c->nx = nx;
     c->ny = ny;
     c->nz = nz;
     return SPR_OK;
+}
+
+template <typename T>
+void launch_minmax(const void* data, size_t n, float* partial, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_minmax<T>), dim3(blocks), dim3(256), 0, s, (const T*)data, n, partial);
 }
 
 }  // namespace
@@ -367,7 +462,7 @@ int spr_destroy(spr_ctx* c) {
         if (c->stream) (void)hipStreamSynchronize(c->stream);
         free_outputs(c);
         free_scratch(c);
-        if (c->d_vol) (void)hipFree(c->d_vol);
+        free_volume(c);
         if (c->d_count) (void)hipFree(c->d_count);
         if (c->d_partial) (void)hipFree(c->d_partial);
         if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -377,6 +472,40 @@ int spr_destroy(spr_ctx* c) {
 }
 
 const char* spr_last_error(const spr_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int spr_set_layout(spr_ctx* c, int layout) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (layout != SPR_LAYOUT_AUTO && layout != SPR_LAYOUT_LINEAR && layout != SPR_LAYOUT_PADDED)
+        return fail(c, SPR_ERR_INVALID_ARG, "unknown layout %d", layout);
+    c->layout_req = layout;
+    return SPR_OK;
+}
+
+int spr_get_layout(spr_ctx* c, int* layout) {
+    if (!c || !layout) return fail(c, SPR_ERR_INVALID_ARG, "NULL argument");
+    if (!has_volume(c)) return fail(c, SPR_ERR_NO_VOLUME, "no volume");
+    *layout = c->layout;
+    return SPR_OK;
+}
+
+int spr_set_tuning(spr_ctx* c, int key, int value) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    switch (key) {
+        case SPR_TUNE_TILE_MAP:
+            if (value < 0 || value > 2) return fail(c, SPR_ERR_INVALID_ARG, "tile map %d", value);
+            c->tile_map = value;
+            return SPR_OK;
+        default: return fail(c, SPR_ERR_INVALID_ARG, "unknown tuning key %d", key);
+    }
+}
+
+int spr_get_tuning(spr_ctx* c, int key, int* value) {
+    if (!c || !value) return fail(c, SPR_ERR_INVALID_ARG, "NULL argument");
+    switch (key) {
+        case SPR_TUNE_TILE_MAP: *value = c->tile_map; return SPR_OK;
+        default: return fail(c, SPR_ERR_INVALID_ARG, "unknown tuning key %d", key);
+    }
+}
 
 int spr_set_volume(spr_ctx* c, const void* host, int dtype, int nx, int ny, int nz) {
     return set_volume_impl(c, host, false, dtype, nx, ny, nz);
@@ -388,24 +517,17 @@ int spr_set_volume_device(spr_ctx* c, const void* dev, int dtype, int nx, int ny
 
 int spr_volume_minmax(spr_ctx* c, float* mn, float* mx) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
-    if (!c->d_vol) return fail(c, SPR_ERR_NO_VOLUME, "no volume");
+    if (!has_volume(c)) return fail(c, SPR_ERR_NO_VOLUME, "no volume");
     if (!mn || !mx) return fail(c, SPR_ERR_INVALID_ARG, "NULL output");
     DeviceGuard g(c->device);
-    const size_t n = (size_t)c->nx * c->ny * c->nz;
+    // the apron replicates edge voxels, so min/max over the padded copy are the volume's
+    const void* data = c->d_pad ? c->d_pad : c->d_vol;
+    const size_t n = (c->d_pad ? c->pad_bytes : c->vol_bytes) / dtype_size(c->dtype);
     const int blocks = 1024;
     switch (c->dtype) {
-        case SPR_DTYPE_F32:
-            hipLaunchKernelGGL((k_minmax<float>), dim3(blocks), dim3(256), 0, c->stream,
-                               (const float*)c->d_vol, n, c->d_partial);
-            break;
-        case SPR_DTYPE_U16:
-            hipLaunchKernelGGL((k_minmax<uint16_t>), dim3(blocks), dim3(256), 0, c->stream,
-                               (const uint16_t*)c->d_vol, n, c->d_partial);
-            break;
-        default:
-            hipLaunchKernelGGL((k_minmax<uint8_t>), dim3(blocks), dim3(256), 0, c->stream,
-                               (const uint8_t*)c->d_vol, n, c->d_partial);
-            break;
+        case SPR_DTYPE_F32: launch_minmax<float>(data, n, c->d_partial, blocks, c->stream); break;
+        case SPR_DTYPE_U16: launch_minmax<uint16_t>(data, n, c->d_partial, blocks, c->stream); break;
+        default: launch_minmax<uint8_t>(data, n, c->d_partial, blocks, c->stream); break;
     }
     HIPCALL(c, hipGetLastError());
     std::vector<float> part(2 * blocks);
@@ -491,7 +613,8 @@ int spr_count_samples(spr_ctx* c, int num_parts, int w, int h, double* samples) 
     DeviceGuard g(c->device);
     RayParams p = make_params(c, w, h);
     HIPCALL(c, hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->stream));
-    hipLaunchKernelGGL(k_count_hits, dim3(p.ntiles), dim3(256), 0, c->stream, p, c->d_count);
+    hipLaunchKernelGGL(k_count_hits, dim3(tile_grid(p.tile_map, p.ntx, p.nty)), dim3(256), 0,
+                       c->stream, p, c->d_count);
     HIPCALL(c, hipGetLastError());
     unsigned long long n = 0;
     HIPCALL(c, hipMemcpyAsync(&n, c->d_count, sizeof(n), hipMemcpyDeviceToHost, c->stream));

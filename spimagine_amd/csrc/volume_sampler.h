@@ -17,6 +17,15 @@ __device__ __forceinline__ float voxel_f(const T* __restrict__ p, uint32_t i) {
     return (float)p[i];
 }
 
+// texel coordinate u' = s*N - 0.5.  When N is a power of two s*N is exact, so one fused
+// multiply-add gives the same single rounding as the two-step form.
+template <bool POW2>
+__device__ __forceinline__ float texel(float s, float fn) {
+    if (POW2) return fmaf(s, fn, -0.5f);
+    float u = s * fn;
+    return u - 0.5f;
+}
+
 // floor of a texel coordinate kept in [-1, N] before the integer conversion; every
 // value outside that range selects the same clamped corner pair
 __device__ __forceinline__ int floor_idx(float f, float fn) { return (int)fminf(fmaxf(f, -1.0f), fn); }
@@ -24,6 +33,7 @@ __device__ __forceinline__ int clampi(int a, int lo, int hi) { return min(max(a,
 
 template <typename T>
 struct LinearVolume {
+    using value_type = T;
     const T* __restrict__ data;
     int nx, ny, nz;
     float fnx, fny, fnz;
@@ -66,5 +76,131 @@ struct LinearVolume {
         return linear(px, py, pz);
     }
 };
+
+// ---------------------------------------------------------------------------------
+// Padded ("apron") layout: a (nz+2, ny+2, nx+2) copy whose outer shell replicates the
+// nearest edge voxel, padded[p] = vol[clamp(p - 1, 0, N - 1)] per axis.  A linear sample
+// with i0 = floor(u') clamped to [-1, N-1] then reads the corner pair (i0+1, i0+2) of the
+// padded grid, which holds exactly the clamp-to-edge pair (vol[clamp(i0)],
+// vol[clamp(i0+1)]): no per-corner clamping, and the x-neighbour pair is one (unaligned)
+// load.  The four pair loads of a sample share one byte offset; the y and z neighbours
+// are added through the buffer instruction's scalar offset.
+
+template <typename T>
+struct PairBits;
+template <>
+struct PairBits<uint16_t> {
+    __device__ __forceinline__ static void load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff,
+                                                float* lo, float* hi) {
+        uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0);
+        *lo = (float)(v & 0xFFFFu);
+        *hi = (float)(v >> 16);
+    }
+    __device__ __forceinline__ static float one(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        return (float)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+    }
+};
+template <>
+struct PairBits<uint8_t> {
+    __device__ __forceinline__ static void load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff,
+                                                float* lo, float* hi) {
+        uint32_t v = __builtin_amdgcn_raw_buffer_load_b16(r, off, soff, 0);
+        *lo = (float)(v & 0xFFu);
+        *hi = (float)((v >> 8) & 0xFFu);
+    }
+    __device__ __forceinline__ static float one(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        return (float)__builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+    }
+};
+template <>
+struct PairBits<float> {
+    __device__ __forceinline__ static void load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff,
+                                                float* lo, float* hi) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
+        *lo = __builtin_bit_cast(float, (uint32_t)v[0]);
+        *hi = __builtin_bit_cast(float, (uint32_t)v[1]);
+    }
+    __device__ __forceinline__ static float one(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    }
+};
+
+// word 3 of a gfx9-family buffer resource (raw buffer, 32-bit data format)
+constexpr int kBufferWord3 = 0x00020000;
+
+template <typename T, bool POW2>
+struct PaddedVolume {
+    using value_type = T;
+    const void* base;             // padded volume
+    uint32_t nbytes;              // its size in bytes (the buffer's num_records)
+    uint32_t px_bytes;            // row pitch (nx+2)*sizeof(T)
+    uint32_t py;                  // padded rows per plane, ny+2
+    uint32_t soff_y, soff_z, soff_yz;  // byte strides of the y, z and y+z neighbours
+    int nx, ny, nz;
+    float fnx, fny, fnz;           // N
+    float mx, my, mz;              // N - 1
+
+    // raw buffer resource over the padded volume (uniform: lives in SGPRs)
+    __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc() const {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)nbytes,
+                                                 kBufferWord3);
+    }
+
+    __device__ __forceinline__ uint32_t byte_offset(uint32_t ix1, uint32_t iy1, uint32_t iz1) const {
+        uint32_t row = __umul24(iz1, py) + iy1;
+        return __umul24(row, px_bytes) + ix1 * (uint32_t)sizeof(T);
+    }
+
+    __device__ __forceinline__ float linear(float px, float py_, float pz) const {
+        float ux = texel<POW2>(px, fnx), uy = texel<POW2>(py_, fny), uz = texel<POW2>(pz, fnz);
+        float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
+        float ax = ux - fx, ay = uy - fy, az = uz - fz;
+        uint32_t ix1 = (uint32_t)((int)fminf(fmaxf(fx, -1.0f), mx) + 1);
+        uint32_t iy1 = (uint32_t)((int)fminf(fmaxf(fy, -1.0f), my) + 1);
+        uint32_t iz1 = (uint32_t)((int)fminf(fmaxf(fz, -1.0f), mz) + 1);
+        const uint32_t off = byte_offset(ix1, iy1, iz1);
+        const __amdgpu_buffer_rsrc_t r = rsrc();
+        float v000, v100, v010, v110, v001, v101, v011, v111;
+        PairBits<T>::load(r, off, 0, &v000, &v100);
+        PairBits<T>::load(r, off, soff_y, &v010, &v110);
+        PairBits<T>::load(r, off, soff_z, &v001, &v101);
+        PairBits<T>::load(r, off, soff_yz, &v011, &v111);
+        float c00 = lerpf(v000, v100, ax);
+        float c10 = lerpf(v010, v110, ax);
+        float c01 = lerpf(v001, v101, ax);
+        float c11 = lerpf(v011, v111, ax);
+        float c0 = lerpf(c00, c10, ay);
+        float c1 = lerpf(c01, c11, ay);
+        return lerpf(c0, c1, az);
+    }
+
+    __device__ __forceinline__ float nearest(float px, float py_, float pz) const {
+        float ux = px * fnx, uy = py_ * fny, uz = pz * fnz;
+        uint32_t ix1 = (uint32_t)((int)fminf(fmaxf(floorf(ux), 0.0f), mx) + 1);
+        uint32_t iy1 = (uint32_t)((int)fminf(fmaxf(floorf(uy), 0.0f), my) + 1);
+        uint32_t iz1 = (uint32_t)((int)fminf(fmaxf(floorf(uz), 0.0f), mz) + 1);
+        return PairBits<T>::one(rsrc(), byte_offset(ix1, iy1, iz1));
+    }
+
+    template <int FILTER>
+    __device__ __forceinline__ float sample(float px, float py_, float pz) const {
+        if (FILTER == 1) return nearest(px, py_, pz);
+        return linear(px, py_, pz);
+    }
+};
+
+// padded[z][y][x] = vol[clamp(z-1)][clamp(y-1)][clamp(x-1)]
+template <typename T>
+__global__ __launch_bounds__(256) void k_pad_volume(const T* __restrict__ src, T* __restrict__ dst,
+                                                    int nx, int ny, int nz) {
+    const int PX = nx + 2, PY = ny + 2;
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= PX) return;
+    const int sx = min(max(x - 1, 0), nx - 1);
+    const int sy = min(max(y - 1, 0), ny - 1);
+    const int sz = min(max(z - 1, 0), nz - 1);
+    dst[((size_t)z * PY + y) * PX + x] = src[((size_t)sz * ny + sy) * nx + sx];
+}
 
 }  // namespace spr

@@ -112,6 +112,54 @@ def test_mip_modes(VR, dtype, interp, alpha_pow, gamma):
     np.testing.assert_array_equal(rend.output_alpha, alpha)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.uint16, np.uint8])
+@pytest.mark.parametrize("shape", [(29, 50, 37), (32, 64, 16), (1, 1, 1), (2, 3, 1)])
+def test_layouts_are_identical(VR, dtype, shape):
+    """The padded (apron) layout and the linear layout give bit-identical frames for every
+    voxel type, both filters and both render modes, including degenerate volumes."""
+    rng = np.random.default_rng(13)
+    vol = (rng.uniform(0, 200, shape)).astype(dtype)
+    outs = {}
+    for layout in (_lib.LAYOUT_LINEAR, _lib.LAYOUT_PADDED):
+        res = []
+        for interp in ("linear", "nearest"):
+            rend = VR((61, 45), interpolation=interp)
+            rend._ctx.set_layout(layout)
+            rend.set_max_steps(64)
+            rend.set_data(vol)
+            assert rend._ctx.layout() == layout
+            rend.set_modelView(np.dot(mat4_translate(0, .1, -3.), mat4_rotation(.9, 1, .5, .2)))
+            rend.render(maxVal=200.)
+            res.append(rend.output.copy())
+            rend.render(maxVal=200., method="iso_surface")
+            res += [rend.output.copy(), rend.output_depth.copy(), rend.output_normals.copy()]
+        outs[layout] = res
+    for a, b in zip(outs[_lib.LAYOUT_LINEAR], outs[_lib.LAYOUT_PADDED]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_tuning_does_not_change_results(VR):
+    vol = synth.blob_volume((40, 48, 56), n_blobs=10, sigma=(2., 5.), amp=(500., 3000.),
+                            background=100., poisson=True, read_noise=5., dtype="uint16", seed=4)
+    frames = []
+    for tile_map in (0, 1, 2):
+        rend = VR((203, 150))
+        rend._ctx.set_tuning("tile_map", tile_map)
+        assert rend._ctx.get_tuning("tile_map") == tile_map
+        rend.set_data(vol)
+        rend.set_modelView(np.dot(mat4_translate(0, 0, -3.), mat4_rotation(.6, 0, 1, 0)))
+        rend.render(minVal=float(vol.min()), maxVal=float(vol.max()))
+        frames.append(rend.output.copy())
+        rend.render(maxVal=1500., method="iso_surface")
+        frames.append(rend.output.copy())
+    np.testing.assert_array_equal(frames[0], frames[2])
+    np.testing.assert_array_equal(frames[0], frames[4])
+    np.testing.assert_array_equal(frames[1], frames[3])
+    np.testing.assert_array_equal(frames[1], frames[5])
+    with pytest.raises(_lib.SprError):
+        rend._ctx.set_tuning("tile_map", 9)
+
+
 def test_multipass_parts_merge(VR):
     vol = synth.blob_volume((40, 40, 40), n_blobs=10, sigma=(2., 4.), amp=(.3, 1.),
                             background=.05, poisson=False, read_noise=0., dtype="float32", seed=2)
