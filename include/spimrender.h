@@ -66,8 +66,11 @@ extern "C" {
    depend on them.
    SPR_TUNE_TILE_MAP  how 16x16 pixel tiles are dealt to workgroups and so to the 8 XCDs:
                       0 = tile rows interleaved over XCDs (default), 1 = one contiguous
-                      band per XCD, 2 = row-major */
+                      band per XCD, 2 = row-major
+   SPR_TUNE_SLAB_KB   per-wave LDS slab (KiB) of the staged max projection: 8, 12
+                      (default) or 16; 0 = sample every chunk from global memory */
 #define SPR_TUNE_TILE_MAP 0
+#define SPR_TUNE_SLAB_KB 1
 
 typedef struct spr_ctx spr_ctx;
 

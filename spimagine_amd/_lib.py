@@ -25,8 +25,8 @@ DTYPE_CODES = {np.dtype(np.float32): DTYPE_F32, np.dtype(np.uint16): DTYPE_U16,
 FILTER_LINEAR, FILTER_NEAREST = 0, 1
 METHOD_MAX_PROJECT, METHOD_ISO_SURFACE = 0, 1
 LAYOUT_AUTO, LAYOUT_LINEAR, LAYOUT_PADDED = 0, 1, 2
-TUNE_TILE_MAP = 0
-TUNING_KEYS = {"tile_map": TUNE_TILE_MAP}
+TUNE_TILE_MAP, TUNE_SLAB_KB = 0, 1
+TUNING_KEYS = {"tile_map": TUNE_TILE_MAP, "slab_kb": TUNE_SLAB_KB}
 
 # every symbol include/spimrender.h declares
 EXPORTS = (

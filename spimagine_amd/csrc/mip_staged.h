@@ -1,0 +1,221 @@
+// mip_staged.h -- max projection with per-wave LDS staging of the volume.
+//
+// Same result, bit for bit, as k_max_project on the padded layout (volume_kernel.cl:20-185,
+// :190-345 with alpha_pow == 0): identical sample positions, weights and lerp order; only
+// where the eight corners are read from changes.
+//
+// Why: the 64 rays of a wave sit ~1 voxel apart on a plane oblique to the volume axes, so
+// a wave-wide global gather touches ~55 distinct 64-byte granules and the vector-L1 tag
+// path, not HBM or VALU, limits the plain kernel.  Here each wave walks its rays in chunks
+// of the reference's 16-sample inner loop: it bounds the padded-voxel box those 16 samples
+// of its 64 rays touch (DPP reductions), copies the box into its own LDS slab with aligned
+// 16-byte row loads (one granule per 4 lanes), then gathers the corners from LDS.  A box
+// larger than the slab (rays of one wave with very different dt, near the box silhouette)
+// is sampled from global memory for that chunk instead.  No workgroup barrier: each wave
+// owns its slab and LDS operations of one wave complete in order.
+#pragma once
+#include "render_kernels.h"
+
+namespace spr {
+
+typedef unsigned short spr_us2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pk_min_u16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(spr_us2, a),
+                                                                  __builtin_bit_cast(spr_us2, b)));
+}
+
+// wave64 minimum of each 16-bit half (row_shr 1/2/4/8, row_bcast 15/31), uniform result
+__device__ __forceinline__ uint32_t wave_min_u16x2(uint32_t v) {
+    v = pk_min_u16(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xf, 0xf, false));
+    v = pk_min_u16(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xf, 0xf, false));
+    v = pk_min_u16(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xf, 0xf, false));
+    v = pk_min_u16(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xf, 0xf, false));
+    v = pk_min_u16(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xa, 0xf, false));
+    v = pk_min_u16(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+template <typename T>
+__device__ __forceinline__ float lds_voxel(const uint8_t* p) {
+    return (float)*reinterpret_cast<const T*>(p);
+}
+
+// 16-byte aligned load / store units
+struct alignas(16) Unit16 {
+    uint32_t w[4];
+};
+
+template <typename T, bool POW2, int SLAB_BYTES>
+__global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2> vol, RayParams p,
+                                                            float* __restrict__ out,
+                                                            float* __restrict__ alpha) {
+    constexpr bool kShort = !std::is_same<T, float>::value;
+    constexpr uint32_t E = 16u / sizeof(T);  // voxels per 16-byte unit
+    constexpr uint32_t TS = sizeof(T);
+    __shared__ Unit16 lds[4 * SLAB_BYTES / 16];
+    uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * SLAB_BYTES;
+    const uint32_t lane = threadIdx.x & 63u;
+
+    uint32_t x = 0, y = 0;
+    const bool inside = tile_pixel(p, &x, &y);
+    bool active = false;
+    float px = 0.5f, py = 0.5f, pz = 0.5f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    float tnear = 0.0f;
+    const uint32_t o = x + p.W * y;
+    if (inside) {
+        F4 orig, direc;
+        ray_setup(p.invP, p.invM, x, y, p.W, p.H, &orig, &direc);
+        float tfar;
+        if (!intersect_box(orig, direc, p.box, &tnear, &tfar)) {
+            out[o] = 0.0f;
+            alpha[o] = kShort ? 0.0f : -1.0f;
+        } else {
+            if (tnear < 0.0f) tnear = 0.0f;
+            const float dt = fabsf(tfar - tnear) / (float)(p.nouter * kLoopUnroll);
+            orig = add4(orig, scl4((float)p.current_part * dt, direc));
+            const F4 delta = scl4(0.5f * dt, direc);
+            const F4 pos = scl4(0.5f, add4(mk4(1.0f + orig.x, 1.0f + orig.y, 1.0f + orig.z,
+                                               1.0f + orig.w),
+                                           scl4(tnear, direc)));
+            px = pos.x;
+            py = pos.y;
+            pz = pos.z;
+            dx = delta.x;
+            dy = delta.y;
+            dz = delta.z;
+            active = true;
+        }
+    }
+    const uint64_t act = __ballot(active);
+    if (act == 0) return;
+    // lanes without a ray shadow the first active lane, so every lane's LDS address is valid
+    // (the shuffles run with every lane of the wave enabled)
+    {
+        const int src = __builtin_ctzll(act);
+        const float spx = __shfl(px, src), spy = __shfl(py, src), spz = __shfl(pz, src);
+        const float sdx = __shfl(dx, src), sdy = __shfl(dy, src), sdz = __shfl(dz, src);
+        px = active ? px : spx;
+        py = active ? py : spy;
+        pz = active ? pz : spz;
+        dx = active ? dx : sdx;
+        dy = active ? dy : sdy;
+        dz = active ? dz : sdz;
+    }
+
+    const uint32_t PY = vol.py, pitch = vol.px_bytes;
+    const __amdgpu_buffer_rsrc_t rsrc = vol.rsrc();
+    // texel-unit margin around the estimated last sample of a chunk (its position is
+    // computed with one fma instead of 15 sequential adds)
+    const float mgx = 1e-5f * vol.fnx + 0.01f, mgy = 1e-5f * vol.fny + 0.01f,
+                mgz = 1e-5f * vol.fnz + 0.01f;
+    float col = 0.0f;
+
+    for (int i = 0; i <= p.nouter; ++i) {
+        // ---- box of padded voxels this chunk's 16 samples of the wave's rays touch
+        uint32_t b0 = 0xFFFFFFFFu, b1 = 0xFFFFFFFFu, b2 = 0xFFFFFFFFu;
+        if (active) {
+            const float ex = fmaf(15.0f, dx, px), ey = fmaf(15.0f, dy, py), ez = fmaf(15.0f, dz, pz);
+            const float u0x = texel<POW2>(px, vol.fnx), u0y = texel<POW2>(py, vol.fny),
+                        u0z = texel<POW2>(pz, vol.fnz);
+            const float u1x = texel<POW2>(ex, vol.fnx), u1y = texel<POW2>(ey, vol.fny),
+                        u1z = texel<POW2>(ez, vol.fnz);
+            const uint32_t a0x = vol.corner1(floorf(u0x), vol.mx);
+            const uint32_t a0y = vol.corner1(floorf(u0y), vol.my);
+            const uint32_t a0z = vol.corner1(floorf(u0z), vol.mz);
+            const uint32_t lx = min(a0x, vol.corner1(floorf(u1x - mgx), vol.mx));
+            const uint32_t ly = min(a0y, vol.corner1(floorf(u1y - mgy), vol.my));
+            const uint32_t lz = min(a0z, vol.corner1(floorf(u1z - mgz), vol.mz));
+            const uint32_t hx = max(a0x, vol.corner1(floorf(u1x + mgx), vol.mx)) + 1u;
+            const uint32_t hy = max(a0y, vol.corner1(floorf(u1y + mgy), vol.my)) + 1u;
+            const uint32_t hz = max(a0z, vol.corner1(floorf(u1z + mgz), vol.mz)) + 1u;
+            b0 = lx | (ly << 16);
+            b1 = lz | ((0xFFFFu - hx) << 16);
+            b2 = (0xFFFFu - hy) | ((0xFFFFu - hz) << 16);
+        }
+        const uint32_t r0 = wave_min_u16x2(b0), r1 = wave_min_u16x2(b1), r2 = wave_min_u16x2(b2);
+        const uint32_t LX = r0 & 0xFFFFu, LY = r0 >> 16, LZ = r1 & 0xFFFFu;
+        const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
+                       HZ = 0xFFFFu - (r2 >> 16);
+        const uint32_t lox = LX & ~(E - 1u);
+        const uint32_t SX = (HX + 1u - lox + E - 1u) & ~(E - 1u);  // voxels, multiple of E
+        const uint32_t SY = HY - LY + 1u, SZ = HZ - LZ + 1u;
+        const uint32_t nu = SX / E;                                 // 16-byte units per row
+        const uint64_t bytes = (uint64_t)SX * SY * SZ * TS;
+
+        if (bytes <= (uint64_t)SLAB_BYTES && nu <= 64u) {
+            // ---- stage: row r of the box (r = z*SY + y) -> slab[r*SX*TS ...]
+            const uint32_t R = 64u / nu;  // rows per pass
+            const uint32_t lr = lane / nu, lu = lane - lr * nu;
+            const uint32_t nrows = SY * SZ;
+            const uint32_t gx = (lox * TS) + lu * 16u;
+            const uint32_t rowb = SX * TS;
+            if (lr < R) {
+                const float inv_sy = 1.0f / (float)SY;
+                for (uint32_t r = lr; r < nrows; r += R) {
+                    uint32_t zr = (uint32_t)((float)r * inv_sy);
+                    int32_t yr = (int32_t)(r - zr * SY);
+                    if (yr < 0) {
+                        zr -= 1u;
+                        yr += (int32_t)SY;
+                    } else if (yr >= (int32_t)SY) {
+                        zr += 1u;
+                        yr -= (int32_t)SY;
+                    }
+                    const uint32_t goff =
+                        __umul24(__umul24(LZ + zr, PY) + LY + (uint32_t)yr, pitch) + gx;
+                    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, goff, 0, 0);
+                    Unit16 u;
+                    u.w[0] = v[0];
+                    u.w[1] = v[1];
+                    u.w[2] = v[2];
+                    u.w[3] = v[3];
+                    *reinterpret_cast<Unit16*>(slab + r * rowb + lu * 16u) = u;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            // ---- 16 samples from the slab
+            const uint32_t sy = rowb, sz = rowb * SY;
+#pragma unroll
+            for (int j = 0; j < kLoopUnroll; ++j) {
+                uint32_t ix1, iy1, iz1;
+                float ax, ay, az;
+                vol.locate(px, py, pz, &ix1, &iy1, &iz1, &ax, &ay, &az);
+                const uint32_t b = __umul24(iz1 - LZ, sz) + __umul24(iy1 - LY, sy) + (ix1 - lox) * TS;
+                const uint8_t* q = slab + b;
+                const float v000 = lds_voxel<T>(q), v100 = lds_voxel<T>(q + TS);
+                const float v010 = lds_voxel<T>(q + sy), v110 = lds_voxel<T>(q + sy + TS);
+                const float v001 = lds_voxel<T>(q + sz), v101 = lds_voxel<T>(q + sz + TS);
+                const float v011 = lds_voxel<T>(q + sy + sz), v111 = lds_voxel<T>(q + sy + sz + TS);
+                col = fmaxf(col, trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az));
+                px = px + dx;
+                py = py + dy;
+                pz = pz + dz;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        } else {
+#pragma unroll
+            for (int j = 0; j < kLoopUnroll; ++j) {
+                col = fmaxf(col, vol.linear(px, py, pz));
+                px = px + dx;
+                py = py + dy;
+                pz = pz + dz;
+            }
+        }
+    }
+    if (!active) return;
+    col = (p.max_val == 0.0f) ? col : (col - p.min_val) / (p.max_val - p.min_val);
+    col = apply_gamma(col, p.gamma);
+    const float a = kShort ? tnear : 1.0f;
+    if (p.current_part == 0) {
+        out[o] = col;
+        alpha[o] = a;
+    } else {
+        out[o] = fmaxf(col, out[o]);
+        alpha[o] = fmaxf(a, alpha[o]);
+    }
+}
+
+}  // namespace spr

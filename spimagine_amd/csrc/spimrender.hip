@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "mip_staged.h"
 #include "render_kernels.h"
 #include "spimrender.h"
 
@@ -42,6 +43,7 @@ struct spr_ctx {
     float* d_partial = nullptr;
     // launch tuning (spr_set_tuning)
     int tile_map = (int)kTileMapInterleaved;
+    int slab_kb = 12;  // per-wave LDS slab of the staged max projection, 0 = off
 };
 
 namespace {
@@ -205,8 +207,8 @@ PaddedVolume<T, POW2> padded_volume(const spr_ctx* c) {
     PaddedVolume<T, POW2> v;
     v.base = c->d_pad;
     v.nbytes = (uint32_t)c->pad_bytes;
-    const uint32_t PX = (uint32_t)c->nx + 2, PY = (uint32_t)c->ny + 2;
-    v.px_bytes = PX * (uint32_t)sizeof(T);
+    const uint32_t PY = (uint32_t)c->ny + 2;
+    v.px_bytes = padded_pitch<T>(c->nx) * (uint32_t)sizeof(T);
     v.py = PY;
     v.soff_y = v.px_bytes;
     v.soff_z = v.px_bytes * PY;
@@ -268,8 +270,25 @@ int render_impl(spr_ctx* c, int method, int num_parts, int current_part, int w, 
         if (!out || !alpha) return fail(c, SPR_ERR_INVALID_ARG, "out/alpha buffers are required");
         p.nouter = (c->params.max_steps / num_parts) / kLoopUnroll;
         p.current_part = current_part;
+        const bool staged = !nearest && c->params.alpha_pow == 0.0f && c->slab_kb > 0;
         hipError_t e = visit_volume(c, [&](auto v) {
             using V = decltype(v);
+            if constexpr (is_padded<V>::value) {
+                if (staged) {
+                    using T = typename V::value_type;
+                    constexpr bool P2 = V::kPow2;
+                    if (c->slab_kb == 8)
+                        hipLaunchKernelGGL((k_max_project_staged<T, P2, 8192>), grid, block, 0, s, v,
+                                           p, out, alpha);
+                    else if (c->slab_kb == 16)
+                        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384>), grid, block, 0, s,
+                                           v, p, out, alpha);
+                    else
+                        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288>), grid, block, 0, s,
+                                           v, p, out, alpha);
+                    return hipGetLastError();
+                }
+            }
             if (nearest)
                 hipLaunchKernelGGL((k_max_project<V, 1>), grid, block, 0, s, v, p, out, alpha);
             else
@@ -322,7 +341,7 @@ int render_impl(spr_ctx* c, int method, int num_parts, int current_part, int w, 
 
 template <typename T>
 void launch_pad(const void* src, void* dst, int nx, int ny, int nz, hipStream_t s) {
-    dim3 grid((nx + 2 + 255) / 256, ny + 2, nz + 2);
+    dim3 grid((padded_pitch<T>(nx) + 255) / 256, ny + 2, nz + 2);
     hipLaunchKernelGGL(k_pad_volume<T>, grid, dim3(256), 0, s, (const T*)src, (T*)dst, nx, ny, nz);
 }
 
@@ -340,10 +359,11 @@ int set_volume_impl(spr_ctx* c, const void* src, bool src_on_device, int dtype, 
         return fail(c, SPR_ERR_INVALID_ARG, "volume of %zu voxels exceeds 2^32-1", nvox);
     DeviceGuard g(c->device);
     const size_t bytes = nvox * es;
-    const size_t pbytes = (size_t)(nx + 2) * (ny + 2) * (nz + 2) * es;
+    const size_t pitch = (size_t)(nx + 2 + 16 / es - 1) / (16 / es) * (16 / es);  // padded_pitch
+    const size_t pbytes = pitch * (ny + 2) * (nz + 2) * es;
     // the padded layout is addressed with 32-bit buffer offsets built from 24-bit products
     const bool pad_fits = pbytes <= 0xFFFFFFFFull && (size_t)(nz + 2) * (ny + 2) < (1u << 24) &&
-                          (size_t)(nx + 2) * es < (1u << 24);
+                          pitch * es < (1u << 24);
     int layout = c->layout_req;
     if (layout == SPR_LAYOUT_AUTO) layout = pad_fits ? SPR_LAYOUT_PADDED : SPR_LAYOUT_LINEAR;
     if (layout == SPR_LAYOUT_PADDED && !pad_fits)
@@ -495,6 +515,11 @@ int spr_set_tuning(spr_ctx* c, int key, int value) {
             if (value < 0 || value > 2) return fail(c, SPR_ERR_INVALID_ARG, "tile map %d", value);
             c->tile_map = value;
             return SPR_OK;
+        case SPR_TUNE_SLAB_KB:
+            if (value != 0 && value != 8 && value != 12 && value != 16)
+                return fail(c, SPR_ERR_INVALID_ARG, "slab size %d KiB (0, 8, 12 or 16)", value);
+            c->slab_kb = value;
+            return SPR_OK;
         default: return fail(c, SPR_ERR_INVALID_ARG, "unknown tuning key %d", key);
     }
 }
@@ -503,6 +528,7 @@ int spr_get_tuning(spr_ctx* c, int key, int* value) {
     if (!c || !value) return fail(c, SPR_ERR_INVALID_ARG, "NULL argument");
     switch (key) {
         case SPR_TUNE_TILE_MAP: *value = c->tile_map; return SPR_OK;
+        case SPR_TUNE_SLAB_KB: *value = c->slab_kb; return SPR_OK;
         default: return fail(c, SPR_ERR_INVALID_ARG, "unknown tuning key %d", key);
     }
 }

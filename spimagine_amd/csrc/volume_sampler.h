@@ -31,6 +31,19 @@ __device__ __forceinline__ float texel(float s, float fn) {
 __device__ __forceinline__ int floor_idx(float f, float fn) { return (int)fminf(fmaxf(f, -1.0f), fn); }
 __device__ __forceinline__ int clampi(int a, int lo, int hi) { return min(max(a, lo), hi); }
 
+// x, then y, then z (the lerp order every sampler and the oracle use)
+__device__ __forceinline__ float trilerp(float v000, float v100, float v010, float v110, float v001,
+                                         float v101, float v011, float v111, float ax, float ay,
+                                         float az) {
+    float c00 = lerpf(v000, v100, ax);
+    float c10 = lerpf(v010, v110, ax);
+    float c01 = lerpf(v001, v101, ax);
+    float c11 = lerpf(v011, v111, ax);
+    float c0 = lerpf(c00, c10, ay);
+    float c1 = lerpf(c01, c11, ay);
+    return lerpf(c0, c1, az);
+}
+
 template <typename T>
 struct LinearVolume {
     using value_type = T;
@@ -131,9 +144,10 @@ constexpr int kBufferWord3 = 0x00020000;
 template <typename T, bool POW2>
 struct PaddedVolume {
     using value_type = T;
+    static constexpr bool kPow2 = POW2;
     const void* base;             // padded volume
     uint32_t nbytes;              // its size in bytes (the buffer's num_records)
-    uint32_t px_bytes;            // row pitch (nx+2)*sizeof(T)
+    uint32_t px_bytes;            // row pitch, padded_pitch<T>(nx)*sizeof(T)
     uint32_t py;                  // padded rows per plane, ny+2
     uint32_t soff_y, soff_z, soff_yz;  // byte strides of the y, z and y+z neighbours
     int nx, ny, nz;
@@ -151,13 +165,29 @@ struct PaddedVolume {
         return __umul24(row, px_bytes) + ix1 * (uint32_t)sizeof(T);
     }
 
-    __device__ __forceinline__ float linear(float px, float py_, float pz) const {
+    // padded index of one axis' lower corner: floor(u') clamped to [-1, N-1], plus one
+    __device__ __forceinline__ static uint32_t corner1(float f, float m) {
+        return (uint32_t)((int)__builtin_amdgcn_fmed3f(f, -1.0f, m) + 1);
+    }
+
+    // lower-corner padded indices and trilinear weights of a normalised position
+    __device__ __forceinline__ void locate(float px, float py_, float pz, uint32_t* ix1,
+                                           uint32_t* iy1, uint32_t* iz1, float* ax, float* ay,
+                                           float* az) const {
         float ux = texel<POW2>(px, fnx), uy = texel<POW2>(py_, fny), uz = texel<POW2>(pz, fnz);
         float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
-        float ax = ux - fx, ay = uy - fy, az = uz - fz;
-        uint32_t ix1 = (uint32_t)((int)fminf(fmaxf(fx, -1.0f), mx) + 1);
-        uint32_t iy1 = (uint32_t)((int)fminf(fmaxf(fy, -1.0f), my) + 1);
-        uint32_t iz1 = (uint32_t)((int)fminf(fmaxf(fz, -1.0f), mz) + 1);
+        *ax = ux - fx;
+        *ay = uy - fy;
+        *az = uz - fz;
+        *ix1 = corner1(fx, mx);
+        *iy1 = corner1(fy, my);
+        *iz1 = corner1(fz, mz);
+    }
+
+    __device__ __forceinline__ float linear(float px, float py_, float pz) const {
+        uint32_t ix1, iy1, iz1;
+        float ax, ay, az;
+        locate(px, py_, pz, &ix1, &iy1, &iz1, &ax, &ay, &az);
         const uint32_t off = byte_offset(ix1, iy1, iz1);
         const __amdgpu_buffer_rsrc_t r = rsrc();
         float v000, v100, v010, v110, v001, v101, v011, v111;
@@ -165,13 +195,7 @@ struct PaddedVolume {
         PairBits<T>::load(r, off, soff_y, &v010, &v110);
         PairBits<T>::load(r, off, soff_z, &v001, &v101);
         PairBits<T>::load(r, off, soff_yz, &v011, &v111);
-        float c00 = lerpf(v000, v100, ax);
-        float c10 = lerpf(v010, v110, ax);
-        float c01 = lerpf(v001, v101, ax);
-        float c11 = lerpf(v011, v111, ax);
-        float c0 = lerpf(c00, c10, ay);
-        float c1 = lerpf(c01, c11, ay);
-        return lerpf(c0, c1, az);
+        return trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az);
     }
 
     __device__ __forceinline__ float nearest(float px, float py_, float pz) const {
@@ -189,18 +213,36 @@ struct PaddedVolume {
     }
 };
 
-// padded[z][y][x] = vol[clamp(z-1)][clamp(y-1)][clamp(x-1)]
+// Row pitch of the padded layout in elements: nx+2 rounded up to a 16-byte multiple, so
+// every padded row starts 16-byte aligned (the staged kernel moves rows in 16-byte units).
+template <typename T>
+__host__ __device__ inline uint32_t padded_pitch(int nx) {
+    const uint32_t e = 16u / (uint32_t)sizeof(T);
+    return ((uint32_t)nx + 2u + e - 1u) / e * e;
+}
+
+template <class V>
+struct is_padded {
+    static constexpr bool value = false;
+};
+template <typename T, bool POW2>
+struct is_padded<PaddedVolume<T, POW2>> {
+    static constexpr bool value = true;
+};
+
+// padded[z][y][x] = vol[clamp(z-1)][clamp(y-1)][clamp(x-1)] for x < pitch (the columns
+// past nx+1 also replicate the last voxel)
 template <typename T>
 __global__ __launch_bounds__(256) void k_pad_volume(const T* __restrict__ src, T* __restrict__ dst,
                                                     int nx, int ny, int nz) {
-    const int PX = nx + 2, PY = ny + 2;
+    const int PXP = (int)padded_pitch<T>(nx), PY = ny + 2;
     const int x = blockIdx.x * 256 + threadIdx.x;
     const int y = blockIdx.y, z = blockIdx.z;
-    if (x >= PX) return;
+    if (x >= PXP) return;
     const int sx = min(max(x - 1, 0), nx - 1);
     const int sy = min(max(y - 1, 0), ny - 1);
     const int sz = min(max(z - 1, 0), nz - 1);
-    dst[((size_t)z * PY + y) * PX + x] = src[((size_t)sz * ny + sy) * nx + sx];
+    dst[((size_t)z * PY + y) * PXP + x] = src[((size_t)sz * ny + sy) * nx + sx];
 }
 
 }  // namespace spr

@@ -138,6 +138,38 @@ def test_layouts_are_identical(VR, dtype, shape):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("dtype", [np.uint16, np.uint8, np.float32])
+@pytest.mark.parametrize("shape,pose", [((64, 128, 96), 0), ((33, 70, 51), 1), ((128, 64, 64), 2),
+                                        ((16, 16, 16), 3)])
+def test_staged_mip_matches_global(VR, dtype, shape, pose):
+    """The LDS-staged max projection (per-wave slabs, with the global fallback for boxes
+    larger than the slab) is bit-identical to the unstaged kernel for every slab size,
+    including multipass parts and poses whose rays graze the box."""
+    vol = synth.blob_volume(shape, n_blobs=15, sigma=(2., 6.), amp=(100., 250.), background=5.,
+                            poisson=False, read_noise=0., dtype=dtype, seed=pose)
+    M = [np.dot(mat4_translate(0, 0, -4), mat4_rotation(np.pi / 6, 0, 1, 0)),
+         np.dot(mat4_translate(.3, -.2, -2.6), mat4_rotation(2.1, 1, .4, -.3)),
+         np.dot(mat4_translate(0, 0, -1.2), mat4_rotation(.8, .2, 1, .1)),  # eye close to the box
+         mat4_identity()][pose]                                          # eye inside the box
+    frames = {}
+    for slab in (0, 8, 12, 16):
+        rend = VR((130, 97))
+        rend._ctx.set_tuning("slab_kb", slab)
+        rend.set_max_steps(160)
+        rend.set_data(vol)
+        rend.set_modelView(M)
+        rend.render(minVal=float(vol.min()), maxVal=float(vol.max()))
+        res = [rend.output.copy(), rend.output_alpha.copy()]
+        for part in range(3):
+            rend.render(numParts=3, currentPart=part)
+        res.append(rend.output.copy())
+        frames[slab] = res
+    for slab in (8, 12, 16):
+        for a, b in zip(frames[0], frames[slab]):
+            np.testing.assert_array_equal(a, b)
+    assert (frames[0][0] > 0).sum() > 200
+
+
 def test_tuning_does_not_change_results(VR):
     vol = synth.blob_volume((40, 48, 56), n_blobs=10, sigma=(2., 5.), amp=(500., 3000.),
                             background=100., poisson=True, read_noise=5., dtype="uint16", seed=4)
