@@ -1,18 +1,28 @@
-"""Headline benchmark: MIP of a 1024x1024x512 uint16 volume into a 1024^2 frame.
+"""Benchmarks of the MI355X volume ray caster.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 5]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5]          # headline (default)
+    python bench.py --workload batch [--timepoints 64] [--poses 16] [--steps 3]
+    python bench.py --workload serve [--requests 400] [--qps 60]
 
-One step = one max-projection frame (one kernel launch) over the resident volume:
-8000 seeded Gaussian blobs, background 100, Poisson + read noise; perspective 45 deg,
-modelView translate(0,0,-4).rotY(30 deg), max_steps 1536 (each hitting ray takes
-(1536/16+1)*16 = 1552 samples, <= 1 voxel per step), min/max = data min/max, linear.
+headline  One step = one max-projection frame (one kernel launch) of the resident
+          1024x1024x512 uint16 volume into 1024^2: 8000 seeded Gaussian blobs, background
+          100, Poisson + read noise; perspective 45 deg, modelView
+          translate(0,0,-4).rotY(30 deg), max_steps 1536 (each hitting ray takes
+          (1536/16+1)*16 = 1552 samples, <= 1 voxel per step), min/max = data min/max,
+          linear.  value = Gsamples/s over the timed steps, where the sample count is the
+          reference's (rays hitting the box x 1552), independent of the implementation.
+batch     64 timepoints of a 1024x1024x256 uint16 blob series (4000 blobs drifting by
+          seeded random walks of 0.5 voxel per timepoint), all resident, x 16 poses = 1024
+          frames of 1024^2 MIP + alpha.  One step = one replay of the HIP graph that holds
+          the whole batch (one persistent launch); launch tuning from the autotuner.
+serve     a Poisson request stream at fixed QPS against the render server: resident
+          volumes from 256^3 float32 to 2048x2048x512 uint16, frames 512^2-2048^2, random
+          poses, max projection / opacity / iso-surface; p50/p99 latency and frames/s.
 
-Prints ONE JSON line.  value = Gsamples/s over the timed steps, where the sample count
-is the reference's (rays hitting the box x 1552), independent of the implementation.
-roofline.achieved = algorithmic bytes per launch (samples x 8 corners x 2 B + the two
-float32 output planes) / average kernel duration (HIP events on the launch stream), over
-the 8 TB/s HBM peak.  cpu_baseline = the C oracle (oracle/spimref.c, OpenMP) timed on a
-bounded block of rows of the same frame on the host cores.
+Every line carries roofline.achieved = algorithmic bytes per launch (samples x 8 corners
+x voxel bytes + the float32 output and alpha planes) / average kernel duration (HIP
+events on the launch stream), over the 8 TB/s HBM peak, and cpu_baseline = the C oracle
+(oracle/spimref.c, OpenMP) timed on a bounded block of rows of one frame of the workload.
 """
 import argparse
 import json
@@ -27,7 +37,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gsamples/s + frames/s, MIP 1024x1024x512 u16 -> 1024^2; p99 frame latency"
 HBM_PEAK_GBS = 8000.0
-PMC_FILE = os.path.join(ROOT, "profiles", "mip_headline_traffic.json")
+PMC_FILES = {"headline": os.path.join(ROOT, "profiles", "mip_headline_traffic.json"),
+             "batch": os.path.join(ROOT, "profiles", "mip_batch_traffic.json")}
+BOX = (-1, 1, -1, 1, -1, 1)
 
 
 def log(*a):
@@ -38,27 +50,67 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--latency-frames", type=int, default=104)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
-                    help="launch tuning passed to spr_set_tuning, e.g. tile_map=1")
-    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="also run a plain device copy (torch.neg on int16) of this many MiB, "
-                         "the reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
-    args = ap.parse_args()
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: the ray caster runs on a single MI355X here "
-              "(multi-GPU sharding is a design note, not a measured path); use --gpus 1",
-              file=sys.stderr)
-        sys.exit(2)
+def traffic_from_profile(name):
+    path = PMC_FILES.get(name)
+    if path and os.path.exists(path):
+        try:
+            return json.load(open(path)).get("traffic_bytes_per_launch")
+        except (OSError, ValueError):
+            return None
+    return None
 
+
+def roofline(b_alg, kernel_ms, traffic):
+    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel_ms": round(kernel_ms, 4), "algorithmic_bytes": int(b_alg)}
+
+
+def cpu_baseline(vol, invP, invM, lo, hi, S, W, H, seconds, gpu_out, gpu_alpha, what):
+    """The oracle on a bounded block of rows around the middle of the frame; returns the
+    cpu_baseline record and the max |GPU - oracle| over those rows (inf if alpha differs)."""
+    import oracle
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    mid = H // 2
+    per_ray = (S // 16 + 1) * 16
+
+    def run_rows(r0, r1):
+        t = time.perf_counter()
+        o, a = oracle.max_project(vol, invP, invM, min_val=lo, max_val=hi, max_steps=S,
+                                  width=W, height=H, rows=(r0, r1), nthreads=cores)
+        return time.perf_counter() - t, o, a
+
+    log("bench: cpu baseline (oracle, %d threads)" % cores)
+    tcal, o, a = run_rows(mid, mid + cores)
+    per_row = tcal / cores
+    nrows = int(max(cores, min(H, (seconds / max(per_row, 1e-6)) // cores * cores)))
+    r0 = max(0, mid - nrows // 2)
+    r1 = min(H, r0 + nrows)
+    tcpu, o, a = run_rows(r0, r1)
+    hits = int((a[r0:r1] > 0).sum())
+    cpu_samples = hits * per_ray
+    cpu = {"value": round(cpu_samples / tcpu / 1e9, 4), "unit": "Gsamples/s", "cores": cores,
+           "kind": "port",
+           "sample": "rows %d-%d of %s (%d hit rays, %d samples), %.1f s"
+                     % (r0, r1 - 1, what, hits, cpu_samples, tcpu)}
+    parity = float(np.abs(gpu_out[r0:r1] - o[r0:r1]).max())
+    if not np.array_equal(gpu_alpha[r0:r1], a[r0:r1]):
+        parity = float("inf")
+    return cpu, parity
+
+
+def result_line(args, value, ms_per_step, dtype, config, **extra):
+    out = {"metric": METRIC, "value": round(value, 3), "unit": "Gsamples/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic", "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+# ------------------------------------------------------------------------------ headline
+def bench_headline(args):
     import torch
     from spimagine_amd import _lib, synth
     from spimagine_amd.volumerender.volumerender import render_matrices
@@ -83,10 +135,9 @@ def main():
     lo, hi = ctx.volume_minmax()
     M, P = synth.headline_camera()
     invP, invM = render_matrices(M, P, (nx, ny, nz))
-    box = (-1, 1, -1, 1, -1, 1)
 
     def set_pose(iP, iM):
-        ctx.set_params(iP, iM, box, lo, hi, 1., 0., .1, 21, 30, S, _lib.FILTER_LINEAR)
+        ctx.set_params(iP, iM, BOX, lo, hi, 1., 0., .1, 21, 30, S, _lib.FILTER_LINEAR)
 
     set_pose(invP, invM)
     out = torch.empty((H, W), dtype=torch.float32, device="cuda")
@@ -157,76 +208,278 @@ def main():
         del src, dst
 
     b_alg = n_samp * 8 * 2 + W * H * 4 * 2
-    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(PMC_FILE):
-        try:
-            traffic = json.load(open(PMC_FILE)).get("traffic_bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel_ms": round(kernel_ms, 4), "algorithmic_bytes": int(b_alg)}
-
-    cpu = None
-    parity = None
+    cpu = parity = None
     if not args.no_cpu:
-        import oracle
         vol = synth.device_volume_to_numpy(vt, np.uint16)
-        cores = min(len(os.sched_getaffinity(0)), 16)
-        mid = H // 2
+        cpu, parity = cpu_baseline(vol, invP, invM, lo, hi, S, W, H, args.cpu_seconds,
+                                   gpu_out, gpu_alpha, "the same 1024^2 frame")
+    result_line(
+        args, gsps, ms_per_step, "f32",
+        {"workload": "MIP 1024x1024x512 uint16 volume (8000 seeded blobs, Poisson + read noise) "
+                     "-> 1024x1024 float32, max_steps 1536, perspective 45 deg, "
+                     "translate(0,0,-4).rotY(30 deg), linear sampler",
+         "voxel": "u16", "frame": [W, H], "max_steps": S, "samples_per_frame": int(n_samp),
+         "samples_per_hit_ray": per_ray,
+         "tuning": {k: ctx.get_tuning(k) for k in _lib.TUNING_KEYS}},
+        frames_per_s=round(1e3 / ms_per_step, 2),
+        latency_ms={"p50": round(pct(lat, 50), 4), "p99": round(pct(lat, 99), 4),
+                    "frames": len(lat_ev), "poses": 8, "gsamples_per_s": round(pose_gsps, 3)},
+        roofline=roofline(b_alg, kernel_ms, traffic_from_profile("headline")),
+        cpu_baseline=cpu, parity_max_abs_diff_vs_oracle=parity)
 
-        def run_rows(r0, r1):
-            t = time.perf_counter()
-            o, a = oracle.max_project(vol, invP, invM, min_val=lo, max_val=hi, max_steps=S,
-                                      width=W, height=H, rows=(r0, r1), nthreads=cores)
-            return time.perf_counter() - t, o, a
 
-        log("bench: cpu baseline (oracle, %d threads)" % cores)
-        tcal, o, a = run_rows(mid, mid + cores)
-        per_row = tcal / cores
-        nrows = int(max(cores, min(H, (args.cpu_seconds / max(per_row, 1e-6)) // cores * cores)))
-        r0 = max(0, mid - nrows // 2)
-        r1 = min(H, r0 + nrows)
-        tcpu, o, a = run_rows(r0, r1)
-        hits = int((a[r0:r1] > 0).sum())
-        cpu_samples = hits * per_ray
-        cpu = {"value": round(cpu_samples / tcpu / 1e9, 4), "unit": "Gsamples/s", "cores": cores,
-               "kind": "port",
-               "sample": "rows %d-%d of the same 1024^2 frame (%d hit rays, %d samples), %.1f s"
-                         % (r0, r1 - 1, hits, cpu_samples, tcpu)}
-        parity = float(np.abs(gpu_out[r0:r1] - o[r0:r1]).max())
-        if not np.array_equal(gpu_alpha[r0:r1], a[r0:r1]):
-            parity = float("inf")
+# ------------------------------------------------------------------------------ batch
+def bench_batch(args):
+    import torch
+    from spimagine_amd import _lib, autotune, synth
+    from spimagine_amd.utils import mat4_perspective
+    from spimagine_amd.volumerender.volumerender import render_matrices
 
-    result = {
-        "metric": METRIC,
-        "value": round(gsps, 3),
-        "unit": "Gsamples/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "MIP 1024x1024x512 uint16 volume (8000 seeded blobs, Poisson + "
-                               "read noise) -> 1024x1024 float32, max_steps 1536, perspective "
-                               "45 deg, translate(0,0,-4).rotY(30 deg), linear sampler",
-                   "voxel": "u16", "frame": [W, H], "max_steps": S,
-                   "samples_per_frame": int(n_samp), "samples_per_hit_ray": per_ray,
-                   "tuning": {k: ctx.get_tuning(k) for k in _lib.TUNING_KEYS}},
-        "frames_per_s": round(1e3 / ms_per_step, 2),
-        "latency_ms": {"p50": round(pct(lat, 50), 4), "p99": round(pct(lat, 99), 4),
-                       "frames": len(lat_ev), "poses": 8,
-                       "gsamples_per_s": round(pose_gsps, 3)},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "parity_max_abs_diff_vs_oracle": parity,
-    }
-    print(json.dumps(result), flush=True)
+    shape = (256, 1024, 1024)
+    nz, ny, nx = shape
+    W = H = args.frame
+    T, NP = args.timepoints, args.poses
+    S = synth.steps_for_shape((nx, ny, nz))
+    ctx = _lib.Context(0)
+    log("bench: generating %d timepoints of %dx%dx%d uint16 on cuda:0" % (T, nx, ny, nz))
+    t = time.time()
+    slots = []
+    lo = hi = None
+    host_t0 = None
+    for k, vt in enumerate(synth.blob_series_device(shape, 4000, (3., 10.), (500., 3000.), 100.,
+                                                    True, 5., T, drift=.5, seed=0)):
+        if k == 0:
+            u = vt.to(torch.int32) & 0xFFFF
+            lo, hi = float(u.min()), float(u.max())
+            del u
+            ctx.set_volume_device(vt.data_ptr(), np.uint16, shape)  # slot 0: sample counts
+            if not args.no_cpu:
+                host_t0 = synth.device_volume_to_numpy(vt, np.uint16)
+        slots.append(ctx.volume_add(dev_ptr=vt.data_ptr(), dtype=np.uint16, shape=shape))
+        del vt
+    torch.cuda.synchronize()
+    log("bench: %d volumes resident in %.1f s" % (T, time.time() - t))
+
+    P = mat4_perspective(45, W / float(H), .1, 10)
+    poses = synth.random_poses(NP, seed=1)
+    mats = [render_matrices(Mi, P, (nx, ny, nz)) for Mi in poses]
+    planes = torch.empty((T * NP, 2, H, W), dtype=torch.float32, device="cuda")
+    jobs = []
+    for ti in range(T):
+        for pi in range(NP):
+            f = ti * NP + pi
+            jobs.append((slots[ti], mats[pi][0], mats[pi][1], planes[f, 0].data_ptr(),
+                         planes[f, 1].data_ptr()))
+    samples_pose = []
+    for iP, iM in mats:
+        ctx.set_params(iP, iM, BOX, lo, hi, 1., 0., .1, 21, 30, S, _lib.FILTER_LINEAR)
+        samples_pose.append(ctx.count_samples(1, W, H))
+    n_samp = T * sum(samples_pose)
+    params = _lib.make_params(mats[0][0], mats[0][1], BOX, lo, hi, 1., 0., .1, 21, 30, S,
+                              _lib.FILTER_LINEAR)
+    ctx.set_params(mats[0][0], mats[0][1], BOX, lo, hi, 1., 0., .1, 21, 30, S, _lib.FILTER_LINEAR)
+
+    if args.tune:
+        for kv in args.tune:
+            k, v = kv.split("=")
+            ctx.set_tuning(k, int(v))
+        tuned = {"source": "--tune"}
+    else:
+        sample = [jobs[ti * NP + pi] for ti in (0, T // 2) for pi in range(0, NP, max(1, NP // 4))]
+        log("bench: autotuning on %d frames" % len(sample))
+        best, table = autotune.tune_batch(ctx, sample, W, H, params)
+        tuned = {"source": "autotune", "candidates_ms": table}
+    tuning = {k: ctx.get_tuning(k) for k in _lib.TUNING_KEYS}
+    log("bench: tuning %s" % tuning)
+
+    graph = ctx.batch_graph(jobs, W, H)
+    stream = torch.cuda.Stream()
+    log("bench: warmup %d" % args.warmup)
+    for _ in range(args.warmup):
+        graph.launch(stream.cuda_stream)
+    stream.synchronize()
+    log("bench: timing %d replays of %d frames" % (args.steps, len(jobs)))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        graph.launch(stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = e0.elapsed_time(e1) / args.steps
+    ms_per_step = 1e3 * wall / args.steps
+
+    # one frame of the batch against the same frame rendered alone
+    f = (T // 3) * NP + NP // 2
+    single = torch.empty((2, H, W), dtype=torch.float32, device="cuda")
+    p1 = _lib.make_params(jobs[f][1], jobs[f][2], BOX, lo, hi, 1., 0., .1, 21, 30, S,
+                          _lib.FILTER_LINEAR)
+    ctx.render_slot(jobs[f][0], p1, _lib.METHOD_MAX_PROJECT, W, H, single[0].data_ptr(),
+                    single[1].data_ptr(), stream=stream.cuda_stream)
+    stream.synchronize()
+    same_alone = bool(torch.equal(single, planes[f]))
+
+    b_alg = n_samp * 8 * 2 + len(jobs) * W * H * 4 * 2
+    cpu = parity = None
+    if not args.no_cpu:
+        g0 = planes[0].cpu().numpy()
+        cpu, parity = cpu_baseline(host_t0, mats[0][0], mats[0][1], lo, hi, S, W, H,
+                                   args.cpu_seconds, g0[0], g0[1], "frame 0 (timepoint 0, pose 0)")
+    graph.close()
+    result_line(
+        args, n_samp * args.steps / wall / 1e9, ms_per_step, "f32",
+        {"workload": "batched MIP + alpha: %d timepoints of a 1024x1024x256 uint16 blob series "
+                     "(4000 blobs, seeded random-walk drift 0.5 voxel/timepoint, Poisson + read "
+                     "noise) x %d poses -> %d frames of %dx%d float32, max_steps %d, one HIP "
+                     "graph replay per step" % (T, NP, len(jobs), W, H, S),
+         "voxel": "u16", "frames_per_step": len(jobs), "frame": [W, H], "max_steps": S,
+         "samples_per_step": int(n_samp), "resident_volume_bytes":
+             int(sum(ctx.volume_info(s)["bytes"] for s in slots)),
+         "tuning": tuning, "tuning_source": tuned},
+        frames_per_s=round(len(jobs) * 1e3 / ms_per_step, 2),
+        roofline=roofline(b_alg, kernel_ms, traffic_from_profile("batch")),
+        cpu_baseline=cpu, parity_max_abs_diff_vs_oracle=parity,
+        batch_frame_equals_single_render=same_alone)
+
+
+# ------------------------------------------------------------------------------ serve
+SERVE_VOLUMES = [  # name, numpy shape (z, y, x), dtype, blobs
+    ("f32_256", (256, 256, 256), np.float32, 500),
+    ("u8_256x512x512", (256, 512, 512), np.uint8, 1000),
+    ("u16_512", (512, 512, 512), np.uint16, 2000),
+    ("u16_512x1024x1024", (512, 1024, 1024), np.uint16, 8000),
+    ("u16_512x2048x2048", (512, 2048, 2048), np.uint16, 16000),
+]
+
+
+def serve_volume(shape, dtype, n_blobs, seed):
+    import torch
+    from spimagine_amd import synth
+    dt = np.dtype(dtype)
+    amp = (.3, 1.) if dt == np.float32 else (60., 200.) if dt == np.uint8 else (500., 3000.)
+    bg = amp[0] / 10 if dt != np.uint16 else 100.
+    _, c, s, a = synth.blob_params(shape, n_blobs, (3., 10.), amp, seed)
+    vol = torch.full(shape, bg, dtype=torch.float32, device="cuda")
+    synth._splat_blobs_device(vol, c, s, a)
+    if dt == np.float32:
+        return vol.cpu().numpy()
+    top = 255 if dt == np.uint8 else 65535
+    v = torch.clamp(torch.round(vol), 0, top).to(torch.int32)
+    del vol
+    if dt == np.uint8:
+        return v.to(torch.uint8).cpu().numpy()
+    return torch.where(v >= 32768, v - 65536, v).to(torch.int16).cpu().numpy().view(np.uint16)
+
+
+def bench_serve(args):
+    import torch
+    from spimagine_amd import _lib, synth
+    from spimagine_amd.serving import RenderServer, Request, poisson_load
+    from spimagine_amd.utils import mat4_perspective
+
+    budget = int(args.hbm_budget_gib * (1 << 30)) if args.hbm_budget_gib else None
+    server = RenderServer(0, hbm_budget=budget, max_batch=args.max_batch, workers=args.workers)
+    info = {}
+    t = time.time()
+    for k, (name, shape, dt, nb) in enumerate(SERVE_VOLUMES):
+        vol = serve_volume(shape, dt, nb, k)
+        if dt == np.float32:
+            rng = (0., float(vol.max()))
+        else:
+            rng = (float(vol.min()), float(vol.max()))
+        server.register(name, vol)
+        info[name] = dict(shape=shape, dtype=np.dtype(dt).name, range=rng,
+                          steps=synth.steps_for_shape(shape[::-1]))
+    log("bench: %d volumes registered in %.1f s" % (len(info), time.time() - t))
+    rng = np.random.default_rng(5)
+    sizes = [512, 1024, 2048]
+    names = sorted(info)
+    reqs = []
+    for i in range(args.requests):
+        name = names[rng.integers(len(names))]
+        d = info[name]
+        side = sizes[rng.integers(len(sizes))]
+        M = synth.random_poses(1, seed=int(rng.integers(1 << 30)))[0]
+        P = mat4_perspective(45, 1., .1, 10)
+        mode = rng.choice(["max_project"] * 8 + ["alpha", "iso_surface"])
+        lo, hi = d["range"]
+        if mode == "iso_surface":
+            r = Request(name, M, P, size=(side, side), method="iso_surface", min_val=lo,
+                        max_val=2 * (lo + .3 * (hi - lo)), max_steps=d["steps"])
+        else:
+            r = Request(name, M, P, size=(side, side), min_val=lo, max_val=hi,
+                        alpha_pow=.6 if mode == "alpha" else 0., max_steps=d["steps"])
+        reqs.append(r)
+    # warm: every volume admitted and every kernel variant loaded once
+    for r in reqs[:args.warmup]:
+        server.render_alone(r)
+    torch.cuda.synchronize()
+    log("bench: %d requests at %.0f QPS" % (len(reqs), args.qps))
+    t0 = time.perf_counter()
+    futs = poisson_load(server, reqs, args.qps, seed=6)
+    res = [f.result() for f in futs]
+    wall = time.perf_counter() - t0
+    lat = [r.latency_s * 1e3 for r in res]
+    batch = [r.batch_size for r in res]
+    px = sum(r.size[0] * r.size[1] for r in reqs)
+    server.close()
+    print(json.dumps({
+        "metric": "serving: p50/p99 frame latency and frames/s, Poisson arrivals over mixed "
+                  "resident volumes", "value": round(len(res) / wall, 2), "unit": "frames/s",
+        "n_gpus": 1, "steps": len(reqs), "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / len(reqs), 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "Poisson arrivals at %.0f QPS, %d requests over %d resident "
+                               "volumes (256^3 f32 .. 2048x2048x512 u16), frames 512^2-2048^2, "
+                               "80%% MIP / 10%% opacity / 10%% iso" % (args.qps, len(reqs), len(info)),
+                   "volumes": {k: {"shape": list(v["shape"]), "dtype": v["dtype"]}
+                               for k, v in info.items()},
+                   "max_batch": args.max_batch, "workers": args.workers,
+                   "hbm_budget_bytes": server.hbm_budget},
+        "latency_ms": {"p50": round(pct(lat, 50), 3), "p99": round(pct(lat, 99), 3),
+                       "mean": round(float(np.mean(lat)), 3)},
+        "mean_batch": round(float(np.mean(batch)), 2), "gpixels_per_s": round(px / wall / 1e9, 4),
+        "batches": server.batches}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=("headline", "batch", "serve"), default="headline")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--latency-frames", type=int, default=104)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="launch tuning passed to spr_set_tuning, e.g. tile_map=1 "
+                         "(batch: replaces the autotuner)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run a plain device copy (torch.neg on int16) of this many MiB, "
+                         "the reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--timepoints", type=int, default=64)
+    ap.add_argument("--poses", type=int, default=16)
+    ap.add_argument("--frame", type=int, default=1024)
+    ap.add_argument("--requests", type=int, default=400)
+    ap.add_argument("--qps", type=float, default=60.)
+    ap.add_argument("--max-batch", type=int, default=16)
+    ap.add_argument("--workers", type=int, default=2)
+    ap.add_argument("--hbm-budget-gib", type=float, default=0.)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: the ray caster runs on a single MI355X here "
+              "(multi-GPU sharding is a design note, not a measured path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    defaults = {"headline": (50, 5), "batch": (3, 1), "serve": (None, 10)}[args.workload]
+    if args.steps is None:
+        args.steps = defaults[0]
+    if args.warmup is None:
+        args.warmup = defaults[1]
+    {"headline": bench_headline, "batch": bench_batch, "serve": bench_serve}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -3,12 +3,16 @@
  * behind spimagine's VolumeRenderer.
  *
  * Every entry point returns an int status (SPR_OK == 0, negative on error); no C++
- * exception crosses the boundary.  spr_last_error(ctx) returns the message of the last
- * failing call on that context (spr_last_error(NULL): last failure of spr_create or of a
- * call that received a NULL context).
+ * exception crosses the boundary.  spr_last_error(ctx) returns the message of the calling
+ * thread's last failing call if that call concerned ctx, else "" (spr_last_error(NULL):
+ * last failure of spr_create or of a call that received a NULL context).
  *
- * A context owns one HIP device, one HIP stream (default priority), the resident volume
- * and the device-side output buffers.  A context is used by one host thread at a time.
+ * A context owns one HIP device, one HIP stream (default priority), the resident volumes
+ * and the device-side output buffers.  The stateful calls (spr_set_*, spr_render,
+ * spr_render_device, spr_volume_minmax, spr_count_samples, the graph calls) are used by one
+ * host thread at a time.  spr_render_batch, spr_render_slot and spr_volume_info may run on
+ * several threads at once, on different streams, concurrently with spr_volume_add and
+ * spr_volume_release of other slots.
  *
  * Reference interfaces replaced (paths relative to the spimagine checkout):
  *   VolumeRenderer.__init__ / resize / reset_buffer   spimagine/volumerender/volumerender.py:72-135,181-197
@@ -151,6 +155,58 @@ int spr_render(spr_ctx *ctx, int method, int num_parts, int current_part, int w,
 int spr_render_device(spr_ctx *ctx, int method, int num_parts, int current_part, int w, int h,
                       float *d_out, float *d_alpha, float *d_depth, float *d_normals,
                       float *d_occlusion, void *stream);
+
+/* ---- Resident volume table and batched frames (no counterpart in the reference, whose
+   renderer holds one volume and draws one frame per render(); its many-frame paths are
+   the keyframe record loop, spimagine/gui/keyframe_view.py:641-653, and the timepoint
+   change, spimagine/gui/glwidget.py:372-374, which re-uploads the volume each time).
+   spr_set_volume* fill slot 0, the volume spr_render / spr_render_device draw; slots >= 1
+   are added and released here and drawn by spr_render_batch. */
+
+/* Upload a (nz,ny,nx) volume (host or device source) into a new slot. */
+int spr_volume_add(spr_ctx *ctx, const void *src, int src_on_device, int dtype, int nx, int ny,
+                   int nz, int *slot);
+/* Free a slot >= 1 (waits for the device to finish work that may read it). */
+int spr_volume_release(spr_ctx *ctx, int slot);
+/* Voxel type, size and device bytes of a slot (0 = the renderer's volume). */
+int spr_volume_info(spr_ctx *ctx, int slot, int *dtype, int *nx, int *ny, int *nz,
+                    size_t *device_bytes);
+
+/* One frame of a batch: a resident volume, the frame's inverse matrices (as update_matrices
+   computes them for that volume and pose) and two w x h float32 device planes. */
+typedef struct spr_job {
+    int slot;
+    float inv_p[16];
+    float inv_m[16];
+    float *d_out;
+    float *d_alpha;
+} spr_job;
+
+/* Max projection (plain or opacity mode) of n frames in one persistent launch, with
+   `params` (NULL: the spr_set_params state; its matrices are ignored, each job carries
+   its own).  Frames share w, h and every parameter except the matrices; all their slots
+   must use the padded layout and one voxel type.  Results equal n separate
+   spr_render_device calls.  Blocks until the batch completes; calls from several host
+   threads on different streams run concurrently on the device. */
+int spr_render_batch(spr_ctx *ctx, const spr_params *params, const spr_job *jobs, int n, int w,
+                     int h, int num_parts, int current_part, void *stream);
+
+/* One frame of any method from any slot into device buffers on `stream`, with explicit
+   `params` (matrices included): the stateless form of spr_render_device for callers that
+   keep many volumes resident and render with per-request settings from several threads. */
+int spr_render_slot(spr_ctx *ctx, int slot, const spr_params *params, int method, int num_parts,
+                    int current_part, int w, int h, float *d_out, float *d_alpha, float *d_depth,
+                    float *d_normals, float *d_occlusion, void *stream);
+
+/* The same batch (num_parts = 1, the spr_set_params state) captured once in a HIP graph: replay it any number of
+   times on any stream, rewrite its jobs (same count; waits for the last replay first) and
+   destroy it.  Launches are asynchronous. */
+typedef struct spr_batch_graph spr_batch_graph;
+int spr_batch_graph_create(spr_ctx *ctx, const spr_job *jobs, int n, int w, int h,
+                           spr_batch_graph **graph);
+int spr_batch_graph_update(spr_ctx *ctx, spr_batch_graph *graph, const spr_job *jobs, int n);
+int spr_batch_graph_launch(spr_ctx *ctx, spr_batch_graph *graph, void *stream);
+int spr_batch_graph_destroy(spr_ctx *ctx, spr_batch_graph *graph);
 
 /* Wait for all work on the context's stream. */
 int spr_synchronize(spr_ctx *ctx);

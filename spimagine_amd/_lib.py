@@ -34,7 +34,9 @@ EXPORTS = (
     "spr_last_error", "spr_set_layout", "spr_get_layout", "spr_set_tuning", "spr_get_tuning",
     "spr_set_volume", "spr_set_volume_device", "spr_volume_minmax",
     "spr_set_params", "spr_render", "spr_render_device", "spr_synchronize", "spr_get_stream",
-    "spr_count_samples",
+    "spr_count_samples", "spr_volume_add", "spr_volume_release", "spr_volume_info",
+    "spr_render_batch", "spr_render_slot", "spr_batch_graph_create", "spr_batch_graph_update",
+    "spr_batch_graph_launch", "spr_batch_graph_destroy",
 )
 
 
@@ -53,6 +55,37 @@ class SprParams(ctypes.Structure):
         ("occ_n_points", ctypes.c_int),
         ("max_steps", ctypes.c_int),
         ("filter", ctypes.c_int),
+    ]
+
+
+def make_params(inv_p, inv_m, box=(-1, 1, -1, 1, -1, 1), min_val=0., max_val=0., gamma=1.,
+                alpha_pow=0., occ_strength=.1, occ_radius=21, occ_n_points=30, max_steps=200,
+                filter=FILTER_LINEAR):
+    """An SprParams from Python values (every float rounded to float32)."""
+    p = SprParams()
+    p.inv_p[:] = [float(v) for v in np.asarray(inv_p, np.float32).ravel()]
+    p.inv_m[:] = [float(v) for v in np.asarray(inv_m, np.float32).ravel()]
+    p.box[:] = [float(v) for v in np.asarray(box, np.float32).ravel()]
+    p.min_val = float(np.float32(min_val))
+    p.max_val = float(np.float32(max_val))
+    p.gamma = float(np.float32(gamma))
+    p.alpha_pow = float(np.float32(alpha_pow))
+    p.occ_strength = float(np.float32(occ_strength))
+    p.occ_radius = int(occ_radius)
+    p.occ_n_points = int(occ_n_points)
+    p.max_steps = int(max_steps)
+    p.filter = int(filter)
+    return p
+
+
+class SprJob(ctypes.Structure):
+    """struct spr_job (include/spimrender.h)."""
+    _fields_ = [
+        ("slot", ctypes.c_int),
+        ("inv_p", ctypes.c_float * 16),
+        ("inv_m", ctypes.c_float * 16),
+        ("d_out", ctypes.c_void_p),
+        ("d_alpha", ctypes.c_void_p),
     ]
 
 
@@ -108,6 +141,23 @@ def load():
     L.spr_synchronize.argtypes = [c_void_p]
     L.spr_get_stream.argtypes = [c_void_p, ctypes.POINTER(c_void_p)]
     L.spr_count_samples.argtypes = [c_void_p, c_int, c_int, c_int, ctypes.POINTER(ctypes.c_double)]
+    L.spr_volume_add.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
+                                 ctypes.POINTER(c_int)]
+    L.spr_volume_release.argtypes = [c_void_p, c_int]
+    L.spr_volume_info.argtypes = [c_void_p, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                                  ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                                  ctypes.POINTER(ctypes.c_size_t)]
+    jp = ctypes.POINTER(SprJob)
+    L.spr_render_batch.argtypes = [c_void_p, ctypes.POINTER(SprParams), jp, c_int, c_int, c_int,
+                                   c_int, c_int, c_void_p]
+    L.spr_render_slot.argtypes = [c_void_p, c_int, ctypes.POINTER(SprParams), c_int, c_int, c_int,
+                                  c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p]
+    L.spr_batch_graph_create.argtypes = [c_void_p, jp, c_int, c_int, c_int,
+                                         ctypes.POINTER(c_void_p)]
+    L.spr_batch_graph_update.argtypes = [c_void_p, c_void_p, jp, c_int]
+    L.spr_batch_graph_launch.argtypes = [c_void_p, c_void_p, c_void_p]
+    L.spr_batch_graph_destroy.argtypes = [c_void_p, c_void_p]
     _lib = L
     return L
 
@@ -203,20 +253,9 @@ class Context:
 
     def set_params(self, inv_p, inv_m, box, min_val, max_val, gamma, alpha_pow, occ_strength,
                    occ_radius, occ_n_points, max_steps, filter):
-        p = SprParams()
-        p.inv_p[:] = [float(v) for v in np.asarray(inv_p, np.float32).ravel()]
-        p.inv_m[:] = [float(v) for v in np.asarray(inv_m, np.float32).ravel()]
-        p.box[:] = [float(v) for v in np.asarray(box, np.float32).ravel()]
-        p.min_val = float(np.float32(min_val))
-        p.max_val = float(np.float32(max_val))
-        p.gamma = float(np.float32(gamma))
-        p.alpha_pow = float(np.float32(alpha_pow))
-        p.occ_strength = float(np.float32(occ_strength))
-        p.occ_radius = int(occ_radius)
-        p.occ_n_points = int(occ_n_points)
-        p.max_steps = int(max_steps)
-        p.filter = int(filter)
-        self._check(self.lib.spr_set_params(self.handle, ctypes.byref(p)))
+        self._check(self.lib.spr_set_params(self.handle, ctypes.byref(make_params(
+            inv_p, inv_m, box, min_val, max_val, gamma, alpha_pow, occ_strength, occ_radius,
+            occ_n_points, max_steps, filter))))
 
     def render(self, method, num_parts, current_part, w, h, out=None, alpha=None, depth=None,
                normals=None, occlusion=None):
@@ -238,7 +277,94 @@ class Context:
         self._check(self.lib.spr_get_stream(self.handle, ctypes.byref(s)))
         return s.value
 
+    # ---- resident volume table and batches
+    def volume_add(self, data=None, dev_ptr=None, dtype=None, shape=None):
+        """Upload a host array (or a device buffer of `dtype`/`shape`) into a new slot."""
+        slot = ctypes.c_int()
+        if data is not None:
+            data = np.ascontiguousarray(data)
+            nz, ny, nx = data.shape
+            self._check(self.lib.spr_volume_add(self.handle, data.ctypes.data, 0,
+                                                DTYPE_CODES[data.dtype], nx, ny, nz,
+                                                ctypes.byref(slot)))
+        else:
+            nz, ny, nx = shape
+            self._check(self.lib.spr_volume_add(self.handle, ctypes.c_void_p(dev_ptr), 1,
+                                                DTYPE_CODES[np.dtype(dtype)], nx, ny, nz,
+                                                ctypes.byref(slot)))
+        return slot.value
+
+    def volume_release(self, slot):
+        self._check(self.lib.spr_volume_release(self.handle, int(slot)))
+
+    def volume_info(self, slot):
+        d, x, y, z = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        b = ctypes.c_size_t()
+        self._check(self.lib.spr_volume_info(self.handle, int(slot), ctypes.byref(d), ctypes.byref(x),
+                                             ctypes.byref(y), ctypes.byref(z), ctypes.byref(b)))
+        return {"dtype": d.value, "shape": (z.value, y.value, x.value), "bytes": b.value}
+
+    @staticmethod
+    def make_jobs(jobs):
+        """jobs: iterable of (slot, inv_p, inv_m, d_out, d_alpha) -> ctypes array."""
+        jobs = list(jobs)
+        arr = (SprJob * len(jobs))()
+        for k, (slot, ip, im, o, a) in enumerate(jobs):
+            arr[k].slot = int(slot)
+            arr[k].inv_p[:] = [float(v) for v in np.asarray(ip, np.float32).ravel()]
+            arr[k].inv_m[:] = [float(v) for v in np.asarray(im, np.float32).ravel()]
+            arr[k].d_out = o
+            arr[k].d_alpha = a
+        return arr
+
+    def render_batch(self, jobs, w, h, num_parts=1, current_part=0, stream=None, params=None):
+        """params: an SprParams (make_params) or None for the spr_set_params state."""
+        arr = jobs if isinstance(jobs, ctypes.Array) else self.make_jobs(jobs)
+        pp = ctypes.byref(params) if params is not None else None
+        self._check(self.lib.spr_render_batch(self.handle, pp, arr, len(arr), w, h, num_parts,
+                                              current_part, stream))
+
+    def render_slot(self, slot, params, method, w, h, d_out, d_alpha, d_depth=None,
+                    d_normals=None, d_occlusion=None, num_parts=1, current_part=0, stream=None):
+        self._check(self.lib.spr_render_slot(self.handle, int(slot), ctypes.byref(params), method,
+                                             num_parts, current_part, w, h, d_out, d_alpha, d_depth,
+                                             d_normals, d_occlusion, stream))
+
+    def batch_graph(self, jobs, w, h):
+        return BatchGraph(self, jobs, w, h)
+
     def count_samples(self, num_parts, w, h):
         n = ctypes.c_double()
         self._check(self.lib.spr_count_samples(self.handle, num_parts, w, h, ctypes.byref(n)))
         return int(n.value)
+
+
+class BatchGraph:
+    """A captured batch (spr_batch_graph_*): replay with launch(), retarget with update()."""
+
+    def __init__(self, ctx, jobs, w, h):
+        self.ctx = ctx
+        arr = jobs if isinstance(jobs, ctypes.Array) else Context.make_jobs(jobs)
+        self.n = len(arr)
+        g = ctypes.c_void_p()
+        ctx._check(ctx.lib.spr_batch_graph_create(ctx.handle, arr, self.n, w, h, ctypes.byref(g)))
+        self.handle = g
+
+    def update(self, jobs):
+        arr = jobs if isinstance(jobs, ctypes.Array) else Context.make_jobs(jobs)
+        self.ctx._check(self.ctx.lib.spr_batch_graph_update(self.ctx.handle, self.handle, arr,
+                                                            len(arr)))
+
+    def launch(self, stream=None):
+        self.ctx._check(self.ctx.lib.spr_batch_graph_launch(self.ctx.handle, self.handle, stream))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.spr_batch_graph_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

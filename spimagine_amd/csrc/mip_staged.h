@@ -47,18 +47,13 @@ struct alignas(16) Unit16 {
 };
 
 template <typename T, bool POW2, int SLAB_BYTES>
-__global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2> vol, RayParams p,
-                                                            float* __restrict__ out,
-                                                            float* __restrict__ alpha) {
+__device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol, const RayParams& p,
+                                                uint32_t x, uint32_t y, bool inside, uint8_t* slab,
+                                                float* __restrict__ out, float* __restrict__ alpha) {
     constexpr bool kShort = !std::is_same<T, float>::value;
     constexpr uint32_t E = 16u / sizeof(T);  // voxels per 16-byte unit
     constexpr uint32_t TS = sizeof(T);
-    __shared__ Unit16 lds[4 * SLAB_BYTES / 16];
-    uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * SLAB_BYTES;
     const uint32_t lane = threadIdx.x & 63u;
-
-    uint32_t x = 0, y = 0;
-    const bool inside = tile_pixel(p, &x, &y);
     bool active = false;
     float px = 0.5f, py = 0.5f, pz = 0.5f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
     float tnear = 0.0f;
@@ -175,15 +170,18 @@ __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2
             }
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            // ---- 16 samples from the slab
+            // ---- 16 samples from the slab; slab offset of corner (c+1) relative to the box,
+            // computed exactly in float (every term is an integer below 2^24)
             const uint32_t sy = rowb, sz = rowb * SY;
+            const float fsy = (float)sy, fsz = (float)sz;
+            const float oy = 1.0f - (float)LY, oz = 1.0f - (float)LZ;
+            const float ox = (1.0f - (float)lox) * (float)TS;
 #pragma unroll
             for (int j = 0; j < kLoopUnroll; ++j) {
-                uint32_t ix1, iy1, iz1;
-                float ax, ay, az;
-                vol.locate(px, py, pz, &ix1, &iy1, &iz1, &ax, &ay, &az);
-                const uint32_t b = __umul24(iz1 - LZ, sz) + __umul24(iy1 - LY, sy) + (ix1 - lox) * TS;
-                const uint8_t* q = slab + b;
+                float cx, cy, cz, ax, ay, az;
+                vol.locate_f(px, py, pz, &cx, &cy, &cz, &ax, &ay, &az);
+                const float bf = fmaf(cz + oz, fsz, fmaf(cy + oy, fsy, fmaf(cx, (float)TS, ox)));
+                const uint8_t* q = slab + (uint32_t)bf;
                 const float v000 = lds_voxel<T>(q), v100 = lds_voxel<T>(q + TS);
                 const float v010 = lds_voxel<T>(q + sy), v110 = lds_voxel<T>(q + sy + TS);
                 const float v001 = lds_voxel<T>(q + sz), v101 = lds_voxel<T>(q + sz + TS);
@@ -215,6 +213,114 @@ __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2
     } else {
         out[o] = fmaxf(col, out[o]);
         alpha[o] = fmaxf(a, alpha[o]);
+    }
+}
+
+// one frame: one workgroup per 16x16 tile (tile map as k_max_project)
+template <typename T, bool POW2, int SLAB_BYTES>
+__global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2> vol, RayParams p,
+                                                            float* __restrict__ out,
+                                                            float* __restrict__ alpha) {
+    __shared__ Unit16 lds[4 * SLAB_BYTES / 16];
+    uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * SLAB_BYTES;
+    uint32_t x = 0, y = 0;
+    const bool inside = tile_pixel(p, &x, &y);
+    mip_staged_rays<T, POW2, SLAB_BYTES>(vol, p, x, y, inside, slab, out, alpha);
+}
+
+// ---------------------------------------------------------------------------------
+// Batched frames (many volumes x poses in one launch).  A job is one frame: a resident
+// padded volume, the pose's inverse matrices and the two output planes; everything else
+// (frame size, min/max/gamma, max_steps, numParts) is shared by the batch.
+
+struct BatchJob {
+    const void* base;  // padded volume of the job (same voxel type across the batch)
+    uint32_t nbytes, px_bytes, py;
+    int nx, ny, nz;
+    float invP[16];
+    float invM[16];
+    float* out;
+    float* alpha;
+};
+
+// Work queue of a persistent batch launch: the (job, tile) items whose tile row is
+// congruent to q mod 8 form queue q, drained first by the workgroups with
+// blockIdx % 8 == q (one XCD under round-robin dispatch, keeping a tile row's neighbours
+// in one L2), then shared by every workgroup once its own queue is empty.
+struct BatchQueues {
+    uint32_t count[8];  // items per queue
+    uint32_t tiles[8];  // tiles per frame in queue q
+};
+
+template <typename T, bool POW2>
+__device__ __forceinline__ PaddedVolume<T, POW2> job_volume(const BatchJob& j) {
+    PaddedVolume<T, POW2> v;
+    v.base = j.base;
+    v.nbytes = j.nbytes;
+    v.px_bytes = j.px_bytes;
+    v.py = j.py;
+    v.soff_y = j.px_bytes;
+    v.soff_z = j.px_bytes * j.py;
+    v.soff_yz = v.soff_y + v.soff_z;
+    v.nx = j.nx;
+    v.ny = j.ny;
+    v.nz = j.nz;
+    v.fnx = (float)j.nx;
+    v.fny = (float)j.ny;
+    v.fnz = (float)j.nz;
+    v.mx = (float)(j.nx - 1);
+    v.my = (float)(j.ny - 1);
+    v.mz = (float)(j.nz - 1);
+    return v;
+}
+
+// zero the queue counters (a kernel, so the batch can be captured in a HIP graph)
+__global__ void k_reset_counters(uint32_t* __restrict__ counters) {
+    if (threadIdx.x < 8) counters[threadIdx.x] = 0u;
+}
+
+// Persistent launch: a fixed grid of workgroups claims (job, tile) items from the eight
+// queues with one atomic per item until every queue is drained; every workgroup leaves
+// after one failed claim on each queue.
+// MODE 0: staged plain max projection (linear filter, alpha_pow == 0); MODE 1 / 2: every
+// sample from global memory with the linear / nearest filter (any alpha_pow)
+template <typename T, bool POW2, int SLAB_BYTES, int MODE>
+__global__ __launch_bounds__(256) void k_max_project_batch(const BatchJob* __restrict__ jobs,
+                                                           RayParams shared, BatchQueues qs,
+                                                           uint32_t* __restrict__ counters) {
+    __shared__ Unit16 lds[4 * SLAB_BYTES / 16];
+    __shared__ uint32_t s_item;
+    uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * SLAB_BYTES;
+    uint32_t q = blockIdx.x & 7u;
+    for (uint32_t drained = 0; drained < 8u;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&counters[q], 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        __syncthreads();
+        if (item >= qs.count[q]) {
+            q = (q + 1u) & 7u;
+            ++drained;
+            continue;
+        }
+        const uint32_t per = qs.tiles[q];
+        const uint32_t jid = item / per, t = item - jid * per;
+        const uint32_t ty = q + 8u * (t / shared.ntx), tx = t % shared.ntx;
+        const BatchJob& j = jobs[jid];
+        RayParams p = shared;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            p.invP[k] = j.invP[k];
+            p.invM[k] = j.invM[k];
+        }
+        uint32_t x, y;
+        tile_thread_pixel(tx, ty, &x, &y);
+        const bool inside = x < p.W && y < p.H;
+        const PaddedVolume<T, POW2> vol = job_volume<T, POW2>(j);
+        if constexpr (MODE == 0)
+            mip_staged_rays<T, POW2, SLAB_BYTES>(vol, p, x, y, inside, slab, j.out, j.alpha);
+        else
+            mip_rays<PaddedVolume<T, POW2>, MODE == 2 ? 1 : 0>(vol, p, x, y, inside, j.out,
+                                                                j.alpha);
     }
 }
 

@@ -41,6 +41,13 @@ __host__ __device__ inline uint32_t tile_grid(uint32_t tile_map, uint32_t ntx, u
     return tile_map == kTileMapInterleaved ? 8u * ((nty + 7u) / 8u) * ntx : ntx * nty;
 }
 
+// pixel of this thread inside 16x16 tile (tx, ty): each wave64 takes an 8x8 quarter
+__device__ __forceinline__ void tile_thread_pixel(uint32_t tx, uint32_t ty, uint32_t* x, uint32_t* y) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    *x = tx * 16u + (wave & 1u) * 8u + (lane & 7u);
+    *y = ty * 16u + (wave >> 1) * 8u + (lane >> 3);
+}
+
 __device__ __forceinline__ bool tile_pixel(const RayParams& p, uint32_t* x, uint32_t* y) {
     const uint32_t b = blockIdx.x;
     uint32_t tx, ty;
@@ -58,9 +65,7 @@ __device__ __forceinline__ bool tile_pixel(const RayParams& p, uint32_t* x, uint
         tx = b % p.ntx;
         ty = b / p.ntx;
     }
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    *x = tx * 16u + (wave & 1u) * 8u + (lane & 7u);
-    *y = ty * 16u + (wave >> 1) * 8u + (lane >> 3);
+    tile_thread_pixel(tx, ty, x, y);
     return ty < p.nty && *x < p.W && *y < p.H;
 }
 
@@ -71,13 +76,13 @@ __device__ __forceinline__ float apply_gamma(float c, float gamma) {
 }
 
 // volume_kernel.cl:20-185 (float) and :190-345 (short, also used for uint8)
+// one ray per thread, every sample read from global memory (all modes and filters)
 template <class Vol, int FILTER>
-__global__ __launch_bounds__(256) void k_max_project(Vol vol, RayParams p,
-                                                     float* __restrict__ out,
-                                                     float* __restrict__ alpha) {
+__device__ __forceinline__ void mip_rays(const Vol& vol, const RayParams& p, uint32_t x, uint32_t y,
+                                         bool inside, float* __restrict__ out,
+                                         float* __restrict__ alpha) {
     constexpr bool kShort = !std::is_same<typename Vol::value_type, float>::value;
-    uint32_t x, y;
-    if (!tile_pixel(p, &x, &y)) return;
+    if (!inside) return;
     const uint32_t o = x + p.W * y;
     F4 orig, direc;
     ray_setup(p.invP, p.invM, x, y, p.W, p.H, &orig, &direc);
@@ -136,6 +141,15 @@ __global__ __launch_bounds__(256) void k_max_project(Vol vol, RayParams p,
         out[o] = fmaxf(col, out[o]);
         alpha[o] = fmaxf(a, alpha[o]);
     }
+}
+
+template <class Vol, int FILTER>
+__global__ __launch_bounds__(256) void k_max_project(Vol vol, RayParams p,
+                                                     float* __restrict__ out,
+                                                     float* __restrict__ alpha) {
+    uint32_t x, y;
+    const bool inside = tile_pixel(p, &x, &y);
+    mip_rays<Vol, FILTER>(vol, p, x, y, inside, out, alpha);
 }
 
 // iso_kernel.cl:17-225

@@ -1,13 +1,15 @@
 // spimrender.hip -- libspimrender.so: C ABI (include/spimrender.h) over the gfx950
 // ray-casting kernels.  Host side of spimagine's VolumeRenderer
 // (spimagine/volumerender/volumerender.py): volume upload and layout, parameter state,
-// kernel dispatch in the reference's launch order, and the device-to-host output copies.
+// kernel dispatch in the reference's launch order, the device-to-host output copies, and
+// the batched (many volumes x poses, HIP graph) launch.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -18,18 +20,42 @@
 
 using namespace spr;
 
-struct spr_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    // resident volume: (nz, ny, nx) C order, x fastest, in one of two layouts
-    int layout_req = SPR_LAYOUT_AUTO;  // requested for the next upload
-    int layout = SPR_LAYOUT_AUTO;      // layout of the resident volume
-    void* d_vol = nullptr;             // SPR_LAYOUT_LINEAR: the volume as uploaded
+namespace {
+
+// one resident volume
+struct VolSlot {
+    bool used = false;
+    int layout = SPR_LAYOUT_AUTO;  // SPR_LAYOUT_LINEAR or SPR_LAYOUT_PADDED once filled
+    void* d_vol = nullptr;         // LINEAR: the volume as uploaded
     size_t vol_bytes = 0;
-    void* d_pad = nullptr;             // SPR_LAYOUT_PADDED: (nz+2, ny+2, nx+2) apron copy
+    void* d_pad = nullptr;         // PADDED: (nz+2, ny+2, pitch) apron copy
     size_t pad_bytes = 0;
     int dtype = -1, nx = 0, ny = 0, nz = 0;
+};
+
+}  // namespace
+
+constexpr size_t kMaxSlots = 4096;
+
+struct spr_batch_graph {
+    int n = 0, w = 0, h = 0;
+    BatchJob* d_jobs = nullptr;
+    uint32_t* d_counters = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t last_stream = nullptr;
+    int dtype = -1;
+};
+
+struct spr_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    std::mutex table_mu;  // serialises spr_volume_add / spr_volume_release
+    int layout_req = SPR_LAYOUT_AUTO;  // layout for the next upload
+    // slot 0: the renderer's volume.  Capacity is reserved once (kMaxSlots) so adding a
+    // slot never moves the others while renders on other threads read them.
+    std::vector<VolSlot> vols;
     spr_params params{};
     bool has_params = false;
     // w x h output buffers kept between spr_render calls (VolumeRenderer.buf*, reset_buffer)
@@ -48,7 +74,10 @@ struct spr_ctx {
 
 namespace {
 
+// last failure of the calling thread and the context it concerned (NULL: spr_create or a
+// NULL-context call); thread-local so concurrent callers of one context never share it
 thread_local std::string g_err;
+thread_local const spr_ctx* g_err_ctx = nullptr;
 
 int fail(spr_ctx* ctx, int code, const char* fmt, ...) {
     char buf[512];
@@ -56,10 +85,8 @@ int fail(spr_ctx* ctx, int code, const char* fmt, ...) {
     va_start(ap, fmt);
     vsnprintf(buf, sizeof(buf), fmt, ap);
     va_end(ap);
-    if (ctx)
-        ctx->err = buf;
-    else
-        g_err = buf;
+    g_err = buf;
+    g_err_ctx = ctx;
     return code;
 }
 
@@ -109,14 +136,16 @@ void free_scratch(spr_ctx* c) {
     c->scratch_px = 0;
 }
 
-void free_volume(spr_ctx* c) {
-    if (c->d_vol) (void)hipFree(c->d_vol);
-    if (c->d_pad) (void)hipFree(c->d_pad);
-    c->d_vol = c->d_pad = nullptr;
-    c->vol_bytes = c->pad_bytes = 0;
+void free_slot(VolSlot& v) {
+    if (v.d_vol) (void)hipFree(v.d_vol);
+    if (v.d_pad) (void)hipFree(v.d_pad);
+    v = VolSlot();
 }
 
-bool has_volume(const spr_ctx* c) { return c->d_vol || c->d_pad; }
+const VolSlot* slot_of(const spr_ctx* c, int slot) {
+    if (slot < 0 || slot >= (int)c->vols.size() || !c->vols[slot].used) return nullptr;
+    return &c->vols[slot];
+}
 
 int ensure_outputs(spr_ctx* c, int w, int h) {
     if (c->w == w && c->h == h && c->d_out) return SPR_OK;
@@ -146,17 +175,17 @@ int ensure_scratch(spr_ctx* c, size_t px) {
     return SPR_OK;
 }
 
-RayParams make_params(const spr_ctx* c, int w, int h) {
+RayParams make_params(const spr_params& prm, int tile_map, int w, int h) {
     RayParams p;
-    memcpy(p.invP, c->params.inv_p, sizeof(p.invP));
-    memcpy(p.invM, c->params.inv_m, sizeof(p.invM));
-    memcpy(p.box, c->params.box, sizeof(p.box));
-    p.min_val = c->params.min_val;
-    p.max_val = c->params.max_val;
-    p.gamma = c->params.gamma;
-    p.alpha_pow = c->params.alpha_pow;
-    p.iso_val = c->params.max_val / 2.0f;
-    p.max_steps = c->params.max_steps;
+    memcpy(p.invP, prm.inv_p, sizeof(p.invP));
+    memcpy(p.invM, prm.inv_m, sizeof(p.invM));
+    memcpy(p.box, prm.box, sizeof(p.box));
+    p.min_val = prm.min_val;
+    p.max_val = prm.max_val;
+    p.gamma = prm.gamma;
+    p.alpha_pow = prm.alpha_pow;
+    p.iso_val = prm.max_val / 2.0f;
+    p.max_steps = prm.max_steps;
     p.nouter = 0;
     p.current_part = 0;
     p.W = (uint32_t)w;
@@ -164,8 +193,18 @@ RayParams make_params(const spr_ctx* c, int w, int h) {
     p.ntx = (uint32_t)((w + 15) / 16);
     p.nty = (uint32_t)((h + 15) / 16);
     p.ntiles = p.ntx * p.nty;
-    p.tile_map = (uint32_t)c->tile_map;
+    p.tile_map = (uint32_t)tile_map;
     return p;
+}
+
+RayParams make_params(const spr_ctx* c, int w, int h) { return make_params(c->params, c->tile_map, w, h); }
+
+int check_params(spr_ctx* c, const spr_params& p) {
+    if (p.filter != SPR_FILTER_LINEAR && p.filter != SPR_FILTER_NEAREST)
+        return fail(c, SPR_ERR_INVALID_ARG, "unknown filter %d", p.filter);
+    if (p.max_steps < 1)
+        return fail(c, SPR_ERR_INVALID_ARG, "max_steps must be positive (got %d)", p.max_steps);
+    return SPR_OK;
 }
 
 // convolve_2d.cl weight exp((float)(-k*(ht-Nh/2.f)*(ht-Nh/2.f)/Nh/Nh)), float32 argument
@@ -188,104 +227,117 @@ ConvWeights conv_weights(int nh, float k) {
 }
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+bool slot_pow2(const VolSlot& v) { return is_pow2(v.nx) && is_pow2(v.ny) && is_pow2(v.nz); }
 
 template <typename T>
-LinearVolume<T> linear_volume(const spr_ctx* c) {
+LinearVolume<T> linear_volume(const VolSlot& s) {
     LinearVolume<T> v;
-    v.data = (const T*)c->d_vol;
-    v.nx = c->nx;
-    v.ny = c->ny;
-    v.nz = c->nz;
-    v.fnx = (float)c->nx;
-    v.fny = (float)c->ny;
-    v.fnz = (float)c->nz;
+    v.data = (const T*)s.d_vol;
+    v.nx = s.nx;
+    v.ny = s.ny;
+    v.nz = s.nz;
+    v.fnx = (float)s.nx;
+    v.fny = (float)s.ny;
+    v.fnz = (float)s.nz;
     return v;
 }
 
 template <typename T, bool POW2>
-PaddedVolume<T, POW2> padded_volume(const spr_ctx* c) {
+PaddedVolume<T, POW2> padded_volume(const VolSlot& s) {
     PaddedVolume<T, POW2> v;
-    v.base = c->d_pad;
-    v.nbytes = (uint32_t)c->pad_bytes;
-    const uint32_t PY = (uint32_t)c->ny + 2;
-    v.px_bytes = padded_pitch<T>(c->nx) * (uint32_t)sizeof(T);
+    v.base = s.d_pad;
+    v.nbytes = (uint32_t)s.pad_bytes;
+    const uint32_t PY = (uint32_t)s.ny + 2;
+    v.px_bytes = padded_pitch<T>(s.nx) * (uint32_t)sizeof(T);
     v.py = PY;
     v.soff_y = v.px_bytes;
     v.soff_z = v.px_bytes * PY;
     v.soff_yz = v.soff_y + v.soff_z;
-    v.nx = c->nx;
-    v.ny = c->ny;
-    v.nz = c->nz;
-    v.fnx = (float)c->nx;
-    v.fny = (float)c->ny;
-    v.fnz = (float)c->nz;
-    v.mx = (float)(c->nx - 1);
-    v.my = (float)(c->ny - 1);
-    v.mz = (float)(c->nz - 1);
+    v.nx = s.nx;
+    v.ny = s.ny;
+    v.nz = s.nz;
+    v.fnx = (float)s.nx;
+    v.fny = (float)s.ny;
+    v.fnz = (float)s.nz;
+    v.mx = (float)(s.nx - 1);
+    v.my = (float)(s.ny - 1);
+    v.mz = (float)(s.nz - 1);
     return v;
 }
 
-// Calls f(volume accessor) with the accessor matching the resident volume's voxel type,
-// layout and (padded layout) whether every dimension is a power of two.
+// Calls f(volume accessor) with the accessor matching the slot's voxel type, layout and
+// (padded layout) whether every dimension is a power of two.
 template <typename T, typename F>
-hipError_t visit_typed(const spr_ctx* c, F&& f) {
-    if (c->layout == SPR_LAYOUT_LINEAR) return f(linear_volume<T>(c));
-    if (is_pow2(c->nx) && is_pow2(c->ny) && is_pow2(c->nz)) return f(padded_volume<T, true>(c));
-    return f(padded_volume<T, false>(c));
+hipError_t visit_typed(const VolSlot& s, F&& f) {
+    if (s.layout == SPR_LAYOUT_LINEAR) return f(linear_volume<T>(s));
+    if (slot_pow2(s)) return f(padded_volume<T, true>(s));
+    return f(padded_volume<T, false>(s));
 }
 
 template <typename F>
-hipError_t visit_volume(const spr_ctx* c, F&& f) {
-    switch (c->dtype) {
-        case SPR_DTYPE_F32: return visit_typed<float>(c, f);
-        case SPR_DTYPE_U16: return visit_typed<uint16_t>(c, f);
-        default: return visit_typed<uint8_t>(c, f);
+hipError_t visit_volume(const VolSlot& s, F&& f) {
+    switch (s.dtype) {
+        case SPR_DTYPE_F32: return visit_typed<float>(s, f);
+        case SPR_DTYPE_U16: return visit_typed<uint16_t>(s, f);
+        default: return visit_typed<uint8_t>(s, f);
     }
 }
 
-int check_ready(spr_ctx* c, int w, int h) {
-    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
-    if (!has_volume(c)) return fail(c, SPR_ERR_NO_VOLUME, "no data provided, set_data(data) before");
+int check_frame(spr_ctx* c, int w, int h) {
     if (!c->has_params) return fail(c, SPR_ERR_INVALID_ARG, "spr_set_params was not called");
     if (w <= 0 || h <= 0 || w > 32768 || h > 32768)
         return fail(c, SPR_ERR_INVALID_ARG, "invalid output size %d x %d", w, h);
     return SPR_OK;
 }
 
+int check_ready(spr_ctx* c, int w, int h) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (!c->vols[0].used) return fail(c, SPR_ERR_NO_VOLUME, "no data provided, set_data(data) before");
+    return check_frame(c, w, h);
+}
+
+int check_parts(spr_ctx* c, const spr_params& prm, int num_parts) {
+    if (num_parts < 1 || (prm.max_steps / num_parts) / kLoopUnroll < 1)
+        return fail(c, SPR_ERR_INVALID_ARG,
+                    "numParts=%d leaves no 16-sample block of max_steps=%d "
+                    "(the reference divides by zero here)",
+                    num_parts, prm.max_steps);
+    return SPR_OK;
+}
+
+template <typename T, bool P2, typename... Args>
+void launch_staged(int slab_kb, dim3 grid, hipStream_t s, Args... args) {
+    if (slab_kb == 8)
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 8192>), grid, dim3(256), 0, s, args...);
+    else if (slab_kb == 16)
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384>), grid, dim3(256), 0, s, args...);
+    else
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288>), grid, dim3(256), 0, s, args...);
+}
+
 // render(method, numParts, currentPart) into device buffers; the reference's launch
 // order (volumerender.py:366-386 for max_project, :451-497 for iso_surface)
-int render_impl(spr_ctx* c, int method, int num_parts, int current_part, int w, int h, float* out,
-                float* alpha, float* depth, float* normals, float* occ, hipStream_t s) {
-    int rc = check_ready(c, w, h);
-    if (rc) return rc;
-    RayParams p = make_params(c, w, h);
+// tmp (w*h floats) and tmp_vec (3*w*h floats) are the iso post-pass scratch
+int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method, int num_parts,
+                int current_part, int w, int h, float* out, float* alpha, float* depth,
+                float* normals, float* occ, float* tmp, float* tmp_vec, hipStream_t s) {
+    RayParams p = make_params(prm, c->tile_map, w, h);
     const dim3 grid(tile_grid(p.tile_map, p.ntx, p.nty)), block(256);
-    const bool nearest = c->params.filter == SPR_FILTER_NEAREST;
+    const bool nearest = prm.filter == SPR_FILTER_NEAREST;
+    int rc;
     if (method == SPR_METHOD_MAX_PROJECT) {
-        if (num_parts < 1 || (c->params.max_steps / num_parts) / kLoopUnroll < 1)
-            return fail(c, SPR_ERR_INVALID_ARG,
-                        "numParts=%d leaves no 16-sample block of max_steps=%d "
-                        "(the reference divides by zero here)",
-                        num_parts, c->params.max_steps);
+        rc = check_parts(c, prm, num_parts);
+        if (rc) return rc;
         if (!out || !alpha) return fail(c, SPR_ERR_INVALID_ARG, "out/alpha buffers are required");
-        p.nouter = (c->params.max_steps / num_parts) / kLoopUnroll;
+        p.nouter = (prm.max_steps / num_parts) / kLoopUnroll;
         p.current_part = current_part;
-        const bool staged = !nearest && c->params.alpha_pow == 0.0f && c->slab_kb > 0;
-        hipError_t e = visit_volume(c, [&](auto v) {
+        const bool staged = !nearest && prm.alpha_pow == 0.0f && c->slab_kb > 0;
+        hipError_t e = visit_volume(vs, [&](auto v) {
             using V = decltype(v);
             if constexpr (is_padded<V>::value) {
                 if (staged) {
-                    using T = typename V::value_type;
-                    constexpr bool P2 = V::kPow2;
-                    if (c->slab_kb == 8)
-                        hipLaunchKernelGGL((k_max_project_staged<T, P2, 8192>), grid, block, 0, s, v,
-                                           p, out, alpha);
-                    else if (c->slab_kb == 16)
-                        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384>), grid, block, 0, s,
-                                           v, p, out, alpha);
-                    else
-                        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288>), grid, block, 0, s,
-                                           v, p, out, alpha);
+                    launch_staged<typename V::value_type, V::kPow2>(c->slab_kb, grid, s, v, p, out,
+                                                                     alpha);
                     return hipGetLastError();
                 }
             }
@@ -299,15 +351,13 @@ int render_impl(spr_ctx* c, int method, int num_parts, int current_part, int w, 
         return SPR_OK;
     }
     if (method == SPR_METHOD_ISO_SURFACE) {
-        if (c->params.max_steps < 2)
+        if (prm.max_steps < 2)
             return fail(c, SPR_ERR_INVALID_ARG, "iso_surface needs max_steps >= 2");
         if (!out || !alpha || !depth || !normals || !occ)
             return fail(c, SPR_ERR_INVALID_ARG, "iso_surface needs all five output buffers");
-        if (c->params.occ_n_points < 1 || c->params.occ_radius < 0)
+        if (prm.occ_n_points < 1 || prm.occ_radius < 0)
             return fail(c, SPR_ERR_INVALID_ARG, "invalid occlusion parameters");
-        rc = ensure_scratch(c, (size_t)w * h);
-        if (rc) return rc;
-        hipError_t e = visit_volume(c, [&](auto v) {
+        hipError_t e = visit_volume(vs, [&](auto v) {
             using V = decltype(v);
             if (nearest)
                 hipLaunchKernelGGL((k_iso_surface<V, 1>), grid, block, 0, s, v, p, out, alpha,
@@ -321,17 +371,16 @@ int render_impl(spr_ctx* c, int method, int num_parts, int current_part, int w, 
         dim3 g2((w + 15) / 16, (h + 15) / 16), b2(256);
         const ConvWeights wv = conv_weights(7, 5.0f), ws = conv_weights(5, 10.0f);
         // _convolve_vec(buf_normals, 7) (volumerender.py:405-416, 470)
-        hipLaunchKernelGGL((k_conv<0, 3>), g2, b2, 0, s, normals, c->d_tmp_vec, w, h, wv);
-        hipLaunchKernelGGL((k_conv<1, 3>), g2, b2, 0, s, c->d_tmp_vec, normals, w, h, wv);
+        hipLaunchKernelGGL((k_conv<0, 3>), g2, b2, 0, s, normals, tmp_vec, w, h, wv);
+        hipLaunchKernelGGL((k_conv<1, 3>), g2, b2, 0, s, tmp_vec, normals, w, h, wv);
         // occlusion (:472-481)
-        hipLaunchKernelGGL(k_occlusion, g2, b2, 0, s, occ, w, h, c->params.occ_radius,
-                           c->params.occ_n_points, depth);
+        hipLaunchKernelGGL(k_occlusion, g2, b2, 0, s, occ, w, h, prm.occ_radius, prm.occ_n_points,
+                           depth);
         // _convolve_scalar(buf_occlusion, 5) (:392-403, 483)
-        hipLaunchKernelGGL((k_conv<0, 1>), g2, b2, 0, s, occ, c->d_tmp, w, h, ws);
-        hipLaunchKernelGGL((k_conv<1, 1>), g2, b2, 0, s, c->d_tmp, occ, w, h, ws);
+        hipLaunchKernelGGL((k_conv<0, 1>), g2, b2, 0, s, occ, tmp, w, h, ws);
+        hipLaunchKernelGGL((k_conv<1, 1>), g2, b2, 0, s, tmp, occ, w, h, ws);
         // shading overwrites the output (:485-497)
-        hipLaunchKernelGGL(k_shading, g2, b2, 0, s, out, p, c->params.occ_strength, normals, depth,
-                           occ);
+        hipLaunchKernelGGL(k_shading, g2, b2, 0, s, out, p, prm.occ_strength, normals, depth, occ);
         HIPCALL(c, hipGetLastError());
         return SPR_OK;
     }
@@ -345,9 +394,9 @@ void launch_pad(const void* src, void* dst, int nx, int ny, int nz, hipStream_t 
     hipLaunchKernelGGL(k_pad_volume<T>, grid, dim3(256), 0, s, (const T*)src, (T*)dst, nx, ny, nz);
 }
 
-int set_volume_impl(spr_ctx* c, const void* src, bool src_on_device, int dtype, int nx, int ny,
-                    int nz) {
-    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+// fill one slot from host or device memory in the requested layout
+int fill_slot(spr_ctx* c, VolSlot& vs, const void* src, bool src_on_device, int dtype, int nx,
+              int ny, int nz) {
     const size_t es = dtype_size(dtype);
     if (!es) return fail(c, SPR_ERR_UNSUPPORTED_DTYPE, "unsupported voxel type code %d", dtype);
     if (!src || nx <= 0 || ny <= 0 || nz <= 0)
@@ -375,54 +424,149 @@ int set_volume_impl(spr_ctx* c, const void* src, bool src_on_device, int dtype, 
 
     // linear copy: kept as the resident volume (LINEAR), or staged for padding (host source)
     const bool need_linear = layout == SPR_LAYOUT_LINEAR || !src_on_device;
-    if (c->d_pad && (layout != SPR_LAYOUT_PADDED || c->pad_bytes != pbytes)) {
-        HIPCALL(c, hipFree(c->d_pad));
-        c->d_pad = nullptr;
-        c->pad_bytes = 0;
+    if (vs.d_pad && (layout != SPR_LAYOUT_PADDED || vs.pad_bytes != pbytes)) {
+        HIPCALL(c, hipFree(vs.d_pad));
+        vs.d_pad = nullptr;
+        vs.pad_bytes = 0;
     }
-    if (c->d_vol && (!need_linear || c->vol_bytes != bytes)) {
-        HIPCALL(c, hipFree(c->d_vol));
-        c->d_vol = nullptr;
-        c->vol_bytes = 0;
+    if (vs.d_vol && (!need_linear || vs.vol_bytes != bytes)) {
+        HIPCALL(c, hipFree(vs.d_vol));
+        vs.d_vol = nullptr;
+        vs.vol_bytes = 0;
     }
-    if (need_linear && !c->d_vol) {
-        HIPCALL(c, hipMalloc(&c->d_vol, bytes));
-        c->vol_bytes = bytes;
+    vs.used = false;
+    if (need_linear && !vs.d_vol) {
+        HIPCALL(c, hipMalloc(&vs.d_vol, bytes));
+        vs.vol_bytes = bytes;
     }
     if (need_linear)
-        HIPCALL(c, hipMemcpyAsync(c->d_vol, src, bytes,
+        HIPCALL(c, hipMemcpyAsync(vs.d_vol, src, bytes,
                                   src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                                   c->stream));
     if (layout == SPR_LAYOUT_PADDED) {
-        if (!c->d_pad) {
-            HIPCALL(c, hipMalloc(&c->d_pad, pbytes));
-            c->pad_bytes = pbytes;
+        if (!vs.d_pad) {
+            HIPCALL(c, hipMalloc(&vs.d_pad, pbytes));
+            vs.pad_bytes = pbytes;
         }
-        const void* from = need_linear ? c->d_vol : src;
+        const void* from = need_linear ? vs.d_vol : src;
         switch (dtype) {
-            case SPR_DTYPE_F32: launch_pad<float>(from, c->d_pad, nx, ny, nz, c->stream); break;
-            case SPR_DTYPE_U16: launch_pad<uint16_t>(from, c->d_pad, nx, ny, nz, c->stream); break;
-            default: launch_pad<uint8_t>(from, c->d_pad, nx, ny, nz, c->stream); break;
+            case SPR_DTYPE_F32: launch_pad<float>(from, vs.d_pad, nx, ny, nz, c->stream); break;
+            case SPR_DTYPE_U16: launch_pad<uint16_t>(from, vs.d_pad, nx, ny, nz, c->stream); break;
+            default: launch_pad<uint8_t>(from, vs.d_pad, nx, ny, nz, c->stream); break;
         }
         HIPCALL(c, hipGetLastError());
     }
     HIPCALL(c, hipStreamSynchronize(c->stream));
-    if (layout == SPR_LAYOUT_PADDED && c->d_vol) {  // staging copy no longer needed
-        HIPCALL(c, hipFree(c->d_vol));
-        c->d_vol = nullptr;
-        c->vol_bytes = 0;
+    if (layout == SPR_LAYOUT_PADDED && vs.d_vol) {  // staging copy no longer needed
+        HIPCALL(c, hipFree(vs.d_vol));
+        vs.d_vol = nullptr;
+        vs.vol_bytes = 0;
     }
-    c->layout = layout;
-    c->dtype = dtype;
-    c->nx = nx;
-    c->ny = ny;
-    c->nz = nz;
+    vs.used = true;
+    vs.layout = layout;
+    vs.dtype = dtype;
+    vs.nx = nx;
+    vs.ny = ny;
+    vs.nz = nz;
     return SPR_OK;
 }
 
 template <typename T>
 void launch_minmax(const void* data, size_t n, float* partial, int blocks, hipStream_t s) {
     hipLaunchKernelGGL((k_minmax<T>), dim3(blocks), dim3(256), 0, s, (const T*)data, n, partial);
+}
+
+// ---------------------------------------------------------------------------------- batch
+
+// Converts the caller's jobs into device job records; checks that every slot is padded and
+// of one voxel type.  *pow2 = every volume's dimensions are powers of two.
+int build_jobs(spr_ctx* c, const spr_job* jobs, int n, std::vector<BatchJob>& out, int* dtype,
+               bool* pow2) {
+    if (!jobs || n <= 0) return fail(c, SPR_ERR_INVALID_ARG, "empty batch");
+    out.resize(n);
+    *dtype = -1;
+    *pow2 = true;
+    for (int i = 0; i < n; ++i) {
+        const VolSlot* vs = slot_of(c, jobs[i].slot);
+        if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "job %d: no volume in slot %d", i, jobs[i].slot);
+        if (vs->layout != SPR_LAYOUT_PADDED)
+            return fail(c, SPR_ERR_INVALID_ARG, "job %d: batched rendering needs the padded layout", i);
+        if (*dtype >= 0 && vs->dtype != *dtype)
+            return fail(c, SPR_ERR_UNSUPPORTED_DTYPE, "job %d: mixed voxel types in one batch", i);
+        if (!jobs[i].d_out || !jobs[i].d_alpha)
+            return fail(c, SPR_ERR_INVALID_ARG, "job %d: NULL output", i);
+        *dtype = vs->dtype;
+        *pow2 = *pow2 && slot_pow2(*vs);
+        BatchJob& b = out[i];
+        b.base = vs->d_pad;
+        b.nbytes = (uint32_t)vs->pad_bytes;
+        b.px_bytes = (uint32_t)(((size_t)vs->nx + 2 + 16 / dtype_size(vs->dtype) - 1) /
+                                (16 / dtype_size(vs->dtype)) * 16);
+        b.py = (uint32_t)vs->ny + 2;
+        b.nx = vs->nx;
+        b.ny = vs->ny;
+        b.nz = vs->nz;
+        memcpy(b.invP, jobs[i].inv_p, sizeof(b.invP));
+        memcpy(b.invM, jobs[i].inv_m, sizeof(b.invM));
+        b.out = jobs[i].d_out;
+        b.alpha = jobs[i].d_alpha;
+    }
+    return SPR_OK;
+}
+
+BatchQueues make_queues(const RayParams& p, int n) {
+    BatchQueues q;
+    for (uint32_t k = 0; k < 8; ++k) {
+        const uint32_t rows = k < p.nty ? (p.nty - k + 7u) / 8u : 0u;
+        q.tiles[k] = rows * p.ntx;
+        q.count[k] = q.tiles[k] * (uint32_t)n;
+    }
+    return q;
+}
+
+// enqueue counter reset + persistent kernel on s
+template <typename T, bool P2, int SLAB, int MODE>
+hipError_t launch_batch_kernel(const spr_ctx* c, const BatchJob* d_jobs, const RayParams& p,
+                               const BatchQueues& q, uint32_t* counters, hipStream_t s) {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, k_max_project_batch<T, P2, SLAB, MODE>, 256, 0);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    const uint32_t grid = (uint32_t)(c->num_cus * per_cu + 7) / 8u * 8u;
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters);
+    hipLaunchKernelGGL((k_max_project_batch<T, P2, SLAB, MODE>), dim3(grid), dim3(256), 0, s, d_jobs,
+                       p, q, counters);
+    return hipGetLastError();
+}
+
+template <typename T, bool P2>
+hipError_t dispatch_batch_typed(const spr_ctx* c, int mode, const BatchJob* d_jobs,
+                                const RayParams& p, const BatchQueues& q, uint32_t* counters,
+                                hipStream_t s) {
+    if (mode == 1) return launch_batch_kernel<T, P2, 16, 1>(c, d_jobs, p, q, counters, s);
+    if (mode == 2) return launch_batch_kernel<T, P2, 16, 2>(c, d_jobs, p, q, counters, s);
+    if (c->slab_kb == 8) return launch_batch_kernel<T, P2, 8192, 0>(c, d_jobs, p, q, counters, s);
+    if (c->slab_kb == 16) return launch_batch_kernel<T, P2, 16384, 0>(c, d_jobs, p, q, counters, s);
+    return launch_batch_kernel<T, P2, 12288, 0>(c, d_jobs, p, q, counters, s);
+}
+
+hipError_t dispatch_batch(const spr_ctx* c, const spr_params& prm, int dtype, bool pow2,
+                          const BatchJob* d_jobs, const RayParams& p, const BatchQueues& q,
+                          uint32_t* counters, hipStream_t s) {
+    const bool nearest = prm.filter == SPR_FILTER_NEAREST;
+    const int mode = nearest ? 2 : ((prm.alpha_pow == 0.0f && c->slab_kb > 0) ? 0 : 1);
+    switch (dtype) {
+        case SPR_DTYPE_F32:
+            return pow2 ? dispatch_batch_typed<float, true>(c, mode, d_jobs, p, q, counters, s)
+                        : dispatch_batch_typed<float, false>(c, mode, d_jobs, p, q, counters, s);
+        case SPR_DTYPE_U16:
+            return pow2 ? dispatch_batch_typed<uint16_t, true>(c, mode, d_jobs, p, q, counters, s)
+                        : dispatch_batch_typed<uint16_t, false>(c, mode, d_jobs, p, q, counters, s);
+        default:
+            return pow2 ? dispatch_batch_typed<uint8_t, true>(c, mode, d_jobs, p, q, counters, s)
+                        : dispatch_batch_typed<uint8_t, false>(c, mode, d_jobs, p, q, counters, s);
+    }
 }
 
 }  // namespace
@@ -462,8 +606,11 @@ int spr_create(int device, spr_ctx** out) {
     spr_ctx* c = new (std::nothrow) spr_ctx();
     if (!c) return fail(nullptr, SPR_ERR_OUT_OF_MEMORY, "out of host memory");
     c->device = device;
+    c->vols.reserve(kMaxSlots);
+    c->vols.emplace_back();
     DeviceGuard g(device);
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    hipError_t e = hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_count, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&c->d_partial, 2 * 1024 * sizeof(float));
     if (e != hipSuccess) {
@@ -482,7 +629,7 @@ int spr_destroy(spr_ctx* c) {
         if (c->stream) (void)hipStreamSynchronize(c->stream);
         free_outputs(c);
         free_scratch(c);
-        free_volume(c);
+        for (VolSlot& v : c->vols) free_slot(v);
         if (c->d_count) (void)hipFree(c->d_count);
         if (c->d_partial) (void)hipFree(c->d_partial);
         if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -491,7 +638,7 @@ int spr_destroy(spr_ctx* c) {
     return SPR_OK;
 }
 
-const char* spr_last_error(const spr_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
+const char* spr_last_error(const spr_ctx* c) { return g_err_ctx == c ? g_err.c_str() : ""; }
 
 int spr_set_layout(spr_ctx* c, int layout) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
@@ -503,8 +650,8 @@ int spr_set_layout(spr_ctx* c, int layout) {
 
 int spr_get_layout(spr_ctx* c, int* layout) {
     if (!c || !layout) return fail(c, SPR_ERR_INVALID_ARG, "NULL argument");
-    if (!has_volume(c)) return fail(c, SPR_ERR_NO_VOLUME, "no volume");
-    *layout = c->layout;
+    if (!c->vols[0].used) return fail(c, SPR_ERR_NO_VOLUME, "no volume");
+    *layout = c->vols[0].layout;
     return SPR_OK;
 }
 
@@ -534,23 +681,70 @@ int spr_get_tuning(spr_ctx* c, int key, int* value) {
 }
 
 int spr_set_volume(spr_ctx* c, const void* host, int dtype, int nx, int ny, int nz) {
-    return set_volume_impl(c, host, false, dtype, nx, ny, nz);
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    return fill_slot(c, c->vols[0], host, false, dtype, nx, ny, nz);
 }
 
 int spr_set_volume_device(spr_ctx* c, const void* dev, int dtype, int nx, int ny, int nz) {
-    return set_volume_impl(c, dev, true, dtype, nx, ny, nz);
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    return fill_slot(c, c->vols[0], dev, true, dtype, nx, ny, nz);
+}
+
+int spr_volume_add(spr_ctx* c, const void* src, int src_on_device, int dtype, int nx, int ny,
+                   int nz, int* slot) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (!slot) return fail(c, SPR_ERR_INVALID_ARG, "NULL slot output");
+    std::lock_guard<std::mutex> lock(c->table_mu);
+    size_t k = 1;
+    while (k < c->vols.size() && c->vols[k].used) ++k;
+    if (k == c->vols.size()) {
+        if (k == kMaxSlots) return fail(c, SPR_ERR_INVALID_ARG, "all %zu volume slots in use", k);
+        c->vols.emplace_back();
+    }
+    int rc = fill_slot(c, c->vols[k], src, src_on_device != 0, dtype, nx, ny, nz);
+    if (rc) {
+        free_slot(c->vols[k]);
+        return rc;
+    }
+    *slot = (int)k;
+    return SPR_OK;
+}
+
+int spr_volume_release(spr_ctx* c, int slot) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (slot <= 0 || !slot_of(c, slot))
+        return fail(c, SPR_ERR_INVALID_ARG, "slot %d is not an added volume", slot);
+    std::lock_guard<std::mutex> lock(c->table_mu);
+    DeviceGuard g(c->device);
+    HIPCALL(c, hipDeviceSynchronize());  // no in-flight batch may still read it
+    free_slot(c->vols[slot]);
+    return SPR_OK;
+}
+
+int spr_volume_info(spr_ctx* c, int slot, int* dtype, int* nx, int* ny, int* nz,
+                    size_t* device_bytes) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    const VolSlot* vs = slot_of(c, slot);
+    if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d", slot);
+    if (dtype) *dtype = vs->dtype;
+    if (nx) *nx = vs->nx;
+    if (ny) *ny = vs->ny;
+    if (nz) *nz = vs->nz;
+    if (device_bytes) *device_bytes = vs->vol_bytes + vs->pad_bytes;
+    return SPR_OK;
 }
 
 int spr_volume_minmax(spr_ctx* c, float* mn, float* mx) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
-    if (!has_volume(c)) return fail(c, SPR_ERR_NO_VOLUME, "no volume");
+    const VolSlot& vs = c->vols[0];
+    if (!vs.used) return fail(c, SPR_ERR_NO_VOLUME, "no volume");
     if (!mn || !mx) return fail(c, SPR_ERR_INVALID_ARG, "NULL output");
     DeviceGuard g(c->device);
     // the apron replicates edge voxels, so min/max over the padded copy are the volume's
-    const void* data = c->d_pad ? c->d_pad : c->d_vol;
-    const size_t n = (c->d_pad ? c->pad_bytes : c->vol_bytes) / dtype_size(c->dtype);
+    const void* data = vs.d_pad ? vs.d_pad : vs.d_vol;
+    const size_t n = (vs.d_pad ? vs.pad_bytes : vs.vol_bytes) / dtype_size(vs.dtype);
     const int blocks = 1024;
-    switch (c->dtype) {
+    switch (vs.dtype) {
         case SPR_DTYPE_F32: launch_minmax<float>(data, n, c->d_partial, blocks, c->stream); break;
         case SPR_DTYPE_U16: launch_minmax<uint16_t>(data, n, c->d_partial, blocks, c->stream); break;
         default: launch_minmax<uint8_t>(data, n, c->d_partial, blocks, c->stream); break;
@@ -573,10 +767,8 @@ int spr_volume_minmax(spr_ctx* c, float* mn, float* mx) {
 int spr_set_params(spr_ctx* c, const spr_params* p) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
     if (!p) return fail(c, SPR_ERR_INVALID_ARG, "NULL params");
-    if (p->filter != SPR_FILTER_LINEAR && p->filter != SPR_FILTER_NEAREST)
-        return fail(c, SPR_ERR_INVALID_ARG, "unknown filter %d", p->filter);
-    if (p->max_steps < 1)
-        return fail(c, SPR_ERR_INVALID_ARG, "max_steps must be positive (got %d)", p->max_steps);
+    int rc = check_params(c, *p);
+    if (rc) return rc;
     c->params = *p;
     c->has_params = true;
     return SPR_OK;
@@ -589,8 +781,13 @@ int spr_render(spr_ctx* c, int method, int num_parts, int current_part, int w, i
     DeviceGuard g(c->device);
     rc = ensure_outputs(c, w, h);
     if (rc) return rc;
-    rc = render_impl(c, method, num_parts, current_part, w, h, c->d_out, c->d_alpha, c->d_depth,
-                     c->d_normals, c->d_occ, c->stream);
+    if (method == SPR_METHOD_ISO_SURFACE) {
+        rc = ensure_scratch(c, (size_t)w * h);
+        if (rc) return rc;
+    }
+    rc = render_impl(c, c->vols[0], c->params, method, num_parts, current_part, w, h, c->d_out,
+                     c->d_alpha, c->d_depth, c->d_normals, c->d_occ, c->d_tmp, c->d_tmp_vec,
+                     c->stream);
     if (rc) return rc;
     const size_t px = (size_t)w * h;
     if (out) HIPCALL(c, hipMemcpyAsync(out, c->d_out, px * 4, hipMemcpyDeviceToHost, c->stream));
@@ -613,8 +810,168 @@ int spr_render_device(spr_ctx* c, int method, int num_parts, int current_part, i
     if (rc) return rc;
     DeviceGuard g(c->device);
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-    return render_impl(c, method, num_parts, current_part, w, h, d_out, d_alpha, d_depth, d_normals,
-                       d_occlusion, s);
+    if (method == SPR_METHOD_ISO_SURFACE) {
+        rc = ensure_scratch(c, (size_t)w * h);
+        if (rc) return rc;
+    }
+    return render_impl(c, c->vols[0], c->params, method, num_parts, current_part, w, h, d_out,
+                       d_alpha, d_depth, d_normals, d_occlusion, c->d_tmp, c->d_tmp_vec, s);
+}
+
+int spr_render_slot(spr_ctx* c, int slot, const spr_params* params, int method, int num_parts,
+                    int current_part, int w, int h, float* d_out, float* d_alpha, float* d_depth,
+                    float* d_normals, float* d_occlusion, void* stream) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    const VolSlot* vs = slot_of(c, slot);
+    if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d", slot);
+    if (!params) return fail(c, SPR_ERR_INVALID_ARG, "NULL params");
+    int rc = check_params(c, *params);
+    if (rc) return rc;
+    if (w <= 0 || h <= 0 || w > 32768 || h > 32768)
+        return fail(c, SPR_ERR_INVALID_ARG, "invalid output size %d x %d", w, h);
+    DeviceGuard g(c->device);
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    float* scratch = nullptr;
+    if (method == SPR_METHOD_ISO_SURFACE)  // per-call scratch: concurrent calls do not share it
+        HIPCALL(c, hipMallocAsync((void**)&scratch, (size_t)w * h * 16, s));
+    rc = render_impl(c, *vs, *params, method, num_parts, current_part, w, h, d_out, d_alpha, d_depth,
+                     d_normals, d_occlusion, scratch, scratch ? scratch + (size_t)w * h : nullptr, s);
+    if (scratch) (void)hipFreeAsync(scratch, s);
+    return rc;
+}
+
+int spr_render_batch(spr_ctx* c, const spr_params* params, const spr_job* jobs, int n, int w, int h,
+                     int num_parts, int current_part, void* stream) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    const spr_params& prm = params ? *params : c->params;
+    if (!params && !c->has_params) return fail(c, SPR_ERR_INVALID_ARG, "no parameters");
+    int rc = check_params(c, prm);
+    if (rc) return rc;
+    if (w <= 0 || h <= 0 || w > 32768 || h > 32768)
+        return fail(c, SPR_ERR_INVALID_ARG, "invalid output size %d x %d", w, h);
+    rc = check_parts(c, prm, num_parts);
+    if (rc) return rc;
+    std::vector<BatchJob> host;
+    int dtype;
+    bool pow2;
+    rc = build_jobs(c, jobs, n, host, &dtype, &pow2);
+    if (rc) return rc;
+    DeviceGuard g(c->device);
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    // per-call job table and queue counters from the stream-ordered allocator, so batches
+    // submitted concurrently on different streams do not share them
+    const size_t jbytes = (size_t)n * sizeof(BatchJob);
+    void* buf = nullptr;
+    HIPCALL(c, hipMallocAsync(&buf, jbytes + 64, s));
+    BatchJob* d_jobs = (BatchJob*)buf;
+    uint32_t* d_counters = (uint32_t*)((char*)buf + jbytes);
+    hipError_t e = hipMemcpyAsync(d_jobs, host.data(), jbytes, hipMemcpyHostToDevice, s);
+    RayParams p = make_params(prm, c->tile_map, w, h);
+    p.nouter = (prm.max_steps / num_parts) / kLoopUnroll;
+    p.current_part = current_part;
+    const BatchQueues q = make_queues(p, n);
+    if (e == hipSuccess) e = dispatch_batch(c, prm, dtype, pow2, d_jobs, p, q, d_counters, s);
+    (void)hipFreeAsync(buf, s);
+    HIPCALL(c, e);
+    // blocking: the host job table must outlive the transfer
+    HIPCALL(c, hipStreamSynchronize(s));
+    return SPR_OK;
+}
+
+int spr_batch_graph_create(spr_ctx* c, const spr_job* jobs, int n, int w, int h,
+                           spr_batch_graph** out) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (!out) return fail(c, SPR_ERR_INVALID_ARG, "NULL output");
+    *out = nullptr;
+    int rc = check_frame(c, w, h);
+    if (rc) return rc;
+    std::vector<BatchJob> host;
+    int dtype;
+    bool pow2;
+    rc = build_jobs(c, jobs, n, host, &dtype, &pow2);
+    if (rc) return rc;
+    DeviceGuard g(c->device);
+    spr_batch_graph* bg = new (std::nothrow) spr_batch_graph();
+    if (!bg) return fail(c, SPR_ERR_OUT_OF_MEMORY, "out of host memory");
+    bg->n = n;
+    bg->w = w;
+    bg->h = h;
+    bg->dtype = dtype;
+    auto cleanup = [&](int code) {
+        spr_batch_graph_destroy(c, bg);
+        return code;
+    };
+    if (hipMalloc(&bg->d_jobs, (size_t)n * sizeof(BatchJob)) != hipSuccess ||
+        hipMalloc(&bg->d_counters, 8 * sizeof(uint32_t)) != hipSuccess)
+        return cleanup(fail(c, SPR_ERR_OUT_OF_MEMORY, "graph buffers"));
+    if (hipMemcpy(bg->d_jobs, host.data(), (size_t)n * sizeof(BatchJob), hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return cleanup(fail(c, SPR_ERR_HIP, "job upload failed"));
+    rc = check_parts(c, c->params, 1);
+    if (rc) return cleanup(rc);
+    RayParams p = make_params(c, w, h);
+    p.nouter = c->params.max_steps / kLoopUnroll;
+    p.current_part = 0;
+    const BatchQueues q = make_queues(p, n);
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(c, SPR_ERR_HIP, "capture stream"));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    hipError_t e2 = hipSuccess;
+    if (e == hipSuccess) {
+        e2 = dispatch_batch(c, c->params, dtype, pow2, bg->d_jobs, p, q, bg->d_counters, cs);
+        e = hipStreamEndCapture(cs, &bg->graph);
+    }
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess || e2 != hipSuccess)
+        return cleanup(fail(c, SPR_ERR_HIP, "graph capture failed: %s",
+                            hipGetErrorString(e != hipSuccess ? e : e2)));
+    if (hipGraphInstantiate(&bg->exec, bg->graph, nullptr, nullptr, 0) != hipSuccess)
+        return cleanup(fail(c, SPR_ERR_HIP, "graph instantiation failed"));
+    *out = bg;
+    return SPR_OK;
+}
+
+int spr_batch_graph_update(spr_ctx* c, spr_batch_graph* bg, const spr_job* jobs, int n) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (!bg) return fail(c, SPR_ERR_INVALID_ARG, "NULL graph");
+    if (n != bg->n) return fail(c, SPR_ERR_INVALID_ARG, "graph holds %d jobs, got %d", bg->n, n);
+    std::vector<BatchJob> host;
+    int dtype;
+    bool pow2;
+    int rc = build_jobs(c, jobs, n, host, &dtype, &pow2);
+    if (rc) return rc;
+    if (dtype != bg->dtype)
+        return fail(c, SPR_ERR_UNSUPPORTED_DTYPE, "voxel type differs from the captured batch");
+    DeviceGuard g(c->device);
+    if (bg->last_stream) HIPCALL(c, hipStreamSynchronize(bg->last_stream));
+    HIPCALL(c, hipMemcpy(bg->d_jobs, host.data(), (size_t)n * sizeof(BatchJob),
+                         hipMemcpyHostToDevice));
+    return SPR_OK;
+}
+
+int spr_batch_graph_launch(spr_ctx* c, spr_batch_graph* bg, void* stream) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (!bg || !bg->exec) return fail(c, SPR_ERR_INVALID_ARG, "NULL graph");
+    DeviceGuard g(c->device);
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    HIPCALL(c, hipGraphLaunch(bg->exec, s));
+    bg->last_stream = s;
+    return SPR_OK;
+}
+
+int spr_batch_graph_destroy(spr_ctx* c, spr_batch_graph* bg) {
+    if (!bg) return SPR_OK;
+    if (c) {
+        DeviceGuard g(c->device);
+        if (bg->last_stream) (void)hipStreamSynchronize(bg->last_stream);
+    }
+    if (bg->exec) (void)hipGraphExecDestroy(bg->exec);
+    if (bg->graph) (void)hipGraphDestroy(bg->graph);
+    if (bg->d_jobs) (void)hipFree(bg->d_jobs);
+    if (bg->d_counters) (void)hipFree(bg->d_counters);
+    delete bg;
+    return SPR_OK;
 }
 
 int spr_synchronize(spr_ctx* c) {
@@ -634,8 +991,8 @@ int spr_count_samples(spr_ctx* c, int num_parts, int w, int h, double* samples) 
     int rc = check_ready(c, w, h);
     if (rc) return rc;
     if (!samples) return fail(c, SPR_ERR_INVALID_ARG, "NULL output");
-    if (num_parts < 1 || (c->params.max_steps / num_parts) / kLoopUnroll < 1)
-        return fail(c, SPR_ERR_INVALID_ARG, "invalid numParts %d", num_parts);
+    rc = check_parts(c, c->params, num_parts);
+    if (rc) return rc;
     DeviceGuard g(c->device);
     RayParams p = make_params(c, w, h);
     HIPCALL(c, hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->stream));

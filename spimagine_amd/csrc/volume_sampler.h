@@ -184,11 +184,33 @@ struct PaddedVolume {
         *iz1 = corner1(fz, mz);
     }
 
+    // Lower-corner floors clamped to [-1, N-1] (as floats) and trilinear weights.  Index
+    // arithmetic on these stays exact in float32 while it stays below 2^24, which the
+    // padded layout guarantees for (nz+2)*(ny+2) and the row pitch.
+    __device__ __forceinline__ void locate_f(float px, float py_, float pz, float* cx, float* cy,
+                                             float* cz, float* ax, float* ay, float* az) const {
+        float ux = texel<POW2>(px, fnx), uy = texel<POW2>(py_, fny), uz = texel<POW2>(pz, fnz);
+        float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
+        *ax = ux - fx;
+        *ay = uy - fy;
+        *az = uz - fz;
+        *cx = __builtin_amdgcn_fmed3f(fx, -1.0f, mx);
+        *cy = __builtin_amdgcn_fmed3f(fy, -1.0f, my);
+        *cz = __builtin_amdgcn_fmed3f(fz, -1.0f, mz);
+    }
+
+    // byte offset of the padded corner (cx+1, cy+1, cz+1): row = (cz+1)*PY + (cy+1) exact
+    // in float, then one 24-bit multiply-add with the row pitch
+    __device__ __forceinline__ uint32_t byte_offset_f(float cx, float cy, float cz) const {
+        const float row = fmaf(cz + 1.0f, (float)py, cy + 1.0f);
+        const uint32_t xb = (uint32_t)fmaf(cx, (float)sizeof(T), (float)sizeof(T));
+        return __umul24((uint32_t)row, px_bytes) + xb;
+    }
+
     __device__ __forceinline__ float linear(float px, float py_, float pz) const {
-        uint32_t ix1, iy1, iz1;
-        float ax, ay, az;
-        locate(px, py_, pz, &ix1, &iy1, &iz1, &ax, &ay, &az);
-        const uint32_t off = byte_offset(ix1, iy1, iz1);
+        float cx, cy, cz, ax, ay, az;
+        locate_f(px, py_, pz, &cx, &cy, &cz, &ax, &ay, &az);
+        const uint32_t off = byte_offset_f(cx, cy, cz);
         const __amdgpu_buffer_rsrc_t r = rsrc();
         float v000, v100, v010, v110, v001, v101, v011, v111;
         PairBits<T>::load(r, off, 0, &v000, &v100);

@@ -108,6 +108,62 @@ def blob_volume_device(shape, n_blobs, sigma, amp, background, poisson, read_noi
     return vol
 
 
+def _splat_blobs_device(vol, centres, sigmas, amps, chunk=96):
+    """Adds the blobs to a float32 device volume in chunks: each blob is evaluated on a
+    64^3 window anchored at floor(c - 3 sigma) (wide enough for sigma <= 10), masked to
+    [floor(c - 3 sigma), ceil(c + 3 sigma)] and the volume, and scattered with index_add_."""
+    import torch
+    dev = vol.device
+    dims = torch.tensor(vol.shape, device=dev, dtype=torch.float32)
+    L = 64
+    if float(np.max(sigmas)) * 6 + 3 > L:
+        raise ValueError("blob sigma too large for the splat window")
+    ar = torch.arange(L, device=dev, dtype=torch.float32)
+    c_all = torch.as_tensor(np.asarray(centres, np.float32), device=dev)
+    s_all = torch.as_tensor(np.asarray(sigmas, np.float32), device=dev)
+    a_all = torch.as_tensor(np.asarray(amps, np.float32), device=dev)
+    ny, nx = vol.shape[1], vol.shape[2]
+    flat = vol.view(-1)
+    for k in range(0, len(amps), chunk):
+        c, s, a = c_all[k:k + chunk], s_all[k:k + chunk], a_all[k:k + chunk]
+        lo = torch.floor(c - 3 * s)
+        hi = torch.ceil(c + 3 * s)
+        idx = lo[:, :, None] + ar                                        # (m, 3, L)
+        g = torch.exp(-.5 * ((idx - c[:, :, None]) / s[:, :, None]) ** 2)
+        ok = (idx >= 0) & (idx < dims[None, :, None]) & (idx <= hi[:, :, None])
+        g = torch.where(ok, g, torch.zeros_like(g))
+        idx = torch.minimum(idx.clamp(min=0), dims[None, :, None] - 1).long()
+        w = (a[:, None, None, None] * g[:, 0, :, None, None] * g[:, 1, None, :, None]
+             * g[:, 2, None, None, :])
+        lin = (idx[:, 0, :, None, None] * ny + idx[:, 1, None, :, None]) * nx \
+            + idx[:, 2, None, None, :]
+        flat.index_add_(0, lin.reshape(-1), w.reshape(-1))
+
+
+def blob_series_device(shape, n_blobs, sigma, amp, background, poisson, read_noise, n_time,
+                       drift=.5, seed=0, device="cuda"):
+    """Yields n_time uint16 timepoints (int16 bit pattern, like blob_volume_device) of one
+    blob population whose centres take seeded random walks of `drift` voxels per step
+    (each step a uniformly random direction)."""
+    import torch
+    rng, centres, sigmas, amps = blob_params(shape, n_blobs, sigma, amp, seed)
+    gen = torch.Generator(device=device)
+    for t in range(n_time):
+        if t:
+            step = rng.normal(size=centres.shape)
+            centres = centres + drift * step / np.linalg.norm(step, axis=1, keepdims=True)
+        vol = torch.full(shape, float(background), dtype=torch.float32, device=device)
+        _splat_blobs_device(vol, centres, sigmas, amps)
+        gen.manual_seed(int(seed) * 1000 + t + 1)
+        if poisson:
+            vol = torch.poisson(vol, generator=gen)
+        if read_noise:
+            vol += float(read_noise) * torch.randn(vol.shape, generator=gen, device=device)
+        v = torch.clamp(torch.round(vol), 0, 65535).to(torch.int32)
+        del vol
+        yield torch.where(v >= 32768, v - 65536, v).to(torch.int16)
+
+
 def device_volume_to_numpy(t, dtype):
     """Host copy of a volume from blob_volume_device, as the numpy dtype it encodes."""
     return t.cpu().numpy().view(np.dtype(dtype))
