@@ -1,0 +1,83 @@
+/*
+ * spimlearn.h -- C ABI of libspimlearn.so: learned reconstruction of a full-resolution
+ * max projection from a coarse one, on gfx950 matrix cores (BF16 operands, FP32
+ * accumulation), with the training step on the device as well.
+ *
+ * No counterpart in the reference.  Its nearest analogue is progressive multipass
+ * rendering (numParts / currentPart, spimagine/volumerender/kernels/volume_kernel.cl:100-106,
+ * 172-183; driven from spimagine/gui/glwidget.py:633-635, 688-694): a cheaper frame first,
+ * the exact one later.  Here the cheap frame is a max projection rendered at half the
+ * resolution with a quarter of the ray samples (max_steps / 4), from the volume and from a
+ * max-filtered copy of it (spr_volume_add_dilated); the network maps those two planes to
+ * the full-resolution frame.  The exact renderer stays the reference and the fallback.
+ *
+ * The network (all 3x3 convolutions, zero padding 1, C = 32 channels):
+ *   a0 = relu(conv_in(x))            x: 2 planes h x w (coarse MIP, coarse dilated MIP)
+ *   e0 = relu(conv(a0))              h   x w
+ *   e1 = relu(conv(avgpool2(e0)))    h/2 x w/2
+ *   e2 = relu(conv(relu(conv(avgpool2(e1)))))   h/4 x w/4
+ *   u1 = relu(conv(up2(e2) + e1))    h/2 x w/2      (up2 = nearest-neighbour x2)
+ *   u0 = relu(conv(up2(u1) + e0))    h   x w
+ *   y  = pixel_shuffle2(head(u0)) + bilinear_up2(x plane 0)     2h x 2w
+ * Parameters: one float32 array of spl_param_count() values, in this order:
+ *   conv_in W[9][2][32], b[32];  six 32->32 layers (e0, e1, e2a, e2, u1, u0) W[9][32][32],
+ *   b[32];  head W[9][32][4], b[4];  tap index t = 3 (dy + 1) + (dx + 1),
+ *   out(y, x, co) = b[co] + sum_t sum_ci in(y + dy, x + dx, ci) W[t][ci][co].
+ * Activations are stored as bf16 (channels fastest), gradients as float32; the master
+ * weights and the Adam state are float32.
+ */
+#ifndef SPIMLEARN_H
+#define SPIMLEARN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPL_ABI_VERSION 1
+
+#define SPL_OK 0
+#define SPL_ERR_INVALID_ARG (-1)
+#define SPL_ERR_OUT_OF_MEMORY (-4)
+#define SPL_ERR_HIP (-5)
+#define SPL_ERR_NO_DEVICE (-6)
+
+typedef struct spl_net spl_net;
+
+int spl_abi_version(void);
+/* Number of float32 parameters (57252). */
+int spl_param_count(void);
+
+/* A network on a device, parameters zero until spl_set_params. */
+int spl_create(int device, spl_net **net);
+int spl_destroy(spl_net *net);
+/* Message of the calling thread's last failing call on `net` ("" if none). */
+const char *spl_last_error(const spl_net *net);
+
+/* Copy all parameters in / out (host float32, n == spl_param_count()).  Setting them
+   resets the Adam state. */
+int spl_set_params(spl_net *net, const float *host, int n);
+int spl_get_params(spl_net *net, float *host, int n);
+
+/* Inference: d_in = n images of 2 float32 planes (n, 2, h, w); d_out = (n, 2h, 2w)
+   float32.  h and w must be multiples of 4.  Enqueued on `stream` (hipStream_t; NULL =
+   the network's own stream); no host synchronisation. */
+int spl_forward(spl_net *net, const float *d_in, int n, int h, int w, float *d_out,
+                void *stream);
+
+/* Gradient of the summed squared error sum (y - target)^2 / (n 4 h w) with respect to
+   every parameter, into host_grad (spl_param_count() floats); *sse = the summed squared
+   error.  Blocks until done.  (The training step's gradient, exposed for testing.) */
+int spl_gradients(spl_net *net, const float *d_in, const float *d_target, int n, int h, int w,
+                  float *host_grad, double *sse, void *stream);
+
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, learning rate lr) on the mean squared
+   error of the batch; *sse = the batch's summed squared error before the step (may be
+   NULL).  Blocks until the loss is read back when sse != NULL. */
+int spl_train_step(spl_net *net, const float *d_in, const float *d_target, int n, int h, int w,
+                   float lr, double *sse, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPIMLEARN_H */

@@ -166,6 +166,14 @@ int spr_render_device(spr_ctx *ctx, int method, int num_parts, int current_part,
 /* Upload a (nz,ny,nx) volume (host or device source) into a new slot. */
 int spr_volume_add(spr_ctx *ctx, const void *src, int src_on_device, int dtype, int nx, int ny,
                    int nz, int *slot);
+/* A new slot holding the max filter of slot src_slot over (2 radius + 1)^3 voxels (edges
+   clamped), computed on the device: the "dilated" volume whose coarse projection bounds
+   the samples a coarse ray step skips (input of the learned reconstruction, spimlearn.h).
+   radius in [1, 16]. */
+int spr_volume_add_dilated(spr_ctx *ctx, int src_slot, int radius, int *slot);
+/* Copy a slot's volume back to host memory as a (nz,ny,nx) C-order array (bytes must be
+   nx ny nz x voxel size); the readback of update_data's image (volumerender.py:294). */
+int spr_volume_read(spr_ctx *ctx, int slot, void *host, size_t bytes);
 /* Free a slot >= 1 (waits for the device to finish work that may read it). */
 int spr_volume_release(spr_ctx *ctx, int slot);
 /* Voxel type, size and device bytes of a slot (0 = the renderer's volume). */

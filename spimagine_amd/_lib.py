@@ -34,7 +34,8 @@ EXPORTS = (
     "spr_last_error", "spr_set_layout", "spr_get_layout", "spr_set_tuning", "spr_get_tuning",
     "spr_set_volume", "spr_set_volume_device", "spr_volume_minmax",
     "spr_set_params", "spr_render", "spr_render_device", "spr_synchronize", "spr_get_stream",
-    "spr_count_samples", "spr_volume_add", "spr_volume_release", "spr_volume_info",
+    "spr_count_samples", "spr_volume_add", "spr_volume_add_dilated", "spr_volume_read",
+    "spr_volume_release", "spr_volume_info",
     "spr_render_batch", "spr_render_slot", "spr_batch_graph_create", "spr_batch_graph_update",
     "spr_batch_graph_launch", "spr_batch_graph_destroy",
 )
@@ -143,6 +144,8 @@ def load():
     L.spr_count_samples.argtypes = [c_void_p, c_int, c_int, c_int, ctypes.POINTER(ctypes.c_double)]
     L.spr_volume_add.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
                                  ctypes.POINTER(c_int)]
+    L.spr_volume_add_dilated.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(c_int)]
+    L.spr_volume_read.argtypes = [c_void_p, c_int, c_void_p, ctypes.c_size_t]
     L.spr_volume_release.argtypes = [c_void_p, c_int]
     L.spr_volume_info.argtypes = [c_void_p, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                                   ctypes.POINTER(c_int), ctypes.POINTER(c_int),
@@ -293,6 +296,21 @@ class Context:
                                                 DTYPE_CODES[np.dtype(dtype)], nx, ny, nz,
                                                 ctypes.byref(slot)))
         return slot.value
+
+    def volume_add_dilated(self, slot, radius):
+        """A new slot: the (2 radius + 1)^3 max filter of `slot`, computed on the device."""
+        out = ctypes.c_int()
+        self._check(self.lib.spr_volume_add_dilated(self.handle, int(slot), int(radius),
+                                                    ctypes.byref(out)))
+        return out.value
+
+    def volume_read(self, slot):
+        """The slot's volume as a host (nz, ny, nx) array."""
+        info = self.volume_info(slot)
+        dt = {DTYPE_F32: np.float32, DTYPE_U16: np.uint16, DTYPE_U8: np.uint8}[info["dtype"]]
+        out = np.empty(info["shape"], dt)
+        self._check(self.lib.spr_volume_read(self.handle, int(slot), out.ctypes.data, out.nbytes))
+        return out
 
     def volume_release(self, slot):
         self._check(self.lib.spr_volume_release(self.handle, int(slot)))

@@ -1,9 +1,10 @@
-"""Build libspimrender.so (gfx950) in-tree with hipcc.
+"""Build libspimrender.so (ray caster) and libspimlearn.so (learned reconstruction) for
+gfx950 in-tree with hipcc.
 
     python -m spimagine_amd.build
 
-The library lands next to this file (spimagine_amd/libspimrender.so) so it travels
-with the source tree; nothing is installed into site-packages.
+The libraries land next to this file (spimagine_amd/lib*.so) so they travel with the
+source tree; nothing is installed into site-packages.
 """
 import os
 import subprocess
@@ -13,9 +14,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspimrender.so")
-SOURCES = [os.path.join(CSRC, "spimrender.hip")]
-HEADERS = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")] + [
-    os.path.join(ROOT, "include", "spimrender.h")]
+LEARN_LIB = os.path.join(HERE, "libspimlearn.so")
+LIBRARIES = {LIB: [os.path.join(CSRC, "spimrender.hip")],
+             LEARN_LIB: [os.path.join(CSRC, "spimlearn.hip")]}
 
 # -ffp-contract=off + correctly rounded fp32 div/sqrt: the kernels' float32 operation
 # sequence is fixed (every FMA is an explicit fmaf).  Packed FP32 instructions are
@@ -39,17 +40,24 @@ def hipcc():
     return p if os.path.exists(p) else "hipcc"
 
 
-def build_library(verbose=False):
-    tmp = LIB + ".tmp"
+def build_one(lib, sources, verbose=False):
+    tmp = lib + ".tmp"
     cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-I", CSRC,
-                                     "-o", tmp] + SOURCES
+                                     "-o", tmp] + sources
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building %s" % LIB)
-    os.replace(tmp, LIB)
+        raise RuntimeError("hipcc failed building %s" % lib)
+    os.replace(tmp, lib)
+    return lib
+
+
+def build_library(verbose=False):
+    """Builds every library; returns the ray caster's path."""
+    for lib, sources in LIBRARIES.items():
+        build_one(lib, sources, verbose)
     return LIB
 
 

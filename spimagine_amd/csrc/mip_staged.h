@@ -41,6 +41,31 @@ __device__ __forceinline__ float lds_voxel(const uint8_t* p) {
     return (float)*reinterpret_cast<const T*>(p);
 }
 
+// The voxel pair (x, x+1) at byte offset b of a 4-byte aligned slab, read as two aligned
+// dwords (one ds_read2_b32) and shifted into place with v_alignbyte: LDS accesses that
+// are not dword aligned stall the LDS pipeline.  Reads the 8 bytes from the dword holding
+// byte b, so up to 6 bytes past the pair (the slab array carries 16 bytes of slack).
+__device__ __forceinline__ void lds_pair(const uint8_t* slab, uint32_t b, float& v0, float& v1,
+                                         uint16_t) {
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(slab + (b & ~3u));
+    const uint32_t w = __builtin_amdgcn_alignbyte(d[1], d[0], b & 3u);
+    v0 = (float)(w & 0xFFFFu);
+    v1 = (float)(w >> 16);
+}
+__device__ __forceinline__ void lds_pair(const uint8_t* slab, uint32_t b, float& v0, float& v1,
+                                         uint8_t) {
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(slab + (b & ~3u));
+    const uint32_t w = __builtin_amdgcn_alignbyte(d[1], d[0], b & 3u);
+    v0 = (float)(w & 0xFFu);
+    v1 = (float)((w >> 8) & 0xFFu);
+}
+__device__ __forceinline__ void lds_pair(const uint8_t* slab, uint32_t b, float& v0, float& v1,
+                                         float) {
+    const float* d = reinterpret_cast<const float*>(slab + b);  // b is a multiple of 4
+    v0 = d[0];
+    v1 = d[1];
+}
+
 // 16-byte aligned load / store units
 struct alignas(16) Unit16 {
     uint32_t w[4];
@@ -181,11 +206,12 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
                 float cx, cy, cz, ax, ay, az;
                 vol.locate_f(px, py, pz, &cx, &cy, &cz, &ax, &ay, &az);
                 const float bf = fmaf(cz + oz, fsz, fmaf(cy + oy, fsy, fmaf(cx, (float)TS, ox)));
-                const uint8_t* q = slab + (uint32_t)bf;
-                const float v000 = lds_voxel<T>(q), v100 = lds_voxel<T>(q + TS);
-                const float v010 = lds_voxel<T>(q + sy), v110 = lds_voxel<T>(q + sy + TS);
-                const float v001 = lds_voxel<T>(q + sz), v101 = lds_voxel<T>(q + sz + TS);
-                const float v011 = lds_voxel<T>(q + sy + sz), v111 = lds_voxel<T>(q + sy + sz + TS);
+                const uint32_t q = (uint32_t)bf;
+                float v000, v100, v010, v110, v001, v101, v011, v111;
+                lds_pair(slab, q, v000, v100, T());
+                lds_pair(slab, q + sy, v010, v110, T());
+                lds_pair(slab, q + sz, v001, v101, T());
+                lds_pair(slab, q + sy + sz, v011, v111, T());
                 col = fmaxf(col, trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az));
                 px = px + dx;
                 py = py + dy;
@@ -221,7 +247,7 @@ template <typename T, bool POW2, int SLAB_BYTES>
 __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2> vol, RayParams p,
                                                             float* __restrict__ out,
                                                             float* __restrict__ alpha) {
-    __shared__ Unit16 lds[4 * SLAB_BYTES / 16];
+    __shared__ Unit16 lds[4 * SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
     uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * SLAB_BYTES;
     uint32_t x = 0, y = 0;
     const bool inside = tile_pixel(p, &x, &y);
@@ -288,7 +314,7 @@ template <typename T, bool POW2, int SLAB_BYTES, int MODE>
 __global__ __launch_bounds__(256) void k_max_project_batch(const BatchJob* __restrict__ jobs,
                                                            RayParams shared, BatchQueues qs,
                                                            uint32_t* __restrict__ counters) {
-    __shared__ Unit16 lds[4 * SLAB_BYTES / 16];
+    __shared__ Unit16 lds[4 * SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
     __shared__ uint32_t s_item;
     uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * SLAB_BYTES;
     uint32_t q = blockIdx.x & 7u;

@@ -394,6 +394,41 @@ void launch_pad(const void* src, void* dst, int nx, int ny, int nz, hipStream_t 
     hipLaunchKernelGGL(k_pad_volume<T>, grid, dim3(256), 0, s, (const T*)src, (T*)dst, nx, ny, nz);
 }
 
+// out(x, y, z) = max of in over [c - r, c + r] along AXIS (0 x, 1 y, 2 z), coordinates
+// clamped to the volume; in is addressed in[((z + o) py + y + o) px + x + o] (o = 1, the
+// padded layout's apron; o = 0, a plain (nz, ny, nx) array); out is a plain array
+template <typename T, int AXIS>
+__global__ __launch_bounds__(256) void k_max_filter(const T* __restrict__ in, T* __restrict__ out,
+                                                    int nx, int ny, int nz, int px, int py, int o,
+                                                    int r) {
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= nx) return;
+    T m = in[((size_t)(z + o) * py + y + o) * px + x + o];
+    for (int k = -r; k <= r; ++k) {
+        int xx = x, yy = y, zz = z;
+        if (AXIS == 0) xx = min(max(x + k, 0), nx - 1);
+        if (AXIS == 1) yy = min(max(y + k, 0), ny - 1);
+        if (AXIS == 2) zz = min(max(z + k, 0), nz - 1);
+        const T v = in[((size_t)(zz + o) * py + yy + o) * px + xx + o];
+        m = v > m ? v : m;
+    }
+    out[((size_t)z * ny + y) * nx + x] = m;
+}
+
+template <typename T>
+void launch_max_filter3(const VolSlot& vs, void* t1, void* t2, int r, hipStream_t s) {
+    const dim3 grid((vs.nx + 255) / 256, vs.ny, vs.nz), block(256);
+    const bool pad = vs.layout == SPR_LAYOUT_PADDED;
+    const T* src = (const T*)(pad ? vs.d_pad : vs.d_vol);
+    const int px = pad ? (int)padded_pitch<T>(vs.nx) : vs.nx, py = pad ? vs.ny + 2 : vs.ny;
+    hipLaunchKernelGGL((k_max_filter<T, 0>), grid, block, 0, s, src, (T*)t1, vs.nx, vs.ny, vs.nz,
+                       px, py, pad ? 1 : 0, r);
+    hipLaunchKernelGGL((k_max_filter<T, 1>), grid, block, 0, s, (const T*)t1, (T*)t2, vs.nx, vs.ny,
+                       vs.nz, vs.nx, vs.ny, 0, r);
+    hipLaunchKernelGGL((k_max_filter<T, 2>), grid, block, 0, s, (const T*)t2, (T*)t1, vs.nx, vs.ny,
+                       vs.nz, vs.nx, vs.ny, 0, r);
+}
+
 // fill one slot from host or device memory in the requested layout
 int fill_slot(spr_ctx* c, VolSlot& vs, const void* src, bool src_on_device, int dtype, int nx,
               int ny, int nz) {
@@ -707,6 +742,63 @@ int spr_volume_add(spr_ctx* c, const void* src, int src_on_device, int dtype, in
         return rc;
     }
     *slot = (int)k;
+    return SPR_OK;
+}
+
+int spr_volume_add_dilated(spr_ctx* c, int src_slot, int radius, int* slot) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (!slot) return fail(c, SPR_ERR_INVALID_ARG, "NULL slot output");
+    const VolSlot* vs = slot_of(c, src_slot);
+    if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d", src_slot);
+    if (radius < 1 || radius > 16) return fail(c, SPR_ERR_INVALID_ARG, "radius %d not in [1, 16]", radius);
+    DeviceGuard g(c->device);
+    const size_t bytes = (size_t)vs->nx * vs->ny * vs->nz * dtype_size(vs->dtype);
+    void *t1 = nullptr, *t2 = nullptr;
+    hipError_t e = hipMalloc(&t1, bytes);
+    if (e == hipSuccess) e = hipMalloc(&t2, bytes);
+    if (e == hipSuccess) {
+        switch (vs->dtype) {
+            case SPR_DTYPE_F32: launch_max_filter3<float>(*vs, t1, t2, radius, c->stream); break;
+            case SPR_DTYPE_U16: launch_max_filter3<uint16_t>(*vs, t1, t2, radius, c->stream); break;
+            default: launch_max_filter3<uint8_t>(*vs, t1, t2, radius, c->stream); break;
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    int rc = SPR_OK;
+    if (e != hipSuccess)
+        rc = fail(c, e == hipErrorOutOfMemory ? SPR_ERR_OUT_OF_MEMORY : SPR_ERR_HIP,
+                  "max filter failed: %s", hipGetErrorString(e));
+    else
+        rc = spr_volume_add(c, t1, 1, vs->dtype, vs->nx, vs->ny, vs->nz, slot);
+    if (t1) (void)hipFree(t1);
+    if (t2) (void)hipFree(t2);
+    return rc;
+}
+
+int spr_volume_read(spr_ctx* c, int slot, void* host, size_t bytes) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    const VolSlot* vs = slot_of(c, slot);
+    if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d", slot);
+    const size_t es = dtype_size(vs->dtype), row = (size_t)vs->nx * es;
+    const size_t need = row * vs->ny * vs->nz;
+    if (!host || bytes != need)
+        return fail(c, SPR_ERR_INVALID_ARG, "host buffer of %zu bytes, the volume has %zu", bytes, need);
+    DeviceGuard g(c->device);
+    if (vs->layout == SPR_LAYOUT_PADDED) {
+        const size_t pitch = (size_t)(vs->dtype == SPR_DTYPE_F32   ? padded_pitch<float>(vs->nx)
+                                      : vs->dtype == SPR_DTYPE_U16 ? padded_pitch<uint16_t>(vs->nx)
+                                                                   : padded_pitch<uint8_t>(vs->nx)) *
+                             es;
+        for (int z = 0; z < vs->nz; ++z) {  // interior rows of each padded plane
+            const char* src = (const char*)vs->d_pad + (((size_t)z + 1) * (vs->ny + 2) + 1) * pitch + es;
+            HIPCALL(c, hipMemcpy2DAsync((char*)host + (size_t)z * vs->ny * row, row, src, pitch, row,
+                                        vs->ny, hipMemcpyDeviceToHost, c->stream));
+        }
+    } else {
+        HIPCALL(c, hipMemcpyAsync(host, vs->d_vol, need, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCALL(c, hipStreamSynchronize(c->stream));
     return SPR_OK;
 }
 

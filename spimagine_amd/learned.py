@@ -1,0 +1,342 @@
+"""Learned reconstruction of a max projection (libspimlearn.so, include/spimlearn.h).
+
+A frame is first rendered coarse -- half the resolution and a quarter of the ray samples
+(max_steps / 4) -- twice: from the volume and from its max-filtered ("dilated") copy, which
+bounds the samples the coarse step skips.  A small convolutional network on the matrix
+cores turns those two planes into the full-resolution frame.  The exact renderer is the
+reference: the network is trained on coarse/fine pairs the renderer produces, and
+``LearnedMIP.validate`` measures PSNR against exact frames; below the threshold the
+renderer falls back to the exact path.  No reference counterpart: the nearest analogue is
+progressive multipass rendering (spimagine/volumerender/kernels/volume_kernel.cl:100-106,
+driven from spimagine/gui/glwidget.py:633-635).
+"""
+import ctypes
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspimlearn.so")
+WEIGHTS_PATH = os.path.join(HERE, "weights", "mip_x2_q4.npz")
+EXPORTS = ("spl_abi_version", "spl_param_count", "spl_create", "spl_destroy", "spl_last_error",
+           "spl_set_params", "spl_get_params", "spl_forward", "spl_gradients", "spl_train_step")
+C = 32
+LAYER_NAMES = ("e0", "e1", "e2a", "e2", "u1", "u0")
+PSNR_THRESHOLD = 40.0
+
+_lib_learn = None
+
+
+def load():
+    global _lib_learn
+    if _lib_learn is not None:
+        return _lib_learn
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("%s is not built; run `python -m spimagine_amd.build`" % LIB_PATH)
+    try:  # one HIP runtime per process (see _lib.load)
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = ctypes.CDLL(LIB_PATH)
+    c_int, c_void_p, c_float = ctypes.c_int, ctypes.c_void_p, ctypes.c_float
+    L.spl_abi_version.argtypes = []
+    L.spl_param_count.argtypes = []
+    L.spl_create.argtypes = [c_int, ctypes.POINTER(c_void_p)]
+    L.spl_destroy.argtypes = [c_void_p]
+    L.spl_last_error.argtypes = [c_void_p]
+    L.spl_last_error.restype = ctypes.c_char_p
+    L.spl_set_params.argtypes = [c_void_p, c_void_p, c_int]
+    L.spl_get_params.argtypes = [c_void_p, c_void_p, c_int]
+    L.spl_forward.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]
+    L.spl_gradients.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p,
+                                ctypes.POINTER(ctypes.c_double), c_void_p]
+    L.spl_train_step.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_float,
+                                 ctypes.POINTER(ctypes.c_double), c_void_p]
+    _lib_learn = L
+    return L
+
+
+def param_layout():
+    """(name, shape) of each parameter block, in the flat order of spl_set_params."""
+    out = [("in.w", (9, 2, C)), ("in.b", (C,))]
+    for n in LAYER_NAMES:
+        out += [(n + ".w", (9, C, C)), (n + ".b", (C,))]
+    out += [("head.w", (9, C, 4)), ("head.b", (4,))]
+    return out
+
+
+def param_count():
+    return sum(int(np.prod(s)) for _, s in param_layout())
+
+
+def split_params(flat):
+    """flat float32 array -> {name: array view of its shape}."""
+    out, k = {}, 0
+    for name, shape in param_layout():
+        n = int(np.prod(shape))
+        out[name] = flat[k:k + n].reshape(shape)
+        k += n
+    return out
+
+
+def init_params(seed=0):
+    """He-normal weights, zero biases, zero head: the untrained network returns the
+    bilinear upsampling of the coarse projection."""
+    rng = np.random.default_rng(seed)
+    flat = np.zeros(param_count(), np.float32)
+    p = split_params(flat)
+    p["in.w"][...] = rng.normal(0, math.sqrt(2. / (9 * 2)), p["in.w"].shape)
+    for n in LAYER_NAMES:
+        p[n + ".w"][...] = rng.normal(0, math.sqrt(2. / (9 * C)), p[n + ".w"].shape)
+    return flat
+
+
+class SplError(RuntimeError):
+    def __init__(self, code, msg):
+        RuntimeError.__init__(self, "libspimlearn error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Net(object):
+    """One network on a device (spl_create)."""
+
+    def __init__(self, device=0, params=None):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        rc = self.lib.spl_create(int(device), ctypes.byref(h))
+        if rc:
+            raise SplError(rc, self.lib.spl_last_error(None).decode())
+        self.handle = h
+        self.n_params = self.lib.spl_param_count()
+        if params is not None:
+            self.set_params(params)
+
+    def _check(self, rc):
+        if rc:
+            raise SplError(rc, self.lib.spl_last_error(self.handle).decode())
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.spl_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, np.float32)
+        self._check(self.lib.spl_set_params(self.handle, flat.ctypes.data, flat.size))
+
+    def get_params(self):
+        out = np.empty(self.n_params, np.float32)
+        self._check(self.lib.spl_get_params(self.handle, out.ctypes.data, out.size))
+        return out
+
+    def forward(self, d_in, n, h, w, d_out, stream=None):
+        self._check(self.lib.spl_forward(self.handle, d_in, n, h, w, d_out, stream))
+
+    def gradients(self, d_in, d_target, n, h, w, stream=None):
+        g = np.empty(self.n_params, np.float32)
+        sse = ctypes.c_double()
+        self._check(self.lib.spl_gradients(self.handle, d_in, d_target, n, h, w, g.ctypes.data,
+                                           ctypes.byref(sse), stream))
+        return g, sse.value
+
+    def train_step(self, d_in, d_target, n, h, w, lr, stream=None, want_loss=True):
+        sse = ctypes.c_double()
+        self._check(self.lib.spl_train_step(self.handle, d_in, d_target, n, h, w, float(lr),
+                                            ctypes.byref(sse) if want_loss else None, stream))
+        return sse.value if want_loss else None
+
+
+def psnr(a, b):
+    """PSNR (dB) of a against b for images in [0, 1]."""
+    mse = float(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2))
+    return float("inf") if mse == 0 else 10. * math.log10(1. / mse)
+
+
+def coarse_params(p):
+    """The coarse pass's parameters: the same pose and transfer, a quarter of the steps."""
+    q = _lib.SprParams()
+    ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(q))
+    q.max_steps = max(16, p.max_steps // 4)
+    return q
+
+
+class LearnedMIP(object):
+    """Learned max projection over the volumes of one renderer context."""
+
+    def __init__(self, ctx, params=None, device=0, radius=2, threshold=PSNR_THRESHOLD, net=None):
+        import torch
+        self.torch = torch
+        self.ctx = ctx
+        if net is None:
+            if params is None:
+                params = np.load(WEIGHTS_PATH)["params"]
+            net = Net(device, params)
+        self.net = net
+        self.radius = radius
+        self.threshold = threshold
+        self.dilated = {}
+        self.enabled = {}   # slot -> validated (min PSNR >= threshold)
+        self._bufs = {}
+
+    def attach(self, slot):
+        if slot not in self.dilated:
+            self.dilated[slot] = self.ctx.volume_add_dilated(slot, self.radius)
+        return self.dilated[slot]
+
+    def _buffer(self, key, shape):
+        if key not in self._bufs:
+            self._bufs[key] = self.torch.empty(shape, device="cuda")
+        return self._bufs[key]
+
+    def coarse(self, slot, p, w, h, planes, k=0, stream=None):
+        """Renders the two coarse planes of frame k of `planes` (n, 2, h/2, w/2)."""
+        dil = self.attach(slot)
+        scratch = self._buffer(("alpha", w, h), (h // 2, w // 2))
+        q = coarse_params(p)
+        for j, s in enumerate((slot, dil)):
+            self.ctx.render_slot(s, q, _lib.METHOD_MAX_PROJECT, w // 2, h // 2,
+                                 planes[k, j].data_ptr(), scratch.data_ptr(), stream=stream)
+
+    def render(self, slot, p, w, h, d_out, stream):
+        """The learned w x h frame of `slot` with parameters p into device buffer d_out."""
+        if w % 8 or h % 8:
+            raise ValueError("learned frames need w and h multiples of 8")
+        planes = self._buffer(("planes", w, h), (1, 2, h // 2, w // 2))
+        self.coarse(slot, p, w, h, planes, 0, stream)
+        self.net.forward(planes.data_ptr(), 1, h // 2, w // 2, d_out, stream)
+
+    def render_or_exact(self, slot, p, w, h, d_out, d_alpha, stream):
+        """Learned frame when the slot validated, else the exact render."""
+        if self.enabled.get(slot):
+            self.render(slot, p, w, h, d_out, stream)
+            return True
+        self.ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, w, h, d_out, d_alpha, stream=stream)
+        return False
+
+    def validate(self, slot, param_list, w, h):
+        """PSNR and max |error| of the learned frames against the exact ones; enables the
+        learned path for `slot` iff every frame reaches the threshold."""
+        torch = self.torch
+        s = torch.cuda.Stream()
+        exact = torch.empty((h, w), device="cuda")
+        alpha = torch.empty_like(exact)
+        learned = torch.empty_like(exact)
+        rep = []
+        for p in param_list:
+            self.ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, w, h, exact.data_ptr(),
+                                 alpha.data_ptr(), stream=s.cuda_stream)
+            self.render(slot, p, w, h, learned.data_ptr(), s.cuda_stream)
+            s.synchronize()
+            e, l = exact.cpu().numpy(), learned.cpu().numpy()
+            rep.append({"psnr": psnr(l, e), "max_abs": float(np.abs(l - e).max())})
+        self.enabled[slot] = bool(rep) and min(r["psnr"] for r in rep) >= self.threshold
+        return rep
+
+
+# ------------------------------------------------------------------------------ training
+def training_volumes(n, shape=(512, 512, 512), seed0=1):
+    """Seeded uint16 blob volumes (2000 blobs per 512^3 recipe), as device int16 tensors."""
+    from . import synth
+    vols = []
+    for k in range(n):
+        nb = int(round(2000 * np.prod(shape) / 512 ** 3))
+        vols.append(next(synth.blob_series_device(shape, nb, (3., 10.), (500., 3000.), 100., True,
+                                                  5., 1, seed=seed0 + k)))
+    return vols
+
+
+def train(ctx, slots, ranges, steps=2500, batch=8, frame=1024, max_steps=896, lr=2e-3,
+          seed=0, params=None, log=None, radius=2):
+    """Trains the network on coarse/fine pairs rendered from random poses of `slots`
+    (ranges: per-slot (min, max) used to normalise).  Returns (params, history)."""
+    import torch
+    from . import synth
+    from .utils import mat4_perspective
+    from .volumerender.volumerender import render_matrices
+    net = Net(0, init_params(seed) if params is None else params)
+    lm = LearnedMIP(ctx, net=net, radius=radius)
+    for s in slots:
+        lm.attach(s)
+    rng = np.random.default_rng(seed)
+    P = mat4_perspective(45, 1., .1, 10)
+    stream = torch.cuda.Stream()
+    W = H = frame
+    planes = torch.empty((batch, 2, H // 2, W // 2), device="cuda")
+    target = torch.empty((batch, H, W), device="cuda")
+    alpha = torch.empty((H, W), device="cuda")
+    hist = []
+    t0 = time.time()
+    for it in range(steps):
+        for k in range(batch):
+            j = int(rng.integers(len(slots)))
+            info = ctx.volume_info(slots[j])
+            nz, ny, nx = info["shape"]
+            M = synth.random_poses(1, seed=int(rng.integers(1 << 30)),
+                                   distance=float(rng.uniform(3., 5.)))[0]
+            iP, iM = render_matrices(M, P, (nx, ny, nz))
+            lo, hi = ranges[j]
+            p = _lib.make_params(iP, iM, (-1, 1, -1, 1, -1, 1), lo, hi, 1., 0., .1, 21, 30,
+                                 max_steps, _lib.FILTER_LINEAR)
+            ctx.render_slot(slots[j], p, _lib.METHOD_MAX_PROJECT, W, H, target[k].data_ptr(),
+                            alpha.data_ptr(), stream=stream.cuda_stream)
+            lm.coarse(slots[j], p, W, H, planes, k, stream.cuda_stream)
+        frac = it / max(1, steps - 1)
+        lr_t = lr * (.05 + .95 * .5 * (1 + math.cos(math.pi * frac)))
+        want = it % 50 == 0 or it == steps - 1
+        sse = net.train_step(planes.data_ptr(), target.data_ptr(), batch, H // 2, W // 2, lr_t,
+                             stream.cuda_stream, want_loss=want)
+        if want:
+            mse = sse / (batch * H * W)
+            hist.append((it, mse))
+            if log:
+                log("train step %d/%d  mse %.3e  (%.1f dB)  lr %.2e  %.0f s" % (
+                    it, steps, mse, 10 * math.log10(1. / max(mse, 1e-30)), lr_t, time.time() - t0))
+    stream.synchronize()
+    return net.get_params(), hist
+
+
+def main():
+    """python -m spimagine_amd.learned [--steps N] [--out PATH]: train on seeded 512^3
+    uint16 blob volumes and save the parameters."""
+    import argparse
+    import sys
+    import torch
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=2500)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--volumes", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default=WEIGHTS_PATH)
+    args = ap.parse_args()
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    ctx = _lib.Context(0)
+    slots, ranges = [], []
+    for v in training_volumes(args.volumes):
+        u = v.to(torch.int32) & 0xFFFF
+        ranges.append((float(u.min()), float(u.max())))
+        del u
+        slots.append(ctx.volume_add(dev_ptr=v.data_ptr(), dtype=np.uint16, shape=tuple(v.shape)))
+        del v
+    log("training on %d volumes, %d steps x %d pairs" % (len(slots), args.steps, args.batch))
+    params, hist = train(ctx, slots, ranges, steps=args.steps, batch=args.batch, lr=args.lr, log=log)
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    np.savez(args.out, params=params, history=np.array(hist, np.float64),
+             pairs=args.steps * args.batch)
+    log("saved %s" % args.out)
+
+
+if __name__ == "__main__":
+    main()

@@ -93,6 +93,7 @@ class RenderServer(object):
         self.table_lock = threading.Lock()  # slot table changes (add / release)
         self.stop = False
         self.batches = 0
+        self._alone_stream = None
         self.threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(workers)]
         for t in self.threads:
             t.start()
@@ -124,9 +125,13 @@ class RenderServer(object):
         vol = self.volumes[req.volume]
         with self.cv:
             vol.users += 1
+            if self._alone_stream is None:
+                self._alone_stream = self.torch.cuda.Stream()
         try:
             self._admit_blocking(vol)
-            res = self._render_single(req, vol, self.torch.cuda.current_stream())
+            # a stream of its own: torch's default stream has handle 0, which the C ABI
+            # reads as the context's stream, unordered with torch's copies
+            res = self._render_single(req, vol, self._alone_stream)
         finally:
             with self.cv:
                 vol.users -= 1
@@ -252,6 +257,7 @@ class RenderServer(object):
 
     def _render_single(self, req, vol, stream):
         torch = self.torch
+        assert stream.cuda_stream, "render on a non-default stream"
         w, h = req.size
         p = self._params(req, vol)
         extra = {}

@@ -60,3 +60,27 @@ def test_dtype_codes_match_header():
                       ("SPR_METHOD_MAX_PROJECT", 0), ("SPR_METHOD_ISO_SURFACE", 1)):
         assert re.search(r"#define %s %d\b" % (name, val), text), name
     assert _lib.DTYPE_CODES[np.dtype(np.uint16)] == 1
+
+
+LEARN_HEADER = os.path.join(ROOT, "include", "spimlearn.h")
+
+
+def test_learn_library_exports_every_declared_symbol():
+    from spimagine_amd import learned
+    text = re.sub(r"/\*.*?\*/", "", open(LEARN_HEADER).read(), flags=re.S)
+    declared = sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(spl_\w+)\s*\(", text, re.M)))
+    assert declared == sorted(learned.EXPORTS)
+    L = learned.load()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert L.spl_abi_version() == 1
+    assert L.spl_param_count() == learned.param_count() == 57252
+
+
+def test_learn_errors_without_network():
+    from spimagine_amd import learned
+    L = learned.load()
+    assert L.spl_forward(None, None, 1, 4, 4, None, None) == -1
+    assert b"NULL network" in L.spl_last_error(None)
+    p = learned.split_params(learned.init_params(0))
+    assert not p["head.w"].any() and not p["e0.b"].any() and p["e0.w"].std() > 0

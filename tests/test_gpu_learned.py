@@ -1,0 +1,193 @@
+"""GPU: the learned reconstruction library (libspimlearn.so) against a plain PyTorch float32
+reference of the same network, and the device-side volume transforms it relies on.
+
+The reference stores activations rounded to bf16 exactly where the kernels do (after each
+32-channel layer, pooling and skip addition) and rounds the 32->32 weights to bf16, so the
+remaining differences are float32 summation order and the bf16 rounding flips it causes:
+forward outputs agree to 2% of their range (1e-3 mean); gradients (whose 32->32 input
+gradients the kernels take in bf16) agree per parameter block to cosine >= 0.99 and 10%
+relative L2 error."""
+import numpy as np
+import pytest
+
+from spimagine_amd import _lib, learned, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert _lib.device_count() > 0
+    return t
+
+
+def ref_forward(torch, params, x):
+    return ref_forward_t(torch, torch.from_numpy(np.asarray(params, np.float32)), x)
+
+
+def random_params(seed):
+    rng = np.random.default_rng(seed)
+    flat = learned.init_params(seed)
+    p = learned.split_params(flat)
+    for k in p:
+        if k.endswith(".b"):
+            p[k][...] = rng.normal(0, .05, p[k].shape)
+    p["head.w"][...] = rng.normal(0, .05, p["head.w"].shape)
+    return flat
+
+
+def smooth_inputs(rng, n, h, w):
+    from scipy.ndimage import gaussian_filter
+    x = rng.uniform(0, 1, (n, 2, h, w)).astype(np.float32)
+    return np.stack([gaussian_filter(v, (0, 1.5, 1.5)) for v in x]).astype(np.float32)
+
+
+@pytest.mark.parametrize("n,h,w", [(1, 64, 64), (2, 48, 80), (1, 36, 100)])
+def test_forward_matches_torch(torch, n, h, w):
+    rng = np.random.default_rng(n * 1000 + h + w)
+    params = random_params(h + w)
+    x = smooth_inputs(rng, n, h, w)
+    net = learned.Net(0, params)
+    dx = torch.from_numpy(x).cuda()
+    dy = torch.full((n, 2 * h, 2 * w), -9., device="cuda")
+    torch.cuda.synchronize()  # the fill runs on torch's stream, the network on its own
+    net.forward(dx.data_ptr(), n, h, w, dy.data_ptr())
+    torch.cuda.synchronize()
+    got = dy.cpu().numpy()
+    ref = ref_forward(torch, params, torch.from_numpy(x)).numpy()
+    scale = np.abs(ref).max()
+    d = np.abs(got - ref)
+    assert d.max() <= 2e-2 * scale, (d.max(), scale)
+    assert d.mean() <= 1e-3 * scale, (d.mean(), scale)
+    np.testing.assert_array_equal(net.get_params(), params.astype(np.float32))
+    net.close()
+
+
+def test_gradients_match_torch_autograd(torch):
+    n, h, w = 2, 32, 48
+    rng = np.random.default_rng(5)
+    params = random_params(11)
+    x = smooth_inputs(rng, n, h, w)
+    tgt = rng.uniform(0, 1, (n, 2 * h, 2 * w)).astype(np.float32)
+    net = learned.Net(0, params)
+    dx, dt = torch.from_numpy(x).cuda(), torch.from_numpy(tgt).cuda()
+    g, sse = net.gradients(dx.data_ptr(), dt.data_ptr(), n, h, w)
+    net.close()
+    pt = torch.from_numpy(params.copy()).requires_grad_(True)
+    # autograd through the reference (bf16 rounding passes the gradient straight through)
+    y = ref_forward_t(torch, pt, torch.from_numpy(x))
+    loss = ((y - torch.from_numpy(tgt)) ** 2).sum() / y.numel()
+    loss.backward()
+    gr = pt.grad.numpy()
+    ref_sse = float(((y.detach() - torch.from_numpy(tgt)) ** 2).sum())
+    assert abs(sse - ref_sse) <= 2e-2 * ref_sse
+    k = 0
+    for name, shape in learned.param_layout():
+        m = int(np.prod(shape))
+        a, b = g[k:k + m].astype(np.float64), gr[k:k + m].astype(np.float64)
+        k += m
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
+        rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+        assert cos >= .99 and rel <= .1, (name, cos, rel)
+
+
+def ref_forward_t(torch, flat, x):
+    """ref_forward on a torch parameter vector (keeps the autograd graph)."""
+    F = torch.nn.functional
+    p, k = {}, 0
+    for name, shape in learned.param_layout():
+        m = int(np.prod(shape))
+        p[name] = flat[k:k + m].reshape(shape)
+        k += m
+
+    def bf(t):
+        return t.to(torch.bfloat16).to(torch.float32)
+
+    def conv(t, w, b, round_w):
+        cin, cout = w.shape[1], w.shape[2]
+        wt = w.reshape(3, 3, cin, cout).permute(3, 2, 0, 1).contiguous()
+        return F.conv2d(t, bf(wt) if round_w else wt, b, padding=1)
+
+    a0 = bf(F.relu(conv(x, p["in.w"], p["in.b"], False)))
+    e0 = bf(F.relu(conv(a0, p["e0.w"], p["e0.b"], True)))
+    p1 = bf(F.avg_pool2d(e0, 2))
+    e1 = bf(F.relu(conv(p1, p["e1.w"], p["e1.b"], True)))
+    p2 = bf(F.avg_pool2d(e1, 2))
+    e2a = bf(F.relu(conv(p2, p["e2a.w"], p["e2a.b"], True)))
+    e2 = bf(F.relu(conv(e2a, p["e2.w"], p["e2.b"], True)))
+    s1 = bf(F.interpolate(e2, scale_factor=2, mode="nearest") + e1)
+    u1 = bf(F.relu(conv(s1, p["u1.w"], p["u1.b"], True)))
+    s0 = bf(F.interpolate(u1, scale_factor=2, mode="nearest") + e0)
+    u0 = bf(F.relu(conv(s0, p["u0.w"], p["u0.b"], True)))
+    r = conv(u0, p["head.w"], p["head.b"], False)
+    base = F.interpolate(x[:, :1], scale_factor=2, mode="bilinear", align_corners=False)
+    return (F.pixel_shuffle(r, 2) + base)[:, 0]
+
+
+def test_training_reduces_the_loss(torch):
+    n, h, w = 4, 32, 32
+    rng = np.random.default_rng(2)
+    x = smooth_inputs(rng, n, h, w)
+    # a target the network can represent: a fixed smooth function of the input planes
+    up = np.repeat(np.repeat(x, 2, axis=2), 2, axis=3)
+    tgt = np.clip(.7 * up[:, 0] + .3 * up[:, 1] ** 2, 0, 1).astype(np.float32)
+    net = learned.Net(0, learned.init_params(3))
+    dx, dt = torch.from_numpy(x).cuda(), torch.from_numpy(tgt).cuda()
+    first = net.train_step(dx.data_ptr(), dt.data_ptr(), n, h, w, 1e-3)
+    for _ in range(60):
+        last = net.train_step(dx.data_ptr(), dt.data_ptr(), n, h, w, 1e-3)
+    assert last < .5 * first, (first, last)
+    net.close()
+
+
+@pytest.mark.parametrize("dtype", [np.uint16, np.float32, np.uint8])
+def test_dilated_volume_is_scipy_max_filter(torch, dtype):
+    from scipy.ndimage import maximum_filter
+    rng = np.random.default_rng(4)
+    shape = (19, 23, 37)
+    if dtype == np.float32:
+        vol = rng.normal(0, 1, shape).astype(np.float32)
+    else:
+        vol = rng.integers(0, np.iinfo(dtype).max, shape).astype(dtype)
+    ctx = _lib.Context(0)
+    for layout in (_lib.LAYOUT_PADDED, _lib.LAYOUT_LINEAR):
+        ctx.set_layout(layout)
+        s = ctx.volume_add(vol)
+        np.testing.assert_array_equal(ctx.volume_read(s), vol)
+        for r in (1, 2):
+            d = ctx.volume_add_dilated(s, r)
+            np.testing.assert_array_equal(ctx.volume_read(d),
+                                          maximum_filter(vol, size=2 * r + 1, mode="nearest"))
+            ctx.volume_release(d)
+        ctx.volume_release(s)
+
+
+def test_learned_render_path_shapes_and_fallback(torch):
+    vol = synth.blob_volume((48, 64, 64), n_blobs=12, sigma=(3., 6.), amp=(500., 3000.),
+                            background=100., poisson=True, read_noise=5., dtype="uint16", seed=1)
+    ctx = _lib.Context(0)
+    slot = ctx.volume_add(vol)
+    # threshold above any PSNR: validation must switch the learned path off
+    lm = learned.LearnedMIP(ctx, params=learned.init_params(0), threshold=float("inf"))
+    from spimagine_amd.utils import mat4_perspective
+    from spimagine_amd.volumerender.volumerender import render_matrices
+    P = mat4_perspective(45, 1., .1, 10)
+    ps = []
+    for M in synth.random_poses(2, seed=3):
+        iP, iM = render_matrices(M, P, (64, 64, 48))
+        ps.append(_lib.make_params(iP, iM, (-1, 1, -1, 1, -1, 1), float(vol.min()), float(vol.max()),
+                                   1., 0., .1, 21, 30, 128, _lib.FILTER_LINEAR))
+    rep = lm.validate(slot, ps, 96, 96)
+    assert len(rep) == 2 and all(np.isfinite(r["psnr"]) for r in rep)
+    assert lm.enabled[slot] is False
+    s = torch.cuda.Stream()
+    o = torch.empty((96, 96), device="cuda")
+    a = torch.empty_like(o)
+    assert lm.render_or_exact(slot, ps[0], 96, 96, o.data_ptr(), a.data_ptr(), s.cuda_stream) is False
+    s.synchronize()
+    ref = torch.empty_like(o)
+    ctx.render_slot(slot, ps[0], _lib.METHOD_MAX_PROJECT, 96, 96, ref.data_ptr(), a.data_ptr(),
+                    stream=s.cuda_stream)
+    s.synchronize()
+    assert torch.equal(o, ref)
