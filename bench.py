@@ -3,6 +3,7 @@
     python bench.py [--gpus 1] [--steps 50] [--warmup 5]          # headline (default)
     python bench.py --workload batch [--timepoints 64] [--poses 16] [--steps 3]
     python bench.py --workload serve [--requests 400] [--qps 60]
+    python bench.py --workload learned
 
 headline  One step = one max-projection frame (one kernel launch) of the resident
           1024x1024x512 uint16 volume into 1024^2: 8000 seeded Gaussian blobs, background
@@ -18,6 +19,9 @@ batch     64 timepoints of a 1024x1024x256 uint16 blob series (4000 blobs drifti
 serve     a Poisson request stream at fixed QPS against the render server: resident
           volumes from 256^3 float32 to 2048x2048x512 uint16, frames 512^2-2048^2, random
           poses, max projection / opacity / iso-surface; p50/p99 latency and frames/s.
+learned   exact 1024^2 MIP of a held-out 512^3 uint16 volume vs the learned frame (two
+          coarse 512^2 renders at max_steps/4 + the conv-net) over 8 seeded poses: frames/s
+          of both, PSNR and max |error| per frame, and whether validation enables it.
 
 Every line carries roofline.achieved = algorithmic bytes per launch (samples x 8 corners
 x voxel bytes + the float32 output and alpha planes) / average kernel duration (HIP
@@ -444,10 +448,95 @@ def bench_serve(args):
         "batches": server.batches}), flush=True)
 
 
+# ------------------------------------------------------------------------------ learned
+def bench_learned(args):
+    import torch
+    from spimagine_amd import _lib, learned, synth
+    from spimagine_amd.utils import mat4_perspective
+    from spimagine_amd.volumerender.volumerender import render_matrices
+
+    shape = (512, 512, 512)
+    W = H = 1024
+    S = synth.steps_for_shape(shape)
+    ctx = _lib.Context(0)
+    # held out: the networks are trained on seeds 1.. of the same recipe
+    vt = next(synth.blob_series_device(shape, 2000, (3., 10.), (500., 3000.), 100., True, 5., 1,
+                                       seed=0))
+    u = vt.to(torch.int32) & 0xFFFF
+    lo, hi = float(u.min()), float(u.max())
+    del u
+    slot = ctx.volume_add(dev_ptr=vt.data_ptr(), dtype=np.uint16, shape=shape)
+    del vt
+    lm = learned.LearnedMIP(ctx, params=np.load(args.weights)["params"])
+    lm.attach(slot)
+    P = mat4_perspective(45, 1., .1, 10)
+    params = []
+    for M in synth.random_poses(8, seed=0):
+        iP, iM = render_matrices(M, P, shape[::-1])
+        params.append(_lib.make_params(iP, iM, BOX, lo, hi, 1., 0., .1, 21, 30, S,
+                                       _lib.FILTER_LINEAR))
+    rep = lm.validate(slot, params, W, H)
+    stream = torch.cuda.Stream()
+    out = torch.empty((H, W), device="cuda")
+    alpha = torch.empty_like(out)
+
+    def timed(fn, reps):
+        for p in params:  # warm
+            fn(p)
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            for p in params:
+                fn(p)
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_time(e1) / (reps * len(params))
+
+    reps = max(1, args.steps // len(params))
+    exact_ms = timed(lambda p: ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W, H,
+                                               out.data_ptr(), alpha.data_ptr(),
+                                               stream=stream.cuda_stream), reps)
+    learned_ms = timed(lambda p: lm.render(slot, p, W, H, out.data_ptr(), stream.cuda_stream), reps)
+    planes = torch.empty((1, 2, H // 2, W // 2), device="cuda")
+    lm.coarse(slot, params[0], W, H, planes, 0, stream.cuda_stream)
+    net_ms = timed(lambda p: lm.net.forward(planes.data_ptr(), 1, H // 2, W // 2, out.data_ptr(),
+                                            stream.cuda_stream), reps)
+    # matrix-core work of one network pass: 6 layers of 32x32x9 MACs per pixel at h*w,
+    # h*w/4 (x2) and h*w/16 (x2), here h = H/2
+    px = (H // 2) * (W // 2)
+    flops = 2 * 9 * 32 * 32 * px * (2 + 2 / 4. + 2 / 16.)
+    achieved = flops / (net_ms * 1e-3) / 1e12
+    ok = lm.enabled[slot]
+    w = np.load(args.weights)
+    print(json.dumps({
+        "metric": "learned reconstruction: frames/s vs the exact MIP and PSNR per frame, "
+                  "512^3 u16 -> 1024^2", "value": round(1e3 / learned_ms, 2), "unit": "frames/s",
+        "n_gpus": 1, "steps": reps * len(params), "warmup": len(params),
+        "ms_per_step": round(learned_ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "MIP 512^3 uint16 (2000 blobs, Poisson + read noise, held-out seed) "
+                               "-> 1024^2, max_steps %d, 8 seeded poses; learned = two coarse "
+                               "512^2 renders at max_steps %d + conv-net" % (S, S // 4),
+                   "training_pairs": int(w["pairs"]) if "pairs" in w else None},
+        "exact_ms": round(exact_ms, 4), "exact_frames_per_s": round(1e3 / exact_ms, 2),
+        "learned_ms": round(learned_ms, 4), "speedup": round(exact_ms / learned_ms, 3),
+        "net_ms": round(net_ms, 4),
+        "psnr_db": [round(r["psnr"], 2) for r in rep],
+        "max_abs_err": [round(r["max_abs"], 4) for r in rep],
+        "psnr_threshold_db": lm.threshold, "learned_path_enabled": ok,
+        "effective_frames_per_s": round(1e3 / (learned_ms if ok else exact_ms), 2),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(achieved / 2500., 5), "traffic": None,
+                     "kernel": "network forward (all launches)"},
+        "cpu_baseline": None}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("headline", "batch", "serve"), default="headline")
+    ap.add_argument("--workload", choices=("headline", "batch", "serve", "learned"),
+                    default="headline")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--latency-frames", type=int, default=104)
@@ -467,6 +556,8 @@ def main():
     ap.add_argument("--max-batch", type=int, default=16)
     ap.add_argument("--workers", type=int, default=2)
     ap.add_argument("--hbm-budget-gib", type=float, default=0.)
+    ap.add_argument("--weights", default=os.path.join(ROOT, "spimagine_amd", "weights",
+                                                      "mip_x2_q4.npz"))
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -474,12 +565,14 @@ def main():
               "(multi-GPU sharding is a design note, not a measured path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
-    defaults = {"headline": (50, 5), "batch": (3, 1), "serve": (None, 10)}[args.workload]
+    defaults = {"headline": (50, 5), "batch": (3, 1), "serve": (None, 10),
+                "learned": (48, 8)}[args.workload]
     if args.steps is None:
         args.steps = defaults[0]
     if args.warmup is None:
         args.warmup = defaults[1]
-    {"headline": bench_headline, "batch": bench_batch, "serve": bench_serve}[args.workload](args)
+    {"headline": bench_headline, "batch": bench_batch, "serve": bench_serve,
+     "learned": bench_learned}[args.workload](args)
 
 
 if __name__ == "__main__":
