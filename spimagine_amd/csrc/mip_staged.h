@@ -71,7 +71,11 @@ struct alignas(16) Unit16 {
     uint32_t w[4];
 };
 
-template <typename T, bool POW2, int SLAB_BYTES>
+// CH: samples staged per box (16, 8 or 4).  The samples, their order and their positions
+// are those of the reference's 16-sample inner loop whatever CH is (a max does not depend
+// on how the sequence is cut); a smaller CH keeps a box small when the ray step is several
+// voxels long.
+template <typename T, bool POW2, int SLAB_BYTES, int CH>
 __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol, const RayParams& p,
                                                 uint32_t x, uint32_t y, bool inside, uint8_t* slab,
                                                 float* __restrict__ out, float* __restrict__ alpha) {
@@ -131,11 +135,14 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
                 mgz = 1e-5f * vol.fnz + 0.01f;
     float col = 0.0f;
 
-    for (int i = 0; i <= p.nouter; ++i) {
-        // ---- box of padded voxels this chunk's 16 samples of the wave's rays touch
+    static_assert(kLoopUnroll % CH == 0, "a chunk is a whole number of boxes");
+    const int nchunks = (p.nouter + 1) * (kLoopUnroll / CH);
+    for (int i = 0; i < nchunks; ++i) {
+        // ---- box of padded voxels this chunk's CH samples of the wave's rays touch
         uint32_t b0 = 0xFFFFFFFFu, b1 = 0xFFFFFFFFu, b2 = 0xFFFFFFFFu;
         if (active) {
-            const float ex = fmaf(15.0f, dx, px), ey = fmaf(15.0f, dy, py), ez = fmaf(15.0f, dz, pz);
+            const float ex = fmaf((float)(CH - 1), dx, px), ey = fmaf((float)(CH - 1), dy, py),
+                        ez = fmaf((float)(CH - 1), dz, pz);
             const float u0x = texel<POW2>(px, vol.fnx), u0y = texel<POW2>(py, vol.fny),
                         u0z = texel<POW2>(pz, vol.fnz);
             const float u1x = texel<POW2>(ex, vol.fnx), u1y = texel<POW2>(ey, vol.fny),
@@ -195,14 +202,14 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
             }
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            // ---- 16 samples from the slab; slab offset of corner (c+1) relative to the box,
+            // ---- CH samples from the slab; slab offset of corner (c+1) relative to the box,
             // computed exactly in float (every term is an integer below 2^24)
             const uint32_t sy = rowb, sz = rowb * SY;
             const float fsy = (float)sy, fsz = (float)sz;
             const float oy = 1.0f - (float)LY, oz = 1.0f - (float)LZ;
             const float ox = (1.0f - (float)lox) * (float)TS;
 #pragma unroll
-            for (int j = 0; j < kLoopUnroll; ++j) {
+            for (int j = 0; j < CH; ++j) {
                 float cx, cy, cz, ax, ay, az;
                 vol.locate_f(px, py, pz, &cx, &cy, &cz, &ax, &ay, &az);
                 const float bf = fmaf(cz + oz, fsz, fmaf(cy + oy, fsy, fmaf(cx, (float)TS, ox)));
@@ -221,7 +228,7 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
             __builtin_amdgcn_wave_barrier();
         } else {
 #pragma unroll
-            for (int j = 0; j < kLoopUnroll; ++j) {
+            for (int j = 0; j < CH; ++j) {
                 col = fmaxf(col, vol.linear(px, py, pz));
                 px = px + dx;
                 py = py + dy;
@@ -243,7 +250,7 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
 }
 
 // one frame: one workgroup per 16x16 tile (tile map as k_max_project)
-template <typename T, bool POW2, int SLAB_BYTES>
+template <typename T, bool POW2, int SLAB_BYTES, int CH>
 __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2> vol, RayParams p,
                                                             float* __restrict__ out,
                                                             float* __restrict__ alpha) {
@@ -251,7 +258,17 @@ __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2
     uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * SLAB_BYTES;
     uint32_t x = 0, y = 0;
     const bool inside = tile_pixel(p, &x, &y);
-    mip_staged_rays<T, POW2, SLAB_BYTES>(vol, p, x, y, inside, slab, out, alpha);
+    mip_staged_rays<T, POW2, SLAB_BYTES, CH>(vol, p, x, y, inside, slab, out, alpha);
+}
+
+// Samples per staged box for a frame: the ray advances at most N_a / (16 nouter) voxels
+// per sample along axis a (its whole path inside the box spans at most the volume), so
+// the largest CH whose CH - 1 steps stay within ~12 voxels keeps boxes inside a slab.
+__host__ inline int staged_chunk(int nouter, int nmax) {
+    const float step = (float)nmax / (16.0f * (float)(nouter > 0 ? nouter : 1));
+    if (15.0f * step <= 12.0f) return 16;
+    if (7.0f * step <= 12.0f) return 8;
+    return 4;
 }
 
 // ---------------------------------------------------------------------------------
@@ -308,9 +325,10 @@ __global__ void k_reset_counters(uint32_t* __restrict__ counters) {
 // Persistent launch: a fixed grid of workgroups claims (job, tile) items from the eight
 // queues with one atomic per item until every queue is drained; every workgroup leaves
 // after one failed claim on each queue.
-// MODE 0: staged plain max projection (linear filter, alpha_pow == 0); MODE 1 / 2: every
-// sample from global memory with the linear / nearest filter (any alpha_pow)
-template <typename T, bool POW2, int SLAB_BYTES, int MODE>
+// MODE 0: staged plain max projection (linear filter, alpha_pow == 0; CH samples per
+// staged box); MODE 1 / 2: every sample from global memory with the linear / nearest
+// filter (any alpha_pow; CH unused)
+template <typename T, bool POW2, int SLAB_BYTES, int MODE, int CH>
 __global__ __launch_bounds__(256) void k_max_project_batch(const BatchJob* __restrict__ jobs,
                                                            RayParams shared, BatchQueues qs,
                                                            uint32_t* __restrict__ counters) {
@@ -343,7 +361,7 @@ __global__ __launch_bounds__(256) void k_max_project_batch(const BatchJob* __res
         const bool inside = x < p.W && y < p.H;
         const PaddedVolume<T, POW2> vol = job_volume<T, POW2>(j);
         if constexpr (MODE == 0)
-            mip_staged_rays<T, POW2, SLAB_BYTES>(vol, p, x, y, inside, slab, j.out, j.alpha);
+            mip_staged_rays<T, POW2, SLAB_BYTES, CH>(vol, p, x, y, inside, slab, j.out, j.alpha);
         else
             mip_rays<PaddedVolume<T, POW2>, MODE == 2 ? 1 : 0>(vol, p, x, y, inside, j.out,
                                                                 j.alpha);

@@ -5,6 +5,7 @@
 // the batched (many volumes x poses, HIP graph) launch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -305,14 +306,24 @@ int check_parts(spr_ctx* c, const spr_params& prm, int num_parts) {
     return SPR_OK;
 }
 
-template <typename T, bool P2, typename... Args>
-void launch_staged(int slab_kb, dim3 grid, hipStream_t s, Args... args) {
+template <typename T, bool P2, int CH, typename... Args>
+void launch_staged_ch(int slab_kb, dim3 grid, hipStream_t s, Args... args) {
     if (slab_kb == 8)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 8192>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 8192, CH>), grid, dim3(256), 0, s, args...);
     else if (slab_kb == 16)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, CH>), grid, dim3(256), 0, s, args...);
     else
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288, CH>), grid, dim3(256), 0, s, args...);
+}
+
+template <typename T, bool P2, typename... Args>
+void launch_staged(int slab_kb, int ch, dim3 grid, hipStream_t s, Args... args) {
+    if (ch == 16)
+        launch_staged_ch<T, P2, 16>(slab_kb, grid, s, args...);
+    else if (ch == 8)
+        launch_staged_ch<T, P2, 8>(slab_kb, grid, s, args...);
+    else
+        launch_staged_ch<T, P2, 4>(slab_kb, grid, s, args...);
 }
 
 // render(method, numParts, currentPart) into device buffers; the reference's launch
@@ -336,8 +347,10 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
             using V = decltype(v);
             if constexpr (is_padded<V>::value) {
                 if (staged) {
-                    launch_staged<typename V::value_type, V::kPow2>(c->slab_kb, grid, s, v, p, out,
-                                                                     alpha);
+                    const int nmax = vs.nx > vs.ny ? (vs.nx > vs.nz ? vs.nx : vs.nz)
+                                                   : (vs.ny > vs.nz ? vs.ny : vs.nz);
+                    launch_staged<typename V::value_type, V::kPow2>(
+                        c->slab_kb, staged_chunk(p.nouter, nmax), grid, s, v, p, out, alpha);
                     return hipGetLastError();
                 }
             }
@@ -560,48 +573,64 @@ BatchQueues make_queues(const RayParams& p, int n) {
 }
 
 // enqueue counter reset + persistent kernel on s
-template <typename T, bool P2, int SLAB, int MODE>
+template <typename T, bool P2, int SLAB, int MODE, int CH>
 hipError_t launch_batch_kernel(const spr_ctx* c, const BatchJob* d_jobs, const RayParams& p,
                                const BatchQueues& q, uint32_t* counters, hipStream_t s) {
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, k_max_project_batch<T, P2, SLAB, MODE>, 256, 0);
+        &per_cu, k_max_project_batch<T, P2, SLAB, MODE, CH>, 256, 0);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     const uint32_t grid = (uint32_t)(c->num_cus * per_cu + 7) / 8u * 8u;
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters);
-    hipLaunchKernelGGL((k_max_project_batch<T, P2, SLAB, MODE>), dim3(grid), dim3(256), 0, s, d_jobs,
-                       p, q, counters);
+    hipLaunchKernelGGL((k_max_project_batch<T, P2, SLAB, MODE, CH>), dim3(grid), dim3(256), 0, s,
+                       d_jobs, p, q, counters);
     return hipGetLastError();
 }
 
-template <typename T, bool P2>
-hipError_t dispatch_batch_typed(const spr_ctx* c, int mode, const BatchJob* d_jobs,
-                                const RayParams& p, const BatchQueues& q, uint32_t* counters,
-                                hipStream_t s) {
-    if (mode == 1) return launch_batch_kernel<T, P2, 16, 1>(c, d_jobs, p, q, counters, s);
-    if (mode == 2) return launch_batch_kernel<T, P2, 16, 2>(c, d_jobs, p, q, counters, s);
-    if (c->slab_kb == 8) return launch_batch_kernel<T, P2, 8192, 0>(c, d_jobs, p, q, counters, s);
-    if (c->slab_kb == 16) return launch_batch_kernel<T, P2, 16384, 0>(c, d_jobs, p, q, counters, s);
-    return launch_batch_kernel<T, P2, 12288, 0>(c, d_jobs, p, q, counters, s);
+template <typename T, bool P2, int SLAB>
+hipError_t launch_batch_staged(int ch, const spr_ctx* c, const BatchJob* d_jobs, const RayParams& p,
+                               const BatchQueues& q, uint32_t* counters, hipStream_t s) {
+    if (ch == 16) return launch_batch_kernel<T, P2, SLAB, 0, 16>(c, d_jobs, p, q, counters, s);
+    if (ch == 8) return launch_batch_kernel<T, P2, SLAB, 0, 8>(c, d_jobs, p, q, counters, s);
+    return launch_batch_kernel<T, P2, SLAB, 0, 4>(c, d_jobs, p, q, counters, s);
 }
 
-hipError_t dispatch_batch(const spr_ctx* c, const spr_params& prm, int dtype, bool pow2,
+template <typename T, bool P2>
+hipError_t dispatch_batch_typed(const spr_ctx* c, int mode, int ch, const BatchJob* d_jobs,
+                                const RayParams& p, const BatchQueues& q, uint32_t* counters,
+                                hipStream_t s) {
+    if (mode == 1) return launch_batch_kernel<T, P2, 16, 1, 16>(c, d_jobs, p, q, counters, s);
+    if (mode == 2) return launch_batch_kernel<T, P2, 16, 2, 16>(c, d_jobs, p, q, counters, s);
+    if (c->slab_kb == 8) return launch_batch_staged<T, P2, 8192>(ch, c, d_jobs, p, q, counters, s);
+    if (c->slab_kb == 16) return launch_batch_staged<T, P2, 16384>(ch, c, d_jobs, p, q, counters, s);
+    return launch_batch_staged<T, P2, 12288>(ch, c, d_jobs, p, q, counters, s);
+}
+
+// nmax: the largest volume dimension over the batch's jobs (sets the staged box length)
+hipError_t dispatch_batch(const spr_ctx* c, const spr_params& prm, int dtype, bool pow2, int nmax,
                           const BatchJob* d_jobs, const RayParams& p, const BatchQueues& q,
                           uint32_t* counters, hipStream_t s) {
     const bool nearest = prm.filter == SPR_FILTER_NEAREST;
     const int mode = nearest ? 2 : ((prm.alpha_pow == 0.0f && c->slab_kb > 0) ? 0 : 1);
+    const int ch = staged_chunk(p.nouter, nmax);
     switch (dtype) {
         case SPR_DTYPE_F32:
-            return pow2 ? dispatch_batch_typed<float, true>(c, mode, d_jobs, p, q, counters, s)
-                        : dispatch_batch_typed<float, false>(c, mode, d_jobs, p, q, counters, s);
+            return pow2 ? dispatch_batch_typed<float, true>(c, mode, ch, d_jobs, p, q, counters, s)
+                        : dispatch_batch_typed<float, false>(c, mode, ch, d_jobs, p, q, counters, s);
         case SPR_DTYPE_U16:
-            return pow2 ? dispatch_batch_typed<uint16_t, true>(c, mode, d_jobs, p, q, counters, s)
-                        : dispatch_batch_typed<uint16_t, false>(c, mode, d_jobs, p, q, counters, s);
+            return pow2 ? dispatch_batch_typed<uint16_t, true>(c, mode, ch, d_jobs, p, q, counters, s)
+                        : dispatch_batch_typed<uint16_t, false>(c, mode, ch, d_jobs, p, q, counters, s);
         default:
-            return pow2 ? dispatch_batch_typed<uint8_t, true>(c, mode, d_jobs, p, q, counters, s)
-                        : dispatch_batch_typed<uint8_t, false>(c, mode, d_jobs, p, q, counters, s);
+            return pow2 ? dispatch_batch_typed<uint8_t, true>(c, mode, ch, d_jobs, p, q, counters, s)
+                        : dispatch_batch_typed<uint8_t, false>(c, mode, ch, d_jobs, p, q, counters, s);
     }
+}
+
+int jobs_nmax(const std::vector<BatchJob>& jobs) {
+    int m = 1;
+    for (const BatchJob& j : jobs) m = std::max(m, std::max(j.nx, std::max(j.ny, j.nz)));
+    return m;
 }
 
 }  // namespace
@@ -962,7 +991,8 @@ int spr_render_batch(spr_ctx* c, const spr_params* params, const spr_job* jobs, 
     p.nouter = (prm.max_steps / num_parts) / kLoopUnroll;
     p.current_part = current_part;
     const BatchQueues q = make_queues(p, n);
-    if (e == hipSuccess) e = dispatch_batch(c, prm, dtype, pow2, d_jobs, p, q, d_counters, s);
+    if (e == hipSuccess)
+        e = dispatch_batch(c, prm, dtype, pow2, jobs_nmax(host), d_jobs, p, q, d_counters, s);
     (void)hipFreeAsync(buf, s);
     HIPCALL(c, e);
     // blocking: the host job table must outlive the transfer
@@ -1011,7 +1041,9 @@ int spr_batch_graph_create(spr_ctx* c, const spr_job* jobs, int n, int w, int h,
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     hipError_t e2 = hipSuccess;
     if (e == hipSuccess) {
-        e2 = dispatch_batch(c, c->params, dtype, pow2, bg->d_jobs, p, q, bg->d_counters, cs);
+        // the box length follows the captured jobs' volumes (speed only, never results)
+        e2 = dispatch_batch(c, c->params, dtype, pow2, jobs_nmax(host), bg->d_jobs, p, q,
+                            bg->d_counters, cs);
         e = hipStreamEndCapture(cs, &bg->graph);
     }
     (void)hipStreamDestroy(cs);

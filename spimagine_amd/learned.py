@@ -5,8 +5,9 @@ A frame is first rendered coarse -- half the resolution and a quarter of the ray
 bounds the samples the coarse step skips.  A small convolutional network on the matrix
 cores turns those two planes into the full-resolution frame.  The exact renderer is the
 reference: the network is trained on coarse/fine pairs the renderer produces, and
-``LearnedMIP.validate`` measures PSNR against exact frames; below the threshold the
-renderer falls back to the exact path.  No reference counterpart: the nearest analogue is
+``LearnedMIP.render_judged`` checks every frame against the exact renderer on a sparse
+pixel grid and falls back to the exact frame below the PSNR threshold (``validate``
+measures full-frame PSNR on a set of poses).  No reference counterpart: the nearest analogue is
 progressive multipass rendering (spimagine/volumerender/kernels/volume_kernel.cl:100-106,
 driven from spimagine/gui/glwidget.py:633-635).
 """
@@ -214,6 +215,35 @@ class LearnedMIP(object):
         planes = self._buffer(("planes", w, h), (1, 2, h // 2, w // 2))
         self.coarse(slot, p, w, h, planes, 0, stream)
         self.net.forward(planes.data_ptr(), 1, h // 2, w // 2, d_out, stream)
+
+    def render_judged(self, slot, p, out, alpha, stream, margin=0.5):
+        """One frame with a per-frame judge: the learned frame, then the exact renderer on
+        every 8th pixel in x and y (a (w/8) x (h/8) render with the same matrices traces
+        exactly the rays of pixels (8i, 8j): 1/64 of the exact cost), the PSNR of the
+        learned frame on those pixels, and the exact full frame when that estimate is
+        below threshold + margin.  out / alpha: (h, w) float32 torch tensors; stream: a
+        torch.cuda.Stream.  Learned frames take alpha from the coarse pass (exact at even
+        pixels, nearest-upsampled).  Returns (learned frame kept, estimated PSNR)."""
+        torch = self.torch
+        h, w = out.shape
+        if w % 64 or h % 64:
+            raise ValueError("judged frames need w and h multiples of 64")
+        sh = stream.cuda_stream
+        self.render(slot, p, w, h, out.data_ptr(), sh)
+        sparse = self._buffer(("judge", w, h), (h // 8, w // 8))
+        salpha = self._buffer(("judge_alpha", w, h), (h // 8, w // 8))
+        self.ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, w // 8, h // 8, sparse.data_ptr(),
+                             salpha.data_ptr(), stream=sh)
+        with torch.cuda.stream(stream):
+            mse = float(((out[::8, ::8] - sparse) ** 2).mean())
+            est = float("inf") if mse == 0 else 10. * math.log10(1. / mse)
+            if est >= self.threshold + margin:
+                coarse_alpha = self._buffer(("alpha", w, h), (h // 2, w // 2))
+                alpha.copy_(coarse_alpha.repeat_interleave(2, 0).repeat_interleave(2, 1))
+                return True, est
+        self.ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, w, h, out.data_ptr(),
+                             alpha.data_ptr(), stream=sh)
+        return False, est
 
     def render_or_exact(self, slot, p, w, h, d_out, d_alpha, stream):
         """Learned frame when the slot validated, else the exact render."""

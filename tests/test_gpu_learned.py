@@ -191,3 +191,50 @@ def test_learned_render_path_shapes_and_fallback(torch):
                     stream=s.cuda_stream)
     s.synchronize()
     assert torch.equal(o, ref)
+
+
+def test_sparse_judge_rays_are_the_full_frame_pixels(torch):
+    """A (w/8) x (h/8) render with the same matrices traces the rays of pixels (8i, 8j) of
+    the w x h frame, bit for bit; render_judged falls back to the exact frame below its
+    threshold and keeps the learned frame above it."""
+    vol = synth.blob_volume((64, 80, 72), n_blobs=14, sigma=(3., 6.), amp=(500., 3000.),
+                            background=100., poisson=True, read_noise=5., dtype="uint16", seed=2)
+    ctx = _lib.Context(0)
+    slot = ctx.volume_add(vol)
+    from spimagine_amd.utils import mat4_perspective
+    from spimagine_amd.volumerender.volumerender import render_matrices
+    P = mat4_perspective(45, 1., .1, 10)
+    M = synth.random_poses(1, seed=9)[0]
+    iP, iM = render_matrices(M, P, (72, 80, 64))
+    p = _lib.make_params(iP, iM, (-1, 1, -1, 1, -1, 1), float(vol.min()), float(vol.max()), 1., 0.,
+                         .1, 21, 30, 160, _lib.FILTER_LINEAR)
+    W = H = 128
+    s = torch.cuda.Stream()
+    full = torch.empty((H, W), device="cuda")
+    fa = torch.empty_like(full)
+    sparse = torch.empty((H // 8, W // 8), device="cuda")
+    sa = torch.empty_like(sparse)
+    ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W, H, full.data_ptr(), fa.data_ptr(),
+                    stream=s.cuda_stream)
+    ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W // 8, H // 8, sparse.data_ptr(), sa.data_ptr(),
+                    stream=s.cuda_stream)
+    s.synchronize()
+    assert torch.equal(sparse, full[::8, ::8]) and torch.equal(sa, fa[::8, ::8])
+    out = torch.empty_like(full)
+    alpha = torch.empty_like(full)
+    strict = learned.LearnedMIP(ctx, params=learned.init_params(0), threshold=float("inf"))
+    kept, est = strict.render_judged(slot, p, out, alpha, s)
+    s.synchronize()
+    assert kept is False and np.isfinite(est)
+    assert torch.equal(out, full) and torch.equal(alpha, fa)
+    lax = learned.LearnedMIP(ctx, params=learned.init_params(0), threshold=-float("inf"))
+    kept, est2 = lax.render_judged(slot, p, out, alpha, s)
+    ref = torch.empty_like(full)
+    lax.render(slot, p, W, H, ref.data_ptr(), s.cuda_stream)
+    s.synchronize()
+    assert kept is True and abs(est2 - est) < 1e-6
+    assert torch.equal(out, ref)
+    mse = float(((ref[::8, ::8] - full[::8, ::8]) ** 2).mean())
+    assert abs(est2 - 10 * np.log10(1. / mse)) < 1e-4
+    # learned frames carry the coarse pass's alpha: exact at even pixels
+    assert torch.equal(alpha[::2, ::2], fa[::2, ::2])

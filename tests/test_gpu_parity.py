@@ -170,6 +170,28 @@ def test_staged_mip_matches_global(VR, dtype, shape, pose):
     assert (frames[0][0] > 0).sum() > 200
 
 
+@pytest.mark.parametrize("dtype", [np.uint16, np.uint8, np.float32])
+@pytest.mark.parametrize("max_steps", [32, 64, 96])
+def test_staged_mip_long_steps(VR, dtype, max_steps):
+    """Ray steps of 1-3 voxels (coarse renders): the staged kernel stages 16, 8 or 4
+    samples per box and stays bit-identical to the unstaged kernel."""
+    vol = synth.blob_volume((80, 96, 88), n_blobs=20, sigma=(2., 6.), amp=(100., 250.),
+                            background=5., poisson=False, read_noise=0., dtype=dtype, seed=7)
+    M = np.dot(mat4_translate(0, .1, -3.5), mat4_rotation(.7, .3, 1, .2))
+    frames = {}
+    for slab in (0, 12):
+        rend = VR((120, 104))
+        rend._ctx.set_tuning("slab_kb", slab)
+        rend.set_max_steps(max_steps)
+        rend.set_data(vol)
+        rend.set_modelView(M)
+        rend.render(minVal=float(vol.min()), maxVal=float(vol.max()))
+        frames[slab] = (rend.output.copy(), rend.output_alpha.copy())
+    np.testing.assert_array_equal(frames[0][0], frames[12][0])
+    np.testing.assert_array_equal(frames[0][1], frames[12][1])
+    assert (frames[0][0] > 0).sum() > 500
+
+
 def test_tuning_does_not_change_results(VR):
     vol = synth.blob_volume((40, 48, 56), n_blobs=10, sigma=(2., 5.), amp=(500., 3000.),
                             background=100., poisson=True, read_noise=5., dtype="uint16", seed=4)
