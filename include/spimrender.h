@@ -72,9 +72,14 @@ extern "C" {
                       0 = tile rows interleaved over XCDs (default), 1 = one contiguous
                       band per XCD, 2 = row-major
    SPR_TUNE_SLAB_KB   per-wave LDS slab (KiB) of the staged max projection: 8, 12
-                      (default) or 16; 0 = sample every chunk from global memory */
+                      (default) or 16; 0 = sample every chunk from global memory
+   SPR_TUNE_STAGING   0 = each wave stages its own box (default), 1 = the four waves of a
+                      workgroup stage one union box into a slab of 4 x SLAB_KB
+   SPR_TUNE_CHUNK     samples per staged box: 0 = from the ray step (default), 4, 8, 16 */
 #define SPR_TUNE_TILE_MAP 0
 #define SPR_TUNE_SLAB_KB 1
+#define SPR_TUNE_STAGING 2
+#define SPR_TUNE_CHUNK 3
 
 typedef struct spr_ctx spr_ctx;
 

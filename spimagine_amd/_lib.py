@@ -26,7 +26,9 @@ FILTER_LINEAR, FILTER_NEAREST = 0, 1
 METHOD_MAX_PROJECT, METHOD_ISO_SURFACE = 0, 1
 LAYOUT_AUTO, LAYOUT_LINEAR, LAYOUT_PADDED = 0, 1, 2
 TUNE_TILE_MAP, TUNE_SLAB_KB = 0, 1
-TUNING_KEYS = {"tile_map": TUNE_TILE_MAP, "slab_kb": TUNE_SLAB_KB}
+TUNE_STAGING, TUNE_CHUNK = 2, 3
+TUNING_KEYS = {"tile_map": TUNE_TILE_MAP, "slab_kb": TUNE_SLAB_KB, "staging": TUNE_STAGING,
+               "chunk": TUNE_CHUNK}
 
 # every symbol include/spimrender.h declares
 EXPORTS = (

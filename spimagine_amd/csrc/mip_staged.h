@@ -1,4 +1,4 @@
-// mip_staged.h -- max projection with per-wave LDS staging of the volume.
+// mip_staged.h -- max projection with LDS staging of the volume (per wave or per workgroup).
 //
 // Same result, bit for bit, as k_max_project on the padded layout (volume_kernel.cl:20-185,
 // :190-345 with alpha_pow == 0): identical sample positions, weights and lerp order; only
@@ -11,8 +11,12 @@
 // of its 64 rays touch (DPP reductions), copies the box into its own LDS slab with aligned
 // 16-byte row loads (one granule per 4 lanes), then gathers the corners from LDS.  A box
 // larger than the slab (rays of one wave with very different dt, near the box silhouette)
-// is sampled from global memory for that chunk instead.  No workgroup barrier: each wave
-// owns its slab and LDS operations of one wave complete in order.
+// is sampled from global memory for that chunk instead.  In the per-wave form there is no
+// workgroup barrier: each wave owns its slab and LDS operations of one wave complete in
+// order.  In the shared form the four waves of a 16x16 tile stage one union box (fewer
+// bytes than four overlapping boxes, and boxes of oblique views fit more often) with two
+// workgroup barriers per box; every ray of a workgroup takes the same number of samples
+// (the reference's fixed count), so the waves advance in step.
 #pragma once
 #include "render_kernels.h"
 
@@ -71,51 +75,198 @@ struct alignas(16) Unit16 {
     uint32_t w[4];
 };
 
+// Ray set-up shared by the staged kernels (volume_kernel.cl:45-106): writes the miss
+// outputs and returns whether the ray hits the box; position and step in texture units.
+template <bool kShort>
+__device__ __forceinline__ bool staged_ray_begin(const RayParams& p, uint32_t x, uint32_t y,
+                                                 bool inside, float* __restrict__ out,
+                                                 float* __restrict__ alpha, float& px, float& py,
+                                                 float& pz, float& dx, float& dy, float& dz,
+                                                 float& tnear) {
+    if (!inside) return false;
+    const uint32_t o = x + p.W * y;
+    F4 orig, direc;
+    ray_setup(p.invP, p.invM, x, y, p.W, p.H, &orig, &direc);
+    float tfar;
+    if (!intersect_box(orig, direc, p.box, &tnear, &tfar)) {
+        out[o] = 0.0f;
+        alpha[o] = kShort ? 0.0f : -1.0f;
+        return false;
+    }
+    if (tnear < 0.0f) tnear = 0.0f;
+    const float dt = fabsf(tfar - tnear) / (float)(p.nouter * kLoopUnroll);
+    orig = add4(orig, scl4((float)p.current_part * dt, direc));
+    const F4 delta = scl4(0.5f * dt, direc);
+    const F4 pos = scl4(0.5f, add4(mk4(1.0f + orig.x, 1.0f + orig.y, 1.0f + orig.z, 1.0f + orig.w),
+                                   scl4(tnear, direc)));
+    px = pos.x;
+    py = pos.y;
+    pz = pos.z;
+    dx = delta.x;
+    dy = delta.y;
+    dz = delta.z;
+    return true;
+}
+
+// Padded-voxel box of the CH samples starting at (px, py, pz), packed for a min-reduction:
+// b0 = lo.x | lo.y << 16, b1 = lo.z | (0xFFFF - hi.x) << 16, b2 = (0xFFFF - hi.y) |
+// (0xFFFF - hi.z) << 16 (hi exclusive).  Inactive lanes give all ones (no effect).  The
+// last sample's position is estimated with one fma; mg* covers its distance from the
+// sequentially added position.
+template <typename T, bool POW2, int CH>
+__device__ __forceinline__ void staged_chunk_box(const PaddedVolume<T, POW2>& vol, bool active,
+                                                 float px, float py, float pz, float dx, float dy,
+                                                 float dz, float mgx, float mgy, float mgz,
+                                                 uint32_t& b0, uint32_t& b1, uint32_t& b2) {
+    b0 = b1 = b2 = 0xFFFFFFFFu;
+    if (!active) return;
+    const float ex = fmaf((float)(CH - 1), dx, px), ey = fmaf((float)(CH - 1), dy, py),
+                ez = fmaf((float)(CH - 1), dz, pz);
+    const float u0x = texel<POW2>(px, vol.fnx), u0y = texel<POW2>(py, vol.fny),
+                u0z = texel<POW2>(pz, vol.fnz);
+    const float u1x = texel<POW2>(ex, vol.fnx), u1y = texel<POW2>(ey, vol.fny),
+                u1z = texel<POW2>(ez, vol.fnz);
+    const uint32_t a0x = vol.corner1(floorf(u0x), vol.mx);
+    const uint32_t a0y = vol.corner1(floorf(u0y), vol.my);
+    const uint32_t a0z = vol.corner1(floorf(u0z), vol.mz);
+    const uint32_t lx = min(a0x, vol.corner1(floorf(u1x - mgx), vol.mx));
+    const uint32_t ly = min(a0y, vol.corner1(floorf(u1y - mgy), vol.my));
+    const uint32_t lz = min(a0z, vol.corner1(floorf(u1z - mgz), vol.mz));
+    const uint32_t hx = max(a0x, vol.corner1(floorf(u1x + mgx), vol.mx)) + 1u;
+    const uint32_t hy = max(a0y, vol.corner1(floorf(u1y + mgy), vol.my)) + 1u;
+    const uint32_t hz = max(a0z, vol.corner1(floorf(u1z + mgz), vol.mz)) + 1u;
+    b0 = lx | (ly << 16);
+    b1 = lz | ((0xFFFFu - hx) << 16);
+    b2 = (0xFFFFu - hy) | ((0xFFFFu - hz) << 16);
+}
+
+// Copies rows r = z SY + y of the box into slab[r SX TS ...] with NT threads (t = this
+// thread's index among them): 16-byte buffer loads, nu units per row, NT / nu rows a pass.
+template <uint32_t NT>
+__device__ __forceinline__ void staged_copy_rows(__amdgpu_buffer_rsrc_t rsrc, uint8_t* slab,
+                                                 uint32_t t, uint32_t LY, uint32_t LZ,
+                                                 uint32_t gx0, uint32_t nu, uint32_t rowb,
+                                                 uint32_t SY, uint32_t SZ, uint32_t PY,
+                                                 uint32_t pitch) {
+    const uint32_t R = NT / nu;
+    const uint32_t lr = t / nu, lu = t - lr * nu;
+    const uint32_t nrows = SY * SZ;
+    const uint32_t gx = gx0 + lu * 16u;
+    if (lr >= R) return;
+    const float inv_sy = 1.0f / (float)SY;
+    for (uint32_t r = lr; r < nrows; r += R) {
+        uint32_t zr = (uint32_t)((float)r * inv_sy);
+        int32_t yr = (int32_t)(r - zr * SY);
+        if (yr < 0) {
+            zr -= 1u;
+            yr += (int32_t)SY;
+        } else if (yr >= (int32_t)SY) {
+            zr += 1u;
+            yr -= (int32_t)SY;
+        }
+        const uint32_t goff = __umul24(__umul24(LZ + zr, PY) + LY + (uint32_t)yr, pitch) + gx;
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, goff, 0, 0);
+        Unit16 u;
+        u.w[0] = v[0];
+        u.w[1] = v[1];
+        u.w[2] = v[2];
+        u.w[3] = v[3];
+        *reinterpret_cast<Unit16*>(slab + r * rowb + lu * 16u) = u;
+    }
+}
+
+// CH samples from the slab holding the box (lox, LY, LZ, row bytes rowb, SY rows a plane);
+// slab offset of corner (c+1) relative to the box computed exactly in float (every term is
+// an integer below 2^24)
+template <typename T, bool POW2, int CH>
+__device__ __forceinline__ void staged_sample_slab(const PaddedVolume<T, POW2>& vol,
+                                                   const uint8_t* slab, uint32_t lox, uint32_t LY,
+                                                   uint32_t LZ, uint32_t rowb, uint32_t SY,
+                                                   float& px, float& py, float& pz, float dx,
+                                                   float dy, float dz, float& col) {
+    constexpr uint32_t TS = sizeof(T);
+    const uint32_t sy = rowb, sz = rowb * SY;
+    const float fsy = (float)sy, fsz = (float)sz;
+    const float oy = 1.0f - (float)LY, oz = 1.0f - (float)LZ;
+    const float ox = (1.0f - (float)lox) * (float)TS;
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+        float cx, cy, cz, ax, ay, az;
+        vol.locate_f(px, py, pz, &cx, &cy, &cz, &ax, &ay, &az);
+        const float bf = fmaf(cz + oz, fsz, fmaf(cy + oy, fsy, fmaf(cx, (float)TS, ox)));
+        const uint32_t q = (uint32_t)bf;
+        float v000, v100, v010, v110, v001, v101, v011, v111;
+        lds_pair(slab, q, v000, v100, T());
+        lds_pair(slab, q + sy, v010, v110, T());
+        lds_pair(slab, q + sz, v001, v101, T());
+        lds_pair(slab, q + sy + sz, v011, v111, T());
+        col = fmaxf(col, trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az));
+        px = px + dx;
+        py = py + dy;
+        pz = pz + dz;
+    }
+}
+
+template <typename T, bool POW2, int CH>
+__device__ __forceinline__ void staged_sample_global(const PaddedVolume<T, POW2>& vol, float& px,
+                                                     float& py, float& pz, float dx, float dy,
+                                                     float dz, float& col) {
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+        col = fmaxf(col, vol.linear(px, py, pz));
+        px = px + dx;
+        py = py + dy;
+        pz = pz + dz;
+    }
+}
+
+template <bool kShort>
+__device__ __forceinline__ void staged_ray_end(const RayParams& p, uint32_t o, float col,
+                                               float tnear, float* __restrict__ out,
+                                               float* __restrict__ alpha) {
+    col = (p.max_val == 0.0f) ? col : (col - p.min_val) / (p.max_val - p.min_val);
+    col = apply_gamma(col, p.gamma);
+    const float a = kShort ? tnear : 1.0f;
+    if (p.current_part == 0) {
+        out[o] = col;
+        alpha[o] = a;
+    } else {
+        out[o] = fmaxf(col, out[o]);
+        alpha[o] = fmaxf(a, alpha[o]);
+    }
+}
+
 // CH: samples staged per box (16, 8 or 4).  The samples, their order and their positions
 // are those of the reference's 16-sample inner loop whatever CH is (a max does not depend
 // on how the sequence is cut); a smaller CH keeps a box small when the ray step is several
 // voxels long.
-template <typename T, bool POW2, int SLAB_BYTES, int CH>
+// SHARED = false: each wave stages the box of its own 64 rays into its own slab
+// (SLAB_BYTES, wave-level ordering only).  SHARED = true: the workgroup's four waves
+// stage the union box of its 256 rays into one slab of SLAB_BYTES with all 256 threads,
+// two workgroup barriers per box; s_box holds the four waves' partial boxes.
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED>
 __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol, const RayParams& p,
                                                 uint32_t x, uint32_t y, bool inside, uint8_t* slab,
-                                                float* __restrict__ out, float* __restrict__ alpha) {
+                                                uint32_t* s_box, float* __restrict__ out,
+                                                float* __restrict__ alpha) {
     constexpr bool kShort = !std::is_same<T, float>::value;
     constexpr uint32_t E = 16u / sizeof(T);  // voxels per 16-byte unit
     constexpr uint32_t TS = sizeof(T);
+    constexpr uint32_t NT = SHARED ? 256u : 64u;
     const uint32_t lane = threadIdx.x & 63u;
-    bool active = false;
-    float px = 0.5f, py = 0.5f, pz = 0.5f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
-    float tnear = 0.0f;
-    const uint32_t o = x + p.W * y;
-    if (inside) {
-        F4 orig, direc;
-        ray_setup(p.invP, p.invM, x, y, p.W, p.H, &orig, &direc);
-        float tfar;
-        if (!intersect_box(orig, direc, p.box, &tnear, &tfar)) {
-            out[o] = 0.0f;
-            alpha[o] = kShort ? 0.0f : -1.0f;
-        } else {
-            if (tnear < 0.0f) tnear = 0.0f;
-            const float dt = fabsf(tfar - tnear) / (float)(p.nouter * kLoopUnroll);
-            orig = add4(orig, scl4((float)p.current_part * dt, direc));
-            const F4 delta = scl4(0.5f * dt, direc);
-            const F4 pos = scl4(0.5f, add4(mk4(1.0f + orig.x, 1.0f + orig.y, 1.0f + orig.z,
-                                               1.0f + orig.w),
-                                           scl4(tnear, direc)));
-            px = pos.x;
-            py = pos.y;
-            pz = pos.z;
-            dx = delta.x;
-            dy = delta.y;
-            dz = delta.z;
-            active = true;
-        }
-    }
+    float px = 0.5f, py = 0.5f, pz = 0.5f, dx = 0.0f, dy = 0.0f, dz = 0.0f, tnear = 0.0f;
+    const bool active = staged_ray_begin<kShort>(p, x, y, inside, out, alpha, px, py, pz, dx, dy,
+                                                 dz, tnear);
     const uint64_t act = __ballot(active);
-    if (act == 0) return;
-    // lanes without a ray shadow the first active lane, so every lane's LDS address is valid
-    // (the shuffles run with every lane of the wave enabled)
-    {
+    if constexpr (SHARED) {
+        if (__syncthreads_or(active ? 1 : 0) == 0) return;  // uniform over the workgroup
+    } else {
+        if (act == 0) return;
+    }
+    const bool live = act != 0;  // uniform over the wave
+    // lanes without a ray shadow the first active lane of their wave, so every lane's LDS
+    // address is valid (the shuffles run with every lane of the wave enabled)
+    if (live) {
         const int src = __builtin_ctzll(act);
         const float spx = __shfl(px, src), spy = __shfl(py, src), spz = __shfl(pz, src);
         const float sdx = __shfl(dx, src), sdy = __shfl(dy, src), sdz = __shfl(dz, src);
@@ -129,8 +280,6 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
 
     const uint32_t PY = vol.py, pitch = vol.px_bytes;
     const __amdgpu_buffer_rsrc_t rsrc = vol.rsrc();
-    // texel-unit margin around the estimated last sample of a chunk (its position is
-    // computed with one fma instead of 15 sequential adds)
     const float mgx = 1e-5f * vol.fnx + 0.01f, mgy = 1e-5f * vol.fny + 0.01f,
                 mgz = 1e-5f * vol.fnz + 0.01f;
     float col = 0.0f;
@@ -138,29 +287,24 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
     static_assert(kLoopUnroll % CH == 0, "a chunk is a whole number of boxes");
     const int nchunks = (p.nouter + 1) * (kLoopUnroll / CH);
     for (int i = 0; i < nchunks; ++i) {
-        // ---- box of padded voxels this chunk's CH samples of the wave's rays touch
-        uint32_t b0 = 0xFFFFFFFFu, b1 = 0xFFFFFFFFu, b2 = 0xFFFFFFFFu;
-        if (active) {
-            const float ex = fmaf((float)(CH - 1), dx, px), ey = fmaf((float)(CH - 1), dy, py),
-                        ez = fmaf((float)(CH - 1), dz, pz);
-            const float u0x = texel<POW2>(px, vol.fnx), u0y = texel<POW2>(py, vol.fny),
-                        u0z = texel<POW2>(pz, vol.fnz);
-            const float u1x = texel<POW2>(ex, vol.fnx), u1y = texel<POW2>(ey, vol.fny),
-                        u1z = texel<POW2>(ez, vol.fnz);
-            const uint32_t a0x = vol.corner1(floorf(u0x), vol.mx);
-            const uint32_t a0y = vol.corner1(floorf(u0y), vol.my);
-            const uint32_t a0z = vol.corner1(floorf(u0z), vol.mz);
-            const uint32_t lx = min(a0x, vol.corner1(floorf(u1x - mgx), vol.mx));
-            const uint32_t ly = min(a0y, vol.corner1(floorf(u1y - mgy), vol.my));
-            const uint32_t lz = min(a0z, vol.corner1(floorf(u1z - mgz), vol.mz));
-            const uint32_t hx = max(a0x, vol.corner1(floorf(u1x + mgx), vol.mx)) + 1u;
-            const uint32_t hy = max(a0y, vol.corner1(floorf(u1y + mgy), vol.my)) + 1u;
-            const uint32_t hz = max(a0z, vol.corner1(floorf(u1z + mgz), vol.mz)) + 1u;
-            b0 = lx | (ly << 16);
-            b1 = lz | ((0xFFFFu - hx) << 16);
-            b2 = (0xFFFFu - hy) | ((0xFFFFu - hz) << 16);
+        uint32_t b0, b1, b2;
+        staged_chunk_box<T, POW2, CH>(vol, active, px, py, pz, dx, dy, dz, mgx, mgy, mgz, b0, b1, b2);
+        uint32_t r0 = wave_min_u16x2(b0), r1 = wave_min_u16x2(b1), r2 = wave_min_u16x2(b2);
+        if constexpr (SHARED) {
+            const uint32_t w = threadIdx.x >> 6;
+            if (lane == 0) {
+                s_box[3 * w] = r0;
+                s_box[3 * w + 1] = r1;
+                s_box[3 * w + 2] = r2;
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                r0 = pk_min_u16(r0, s_box[3 * k]);
+                r1 = pk_min_u16(r1, s_box[3 * k + 1]);
+                r2 = pk_min_u16(r2, s_box[3 * k + 2]);
+            }
         }
-        const uint32_t r0 = wave_min_u16x2(b0), r1 = wave_min_u16x2(b1), r2 = wave_min_u16x2(b2);
         const uint32_t LX = r0 & 0xFFFFu, LY = r0 >> 16, LZ = r1 & 0xFFFFu;
         const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
                        HZ = 0xFFFFu - (r2 >> 16);
@@ -169,96 +313,44 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
         const uint32_t SY = HY - LY + 1u, SZ = HZ - LZ + 1u;
         const uint32_t nu = SX / E;                                 // 16-byte units per row
         const uint64_t bytes = (uint64_t)SX * SY * SZ * TS;
-
-        if (bytes <= (uint64_t)SLAB_BYTES && nu <= 64u) {
-            // ---- stage: row r of the box (r = z*SY + y) -> slab[r*SX*TS ...]
-            const uint32_t R = 64u / nu;  // rows per pass
-            const uint32_t lr = lane / nu, lu = lane - lr * nu;
-            const uint32_t nrows = SY * SZ;
-            const uint32_t gx = (lox * TS) + lu * 16u;
-            const uint32_t rowb = SX * TS;
-            if (lr < R) {
-                const float inv_sy = 1.0f / (float)SY;
-                for (uint32_t r = lr; r < nrows; r += R) {
-                    uint32_t zr = (uint32_t)((float)r * inv_sy);
-                    int32_t yr = (int32_t)(r - zr * SY);
-                    if (yr < 0) {
-                        zr -= 1u;
-                        yr += (int32_t)SY;
-                    } else if (yr >= (int32_t)SY) {
-                        zr += 1u;
-                        yr -= (int32_t)SY;
-                    }
-                    const uint32_t goff =
-                        __umul24(__umul24(LZ + zr, PY) + LY + (uint32_t)yr, pitch) + gx;
-                    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, goff, 0, 0);
-                    Unit16 u;
-                    u.w[0] = v[0];
-                    u.w[1] = v[1];
-                    u.w[2] = v[2];
-                    u.w[3] = v[3];
-                    *reinterpret_cast<Unit16*>(slab + r * rowb + lu * 16u) = u;
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            // ---- CH samples from the slab; slab offset of corner (c+1) relative to the box,
-            // computed exactly in float (every term is an integer below 2^24)
-            const uint32_t sy = rowb, sz = rowb * SY;
-            const float fsy = (float)sy, fsz = (float)sz;
-            const float oy = 1.0f - (float)LY, oz = 1.0f - (float)LZ;
-            const float ox = (1.0f - (float)lox) * (float)TS;
-#pragma unroll
-            for (int j = 0; j < CH; ++j) {
-                float cx, cy, cz, ax, ay, az;
-                vol.locate_f(px, py, pz, &cx, &cy, &cz, &ax, &ay, &az);
-                const float bf = fmaf(cz + oz, fsz, fmaf(cy + oy, fsy, fmaf(cx, (float)TS, ox)));
-                const uint32_t q = (uint32_t)bf;
-                float v000, v100, v010, v110, v001, v101, v011, v111;
-                lds_pair(slab, q, v000, v100, T());
-                lds_pair(slab, q + sy, v010, v110, T());
-                lds_pair(slab, q + sz, v001, v101, T());
-                lds_pair(slab, q + sy + sz, v011, v111, T());
-                col = fmaxf(col, trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az));
-                px = px + dx;
-                py = py + dy;
-                pz = pz + dz;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
+        const bool fits = bytes <= (uint64_t)SLAB_BYTES && nu <= NT;
+        if (fits)
+            staged_copy_rows<NT>(rsrc, slab, SHARED ? threadIdx.x : lane, LY, LZ, lox * TS, nu,
+                                 SX * TS, SY, SZ, PY, pitch);
+        if constexpr (SHARED) {
+            __syncthreads();
         } else {
-#pragma unroll
-            for (int j = 0; j < CH; ++j) {
-                col = fmaxf(col, vol.linear(px, py, pz));
-                px = px + dx;
-                py = py + dy;
-                pz = pz + dz;
-            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+        if (live) {
+            if (fits)
+                staged_sample_slab<T, POW2, CH>(vol, slab, lox, LY, LZ, SX * TS, SY, px, py, pz, dx,
+                                                dy, dz, col);
+            else
+                staged_sample_global<T, POW2, CH>(vol, px, py, pz, dx, dy, dz, col);
+        }
+        if constexpr (!SHARED) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
     }
-    if (!active) return;
-    col = (p.max_val == 0.0f) ? col : (col - p.min_val) / (p.max_val - p.min_val);
-    col = apply_gamma(col, p.gamma);
-    const float a = kShort ? tnear : 1.0f;
-    if (p.current_part == 0) {
-        out[o] = col;
-        alpha[o] = a;
-    } else {
-        out[o] = fmaxf(col, out[o]);
-        alpha[o] = fmaxf(a, alpha[o]);
-    }
+    if (active) staged_ray_end<kShort>(p, x + p.W * y, col, tnear, out, alpha);
 }
 
 // one frame: one workgroup per 16x16 tile (tile map as k_max_project)
-template <typename T, bool POW2, int SLAB_BYTES, int CH>
+// SLAB_BYTES: per wave (SHARED = false) or per workgroup (SHARED = true)
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED>
 __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2> vol, RayParams p,
                                                             float* __restrict__ out,
                                                             float* __restrict__ alpha) {
-    __shared__ Unit16 lds[4 * SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
-    uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * SLAB_BYTES;
+    constexpr int kLds = SHARED ? SLAB_BYTES : 4 * SLAB_BYTES;
+    __shared__ Unit16 lds[kLds / 16 + 1];  // +16 B: lds_pair reads past a pair
+    __shared__ uint32_t s_box[12];
+    uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (SHARED ? 0u : (threadIdx.x >> 6) * SLAB_BYTES);
     uint32_t x = 0, y = 0;
     const bool inside = tile_pixel(p, &x, &y);
-    mip_staged_rays<T, POW2, SLAB_BYTES, CH>(vol, p, x, y, inside, slab, out, alpha);
+    mip_staged_rays<T, POW2, SLAB_BYTES, CH, SHARED>(vol, p, x, y, inside, slab, s_box, out, alpha);
 }
 
 // Samples per staged box for a frame: the ray advances at most N_a / (16 nouter) voxels
@@ -327,14 +419,16 @@ __global__ void k_reset_counters(uint32_t* __restrict__ counters) {
 // after one failed claim on each queue.
 // MODE 0: staged plain max projection (linear filter, alpha_pow == 0; CH samples per
 // staged box); MODE 1 / 2: every sample from global memory with the linear / nearest
-// filter (any alpha_pow; CH unused)
-template <typename T, bool POW2, int SLAB_BYTES, int MODE, int CH>
+// filter (any alpha_pow; CH and SHARED unused)
+template <typename T, bool POW2, int SLAB_BYTES, int MODE, int CH, bool SHARED>
 __global__ __launch_bounds__(256) void k_max_project_batch(const BatchJob* __restrict__ jobs,
                                                            RayParams shared, BatchQueues qs,
                                                            uint32_t* __restrict__ counters) {
-    __shared__ Unit16 lds[4 * SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
+    constexpr int kLds = SHARED ? SLAB_BYTES : 4 * SLAB_BYTES;
+    __shared__ Unit16 lds[kLds / 16 + 1];  // +16 B: lds_pair reads past a pair
+    __shared__ uint32_t s_box[12];
     __shared__ uint32_t s_item;
-    uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * SLAB_BYTES;
+    uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (SHARED ? 0u : (threadIdx.x >> 6) * SLAB_BYTES);
     uint32_t q = blockIdx.x & 7u;
     for (uint32_t drained = 0; drained < 8u;) {
         if (threadIdx.x == 0) s_item = atomicAdd(&counters[q], 1u);
@@ -361,7 +455,8 @@ __global__ __launch_bounds__(256) void k_max_project_batch(const BatchJob* __res
         const bool inside = x < p.W && y < p.H;
         const PaddedVolume<T, POW2> vol = job_volume<T, POW2>(j);
         if constexpr (MODE == 0)
-            mip_staged_rays<T, POW2, SLAB_BYTES, CH>(vol, p, x, y, inside, slab, j.out, j.alpha);
+            mip_staged_rays<T, POW2, SLAB_BYTES, CH, SHARED>(vol, p, x, y, inside, slab, s_box,
+                                                             j.out, j.alpha);
         else
             mip_rays<PaddedVolume<T, POW2>, MODE == 2 ? 1 : 0>(vol, p, x, y, inside, j.out,
                                                                 j.alpha);

@@ -71,6 +71,8 @@ struct spr_ctx {
     // launch tuning (spr_set_tuning)
     int tile_map = (int)kTileMapInterleaved;
     int slab_kb = 12;  // per-wave LDS slab of the staged max projection, 0 = off
+    int staging = 0;   // 0 per-wave boxes, 1 one box per workgroup
+    int chunk = 0;     // samples per staged box, 0 = from the ray step
 };
 
 namespace {
@@ -306,24 +308,38 @@ int check_parts(spr_ctx* c, const spr_params& prm, int num_parts) {
     return SPR_OK;
 }
 
-template <typename T, bool P2, int CH, typename... Args>
-void launch_staged_ch(int slab_kb, dim3 grid, hipStream_t s, Args... args) {
+template <typename T, bool P2, int CH, bool SH, typename... Args>
+void launch_staged_sh(int slab_kb, dim3 grid, hipStream_t s, Args... args) {
+    constexpr int F = SH ? 4 : 1;  // a workgroup slab holds the four waves' share
     if (slab_kb == 8)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 8192, CH>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 8192 * F, CH, SH>), grid, dim3(256), 0, s, args...);
     else if (slab_kb == 16)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, CH>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384 * F, CH, SH>), grid, dim3(256), 0, s, args...);
     else
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288, CH>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288 * F, CH, SH>), grid, dim3(256), 0, s, args...);
+}
+
+template <typename T, bool P2, int CH, typename... Args>
+void launch_staged_ch(int slab_kb, bool shared, dim3 grid, hipStream_t s, Args... args) {
+    if (shared)
+        launch_staged_sh<T, P2, CH, true>(slab_kb, grid, s, args...);
+    else
+        launch_staged_sh<T, P2, CH, false>(slab_kb, grid, s, args...);
+}
+
+// samples per staged box: the tuning value, else from the ray step
+int chunk_for(const spr_ctx* c, int nouter, int nmax) {
+    return c->chunk ? c->chunk : staged_chunk(nouter, nmax);
 }
 
 template <typename T, bool P2, typename... Args>
-void launch_staged(int slab_kb, int ch, dim3 grid, hipStream_t s, Args... args) {
+void launch_staged(int slab_kb, bool shared, int ch, dim3 grid, hipStream_t s, Args... args) {
     if (ch == 16)
-        launch_staged_ch<T, P2, 16>(slab_kb, grid, s, args...);
+        launch_staged_ch<T, P2, 16>(slab_kb, shared, grid, s, args...);
     else if (ch == 8)
-        launch_staged_ch<T, P2, 8>(slab_kb, grid, s, args...);
+        launch_staged_ch<T, P2, 8>(slab_kb, shared, grid, s, args...);
     else
-        launch_staged_ch<T, P2, 4>(slab_kb, grid, s, args...);
+        launch_staged_ch<T, P2, 4>(slab_kb, shared, grid, s, args...);
 }
 
 // render(method, numParts, currentPart) into device buffers; the reference's launch
@@ -350,7 +366,8 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
                     const int nmax = vs.nx > vs.ny ? (vs.nx > vs.nz ? vs.nx : vs.nz)
                                                    : (vs.ny > vs.nz ? vs.ny : vs.nz);
                     launch_staged<typename V::value_type, V::kPow2>(
-                        c->slab_kb, staged_chunk(p.nouter, nmax), grid, s, v, p, out, alpha);
+                        c->slab_kb, c->staging == 1, chunk_for(c, p.nouter, nmax), grid, s, v, p,
+                        out, alpha);
                     return hipGetLastError();
                 }
             }
@@ -573,38 +590,48 @@ BatchQueues make_queues(const RayParams& p, int n) {
 }
 
 // enqueue counter reset + persistent kernel on s
-template <typename T, bool P2, int SLAB, int MODE, int CH>
+template <typename T, bool P2, int SLAB, int MODE, int CH, bool SH>
 hipError_t launch_batch_kernel(const spr_ctx* c, const BatchJob* d_jobs, const RayParams& p,
                                const BatchQueues& q, uint32_t* counters, hipStream_t s) {
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, k_max_project_batch<T, P2, SLAB, MODE, CH>, 256, 0);
+        &per_cu, k_max_project_batch<T, P2, SLAB, MODE, CH, SH>, 256, 0);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     const uint32_t grid = (uint32_t)(c->num_cus * per_cu + 7) / 8u * 8u;
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters);
-    hipLaunchKernelGGL((k_max_project_batch<T, P2, SLAB, MODE, CH>), dim3(grid), dim3(256), 0, s,
+    hipLaunchKernelGGL((k_max_project_batch<T, P2, SLAB, MODE, CH, SH>), dim3(grid), dim3(256), 0, s,
                        d_jobs, p, q, counters);
     return hipGetLastError();
 }
 
-template <typename T, bool P2, int SLAB>
+template <typename T, bool P2, int SLAB, bool SH>
 hipError_t launch_batch_staged(int ch, const spr_ctx* c, const BatchJob* d_jobs, const RayParams& p,
                                const BatchQueues& q, uint32_t* counters, hipStream_t s) {
-    if (ch == 16) return launch_batch_kernel<T, P2, SLAB, 0, 16>(c, d_jobs, p, q, counters, s);
-    if (ch == 8) return launch_batch_kernel<T, P2, SLAB, 0, 8>(c, d_jobs, p, q, counters, s);
-    return launch_batch_kernel<T, P2, SLAB, 0, 4>(c, d_jobs, p, q, counters, s);
+    constexpr int B = SLAB * (SH ? 4 : 1);
+    if (ch == 16) return launch_batch_kernel<T, P2, B, 0, 16, SH>(c, d_jobs, p, q, counters, s);
+    if (ch == 8) return launch_batch_kernel<T, P2, B, 0, 8, SH>(c, d_jobs, p, q, counters, s);
+    return launch_batch_kernel<T, P2, B, 0, 4, SH>(c, d_jobs, p, q, counters, s);
+}
+
+template <typename T, bool P2, int SLAB>
+hipError_t launch_batch_slab(bool shared, int ch, const spr_ctx* c, const BatchJob* d_jobs,
+                             const RayParams& p, const BatchQueues& q, uint32_t* counters,
+                             hipStream_t s) {
+    return shared ? launch_batch_staged<T, P2, SLAB, true>(ch, c, d_jobs, p, q, counters, s)
+                  : launch_batch_staged<T, P2, SLAB, false>(ch, c, d_jobs, p, q, counters, s);
 }
 
 template <typename T, bool P2>
 hipError_t dispatch_batch_typed(const spr_ctx* c, int mode, int ch, const BatchJob* d_jobs,
                                 const RayParams& p, const BatchQueues& q, uint32_t* counters,
                                 hipStream_t s) {
-    if (mode == 1) return launch_batch_kernel<T, P2, 16, 1, 16>(c, d_jobs, p, q, counters, s);
-    if (mode == 2) return launch_batch_kernel<T, P2, 16, 2, 16>(c, d_jobs, p, q, counters, s);
-    if (c->slab_kb == 8) return launch_batch_staged<T, P2, 8192>(ch, c, d_jobs, p, q, counters, s);
-    if (c->slab_kb == 16) return launch_batch_staged<T, P2, 16384>(ch, c, d_jobs, p, q, counters, s);
-    return launch_batch_staged<T, P2, 12288>(ch, c, d_jobs, p, q, counters, s);
+    if (mode == 1) return launch_batch_kernel<T, P2, 16, 1, 16, false>(c, d_jobs, p, q, counters, s);
+    if (mode == 2) return launch_batch_kernel<T, P2, 16, 2, 16, false>(c, d_jobs, p, q, counters, s);
+    const bool sh = c->staging == 1;
+    if (c->slab_kb == 8) return launch_batch_slab<T, P2, 8192>(sh, ch, c, d_jobs, p, q, counters, s);
+    if (c->slab_kb == 16) return launch_batch_slab<T, P2, 16384>(sh, ch, c, d_jobs, p, q, counters, s);
+    return launch_batch_slab<T, P2, 12288>(sh, ch, c, d_jobs, p, q, counters, s);
 }
 
 // nmax: the largest volume dimension over the batch's jobs (sets the staged box length)
@@ -613,7 +640,7 @@ hipError_t dispatch_batch(const spr_ctx* c, const spr_params& prm, int dtype, bo
                           uint32_t* counters, hipStream_t s) {
     const bool nearest = prm.filter == SPR_FILTER_NEAREST;
     const int mode = nearest ? 2 : ((prm.alpha_pow == 0.0f && c->slab_kb > 0) ? 0 : 1);
-    const int ch = staged_chunk(p.nouter, nmax);
+    const int ch = chunk_for(c, p.nouter, nmax);
     switch (dtype) {
         case SPR_DTYPE_F32:
             return pow2 ? dispatch_batch_typed<float, true>(c, mode, ch, d_jobs, p, q, counters, s)
@@ -731,6 +758,15 @@ int spr_set_tuning(spr_ctx* c, int key, int value) {
                 return fail(c, SPR_ERR_INVALID_ARG, "slab size %d KiB (0, 8, 12 or 16)", value);
             c->slab_kb = value;
             return SPR_OK;
+        case SPR_TUNE_STAGING:
+            if (value != 0 && value != 1) return fail(c, SPR_ERR_INVALID_ARG, "staging %d (0 or 1)", value);
+            c->staging = value;
+            return SPR_OK;
+        case SPR_TUNE_CHUNK:
+            if (value != 0 && value != 4 && value != 8 && value != 16)
+                return fail(c, SPR_ERR_INVALID_ARG, "chunk %d (0, 4, 8 or 16)", value);
+            c->chunk = value;
+            return SPR_OK;
         default: return fail(c, SPR_ERR_INVALID_ARG, "unknown tuning key %d", key);
     }
 }
@@ -740,6 +776,8 @@ int spr_get_tuning(spr_ctx* c, int key, int* value) {
     switch (key) {
         case SPR_TUNE_TILE_MAP: *value = c->tile_map; return SPR_OK;
         case SPR_TUNE_SLAB_KB: *value = c->slab_kb; return SPR_OK;
+        case SPR_TUNE_STAGING: *value = c->staging; return SPR_OK;
+        case SPR_TUNE_CHUNK: *value = c->chunk; return SPR_OK;
         default: return fail(c, SPR_ERR_INVALID_ARG, "unknown tuning key %d", key);
     }
 }

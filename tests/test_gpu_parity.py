@@ -152,9 +152,10 @@ def test_staged_mip_matches_global(VR, dtype, shape, pose):
          np.dot(mat4_translate(0, 0, -1.2), mat4_rotation(.8, .2, 1, .1)),  # eye close to the box
          mat4_identity()][pose]                                          # eye inside the box
     frames = {}
-    for slab in (0, 8, 12, 16):
+    for slab, staging in ((0, 0), (8, 0), (12, 0), (16, 0), (8, 1), (12, 1)):
         rend = VR((130, 97))
         rend._ctx.set_tuning("slab_kb", slab)
+        rend._ctx.set_tuning("staging", staging)
         rend.set_max_steps(160)
         rend.set_data(vol)
         rend.set_modelView(M)
@@ -163,11 +164,11 @@ def test_staged_mip_matches_global(VR, dtype, shape, pose):
         for part in range(3):
             rend.render(numParts=3, currentPart=part)
         res.append(rend.output.copy())
-        frames[slab] = res
-    for slab in (8, 12, 16):
-        for a, b in zip(frames[0], frames[slab]):
+        frames[slab, staging] = res
+    for key in frames:
+        for a, b in zip(frames[0, 0], frames[key]):
             np.testing.assert_array_equal(a, b)
-    assert (frames[0][0] > 0).sum() > 200
+    assert (frames[0, 0][0] > 0).sum() > 200
 
 
 @pytest.mark.parametrize("dtype", [np.uint16, np.uint8, np.float32])
@@ -179,17 +180,21 @@ def test_staged_mip_long_steps(VR, dtype, max_steps):
                             background=5., poisson=False, read_noise=0., dtype=dtype, seed=7)
     M = np.dot(mat4_translate(0, .1, -3.5), mat4_rotation(.7, .3, 1, .2))
     frames = {}
-    for slab in (0, 12):
+    for slab, staging, chunk in ((0, 0, 0), (12, 0, 0), (12, 1, 0), (12, 0, 16), (12, 1, 4),
+                                 (8, 0, 8)):
         rend = VR((120, 104))
         rend._ctx.set_tuning("slab_kb", slab)
+        rend._ctx.set_tuning("staging", staging)
+        rend._ctx.set_tuning("chunk", chunk)
         rend.set_max_steps(max_steps)
         rend.set_data(vol)
         rend.set_modelView(M)
         rend.render(minVal=float(vol.min()), maxVal=float(vol.max()))
-        frames[slab] = (rend.output.copy(), rend.output_alpha.copy())
-    np.testing.assert_array_equal(frames[0][0], frames[12][0])
-    np.testing.assert_array_equal(frames[0][1], frames[12][1])
-    assert (frames[0][0] > 0).sum() > 500
+        frames[slab, staging, chunk] = (rend.output.copy(), rend.output_alpha.copy())
+    for key in frames:
+        np.testing.assert_array_equal(frames[0, 0, 0][0], frames[key][0])
+        np.testing.assert_array_equal(frames[0, 0, 0][1], frames[key][1])
+    assert (frames[0, 0, 0][0] > 0).sum() > 500
 
 
 def test_tuning_does_not_change_results(VR):
