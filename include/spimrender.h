@@ -211,6 +211,15 @@ int spr_render_slot(spr_ctx *ctx, int slot, const spr_params *params, int method
                     int current_part, int w, int h, float *d_out, float *d_alpha, float *d_depth,
                     float *d_normals, float *d_occlusion, void *stream);
 
+/* Coarse max projections of two slots of one voxel type and shape (a volume and its
+   spr_volume_add_dilated copy) with the same params, in one launch: d_out holds two w x h
+   planes (slot_a's, then slot_b's), d_alpha slot_a's alpha.  Not the reference's sample
+   sequence: sample k of a ray is at p0 + k dt d (one fma instead of a running sum), which
+   lets four waves share a ray's samples; the last bits of a sample position can differ
+   from spr_render_slot's.  The input of the learned reconstruction (spimlearn.h). */
+int spr_render_coarse_pair(spr_ctx *ctx, int slot_a, int slot_b, const spr_params *params, int w,
+                           int h, float *d_out, float *d_alpha, void *stream);
+
 /* The same batch (num_parts = 1, the spr_set_params state) captured once in a HIP graph: replay it any number of
    times on any stream, rewrite its jobs (same count; waits for the last replay first) and
    destroy it.  Launches are asynchronous. */

@@ -38,7 +38,8 @@ EXPORTS = (
     "spr_set_params", "spr_render", "spr_render_device", "spr_synchronize", "spr_get_stream",
     "spr_count_samples", "spr_volume_add", "spr_volume_add_dilated", "spr_volume_read",
     "spr_volume_release", "spr_volume_info",
-    "spr_render_batch", "spr_render_slot", "spr_batch_graph_create", "spr_batch_graph_update",
+    "spr_render_batch", "spr_render_slot", "spr_render_coarse_pair", "spr_batch_graph_create",
+    "spr_batch_graph_update",
     "spr_batch_graph_launch", "spr_batch_graph_destroy",
 )
 
@@ -158,6 +159,8 @@ def load():
     L.spr_render_slot.argtypes = [c_void_p, c_int, ctypes.POINTER(SprParams), c_int, c_int, c_int,
                                   c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p]
+    L.spr_render_coarse_pair.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(SprParams), c_int,
+                                         c_int, c_void_p, c_void_p, c_void_p]
     L.spr_batch_graph_create.argtypes = [c_void_p, jp, c_int, c_int, c_int,
                                          ctypes.POINTER(c_void_p)]
     L.spr_batch_graph_update.argtypes = [c_void_p, c_void_p, jp, c_int]
@@ -349,6 +352,12 @@ class Context:
         self._check(self.lib.spr_render_slot(self.handle, int(slot), ctypes.byref(params), method,
                                              num_parts, current_part, w, h, d_out, d_alpha, d_depth,
                                              d_normals, d_occlusion, stream))
+
+    def render_coarse_pair(self, slot_a, slot_b, params, w, h, d_out, d_alpha, stream=None):
+        """Two coarse planes (slot_a, slot_b) into d_out (2, h, w); see spr_render_coarse_pair."""
+        self._check(self.lib.spr_render_coarse_pair(self.handle, int(slot_a), int(slot_b),
+                                                    ctypes.byref(params), w, h, d_out, d_alpha,
+                                                    stream))
 
     def batch_graph(self, jobs, w, h):
         return BatchGraph(self, jobs, w, h)

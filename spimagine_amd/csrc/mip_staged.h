@@ -464,3 +464,138 @@ __global__ __launch_bounds__(256) void k_max_project_batch(const BatchJob* __res
 }
 
 }  // namespace spr
+
+namespace spr {
+
+// ---------------------------------------------------------------------------------
+// Coarse max projection of two volumes (the learned reconstruction's input planes).
+// Not the reference's sample sequence: sample k of a ray sits at p0 + k d (one fma, no
+// running sum), so the samples of a ray can be split between waves.  A workgroup serves
+// one 8x8 pixel tile of one volume (blockIdx.y); wave w marches samples
+// [w L, (w+1) L) of all 64 rays (L = a quarter of the (nouter+1)*16 samples) with the
+// per-wave LDS staging of the exact kernels, and the four partial maxima meet in LDS.
+// Output: out[v][y][x] and alpha[y][x] (alpha from volume 0), transfer as
+// volume_kernel.cl:159-170.
+template <typename T, bool POW2, int SLAB_BYTES, int CH>
+__global__ __launch_bounds__(256) void k_coarse_mip_pair(PaddedVolume<T, POW2> va,
+                                                         PaddedVolume<T, POW2> vb, RayParams p,
+                                                         float* __restrict__ out,
+                                                         float* __restrict__ alpha) {
+    constexpr bool kShort = !std::is_same<T, float>::value;
+    constexpr uint32_t E = 16u / sizeof(T);
+    constexpr uint32_t TS = sizeof(T);
+    __shared__ Unit16 lds[4 * SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
+    __shared__ float red[4][64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + wave * SLAB_BYTES;
+    const PaddedVolume<T, POW2> vol = blockIdx.y == 0 ? va : vb;
+    float* outv = out + (size_t)blockIdx.y * p.W * p.H;
+    const uint32_t ntx8 = (p.W + 7u) / 8u;
+    const uint32_t x = (blockIdx.x % ntx8) * 8u + (lane & 7u), y = (blockIdx.x / ntx8) * 8u + (lane >> 3);
+    const bool inside = x < p.W && y < p.H;
+    float p0x = 0.5f, p0y = 0.5f, p0z = 0.5f, dx = 0.0f, dy = 0.0f, dz = 0.0f, tnear = 0.0f;
+    bool active = false;
+    if (inside) {
+        F4 orig, direc;
+        ray_setup(p.invP, p.invM, x, y, p.W, p.H, &orig, &direc);
+        float tfar;
+        if (intersect_box(orig, direc, p.box, &tnear, &tfar)) {
+            if (tnear < 0.0f) tnear = 0.0f;
+            const float dt = fabsf(tfar - tnear) / (float)(p.nouter * kLoopUnroll);
+            const F4 delta = scl4(0.5f * dt, direc);
+            const F4 pos = scl4(0.5f, add4(mk4(1.0f + orig.x, 1.0f + orig.y, 1.0f + orig.z,
+                                               1.0f + orig.w),
+                                           scl4(tnear, direc)));
+            p0x = pos.x;
+            p0y = pos.y;
+            p0z = pos.z;
+            dx = delta.x;
+            dy = delta.y;
+            dz = delta.z;
+            active = true;
+        }
+    }
+    const uint64_t act = __ballot(active);
+    float col = 0.0f;
+    if (act != 0) {  // uniform over the wave (every wave of the block sees the same rays)
+        const int src = __builtin_ctzll(act);
+        const float sx = __shfl(p0x, src), sy = __shfl(p0y, src), sz = __shfl(p0z, src);
+        const float sdx = __shfl(dx, src), sdy = __shfl(dy, src), sdz = __shfl(dz, src);
+        p0x = active ? p0x : sx;
+        p0y = active ? p0y : sy;
+        p0z = active ? p0z : sz;
+        dx = active ? dx : sdx;
+        dy = active ? dy : sdy;
+        dz = active ? dz : sdz;
+        const uint32_t PY = vol.py, pitch = vol.px_bytes;
+        const __amdgpu_buffer_rsrc_t rsrc = vol.rsrc();
+        const float mgx = 1e-5f * vol.fnx + 0.01f, mgy = 1e-5f * vol.fny + 0.01f,
+                    mgz = 1e-5f * vol.fnz + 0.01f;
+        static_assert(CH == 4, "a quarter of (nouter+1)*16 samples is a multiple of 4 only");
+        const int L = (p.nouter + 1) * kLoopUnroll / 4;
+        const int k0 = (int)wave * L;
+        for (int c = 0; c < L; c += CH) {
+            const float kc = (float)(k0 + c);
+            const float px = fmaf(kc, dx, p0x), py = fmaf(kc, dy, p0y), pz = fmaf(kc, dz, p0z);
+            uint32_t b0, b1, b2;
+            // the margins cover fma(k + CH - 1, d, p0) against fma(CH - 1, d, fma(k, d, p0))
+            staged_chunk_box<T, POW2, CH>(vol, active, px, py, pz, dx, dy, dz, mgx, mgy, mgz, b0,
+                                          b1, b2);
+            const uint32_t r0 = wave_min_u16x2(b0), r1 = wave_min_u16x2(b1), r2 = wave_min_u16x2(b2);
+            const uint32_t LX = r0 & 0xFFFFu, LY = r0 >> 16, LZ = r1 & 0xFFFFu;
+            const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
+                           HZ = 0xFFFFu - (r2 >> 16);
+            const uint32_t lox = LX & ~(E - 1u);
+            const uint32_t SX = (HX + 1u - lox + E - 1u) & ~(E - 1u);
+            const uint32_t SY = HY - LY + 1u, SZ = HZ - LZ + 1u;
+            const uint32_t nu = SX / E;
+            const bool fits = (uint64_t)SX * SY * SZ * TS <= (uint64_t)SLAB_BYTES && nu <= 64u;
+            if (fits) {
+                staged_copy_rows<64>(rsrc, slab, lane, LY, LZ, lox * TS, nu, SX * TS, SY, SZ, PY, pitch);
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                const uint32_t rowb = SX * TS, szb = rowb * SY;
+                const float fsy = (float)rowb, fsz = (float)szb;
+                const float oy = 1.0f - (float)LY, oz = 1.0f - (float)LZ;
+                const float ox = (1.0f - (float)lox) * (float)TS;
+#pragma unroll
+                for (int j = 0; j < CH; ++j) {
+                    const float kj = kc + (float)j;
+                    float cx, cy, cz, ax, ay, az;
+                    vol.locate_f(fmaf(kj, dx, p0x), fmaf(kj, dy, p0y), fmaf(kj, dz, p0z), &cx, &cy,
+                                 &cz, &ax, &ay, &az);
+                    const uint32_t q = (uint32_t)fmaf(cz + oz, fsz, fmaf(cy + oy, fsy, fmaf(cx, (float)TS, ox)));
+                    float v000, v100, v010, v110, v001, v101, v011, v111;
+                    lds_pair(slab, q, v000, v100, T());
+                    lds_pair(slab, q + rowb, v010, v110, T());
+                    lds_pair(slab, q + szb, v001, v101, T());
+                    lds_pair(slab, q + rowb + szb, v011, v111, T());
+                    col = fmaxf(col, trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az));
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            } else {
+#pragma unroll
+                for (int j = 0; j < CH; ++j) {
+                    const float kj = kc + (float)j;
+                    col = fmaxf(col, vol.linear(fmaf(kj, dx, p0x), fmaf(kj, dy, p0y), fmaf(kj, dz, p0z)));
+                }
+            }
+        }
+    }
+    red[wave][lane] = col;
+    __syncthreads();
+    if (wave != 0 || !inside) return;
+    const uint32_t o = x + p.W * y;
+    if (!active) {
+        outv[o] = 0.0f;
+        if (blockIdx.y == 0) alpha[o] = kShort ? 0.0f : -1.0f;
+        return;
+    }
+    col = fmaxf(fmaxf(red[0][lane], red[1][lane]), fmaxf(red[2][lane], red[3][lane]));
+    col = (p.max_val == 0.0f) ? col : (col - p.min_val) / (p.max_val - p.min_val);
+    outv[o] = apply_gamma(col, p.gamma);
+    if (blockIdx.y == 0) alpha[o] = kShort ? tnear : 1.0f;
+}
+
+}  // namespace spr

@@ -1038,6 +1038,60 @@ int spr_render_batch(spr_ctx* c, const spr_params* params, const spr_job* jobs, 
     return SPR_OK;
 }
 
+int spr_render_coarse_pair(spr_ctx* c, int slot_a, int slot_b, const spr_params* params, int w,
+                           int h, float* d_out, float* d_alpha, void* stream) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    const VolSlot* a = slot_of(c, slot_a);
+    const VolSlot* b = slot_of(c, slot_b);
+    if (!a || !b) return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d or %d", slot_a, slot_b);
+    if (!params) return fail(c, SPR_ERR_INVALID_ARG, "NULL params");
+    int rc = check_params(c, *params);
+    if (rc) return rc;
+    rc = check_parts(c, *params, 1);
+    if (rc) return rc;
+    if (a->layout != SPR_LAYOUT_PADDED || b->layout != SPR_LAYOUT_PADDED || a->dtype != b->dtype ||
+        a->nx != b->nx || a->ny != b->ny || a->nz != b->nz)
+        return fail(c, SPR_ERR_INVALID_ARG, "coarse pairs need two padded volumes of one type and shape");
+    if (w <= 0 || h <= 0 || w > 32768 || h > 32768)
+        return fail(c, SPR_ERR_INVALID_ARG, "invalid output size %d x %d", w, h);
+    if (!d_out || !d_alpha) return fail(c, SPR_ERR_INVALID_ARG, "NULL output");
+    DeviceGuard g(c->device);
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    RayParams p = make_params(*params, c->tile_map, w, h);
+    p.nouter = params->max_steps / kLoopUnroll;
+    p.current_part = 0;
+    const dim3 grid((unsigned)(((w + 7) / 8) * ((h + 7) / 8)), 2), block(256);
+    const bool p2 = slot_pow2(*a);
+    switch (a->dtype) {
+        case SPR_DTYPE_F32:
+            if (p2)
+                hipLaunchKernelGGL((k_coarse_mip_pair<float, true, 12288, 4>), grid, block, 0, s,
+                                   padded_volume<float, true>(*a), padded_volume<float, true>(*b), p, d_out, d_alpha);
+            else
+                hipLaunchKernelGGL((k_coarse_mip_pair<float, false, 12288, 4>), grid, block, 0, s,
+                                   padded_volume<float, false>(*a), padded_volume<float, false>(*b), p, d_out, d_alpha);
+            break;
+        case SPR_DTYPE_U16:
+            if (p2)
+                hipLaunchKernelGGL((k_coarse_mip_pair<uint16_t, true, 12288, 4>), grid, block, 0, s,
+                                   padded_volume<uint16_t, true>(*a), padded_volume<uint16_t, true>(*b), p, d_out, d_alpha);
+            else
+                hipLaunchKernelGGL((k_coarse_mip_pair<uint16_t, false, 12288, 4>), grid, block, 0, s,
+                                   padded_volume<uint16_t, false>(*a), padded_volume<uint16_t, false>(*b), p, d_out, d_alpha);
+            break;
+        default:
+            if (p2)
+                hipLaunchKernelGGL((k_coarse_mip_pair<uint8_t, true, 12288, 4>), grid, block, 0, s,
+                                   padded_volume<uint8_t, true>(*a), padded_volume<uint8_t, true>(*b), p, d_out, d_alpha);
+            else
+                hipLaunchKernelGGL((k_coarse_mip_pair<uint8_t, false, 12288, 4>), grid, block, 0, s,
+                                   padded_volume<uint8_t, false>(*a), padded_volume<uint8_t, false>(*b), p, d_out, d_alpha);
+            break;
+    }
+    HIPCALL(c, hipGetLastError());
+    return SPR_OK;
+}
+
 int spr_batch_graph_create(spr_ctx* c, const spr_job* jobs, int n, int w, int h,
                            spr_batch_graph** out) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");

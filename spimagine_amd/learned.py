@@ -200,13 +200,12 @@ class LearnedMIP(object):
         return self._bufs[key]
 
     def coarse(self, slot, p, w, h, planes, k=0, stream=None):
-        """Renders the two coarse planes of frame k of `planes` (n, 2, h/2, w/2)."""
+        """Renders the two coarse planes of frame k of `planes` (n, 2, h/2, w/2) in one
+        launch (spr_render_coarse_pair; the coarse alpha lands in the "alpha" buffer)."""
         dil = self.attach(slot)
         scratch = self._buffer(("alpha", w, h), (h // 2, w // 2))
-        q = coarse_params(p)
-        for j, s in enumerate((slot, dil)):
-            self.ctx.render_slot(s, q, _lib.METHOD_MAX_PROJECT, w // 2, h // 2,
-                                 planes[k, j].data_ptr(), scratch.data_ptr(), stream=stream)
+        self.ctx.render_coarse_pair(slot, dil, coarse_params(p), w // 2, h // 2,
+                                    planes[k].data_ptr(), scratch.data_ptr(), stream=stream)
 
     def render(self, slot, p, w, h, d_out, stream):
         """The learned w x h frame of `slot` with parameters p into device buffer d_out."""

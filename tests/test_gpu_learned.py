@@ -238,3 +238,45 @@ def test_sparse_judge_rays_are_the_full_frame_pixels(torch):
     assert abs(est2 - 10 * np.log10(1. / mse)) < 1e-4
     # learned frames carry the coarse pass's alpha: exact at even pixels
     assert torch.equal(alpha[::2, ::2], fa[::2, ::2])
+
+
+@pytest.mark.parametrize("dtype,shape", [(np.uint16, (64, 80, 72)), (np.float32, (48, 64, 64)),
+                                         (np.uint8, (40, 52, 60))])
+def test_coarse_pair_matches_exact_renders(torch, dtype, shape):
+    """spr_render_coarse_pair (sample k at p0 + k d, rays split over four waves, both slots
+    in one launch) against spr_render_slot on each slot: the sample positions differ only
+    in their last bits, so the planes agree to 1e-4 of the normalised range (the
+    north-star bar); alpha is identical."""
+    amp = (.3, 1.) if dtype == np.float32 else (500., 3000.) if dtype == np.uint16 else (60., 200.)
+    vol = synth.blob_volume(shape, n_blobs=14, sigma=(3., 6.), amp=amp, background=amp[0] / 10,
+                            poisson=False, read_noise=0., dtype=dtype, seed=5)
+    ctx = _lib.Context(0)
+    a = ctx.volume_add(vol)
+    b = ctx.volume_add_dilated(a, 2)
+    from spimagine_amd.utils import mat4_perspective
+    from spimagine_amd.volumerender.volumerender import render_matrices
+    P = mat4_perspective(45, 1., .1, 10)
+    s = torch.cuda.Stream()
+    for M in synth.random_poses(3, seed=4):
+        iP, iM = render_matrices(M, P, shape[::-1])
+        for steps in (64, 128):
+            p = _lib.make_params(iP, iM, (-1, 1, -1, 1, -1, 1), float(vol.min()), float(vol.max()),
+                                 1., 0., .1, 21, 30, steps, _lib.FILTER_LINEAR)
+            W, H = 100, 84
+            pair = torch.full((2, H, W), -5., device="cuda")
+            pa = torch.full((H, W), -5., device="cuda")
+            torch.cuda.synchronize()
+            ctx.render_coarse_pair(a, b, p, W, H, pair.data_ptr(), pa.data_ptr(), stream=s.cuda_stream)
+            refs = []
+            for slot in (a, b):
+                o = torch.empty((H, W), device="cuda")
+                al = torch.empty_like(o)
+                ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W, H, o.data_ptr(), al.data_ptr(),
+                                stream=s.cuda_stream)
+                refs.append((o, al))
+            s.synchronize()
+            for j in range(2):
+                d = (pair[j] - refs[j][0]).abs().max().item()
+                assert d <= 1e-4, (j, steps, d)
+            assert torch.equal(pa, refs[0][1])
+            assert (refs[0][0] > 0).sum().item() > 1000
