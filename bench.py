@@ -516,10 +516,10 @@ def bench_learned(args):
     judged_ms = 1e3 * (time.perf_counter() - t0) / (reps * len(params))
     kept = [k for k, _ in judged[:len(params)]]
     est = [e for _, e in judged[:len(params)]]
-    # matrix-core work of one network pass: 6 layers of 32x32x9 MACs per pixel at h*w,
-    # h*w/4 (x2) and h*w/16 (x2), here h = H/2
+    # matrix-core work of one network pass: 32x32x9 MACs per pixel in 4 layers at h*w,
+    # 3 at h*w/4 and 2 at h*w/16, here h = H/2
     px = (H // 2) * (W // 2)
-    flops = 2 * 9 * 32 * 32 * px * (2 + 2 / 4. + 2 / 16.)
+    flops = 2 * 9 * 32 * 32 * px * (4 + 3 / 4. + 2 / 16.)
     achieved = flops / (net_ms * 1e-3) / 1e12
     ok = lm.enabled[slot]
     w = np.load(args.weights)

@@ -12,16 +12,16 @@
  * the full-resolution frame.  The exact renderer stays the reference and the fallback.
  *
  * The network (all 3x3 convolutions, zero padding 1, C = 32 channels):
- *   a0 = relu(conv_in(x))            x: 2 planes h x w (coarse MIP, coarse dilated MIP)
- *   e0 = relu(conv(a0))              h   x w
- *   e1 = relu(conv(avgpool2(e0)))    h/2 x w/2
- *   e2 = relu(conv(relu(conv(avgpool2(e1)))))   h/4 x w/4
- *   u1 = relu(conv(up2(e2) + e1))    h/2 x w/2      (up2 = nearest-neighbour x2)
- *   u0 = relu(conv(up2(u1) + e0))    h   x w
- *   y  = pixel_shuffle2(head(u0)) + bilinear_up2(x plane 0)     2h x 2w
+ *   a0  = relu(conv_in(x))             x: 2 planes h x w (coarse MIP, coarse dilated MIP)
+ *   e0  = relu(conv(a0)), e0b = relu(conv(e0))                 h   x w
+ *   e1  = relu(conv(avgpool2(e0b))), e1b = relu(conv(e1))      h/2 x w/2
+ *   e2  = relu(conv(relu(conv(avgpool2(e1b)))))                h/4 x w/4
+ *   u1  = relu(conv(up2(e2) + e1b))                            h/2 x w/2  (up2: nearest x2)
+ *   u0  = relu(conv(up2(u1) + e0b)), u0b = relu(conv(u0))      h   x w
+ *   y   = pixel_shuffle2(head(u0b)) + bilinear_up2(x plane 0)  2h x 2w
  * Parameters: one float32 array of spl_param_count() values, in this order:
- *   conv_in W[9][2][32], b[32];  six 32->32 layers (e0, e1, e2a, e2, u1, u0) W[9][32][32],
- *   b[32];  head W[9][32][4], b[4];  tap index t = 3 (dy + 1) + (dx + 1),
+ *   conv_in W[9][2][32], b[32];  nine 32->32 layers (e0, e0b, e1, e1b, e2a, e2, u1, u0,
+ *   u0b) W[9][32][32], b[32];  head W[9][32][4], b[4];  tap index t = 3 (dy + 1) + (dx + 1),
  *   out(y, x, co) = b[co] + sum_t sum_ci in(y + dy, x + dx, ci) W[t][ci][co].
  * Activations are stored as bf16 (channels fastest), gradients as float32; the master
  * weights and the Adam state are float32.
@@ -44,7 +44,7 @@ extern "C" {
 typedef struct spl_net spl_net;
 
 int spl_abi_version(void);
-/* Number of float32 parameters (57252). */
+/* Number of float32 parameters (84,996). */
 int spl_param_count(void);
 
 /* A network on a device, parameters zero until spl_set_params. */

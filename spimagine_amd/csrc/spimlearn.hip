@@ -27,14 +27,15 @@
 namespace {
 
 constexpr int C = 32;
-constexpr int kLayers = 6;                    // 32->32 layers: e0, e1, e2a, e2, u1, u0
+// 32->32 layers in parameter order
+enum Layer { kE0, kE0b, kE1, kE1b, kE2a, kE2, kU1, kU0, kU0b, kLayers };
 constexpr int kInW = 9 * 2 * C, kInParams = kInW + C;
 constexpr int kL = 9 * C * C, kLParams = kL + C;
 constexpr int kHeadW = 9 * C * 4, kHeadParams = kHeadW + 4;
 constexpr int kOffIn = 0;
 constexpr int kOffL0 = kInParams;
 constexpr int kOffHead = kOffL0 + kLayers * kLParams;
-constexpr int kParams = kOffHead + kHeadParams;  // 57252
+constexpr int kParams = kOffHead + kHeadParams;  // 84,996
 constexpr int kFragPerLayer = 18 * 64;           // uint4 fragments per layer
 constexpr int kMaxWgradBlocks = 512;
 
@@ -495,8 +496,9 @@ struct spl_net {
     int adam_t = 0;
     // workspace for (n, h, w)
     size_t cap = 0;  // n h w the buffers hold
-    uint16_t *a0 = nullptr, *e0 = nullptr, *s0 = nullptr, *u0 = nullptr;  // h x w
-    uint16_t *p1 = nullptr, *e1 = nullptr, *s1 = nullptr, *u1 = nullptr;  // h/2
+    uint16_t *a0 = nullptr, *e0 = nullptr, *e0b = nullptr, *s0 = nullptr, *u0 = nullptr,
+             *u0b = nullptr;                                                // h x w
+    uint16_t *p1 = nullptr, *e1 = nullptr, *e1b = nullptr, *s1 = nullptr, *u1 = nullptr;  // h/2
     uint16_t *p2 = nullptr, *e2a = nullptr, *e2 = nullptr;                // h/4
     float *gH0 = nullptr, *gH1 = nullptr, *gM0 = nullptr, *gM1 = nullptr, *gQ0 = nullptr,
           *gQ1 = nullptr, *dr = nullptr, *y = nullptr;
@@ -541,11 +543,13 @@ struct DeviceGuard {
 };
 
 void free_work(spl_net* t) {
-    void* bufs[] = {t->a0, t->e0, t->s0, t->u0, t->p1, t->e1, t->s1, t->u1, t->p2, t->e2a, t->e2,
-                    t->gH0, t->gH1, t->gM0, t->gM1, t->gQ0, t->gQ1, t->dr, t->y, t->lpart};
+    void* bufs[] = {t->a0,  t->e0,  t->e0b, t->s0,  t->u0,  t->u0b, t->p1,  t->e1,
+                    t->e1b, t->s1,  t->u1,  t->p2,  t->e2a, t->e2,  t->gH0, t->gH1,
+                    t->gM0, t->gM1, t->gQ0, t->gQ1, t->dr,  t->y,   t->lpart};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
-    t->a0 = t->e0 = t->s0 = t->u0 = t->p1 = t->e1 = t->s1 = t->u1 = t->p2 = t->e2a = t->e2 = nullptr;
+    t->a0 = t->e0 = t->e0b = t->s0 = t->u0 = t->u0b = nullptr;
+    t->p1 = t->e1 = t->e1b = t->s1 = t->u1 = t->p2 = t->e2a = t->e2 = nullptr;
     t->gH0 = t->gH1 = t->gM0 = t->gM1 = t->gQ0 = t->gQ1 = t->dr = t->y = t->lpart = nullptr;
     t->cap = 0;
     t->lpart_n = 0;
@@ -569,7 +573,9 @@ int ensure_work(spl_net* t, int n, int h, int w) {
         size_t bytes;
     } plan[] = {
         {(void**)&t->a0, b16},      {(void**)&t->e0, b16},      {(void**)&t->s0, b16},
-        {(void**)&t->u0, b16},      {(void**)&t->p1, b16 / 4},  {(void**)&t->e1, b16 / 4},
+        {(void**)&t->u0, b16},      {(void**)&t->e0b, b16},     {(void**)&t->u0b, b16},
+        {(void**)&t->e1b, b16 / 4},
+        {(void**)&t->p1, b16 / 4},  {(void**)&t->e1, b16 / 4},
         {(void**)&t->s1, b16 / 4},  {(void**)&t->u1, b16 / 4},  {(void**)&t->p2, b16 / 16},
         {(void**)&t->e2a, b16 / 16}, {(void**)&t->e2, b16 / 16}, {(void**)&t->gH0, f32},
         {(void**)&t->gH1, f32},     {(void**)&t->gM0, f32 / 4}, {(void**)&t->gM1, f32 / 4},
@@ -614,52 +620,66 @@ void wgrad(spl_net* t, const void* x, const float* g, const uint16_t* mask, floa
 void forward(spl_net* t, const float* x, int n, int h, int w, float* y, hipStream_t s) {
     const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
     hipLaunchKernelGGL(k_conv_in, grid16(n, h, w), dim3(256), 0, s, x, t->params, t->a0, h, w);
-    conv32_fwd(t, 0, t->a0, t->e0, n, h, w, s);
-    hipLaunchKernelGGL(k_pool2, dim3(blocks_for((size_t)n * h2 * w2 * 4)), dim3(256), 0, s, t->e0, t->p1, n, h, w);
-    conv32_fwd(t, 1, t->p1, t->e1, n, h2, w2, s);
-    hipLaunchKernelGGL(k_pool2, dim3(blocks_for((size_t)n * h4 * w4 * 4)), dim3(256), 0, s, t->e1, t->p2, n, h2, w2);
-    conv32_fwd(t, 2, t->p2, t->e2a, n, h4, w4, s);
-    conv32_fwd(t, 3, t->e2a, t->e2, n, h4, w4, s);
-    hipLaunchKernelGGL(k_up2_add, dim3(blocks_for((size_t)n * h2 * w2 * 4)), dim3(256), 0, s, t->e2, t->e1, t->s1, n, h2, w2);
-    conv32_fwd(t, 4, t->s1, t->u1, n, h2, w2, s);
-    hipLaunchKernelGGL(k_up2_add, dim3(blocks_for((size_t)n * h * w * 4)), dim3(256), 0, s, t->u1, t->e0, t->s0, n, h, w);
-    conv32_fwd(t, 5, t->s0, t->u0, n, h, w, s);
-    hipLaunchKernelGGL(k_head, grid16(n, h, w), dim3(256), 0, s, t->u0, x, t->params, y, h, w);
+    conv32_fwd(t, kE0, t->a0, t->e0, n, h, w, s);
+    conv32_fwd(t, kE0b, t->e0, t->e0b, n, h, w, s);
+    hipLaunchKernelGGL(k_pool2, dim3(blocks_for((size_t)n * h2 * w2 * 4)), dim3(256), 0, s, t->e0b, t->p1, n, h, w);
+    conv32_fwd(t, kE1, t->p1, t->e1, n, h2, w2, s);
+    conv32_fwd(t, kE1b, t->e1, t->e1b, n, h2, w2, s);
+    hipLaunchKernelGGL(k_pool2, dim3(blocks_for((size_t)n * h4 * w4 * 4)), dim3(256), 0, s, t->e1b, t->p2, n, h2, w2);
+    conv32_fwd(t, kE2a, t->p2, t->e2a, n, h4, w4, s);
+    conv32_fwd(t, kE2, t->e2a, t->e2, n, h4, w4, s);
+    hipLaunchKernelGGL(k_up2_add, dim3(blocks_for((size_t)n * h2 * w2 * 4)), dim3(256), 0, s, t->e2, t->e1b, t->s1, n, h2, w2);
+    conv32_fwd(t, kU1, t->s1, t->u1, n, h2, w2, s);
+    hipLaunchKernelGGL(k_up2_add, dim3(blocks_for((size_t)n * h * w * 4)), dim3(256), 0, s, t->u1, t->e0b, t->s0, n, h, w);
+    conv32_fwd(t, kU0, t->s0, t->u0, n, h, w, s);
+    conv32_fwd(t, kU0b, t->u0, t->u0b, n, h, w, s);
+    hipLaunchKernelGGL(k_head, grid16(n, h, w), dim3(256), 0, s, t->u0b, x, t->params, y, h, w);
 }
 
-// loss + backward into t->grad; leaves the per-block squared-error sums in t->lpart
+float* layer_grad(spl_net* t, int L) { return t->grad + kOffL0 + L * kLParams; }
+
+// loss + backward into t->grad; leaves the per-block squared-error sums in t->lpart.
+// Gradient buffers: gH* at h x w, gM* at h/2, gQ* at h/4 (float32, 32 channels).
 void backward(spl_net* t, const float* x, const float* tgt, int n, int h, int w, hipStream_t s) {
     const int h2 = h / 2, w2 = w / 2, h4 = h / 4, w4 = w / 4;
-    float* G = t->grad;
     const float scale = 2.0f / (float)((size_t)n * 4 * h * w);
     hipLaunchKernelGGL(k_loss, dim3(t->lpart_n), dim3(256), 0, s, t->y, tgt, t->dr, t->lpart, n, h, w, scale);
-    // head
-    wgrad<C, 4, false, false>(t, t->u0, t->dr, nullptr, G + kOffHead, n, h, w, s);
+    // head(u0b)
+    wgrad<C, 4, false, false>(t, t->u0b, t->dr, nullptr, t->grad + kOffHead, n, h, w, s);
     hipLaunchKernelGGL(k_head_dgrad, grid16(n, h, w), dim3(256), 0, s, t->dr, t->params, t->gH0, h, w);
-    // u0 = relu(conv5(s0)), s0 = up2(u1) + e0
-    wgrad<C, C, false, true>(t, t->s0, t->gH0, t->u0, G + kOffL0 + 5 * kLParams, n, h, w, s);
-    conv32_dgrad(t, 5, t->gH0, t->u0, t->gH1, n, h, w, s);  // d s0 (and the e0 share)
-    hipLaunchKernelGGL(k_up2_bwd, dim3(blocks_for((size_t)n * h2 * w2 * 8)), dim3(256), 0, s, t->gH1, t->gM0, n, h, w);
-    // u1 = relu(conv4(s1)), s1 = up2(e2) + e1
-    wgrad<C, C, false, true>(t, t->s1, t->gM0, t->u1, G + kOffL0 + 4 * kLParams, n, h2, w2, s);
-    conv32_dgrad(t, 4, t->gM0, t->u1, t->gM1, n, h2, w2, s);  // d s1 (and the e1 share)
+    // u0b = relu(conv(u0))
+    wgrad<C, C, false, true>(t, t->u0, t->gH0, t->u0b, layer_grad(t, kU0b), n, h, w, s);
+    conv32_dgrad(t, kU0b, t->gH0, t->u0b, t->gH1, n, h, w, s);  // d u0
+    // u0 = relu(conv(s0)), s0 = up2(u1) + e0b
+    wgrad<C, C, false, true>(t, t->s0, t->gH1, t->u0, layer_grad(t, kU0), n, h, w, s);
+    conv32_dgrad(t, kU0, t->gH1, t->u0, t->gH0, n, h, w, s);  // d s0 (= the e0b share)
+    hipLaunchKernelGGL(k_up2_bwd, dim3(blocks_for((size_t)n * h2 * w2 * 8)), dim3(256), 0, s, t->gH0, t->gM0, n, h, w);
+    // u1 = relu(conv(s1)), s1 = up2(e2) + e1b
+    wgrad<C, C, false, true>(t, t->s1, t->gM0, t->u1, layer_grad(t, kU1), n, h2, w2, s);
+    conv32_dgrad(t, kU1, t->gM0, t->u1, t->gM1, n, h2, w2, s);  // d s1 (= the e1b share)
     hipLaunchKernelGGL(k_up2_bwd, dim3(blocks_for((size_t)n * h4 * w4 * 8)), dim3(256), 0, s, t->gM1, t->gQ0, n, h2, w2);
-    // e2 = relu(conv3(e2a))
-    wgrad<C, C, false, true>(t, t->e2a, t->gQ0, t->e2, G + kOffL0 + 3 * kLParams, n, h4, w4, s);
-    conv32_dgrad(t, 3, t->gQ0, t->e2, t->gQ1, n, h4, w4, s);
-    // e2a = relu(conv2(p2))
-    wgrad<C, C, false, true>(t, t->p2, t->gQ1, t->e2a, G + kOffL0 + 2 * kLParams, n, h4, w4, s);
-    conv32_dgrad(t, 2, t->gQ1, t->e2a, t->gQ0, n, h4, w4, s);  // d p2
+    // e2 = relu(conv(e2a))
+    wgrad<C, C, false, true>(t, t->e2a, t->gQ0, t->e2, layer_grad(t, kE2), n, h4, w4, s);
+    conv32_dgrad(t, kE2, t->gQ0, t->e2, t->gQ1, n, h4, w4, s);
+    // e2a = relu(conv(p2)), p2 = pool(e1b)
+    wgrad<C, C, false, true>(t, t->p2, t->gQ1, t->e2a, layer_grad(t, kE2a), n, h4, w4, s);
+    conv32_dgrad(t, kE2a, t->gQ1, t->e2a, t->gQ0, n, h4, w4, s);  // d p2
     hipLaunchKernelGGL(k_pool2_bwd, dim3(blocks_for((size_t)n * h2 * w2 * 8)), dim3(256), 0, s, t->gQ0, t->gM1, n, h2, w2);
-    // e1 = relu(conv1(p1))
-    wgrad<C, C, false, true>(t, t->p1, t->gM1, t->e1, G + kOffL0 + 1 * kLParams, n, h2, w2, s);
-    conv32_dgrad(t, 1, t->gM1, t->e1, t->gM0, n, h2, w2, s);  // d p1
-    hipLaunchKernelGGL(k_pool2_bwd, dim3(blocks_for((size_t)n * h * w * 8)), dim3(256), 0, s, t->gM0, t->gH1, n, h, w);
-    // e0 = relu(conv0(a0))
-    wgrad<C, C, false, true>(t, t->a0, t->gH1, t->e0, G + kOffL0, n, h, w, s);
-    conv32_dgrad(t, 0, t->gH1, t->e0, t->gH0, n, h, w, s);  // d a0
+    // e1b = relu(conv(e1))
+    wgrad<C, C, false, true>(t, t->e1, t->gM1, t->e1b, layer_grad(t, kE1b), n, h2, w2, s);
+    conv32_dgrad(t, kE1b, t->gM1, t->e1b, t->gM0, n, h2, w2, s);  // d e1
+    // e1 = relu(conv(p1)), p1 = pool(e0b)
+    wgrad<C, C, false, true>(t, t->p1, t->gM0, t->e1, layer_grad(t, kE1), n, h2, w2, s);
+    conv32_dgrad(t, kE1, t->gM0, t->e1, t->gM1, n, h2, w2, s);  // d p1
+    hipLaunchKernelGGL(k_pool2_bwd, dim3(blocks_for((size_t)n * h * w * 8)), dim3(256), 0, s, t->gM1, t->gH0, n, h, w);
+    // e0b = relu(conv(e0))
+    wgrad<C, C, false, true>(t, t->e0, t->gH0, t->e0b, layer_grad(t, kE0b), n, h, w, s);
+    conv32_dgrad(t, kE0b, t->gH0, t->e0b, t->gH1, n, h, w, s);  // d e0
+    // e0 = relu(conv(a0))
+    wgrad<C, C, false, true>(t, t->a0, t->gH1, t->e0, layer_grad(t, kE0), n, h, w, s);
+    conv32_dgrad(t, kE0, t->gH1, t->e0, t->gH0, n, h, w, s);  // d a0
     // a0 = relu(conv_in(x))
-    wgrad<2, C, true, true>(t, x, t->gH0, t->a0, G + kOffIn, n, h, w, s);
+    wgrad<2, C, true, true>(t, x, t->gH0, t->a0, t->grad + kOffIn, n, h, w, s);
 }
 
 int read_sse(spl_net* t, double* sse, hipStream_t s) {

@@ -26,7 +26,7 @@ WEIGHTS_PATH = os.path.join(HERE, "weights", "mip_x2_q4.npz")
 EXPORTS = ("spl_abi_version", "spl_param_count", "spl_create", "spl_destroy", "spl_last_error",
            "spl_set_params", "spl_get_params", "spl_forward", "spl_gradients", "spl_train_step")
 C = 32
-LAYER_NAMES = ("e0", "e1", "e2a", "e2", "u1", "u0")
+LAYER_NAMES = ("e0", "e0b", "e1", "e1b", "e2a", "e2", "u1", "u0", "u0b")
 PSNR_THRESHOLD = 40.0
 
 _lib_learn = None

@@ -74,7 +74,7 @@ def test_learn_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(L, name), name
     assert L.spl_abi_version() == 1
-    assert L.spl_param_count() == learned.param_count() == 57252
+    assert L.spl_param_count() == learned.param_count() == 84996
 
 
 def test_learn_errors_without_network():

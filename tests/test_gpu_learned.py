@@ -111,16 +111,19 @@ def ref_forward_t(torch, flat, x):
 
     a0 = bf(F.relu(conv(x, p["in.w"], p["in.b"], False)))
     e0 = bf(F.relu(conv(a0, p["e0.w"], p["e0.b"], True)))
-    p1 = bf(F.avg_pool2d(e0, 2))
+    e0b = bf(F.relu(conv(e0, p["e0b.w"], p["e0b.b"], True)))
+    p1 = bf(F.avg_pool2d(e0b, 2))
     e1 = bf(F.relu(conv(p1, p["e1.w"], p["e1.b"], True)))
-    p2 = bf(F.avg_pool2d(e1, 2))
+    e1b = bf(F.relu(conv(e1, p["e1b.w"], p["e1b.b"], True)))
+    p2 = bf(F.avg_pool2d(e1b, 2))
     e2a = bf(F.relu(conv(p2, p["e2a.w"], p["e2a.b"], True)))
     e2 = bf(F.relu(conv(e2a, p["e2.w"], p["e2.b"], True)))
-    s1 = bf(F.interpolate(e2, scale_factor=2, mode="nearest") + e1)
+    s1 = bf(F.interpolate(e2, scale_factor=2, mode="nearest") + e1b)
     u1 = bf(F.relu(conv(s1, p["u1.w"], p["u1.b"], True)))
-    s0 = bf(F.interpolate(u1, scale_factor=2, mode="nearest") + e0)
+    s0 = bf(F.interpolate(u1, scale_factor=2, mode="nearest") + e0b)
     u0 = bf(F.relu(conv(s0, p["u0.w"], p["u0.b"], True)))
-    r = conv(u0, p["head.w"], p["head.b"], False)
+    u0b = bf(F.relu(conv(u0, p["u0b.w"], p["u0b.b"], True)))
+    r = conv(u0b, p["head.w"], p["head.b"], False)
     base = F.interpolate(x[:, :1], scale_factor=2, mode="bilinear", align_corners=False)
     return (F.pixel_shuffle(r, 2) + base)[:, 0]
 
