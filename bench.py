@@ -467,7 +467,8 @@ def bench_learned(args):
     del u
     slot = ctx.volume_add(dev_ptr=vt.data_ptr(), dtype=np.uint16, shape=shape)
     del vt
-    lm = learned.LearnedMIP(ctx, params=np.load(args.weights)["params"])
+    wparams, winputs = learned.load_weights(args.weights)
+    lm = learned.LearnedMIP(ctx, params=wparams, inputs=winputs)
     lm.attach(slot)
     P = mat4_perspective(45, 1., .1, 10)
     params = []
@@ -531,7 +532,8 @@ def bench_learned(args):
         "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
         "config": {"workload": "MIP 512^3 uint16 (2000 blobs, Poisson + read noise, held-out seed) "
                                "-> 1024^2, max_steps %d, 8 seeded poses; learned = two coarse "
-                               "512^2 renders at max_steps %d + conv-net" % (S, S // 4),
+                               "512^2 renders at max_steps %d (inputs: %s) + conv-net"
+                               % (S, S // 4, lm.inputs),
                    "training_pairs": int(w["pairs"]) if "pairs" in w else None},
         "exact_ms": round(exact_ms, 4), "exact_frames_per_s": round(1e3 / exact_ms, 2),
         "learned_ms": round(learned_ms, 4), "speedup": round(exact_ms / learned_ms, 3),

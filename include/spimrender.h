@@ -61,7 +61,9 @@ extern "C" {
    PADDED  a (nz+2, ny+2, nx+2) copy with replicated edges (clamp-to-edge without
            per-sample clamping), addressed with 32-bit offsets: up to 4 GiB padded
    LINEAR  the (nz, ny, nx) volume as uploaded, any size up to 2^32-1 voxels
-   AUTO    PADDED when it fits, else LINEAR (default) */
+   AUTO    PADDED when it fits (default); above 4 GiB, LINEAR plus a padded copy that only
+           the staged max projection reads (addressed per z-plane); such a slot takes both
+           copies' bytes and is not batchable */
 #define SPR_LAYOUT_AUTO 0
 #define SPR_LAYOUT_LINEAR 1
 #define SPR_LAYOUT_PADDED 2
@@ -71,8 +73,8 @@ extern "C" {
    SPR_TUNE_TILE_MAP  how 16x16 pixel tiles are dealt to workgroups and so to the 8 XCDs:
                       0 = tile rows interleaved over XCDs (default), 1 = one contiguous
                       band per XCD, 2 = row-major
-   SPR_TUNE_SLAB_KB   per-wave LDS slab (KiB) of the staged max projection: 8, 12
-                      (default) or 16; 0 = sample every chunk from global memory
+   SPR_TUNE_SLAB_KB   per-wave LDS slab (KiB) of the staged max projection: 8, 12 or 16
+                      (default); 0 = sample every chunk from global memory
    SPR_TUNE_STAGING   0 = each wave stages its own box (default), 1 = the four waves of a
                       workgroup stage one union box into a slab of 4 x SLAB_KB
    SPR_TUNE_CHUNK     samples per staged box: 0 = from the ray step (default), 4, 8, 16 */
@@ -176,6 +178,11 @@ int spr_volume_add(spr_ctx *ctx, const void *src, int src_on_device, int dtype, 
    the samples a coarse ray step skips (input of the learned reconstruction, spimlearn.h).
    radius in [1, 16]. */
 int spr_volume_add_dilated(spr_ctx *ctx, int src_slot, int radius, int *slot);
+/* A new slot holding the volume of slot src_slot pooled by 2 along each axis
+   (ceil(n/2) voxels): mode 0 = mean of each 2x2x2 block (rounded to the voxel type),
+   1 = max (edges clamped); one level of a mean / max pyramid, computed on the device (the
+   coarse inputs of the learned reconstruction, spimlearn.h). */
+int spr_volume_add_pooled(spr_ctx *ctx, int src_slot, int mode, int *slot);
 /* Copy a slot's volume back to host memory as a (nz,ny,nx) C-order array (bytes must be
    nx ny nz x voxel size); the readback of update_data's image (volumerender.py:294). */
 int spr_volume_read(spr_ctx *ctx, int slot, void *host, size_t bytes);

@@ -36,7 +36,8 @@ EXPORTS = (
     "spr_last_error", "spr_set_layout", "spr_get_layout", "spr_set_tuning", "spr_get_tuning",
     "spr_set_volume", "spr_set_volume_device", "spr_volume_minmax",
     "spr_set_params", "spr_render", "spr_render_device", "spr_synchronize", "spr_get_stream",
-    "spr_count_samples", "spr_volume_add", "spr_volume_add_dilated", "spr_volume_read",
+    "spr_count_samples", "spr_volume_add", "spr_volume_add_dilated", "spr_volume_add_pooled",
+    "spr_volume_read",
     "spr_volume_release", "spr_volume_info",
     "spr_render_batch", "spr_render_slot", "spr_render_coarse_pair", "spr_batch_graph_create",
     "spr_batch_graph_update",
@@ -148,6 +149,7 @@ def load():
     L.spr_volume_add.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
                                  ctypes.POINTER(c_int)]
     L.spr_volume_add_dilated.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(c_int)]
+    L.spr_volume_add_pooled.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(c_int)]
     L.spr_volume_read.argtypes = [c_void_p, c_int, c_void_p, ctypes.c_size_t]
     L.spr_volume_release.argtypes = [c_void_p, c_int]
     L.spr_volume_info.argtypes = [c_void_p, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
@@ -307,6 +309,13 @@ class Context:
         out = ctypes.c_int()
         self._check(self.lib.spr_volume_add_dilated(self.handle, int(slot), int(radius),
                                                     ctypes.byref(out)))
+        return out.value
+
+    def volume_add_pooled(self, slot, mode):
+        """A new slot: `slot` pooled 2x2x2 by mean (mode 0) or max (mode 1) on the device."""
+        out = ctypes.c_int()
+        self._check(self.lib.spr_volume_add_pooled(self.handle, int(slot), int(mode),
+                                                   ctypes.byref(out)))
         return out.value
 
     def volume_read(self, slot):

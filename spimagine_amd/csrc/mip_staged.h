@@ -70,6 +70,28 @@ __device__ __forceinline__ void lds_pair(const uint8_t* slab, uint32_t b, float&
     v1 = d[1];
 }
 
+// The same from a dword-aligned address a and the byte shift of the pair within it (the
+// four corner rows of a sample share the shift: row and plane strides are multiples of 16)
+__device__ __forceinline__ void lds_pair_at(const uint8_t* a, uint32_t sh, float& v0, float& v1,
+                                            uint16_t) {
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(a);
+    const uint32_t w = __builtin_amdgcn_alignbyte(d[1], d[0], sh);
+    v0 = (float)(w & 0xFFFFu);
+    v1 = (float)(w >> 16);
+}
+__device__ __forceinline__ void lds_pair_at(const uint8_t* a, uint32_t sh, float& v0, float& v1,
+                                            uint8_t) {
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(a);
+    const uint32_t w = __builtin_amdgcn_alignbyte(d[1], d[0], sh);
+    v0 = (float)(w & 0xFFu);
+    v1 = (float)((w >> 8) & 0xFFu);
+}
+__device__ __forceinline__ void lds_pair_at(const uint8_t* a, uint32_t, float& v0, float& v1, float) {
+    const float* d = reinterpret_cast<const float*>(a);
+    v0 = d[0];
+    v1 = d[1];
+}
+
 // 16-byte aligned load / store units
 struct alignas(16) Unit16 {
     uint32_t w[4];
@@ -113,8 +135,8 @@ __device__ __forceinline__ bool staged_ray_begin(const RayParams& p, uint32_t x,
 // (0xFFFF - hi.z) << 16 (hi exclusive).  Inactive lanes give all ones (no effect).  The
 // last sample's position is estimated with one fma; mg* covers its distance from the
 // sequentially added position.
-template <typename T, bool POW2, int CH>
-__device__ __forceinline__ void staged_chunk_box(const PaddedVolume<T, POW2>& vol, bool active,
+template <typename V, int CH>
+__device__ __forceinline__ void staged_chunk_box(const V& vol, bool active,
                                                  float px, float py, float pz, float dx, float dy,
                                                  float dz, float mgx, float mgy, float mgz,
                                                  uint32_t& b0, uint32_t& b1, uint32_t& b2) {
@@ -122,10 +144,11 @@ __device__ __forceinline__ void staged_chunk_box(const PaddedVolume<T, POW2>& vo
     if (!active) return;
     const float ex = fmaf((float)(CH - 1), dx, px), ey = fmaf((float)(CH - 1), dy, py),
                 ez = fmaf((float)(CH - 1), dz, pz);
-    const float u0x = texel<POW2>(px, vol.fnx), u0y = texel<POW2>(py, vol.fny),
-                u0z = texel<POW2>(pz, vol.fnz);
-    const float u1x = texel<POW2>(ex, vol.fnx), u1y = texel<POW2>(ey, vol.fny),
-                u1z = texel<POW2>(ez, vol.fnz);
+    constexpr bool P2 = V::kPow2;
+    const float u0x = texel<P2>(px, vol.fnx), u0y = texel<P2>(py, vol.fny),
+                u0z = texel<P2>(pz, vol.fnz);
+    const float u1x = texel<P2>(ex, vol.fnx), u1y = texel<P2>(ey, vol.fny),
+                u1z = texel<P2>(ez, vol.fnz);
     const uint32_t a0x = vol.corner1(floorf(u0x), vol.mx);
     const uint32_t a0y = vol.corner1(floorf(u0y), vol.my);
     const uint32_t a0z = vol.corner1(floorf(u0z), vol.mz);
@@ -178,12 +201,13 @@ __device__ __forceinline__ void staged_copy_rows(__amdgpu_buffer_rsrc_t rsrc, ui
 // CH samples from the slab holding the box (lox, LY, LZ, row bytes rowb, SY rows a plane);
 // slab offset of corner (c+1) relative to the box computed exactly in float (every term is
 // an integer below 2^24)
-template <typename T, bool POW2, int CH>
-__device__ __forceinline__ void staged_sample_slab(const PaddedVolume<T, POW2>& vol,
+template <typename V, int CH>
+__device__ __forceinline__ void staged_sample_slab(const V& vol,
                                                    const uint8_t* slab, uint32_t lox, uint32_t LY,
                                                    uint32_t LZ, uint32_t rowb, uint32_t SY,
                                                    float& px, float& py, float& pz, float dx,
                                                    float dy, float dz, float& col) {
+    using T = typename V::value_type;
     constexpr uint32_t TS = sizeof(T);
     const uint32_t sy = rowb, sz = rowb * SY;
     const float fsy = (float)sy, fsz = (float)sz;
@@ -195,11 +219,13 @@ __device__ __forceinline__ void staged_sample_slab(const PaddedVolume<T, POW2>& 
         vol.locate_f(px, py, pz, &cx, &cy, &cz, &ax, &ay, &az);
         const float bf = fmaf(cz + oz, fsz, fmaf(cy + oy, fsy, fmaf(cx, (float)TS, ox)));
         const uint32_t q = (uint32_t)bf;
+        const uint8_t* a = slab + (q & ~3u);  // slab is 16-byte aligned, sy and sz are
+        const uint32_t sh = q & 3u;            // multiples of 16
         float v000, v100, v010, v110, v001, v101, v011, v111;
-        lds_pair(slab, q, v000, v100, T());
-        lds_pair(slab, q + sy, v010, v110, T());
-        lds_pair(slab, q + sz, v001, v101, T());
-        lds_pair(slab, q + sy + sz, v011, v111, T());
+        lds_pair_at(a, sh, v000, v100, T());
+        lds_pair_at(a + sy, sh, v010, v110, T());
+        lds_pair_at(a + sz, sh, v001, v101, T());
+        lds_pair_at(a + sy + sz, sh, v011, v111, T());
         col = fmaxf(col, trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az));
         px = px + dx;
         py = py + dy;
@@ -207,8 +233,8 @@ __device__ __forceinline__ void staged_sample_slab(const PaddedVolume<T, POW2>& 
     }
 }
 
-template <typename T, bool POW2, int CH>
-__device__ __forceinline__ void staged_sample_global(const PaddedVolume<T, POW2>& vol, float& px,
+template <typename V, int CH>
+__device__ __forceinline__ void staged_sample_global(const V& vol, float& px,
                                                      float& py, float& pz, float dx, float dy,
                                                      float dz, float& col) {
 #pragma unroll
@@ -244,8 +270,8 @@ __device__ __forceinline__ void staged_ray_end(const RayParams& p, uint32_t o, f
 // (SLAB_BYTES, wave-level ordering only).  SHARED = true: the workgroup's four waves
 // stage the union box of its 256 rays into one slab of SLAB_BYTES with all 256 threads,
 // two workgroup barriers per box; s_box holds the four waves' partial boxes.
-template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED>
-__device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol, const RayParams& p,
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE = false>
+__device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE>& vol, const RayParams& p,
                                                 uint32_t x, uint32_t y, bool inside, uint8_t* slab,
                                                 uint32_t* s_box, float* __restrict__ out,
                                                 float* __restrict__ alpha) {
@@ -279,7 +305,6 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
     }
 
     const uint32_t PY = vol.py, pitch = vol.px_bytes;
-    const __amdgpu_buffer_rsrc_t rsrc = vol.rsrc();
     const float mgx = 1e-5f * vol.fnx + 0.01f, mgy = 1e-5f * vol.fny + 0.01f,
                 mgz = 1e-5f * vol.fnz + 0.01f;
     float col = 0.0f;
@@ -288,7 +313,8 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
     const int nchunks = (p.nouter + 1) * (kLoopUnroll / CH);
     for (int i = 0; i < nchunks; ++i) {
         uint32_t b0, b1, b2;
-        staged_chunk_box<T, POW2, CH>(vol, active, px, py, pz, dx, dy, dz, mgx, mgy, mgz, b0, b1, b2);
+        staged_chunk_box<PaddedVolume<T, POW2, WIDE>, CH>(vol, active, px, py, pz, dx, dy, dz, mgx,
+                                                          mgy, mgz, b0, b1, b2);
         uint32_t r0 = wave_min_u16x2(b0), r1 = wave_min_u16x2(b1), r2 = wave_min_u16x2(b2);
         if constexpr (SHARED) {
             const uint32_t w = threadIdx.x >> 6;
@@ -314,9 +340,10 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
         const uint32_t nu = SX / E;                                 // 16-byte units per row
         const uint64_t bytes = (uint64_t)SX * SY * SZ * TS;
         const bool fits = bytes <= (uint64_t)SLAB_BYTES && nu <= NT;
+        // rows addressed from plane LZ on (32-bit offsets whatever the volume's size)
         if (fits)
-            staged_copy_rows<NT>(rsrc, slab, SHARED ? threadIdx.x : lane, LY, LZ, lox * TS, nu,
-                                 SX * TS, SY, SZ, PY, pitch);
+            staged_copy_rows<NT>(vol.rsrc_plane(LZ), slab, SHARED ? threadIdx.x : lane, LY, 0u,
+                                 lox * TS, nu, SX * TS, SY, SZ, PY, pitch);
         if constexpr (SHARED) {
             __syncthreads();
         } else {
@@ -325,10 +352,11 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
         }
         if (live) {
             if (fits)
-                staged_sample_slab<T, POW2, CH>(vol, slab, lox, LY, LZ, SX * TS, SY, px, py, pz, dx,
-                                                dy, dz, col);
+                staged_sample_slab<PaddedVolume<T, POW2, WIDE>, CH>(vol, slab, lox, LY, LZ, SX * TS,
+                                                                    SY, px, py, pz, dx, dy, dz, col);
             else
-                staged_sample_global<T, POW2, CH>(vol, px, py, pz, dx, dy, dz, col);
+                staged_sample_global<PaddedVolume<T, POW2, WIDE>, CH>(vol, px, py, pz, dx, dy, dz,
+                                                                      col);
         }
         if constexpr (!SHARED) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -340,8 +368,8 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2>& vol
 
 // one frame: one workgroup per 16x16 tile (tile map as k_max_project)
 // SLAB_BYTES: per wave (SHARED = false) or per workgroup (SHARED = true)
-template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED>
-__global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2> vol, RayParams p,
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE = false>
+__global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2, WIDE> vol, RayParams p,
                                                             float* __restrict__ out,
                                                             float* __restrict__ alpha) {
     constexpr int kLds = SHARED ? SLAB_BYTES : 4 * SLAB_BYTES;
@@ -350,7 +378,8 @@ __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2
     uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (SHARED ? 0u : (threadIdx.x >> 6) * SLAB_BYTES);
     uint32_t x = 0, y = 0;
     const bool inside = tile_pixel(p, &x, &y);
-    mip_staged_rays<T, POW2, SLAB_BYTES, CH, SHARED>(vol, p, x, y, inside, slab, s_box, out, alpha);
+    mip_staged_rays<T, POW2, SLAB_BYTES, CH, SHARED, WIDE>(vol, p, x, y, inside, slab, s_box, out,
+                                                           alpha);
 }
 
 // Samples per staged box for a frame: the ray advances at most N_a / (16 nouter) voxels
@@ -392,6 +421,7 @@ __device__ __forceinline__ PaddedVolume<T, POW2> job_volume(const BatchJob& j) {
     PaddedVolume<T, POW2> v;
     v.base = j.base;
     v.nbytes = j.nbytes;
+    v.total_bytes = j.nbytes;
     v.px_bytes = j.px_bytes;
     v.py = j.py;
     v.soff_y = j.px_bytes;
@@ -528,7 +558,6 @@ __global__ __launch_bounds__(256) void k_coarse_mip_pair(PaddedVolume<T, POW2> v
         dy = active ? dy : sdy;
         dz = active ? dz : sdz;
         const uint32_t PY = vol.py, pitch = vol.px_bytes;
-        const __amdgpu_buffer_rsrc_t rsrc = vol.rsrc();
         const float mgx = 1e-5f * vol.fnx + 0.01f, mgy = 1e-5f * vol.fny + 0.01f,
                     mgz = 1e-5f * vol.fnz + 0.01f;
         static_assert(CH == 4, "a quarter of (nouter+1)*16 samples is a multiple of 4 only");
@@ -539,8 +568,8 @@ __global__ __launch_bounds__(256) void k_coarse_mip_pair(PaddedVolume<T, POW2> v
             const float px = fmaf(kc, dx, p0x), py = fmaf(kc, dy, p0y), pz = fmaf(kc, dz, p0z);
             uint32_t b0, b1, b2;
             // the margins cover fma(k + CH - 1, d, p0) against fma(CH - 1, d, fma(k, d, p0))
-            staged_chunk_box<T, POW2, CH>(vol, active, px, py, pz, dx, dy, dz, mgx, mgy, mgz, b0,
-                                          b1, b2);
+            staged_chunk_box<PaddedVolume<T, POW2>, CH>(vol, active, px, py, pz, dx, dy, dz, mgx,
+                                                        mgy, mgz, b0, b1, b2);
             const uint32_t r0 = wave_min_u16x2(b0), r1 = wave_min_u16x2(b1), r2 = wave_min_u16x2(b2);
             const uint32_t LX = r0 & 0xFFFFu, LY = r0 >> 16, LZ = r1 & 0xFFFFu;
             const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
@@ -551,7 +580,8 @@ __global__ __launch_bounds__(256) void k_coarse_mip_pair(PaddedVolume<T, POW2> v
             const uint32_t nu = SX / E;
             const bool fits = (uint64_t)SX * SY * SZ * TS <= (uint64_t)SLAB_BYTES && nu <= 64u;
             if (fits) {
-                staged_copy_rows<64>(rsrc, slab, lane, LY, LZ, lox * TS, nu, SX * TS, SY, SZ, PY, pitch);
+                staged_copy_rows<64>(vol.rsrc_plane(LZ), slab, lane, LY, 0u, lox * TS, nu, SX * TS, SY,
+                                     SZ, PY, pitch);
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 const uint32_t rowb = SX * TS, szb = rowb * SY;
@@ -566,10 +596,12 @@ __global__ __launch_bounds__(256) void k_coarse_mip_pair(PaddedVolume<T, POW2> v
                                  &cz, &ax, &ay, &az);
                     const uint32_t q = (uint32_t)fmaf(cz + oz, fsz, fmaf(cy + oy, fsy, fmaf(cx, (float)TS, ox)));
                     float v000, v100, v010, v110, v001, v101, v011, v111;
-                    lds_pair(slab, q, v000, v100, T());
-                    lds_pair(slab, q + rowb, v010, v110, T());
-                    lds_pair(slab, q + szb, v001, v101, T());
-                    lds_pair(slab, q + rowb + szb, v011, v111, T());
+                    const uint8_t* aq = slab + (q & ~3u);
+                    const uint32_t sh = q & 3u;
+                    lds_pair_at(aq, sh, v000, v100, T());
+                    lds_pair_at(aq + rowb, sh, v010, v110, T());
+                    lds_pair_at(aq + szb, sh, v001, v101, T());
+                    lds_pair_at(aq + rowb + szb, sh, v011, v111, T());
                     col = fmaxf(col, trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az));
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");

@@ -13,6 +13,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mip_staged.h"
@@ -31,6 +32,7 @@ struct VolSlot {
     size_t vol_bytes = 0;
     void* d_pad = nullptr;         // PADDED: (nz+2, ny+2, pitch) apron copy
     size_t pad_bytes = 0;
+    bool pad_wide = false;         // LINEAR volume above 4 GiB with a padded copy (d_pad)
     int dtype = -1, nx = 0, ny = 0, nz = 0;
 };
 
@@ -70,7 +72,7 @@ struct spr_ctx {
     float* d_partial = nullptr;
     // launch tuning (spr_set_tuning)
     int tile_map = (int)kTileMapInterleaved;
-    int slab_kb = 12;  // per-wave LDS slab of the staged max projection, 0 = off
+    int slab_kb = 16;  // per-wave LDS slab of the staged max projection, 0 = off
     int staging = 0;   // 0 per-wave boxes, 1 one box per workgroup
     int chunk = 0;     // samples per staged box, 0 = from the ray step
 };
@@ -245,11 +247,12 @@ LinearVolume<T> linear_volume(const VolSlot& s) {
     return v;
 }
 
-template <typename T, bool POW2>
-PaddedVolume<T, POW2> padded_volume(const VolSlot& s) {
-    PaddedVolume<T, POW2> v;
+template <typename T, bool POW2, bool WIDE = false>
+PaddedVolume<T, POW2, WIDE> padded_volume(const VolSlot& s) {
+    PaddedVolume<T, POW2, WIDE> v;
     v.base = s.d_pad;
-    v.nbytes = (uint32_t)s.pad_bytes;
+    v.nbytes = WIDE ? 0xFFFFFFFFu : (uint32_t)s.pad_bytes;
+    v.total_bytes = s.pad_bytes;
     const uint32_t PY = (uint32_t)s.ny + 2;
     v.px_bytes = padded_pitch<T>(s.nx) * (uint32_t)sizeof(T);
     v.py = PY;
@@ -332,6 +335,38 @@ int chunk_for(const spr_ctx* c, int nouter, int nmax) {
     return c->chunk ? c->chunk : staged_chunk(nouter, nmax);
 }
 
+// volumes above 4 GiB: per-wave 12 KiB slabs only
+template <typename T, bool P2>
+void launch_staged_wide_t(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
+                          float* out, float* alpha) {
+    const PaddedVolume<T, P2, true> v = padded_volume<T, P2, true>(vs);
+    if (ch == 16)
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288, 16, false, true>), grid, dim3(256), 0, s, v, p, out, alpha);
+    else if (ch == 8)
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288, 8, false, true>), grid, dim3(256), 0, s, v, p, out, alpha);
+    else
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288, 4, false, true>), grid, dim3(256), 0, s, v, p, out, alpha);
+}
+
+void launch_staged_wide(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
+                        float* out, float* alpha) {
+    const bool p2 = slot_pow2(vs);
+    switch (vs.dtype) {
+        case SPR_DTYPE_F32:
+            p2 ? launch_staged_wide_t<float, true>(vs, ch, grid, s, p, out, alpha)
+               : launch_staged_wide_t<float, false>(vs, ch, grid, s, p, out, alpha);
+            break;
+        case SPR_DTYPE_U16:
+            p2 ? launch_staged_wide_t<uint16_t, true>(vs, ch, grid, s, p, out, alpha)
+               : launch_staged_wide_t<uint16_t, false>(vs, ch, grid, s, p, out, alpha);
+            break;
+        default:
+            p2 ? launch_staged_wide_t<uint8_t, true>(vs, ch, grid, s, p, out, alpha)
+               : launch_staged_wide_t<uint8_t, false>(vs, ch, grid, s, p, out, alpha);
+            break;
+    }
+}
+
 template <typename T, bool P2, typename... Args>
 void launch_staged(int slab_kb, bool shared, int ch, dim3 grid, hipStream_t s, Args... args) {
     if (ch == 16)
@@ -359,6 +394,12 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
         p.nouter = (prm.max_steps / num_parts) / kLoopUnroll;
         p.current_part = current_part;
         const bool staged = !nearest && prm.alpha_pow == 0.0f && c->slab_kb > 0;
+        if (staged && vs.pad_wide) {
+            const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
+            launch_staged_wide(vs, chunk_for(c, p.nouter, nmax), grid, s, p, out, alpha);
+            HIPCALL(c, hipGetLastError());
+            return SPR_OK;
+        }
         hipError_t e = visit_volume(vs, [&](auto v) {
             using V = decltype(v);
             if constexpr (is_padded<V>::value) {
@@ -445,6 +486,45 @@ __global__ __launch_bounds__(256) void k_max_filter(const T* __restrict__ in, T*
     out[((size_t)z * ny + y) * nx + x] = m;
 }
 
+// out (ceil(nz/2), ceil(ny/2), ceil(nx/2)) = mean (MODE 0, rounded to T for integer
+// types) or max (MODE 1) of each 2x2x2 block of in (coordinates clamped to the volume);
+// in addressed as in k_max_filter
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_pool2_volume(const T* __restrict__ in, T* __restrict__ out,
+                                                      int nx, int ny, int nz, int px, int py, int o) {
+    const int ox = (nx + 1) / 2, oy = (ny + 1) / 2;
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= ox) return;
+    float acc = MODE == 0 ? 0.0f : -INFINITY;
+    for (int k = 0; k < 8; ++k) {
+        const int xx = min(2 * x + (k & 1), nx - 1), yy = min(2 * y + ((k >> 1) & 1), ny - 1),
+                  zz = min(2 * z + (k >> 2), nz - 1);
+        const float v = (float)in[((size_t)(zz + o) * py + yy + o) * px + xx + o];
+        acc = MODE == 0 ? acc + v : fmaxf(acc, v);
+    }
+    if (MODE == 0) acc *= 0.125f;
+    T r;
+    if constexpr (std::is_same<T, float>::value)
+        r = acc;
+    else
+        r = (T)__float2uint_rn(acc);
+    out[((size_t)z * oy + y) * ox + x] = r;
+}
+
+template <typename T>
+void launch_pool2(const VolSlot& vs, void* dst, int mode, hipStream_t s) {
+    const dim3 grid(((vs.nx + 1) / 2 + 255) / 256, (vs.ny + 1) / 2, (vs.nz + 1) / 2), block(256);
+    const bool pad = vs.layout == SPR_LAYOUT_PADDED;
+    const T* src = (const T*)(pad ? vs.d_pad : vs.d_vol);
+    const int px = pad ? (int)padded_pitch<T>(vs.nx) : vs.nx, py = pad ? vs.ny + 2 : vs.ny;
+    if (mode == 0)
+        hipLaunchKernelGGL((k_pool2_volume<T, 0>), grid, block, 0, s, src, (T*)dst, vs.nx, vs.ny, vs.nz,
+                           px, py, pad ? 1 : 0);
+    else
+        hipLaunchKernelGGL((k_pool2_volume<T, 1>), grid, block, 0, s, src, (T*)dst, vs.nx, vs.ny, vs.nz,
+                           px, py, pad ? 1 : 0);
+}
+
 template <typename T>
 void launch_max_filter3(const VolSlot& vs, void* t1, void* t2, int r, hipStream_t s) {
     const dim3 grid((vs.nx + 255) / 256, vs.ny, vs.nz), block(256);
@@ -478,10 +558,15 @@ int fill_slot(spr_ctx* c, VolSlot& vs, const void* src, bool src_on_device, int 
     // the padded layout is addressed with 32-bit buffer offsets built from 24-bit products
     const bool pad_fits = pbytes <= 0xFFFFFFFFull && (size_t)(nz + 2) * (ny + 2) < (1u << 24) &&
                           pitch * es < (1u << 24);
+    // above 4 GiB (AUTO layout) the volume stays linear for every kernel and also gets a
+    // padded copy that the staged max projection addresses per z-plane ("wide")
+    const bool wide = c->layout_req == SPR_LAYOUT_AUTO && !pad_fits &&
+                      (size_t)(nz + 2) * (ny + 2) < (1u << 24) && pitch * es < (1u << 24);
     int layout = c->layout_req;
     if (layout == SPR_LAYOUT_AUTO) layout = pad_fits ? SPR_LAYOUT_PADDED : SPR_LAYOUT_LINEAR;
     if (layout == SPR_LAYOUT_PADDED && !pad_fits)
         return fail(c, SPR_ERR_INVALID_ARG, "padded volume of %zu bytes exceeds 4 GiB", pbytes);
+    const bool want_pad = layout == SPR_LAYOUT_PADDED || wide;
     HIPCALL(c, hipStreamSynchronize(c->stream));
     // a device source may still be in production on another stream (the context's stream
     // does not synchronise with the null stream): wait for all prior work on the device
@@ -489,7 +574,7 @@ int fill_slot(spr_ctx* c, VolSlot& vs, const void* src, bool src_on_device, int 
 
     // linear copy: kept as the resident volume (LINEAR), or staged for padding (host source)
     const bool need_linear = layout == SPR_LAYOUT_LINEAR || !src_on_device;
-    if (vs.d_pad && (layout != SPR_LAYOUT_PADDED || vs.pad_bytes != pbytes)) {
+    if (vs.d_pad && (!want_pad || vs.pad_bytes != pbytes)) {
         HIPCALL(c, hipFree(vs.d_pad));
         vs.d_pad = nullptr;
         vs.pad_bytes = 0;
@@ -508,7 +593,7 @@ int fill_slot(spr_ctx* c, VolSlot& vs, const void* src, bool src_on_device, int 
         HIPCALL(c, hipMemcpyAsync(vs.d_vol, src, bytes,
                                   src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                                   c->stream));
-    if (layout == SPR_LAYOUT_PADDED) {
+    if (want_pad) {
         if (!vs.d_pad) {
             HIPCALL(c, hipMalloc(&vs.d_pad, pbytes));
             vs.pad_bytes = pbytes;
@@ -529,6 +614,7 @@ int fill_slot(spr_ctx* c, VolSlot& vs, const void* src, bool src_on_device, int 
     }
     vs.used = true;
     vs.layout = layout;
+    vs.pad_wide = wide;
     vs.dtype = dtype;
     vs.nx = nx;
     vs.ny = ny;
@@ -840,6 +926,36 @@ int spr_volume_add_dilated(spr_ctx* c, int src_slot, int radius, int* slot) {
         rc = spr_volume_add(c, t1, 1, vs->dtype, vs->nx, vs->ny, vs->nz, slot);
     if (t1) (void)hipFree(t1);
     if (t2) (void)hipFree(t2);
+    return rc;
+}
+
+int spr_volume_add_pooled(spr_ctx* c, int src_slot, int mode, int* slot) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (!slot) return fail(c, SPR_ERR_INVALID_ARG, "NULL slot output");
+    const VolSlot* vs = slot_of(c, src_slot);
+    if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d", src_slot);
+    if (mode != 0 && mode != 1) return fail(c, SPR_ERR_INVALID_ARG, "pooling mode %d (0 mean, 1 max)", mode);
+    DeviceGuard g(c->device);
+    const int ox = (vs->nx + 1) / 2, oy = (vs->ny + 1) / 2, oz = (vs->nz + 1) / 2;
+    const size_t bytes = (size_t)ox * oy * oz * dtype_size(vs->dtype);
+    void* t = nullptr;
+    hipError_t e = hipMalloc(&t, bytes);
+    if (e == hipSuccess) {
+        switch (vs->dtype) {
+            case SPR_DTYPE_F32: launch_pool2<float>(*vs, t, mode, c->stream); break;
+            case SPR_DTYPE_U16: launch_pool2<uint16_t>(*vs, t, mode, c->stream); break;
+            default: launch_pool2<uint8_t>(*vs, t, mode, c->stream); break;
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    int rc = SPR_OK;
+    if (e != hipSuccess)
+        rc = fail(c, e == hipErrorOutOfMemory ? SPR_ERR_OUT_OF_MEMORY : SPR_ERR_HIP, "pooling failed: %s",
+                  hipGetErrorString(e));
+    else
+        rc = spr_volume_add(c, t, 1, vs->dtype, ox, oy, oz, slot);
+    if (t) (void)hipFree(t);
     return rc;
 }
 

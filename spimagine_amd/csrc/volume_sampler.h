@@ -141,15 +141,20 @@ struct PairBits<float> {
 // word 3 of a gfx9-family buffer resource (raw buffer, 32-bit data format)
 constexpr int kBufferWord3 = 0x00020000;
 
-template <typename T, bool POW2>
+// WIDE: the padded volume may exceed 4 GiB, beyond a buffer resource's 32-bit offsets.
+// Staging then addresses rows relative to a resource rebased at the box's first z-plane
+// (rsrc_plane), and linear() gathers with 64-bit flat loads.
+template <typename T, bool POW2, bool WIDE = false>
 struct PaddedVolume {
     using value_type = T;
     static constexpr bool kPow2 = POW2;
+    static constexpr bool kWide = WIDE;
     const void* base;             // padded volume
-    uint32_t nbytes;              // its size in bytes (the buffer's num_records)
+    uint32_t nbytes;              // its size in bytes (the buffer's num_records; !WIDE)
     uint32_t px_bytes;            // row pitch, padded_pitch<T>(nx)*sizeof(T)
     uint32_t py;                  // padded rows per plane, ny+2
     uint32_t soff_y, soff_z, soff_yz;  // byte strides of the y, z and y+z neighbours
+    uint64_t total_bytes;         // size in bytes (any size)
     int nx, ny, nz;
     float fnx, fny, fnz;           // N
     float mx, my, mz;              // N - 1
@@ -158,6 +163,17 @@ struct PaddedVolume {
     __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc() const {
         return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)nbytes,
                                                  kBufferWord3);
+    }
+
+    // raw buffer resource starting at padded plane z (uniform z): offsets of rows from
+    // plane z on, up to 4 GiB past it
+    __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_plane(uint32_t z) const {
+        const uint64_t skip = (uint64_t)z * soff_z;
+        const uint64_t rest = total_bytes - skip;
+        const uint32_t n = rest > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)rest;
+        return __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint8_t*>(static_cast<const uint8_t*>(base) + skip), (short)0, (int)n,
+            kBufferWord3);
     }
 
     __device__ __forceinline__ uint32_t byte_offset(uint32_t ix1, uint32_t iy1, uint32_t iz1) const {
@@ -210,13 +226,31 @@ struct PaddedVolume {
     __device__ __forceinline__ float linear(float px, float py_, float pz) const {
         float cx, cy, cz, ax, ay, az;
         locate_f(px, py_, pz, &cx, &cy, &cz, &ax, &ay, &az);
-        const uint32_t off = byte_offset_f(cx, cy, cz);
-        const __amdgpu_buffer_rsrc_t r = rsrc();
         float v000, v100, v010, v110, v001, v101, v011, v111;
-        PairBits<T>::load(r, off, 0, &v000, &v100);
-        PairBits<T>::load(r, off, soff_y, &v010, &v110);
-        PairBits<T>::load(r, off, soff_z, &v001, &v101);
-        PairBits<T>::load(r, off, soff_yz, &v011, &v111);
+        if constexpr (WIDE) {
+            const uint32_t row = (uint32_t)fmaf(cz + 1.0f, (float)py, cy + 1.0f);
+            const uint32_t xb = (uint32_t)fmaf(cx, (float)sizeof(T), (float)sizeof(T));
+            const T* q = reinterpret_cast<const T*>(static_cast<const uint8_t*>(base) +
+                                                    (uint64_t)row * px_bytes + xb);
+            const T* qy = reinterpret_cast<const T*>(reinterpret_cast<const uint8_t*>(q) + soff_y);
+            const T* qz = reinterpret_cast<const T*>(reinterpret_cast<const uint8_t*>(q) + soff_z);
+            const T* qyz = reinterpret_cast<const T*>(reinterpret_cast<const uint8_t*>(q) + soff_yz);
+            v000 = (float)q[0];
+            v100 = (float)q[1];
+            v010 = (float)qy[0];
+            v110 = (float)qy[1];
+            v001 = (float)qz[0];
+            v101 = (float)qz[1];
+            v011 = (float)qyz[0];
+            v111 = (float)qyz[1];
+        } else {
+            const uint32_t off = byte_offset_f(cx, cy, cz);
+            const __amdgpu_buffer_rsrc_t r = rsrc();
+            PairBits<T>::load(r, off, 0, &v000, &v100);
+            PairBits<T>::load(r, off, soff_y, &v010, &v110);
+            PairBits<T>::load(r, off, soff_z, &v001, &v101);
+            PairBits<T>::load(r, off, soff_yz, &v011, &v111);
+        }
         return trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az);
     }
 
@@ -247,8 +281,8 @@ template <class V>
 struct is_padded {
     static constexpr bool value = false;
 };
-template <typename T, bool POW2>
-struct is_padded<PaddedVolume<T, POW2>> {
+template <typename T, bool POW2, bool WIDE>
+struct is_padded<PaddedVolume<T, POW2, WIDE>> {
     static constexpr bool value = true;
 };
 

@@ -157,6 +157,20 @@ class Net(object):
         return sse.value if want_loss else None
 
 
+# coarse input planes: "dilated" = the volume and its 5^3 max filter; "pooled" = the volume
+# pooled 2x2x2 by mean and by max (a quarter-size pyramid level: a coarse ray step of 4
+# voxels is 2 pooled voxels, so the coarse render keeps the locality of a fine one)
+INPUTS = ("dilated", "pooled")
+INPUTS_DEFAULT = "dilated"
+
+
+def load_weights(path):
+    """(params, inputs) stored by `python -m spimagine_amd.learned`."""
+    w = np.load(path)
+    inputs = str(w["inputs"]) if "inputs" in w else "dilated"
+    return w["params"], inputs
+
+
 def psnr(a, b):
     """PSNR (dB) of a against b for images in [0, 1]."""
     mse = float(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2))
@@ -174,24 +188,34 @@ def coarse_params(p):
 class LearnedMIP(object):
     """Learned max projection over the volumes of one renderer context."""
 
-    def __init__(self, ctx, params=None, device=0, radius=2, threshold=PSNR_THRESHOLD, net=None):
+    def __init__(self, ctx, params=None, device=0, radius=2, threshold=PSNR_THRESHOLD, net=None,
+                 inputs=None):
         import torch
         self.torch = torch
         self.ctx = ctx
         if net is None:
             if params is None:
-                params = np.load(WEIGHTS_PATH)["params"]
+                params, stored = load_weights(WEIGHTS_PATH)
+                inputs = inputs or stored
             net = Net(device, params)
         self.net = net
         self.radius = radius
         self.threshold = threshold
+        if (inputs or INPUTS_DEFAULT) not in INPUTS:
+            raise ValueError("unknown coarse inputs %r" % inputs)
+        self.inputs = inputs or INPUTS_DEFAULT
         self.dilated = {}
         self.enabled = {}   # slot -> validated (min PSNR >= threshold)
         self._bufs = {}
 
     def attach(self, slot):
+        """The two slots whose coarse projections feed the network (created once)."""
         if slot not in self.dilated:
-            self.dilated[slot] = self.ctx.volume_add_dilated(slot, self.radius)
+            if self.inputs == "pooled":
+                self.dilated[slot] = (self.ctx.volume_add_pooled(slot, 0),
+                                      self.ctx.volume_add_pooled(slot, 1))
+            else:
+                self.dilated[slot] = (slot, self.ctx.volume_add_dilated(slot, self.radius))
         return self.dilated[slot]
 
     def _buffer(self, key, shape):
@@ -202,9 +226,9 @@ class LearnedMIP(object):
     def coarse(self, slot, p, w, h, planes, k=0, stream=None):
         """Renders the two coarse planes of frame k of `planes` (n, 2, h/2, w/2) in one
         launch (spr_render_coarse_pair; the coarse alpha lands in the "alpha" buffer)."""
-        dil = self.attach(slot)
+        a, b = self.attach(slot)
         scratch = self._buffer(("alpha", w, h), (h // 2, w // 2))
-        self.ctx.render_coarse_pair(slot, dil, coarse_params(p), w // 2, h // 2,
+        self.ctx.render_coarse_pair(a, b, coarse_params(p), w // 2, h // 2,
                                     planes[k].data_ptr(), scratch.data_ptr(), stream=stream)
 
     def render(self, slot, p, w, h, d_out, stream):
@@ -285,7 +309,7 @@ def training_volumes(n, shape=(512, 512, 512), seed0=1):
 
 
 def train(ctx, slots, ranges, steps=2500, batch=8, frame=1024, max_steps=896, lr=2e-3,
-          seed=0, params=None, log=None, radius=2):
+          seed=0, params=None, log=None, radius=2, inputs=INPUTS_DEFAULT):
     """Trains the network on coarse/fine pairs rendered from random poses of `slots`
     (ranges: per-slot (min, max) used to normalise).  Returns (params, history)."""
     import torch
@@ -293,7 +317,7 @@ def train(ctx, slots, ranges, steps=2500, batch=8, frame=1024, max_steps=896, lr
     from .utils import mat4_perspective
     from .volumerender.volumerender import render_matrices
     net = Net(0, init_params(seed) if params is None else params)
-    lm = LearnedMIP(ctx, net=net, radius=radius)
+    lm = LearnedMIP(ctx, net=net, radius=radius, inputs=inputs)
     for s in slots:
         lm.attach(s)
     rng = np.random.default_rng(seed)
@@ -346,6 +370,7 @@ def main():
     ap.add_argument("--volumes", type=int, default=4)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=WEIGHTS_PATH)
+    ap.add_argument("--inputs", choices=INPUTS, default=INPUTS_DEFAULT)
     args = ap.parse_args()
 
     def log(*a):
@@ -360,10 +385,11 @@ def main():
         slots.append(ctx.volume_add(dev_ptr=v.data_ptr(), dtype=np.uint16, shape=tuple(v.shape)))
         del v
     log("training on %d volumes, %d steps x %d pairs" % (len(slots), args.steps, args.batch))
-    params, hist = train(ctx, slots, ranges, steps=args.steps, batch=args.batch, lr=args.lr, log=log)
+    params, hist = train(ctx, slots, ranges, steps=args.steps, batch=args.batch, lr=args.lr, log=log,
+                         inputs=args.inputs)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.savez(args.out, params=params, history=np.array(hist, np.float64),
-             pairs=args.steps * args.batch)
+             pairs=args.steps * args.batch, inputs=args.inputs)
     log("saved %s" % args.out)
 
 

@@ -150,7 +150,7 @@ class RenderServer(object):
         """Make `vol` resident if the budget allows (evicting idle volumes). Caller holds cv."""
         if vol.slot is not None:
             return True
-        need = padded_bytes(vol.data.shape, vol.data.dtype)
+        need = device_bytes(vol.data.shape, vol.data.dtype)
         if need > self.hbm_budget:
             raise MemoryError("volume %r larger than the HBM budget" % vol.name)
         idle = sorted((v for v in self.volumes.values() if v.slot is not None and v.users == 0
@@ -163,7 +163,8 @@ class RenderServer(object):
             vol.slot = self.ctx.volume_add(vol.data)
             info = self.ctx.volume_info(vol.slot)
         vol.bytes = info["bytes"]
-        vol.padded = vol.bytes != vol.data.nbytes
+        # the padded layout alone (batchable); above 4 GiB a slot is linear + a padded copy
+        vol.padded = vol.bytes == padded_bytes(vol.data.shape, vol.data.dtype)
         self.resident_bytes += vol.bytes
         return True
 
@@ -313,6 +314,15 @@ def padded_bytes(shape, dtype):
     es = np.dtype(dtype).itemsize
     per16 = 16 // es
     return (nz + 2) * (ny + 2) * ((nx + 2 + per16 - 1) // per16) * 16
+
+
+def device_bytes(shape, dtype):
+    """Device bytes spr_volume_add takes for a volume: the padded copy, plus the linear
+    volume when the padded copy exceeds 4 GiB."""
+    pb = padded_bytes(shape, dtype)
+    if pb <= 0xFFFFFFFF:
+        return pb
+    return pb + int(np.prod(shape)) * np.dtype(dtype).itemsize
 
 
 def poisson_load(server, requests, qps, seed=0):

@@ -283,3 +283,30 @@ def test_coarse_pair_matches_exact_renders(torch, dtype, shape):
                 assert d <= 1e-4, (j, steps, d)
             assert torch.equal(pa, refs[0][1])
             assert (refs[0][0] > 0).sum().item() > 1000
+
+
+@pytest.mark.parametrize("dtype", [np.uint16, np.uint8, np.float32])
+def test_pooled_volumes(torch, dtype):
+    """spr_volume_add_pooled: 2x2x2 mean (rounded to the voxel type) and max, odd sizes
+    clamped at the far edge, from padded and linear slots."""
+    rng = np.random.default_rng(8)
+    shape = (17, 22, 35)
+    if dtype == np.float32:
+        vol = rng.normal(0, 1, shape).astype(np.float32)
+    else:
+        vol = rng.integers(0, np.iinfo(dtype).max, shape).astype(dtype)
+    pad = np.pad(vol, ((0, 1), (0, 0), (0, 1)), mode="edge")  # odd z and x: clamp
+    blocks = pad.reshape(9, 2, 11, 2, 18, 2).astype(np.float64)
+    want_max = blocks.max(axis=(1, 3, 5)).astype(dtype)
+    mean = blocks.sum(axis=(1, 3, 5)) / 8.
+    ctx = _lib.Context(0)
+    for layout in (_lib.LAYOUT_PADDED, _lib.LAYOUT_LINEAR):
+        ctx.set_layout(layout)
+        s = ctx.volume_add(vol)
+        mx = ctx.volume_read(ctx.volume_add_pooled(s, 1))
+        mn = ctx.volume_read(ctx.volume_add_pooled(s, 0))
+        np.testing.assert_array_equal(mx, want_max)
+        if dtype == np.float32:
+            np.testing.assert_allclose(mn, mean, rtol=1e-6, atol=1e-6)
+        else:
+            np.testing.assert_array_equal(mn, np.rint(mean).astype(dtype))
