@@ -213,8 +213,9 @@ def bench_headline(args):
 
     b_alg = n_samp * 8 * 2 + W * H * 4 * 2
     cpu = parity = None
+    vol = synth.device_volume_to_numpy(vt, np.uint16)
+    upload = measure_upload(ctx, vol)
     if not args.no_cpu:
-        vol = synth.device_volume_to_numpy(vt, np.uint16)
         cpu, parity = cpu_baseline(vol, invP, invM, lo, hi, S, W, H, args.cpu_seconds,
                                    gpu_out, gpu_alpha, "the same 1024^2 frame")
     result_line(
@@ -229,7 +230,24 @@ def bench_headline(args):
         latency_ms={"p50": round(pct(lat, 50), 4), "p99": round(pct(lat, 99), 4),
                     "frames": len(lat_ev), "poses": 8, "gsamples_per_s": round(pose_gsps, 3)},
         roofline=roofline(b_alg, kernel_ms, traffic_from_profile("headline")),
-        cpu_baseline=cpu, parity_max_abs_diff_vs_oracle=parity)
+        cpu_baseline=cpu, parity_max_abs_diff_vs_oracle=parity, upload=upload)
+
+
+def measure_upload(ctx, vol):
+    """The format path on the same volume, outside the timed frames: host (pageable numpy)
+    -> device upload + padded-layout build (spr_volume_add), and the GPU min/max."""
+    import torch
+    t0 = time.perf_counter()
+    slot = ctx.volume_add(vol)
+    t1 = time.perf_counter()
+    ctx.volume_release(slot)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    lo, hi = ctx.volume_minmax()
+    t3 = time.perf_counter()
+    return {"bytes": int(vol.nbytes), "host_to_padded_ms": round(1e3 * (t1 - t0), 2),
+            "host_to_padded_GBps": round(vol.nbytes / (t1 - t0) / 1e9, 2),
+            "minmax_ms": round(1e3 * (t3 - t2), 3), "minmax": [lo, hi]}
 
 
 # ------------------------------------------------------------------------------ batch

@@ -77,10 +77,13 @@ class _Volume(object):
 class RenderServer(object):
     METHODS = {"max_project": _lib.METHOD_MAX_PROJECT, "iso_surface": _lib.METHOD_ISO_SURFACE}
 
-    def __init__(self, device=0, hbm_budget=None, max_batch=16, workers=2):
-        import torch
-        self.torch = torch
-        self.ctx = _lib.Context(device)
+    def __init__(self, device=0, hbm_budget=None, max_batch=16, workers=2, ctx=None):
+        """ctx: a context to serve from (default: a new one on `device`); workers = 0 starts
+        no worker threads (the scheduler is then driven by the caller)."""
+        if workers:
+            import torch
+            self.torch = torch
+        self.ctx = ctx if ctx is not None else _lib.Context(device)
         if hbm_budget is None:
             free, _ = _lib.device_memory(device)
             hbm_budget = int(.6 * free)
