@@ -522,19 +522,17 @@ def bench_learned(args):
     coarse_ms = timed(lambda p: lm.coarse(slot, p, W, H, planes, 0, stream.cuda_stream), reps)
     net_ms = timed(lambda p: lm.net.forward(planes.data_ptr(), 1, H // 2, W // 2, out.data_ptr(),
                                             stream.cuda_stream), reps)
-    # per-frame judged rendering: wall clock per frame, fallbacks included
-    judged = []
-    for p in params:  # warm
-        lm.render_judged(slot, p, out, alpha, stream)
+    # per-frame judged rendering of the 8 poses as one batch (one host synchronisation per
+    # batch): wall clock per frame, the exact re-renders of rejected frames included
+    outs = [torch.empty((H, W), device="cuda") for _ in params]
+    alphas = [torch.empty_like(o) for o in outs]
+    lm.render_judged_many(slot, params, outs, alphas, stream)  # warm
     stream.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
-        for p in params:
-            judged.append(lm.render_judged(slot, p, out, alpha, stream))
+        kept, est = lm.render_judged_many(slot, params, outs, alphas, stream)
     stream.synchronize()
     judged_ms = 1e3 * (time.perf_counter() - t0) / (reps * len(params))
-    kept = [k for k, _ in judged[:len(params)]]
-    est = [e for _, e in judged[:len(params)]]
     # matrix-core work of one network pass: 32x32x9 MACs per pixel in 4 layers at h*w,
     # 3 at h*w/4 and 2 at h*w/16, here h = H/2
     px = (H // 2) * (W // 2)

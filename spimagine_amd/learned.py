@@ -161,7 +161,7 @@ class Net(object):
 # pooled 2x2x2 by mean and by max (a quarter-size pyramid level: a coarse ray step of 4
 # voxels is 2 pooled voxels, so the coarse render keeps the locality of a fine one)
 INPUTS = ("dilated", "pooled")
-INPUTS_DEFAULT = "dilated"
+INPUTS_DEFAULT = "pooled"
 
 
 def load_weights(path):
@@ -175,6 +175,27 @@ def psnr(a, b):
     """PSNR (dB) of a against b for images in [0, 1]."""
     mse = float(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2))
     return float("inf") if mse == 0 else 10. * math.log10(1. / mse)
+
+
+JUDGE_STRIDE = 8
+JUDGE_OFFSET = 1   # judge pixels (8i+1, 8j+1): between the coarse rays, where errors are largest
+
+
+def sparse_params(p, w, h, stride=JUDGE_STRIDE, offset=JUDGE_OFFSET):
+    """Parameters of a (w/stride) x (h/stride) exact render whose pixel (i, j) traces the ray
+    of pixel (stride i + offset, stride j + offset) of the w x h frame: the pixel mapping
+    u = x / W * 2 - 1 of the small frame is shifted by 2 offset / w in clip space, i.e.
+    inv_p is multiplied by a clip-space translation (offset 0: the rays of pixels
+    (stride i, stride j) exactly, bit for bit)."""
+    q = _lib.SprParams()
+    ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(q))
+    if offset:
+        T = np.eye(4)
+        T[0, 3] = 2. * offset / w
+        T[1, 3] = 2. * offset / h
+        iP = np.asarray(list(p.inv_p), np.float64).reshape(4, 4).dot(T)
+        q.inv_p[:] = [float(v) for v in iP.astype(np.float32).ravel()]
+    return q
 
 
 def coarse_params(p):
@@ -240,33 +261,45 @@ class LearnedMIP(object):
         self.net.forward(planes.data_ptr(), 1, h // 2, w // 2, d_out, stream)
 
     def render_judged(self, slot, p, out, alpha, stream, margin=0.5):
-        """One frame with a per-frame judge: the learned frame, then the exact renderer on
-        every 8th pixel in x and y (a (w/8) x (h/8) render with the same matrices traces
-        exactly the rays of pixels (8i, 8j): 1/64 of the exact cost), the PSNR of the
-        learned frame on those pixels, and the exact full frame when that estimate is
-        below threshold + margin.  out / alpha: (h, w) float32 torch tensors; stream: a
-        torch.cuda.Stream.  Learned frames take alpha from the coarse pass (exact at even
-        pixels, nearest-upsampled).  Returns (learned frame kept, estimated PSNR)."""
+        """One frame with a per-frame judge (render_judged_many for one frame)."""
+        kept, est = self.render_judged_many(slot, [p], [out], [alpha], stream, margin)
+        return kept[0], est[0]
+
+    def render_judged_many(self, slot, plist, outs, alphas, stream, margin=0.5):
+        """Frames with a per-frame judge: each learned frame, then the exact renderer on one
+        pixel in 8 x 8 (a (w/8) x (h/8) render, sparse_params: 1/64 of the exact cost) and
+        the learned frame's PSNR on those pixels, computed on the device; one host
+        synchronisation for all frames; the exact full frame for every frame whose
+        estimate is below threshold + margin.  outs / alphas: (h, w) float32 torch tensors;
+        stream: a torch.cuda.Stream.  Learned frames take alpha from the coarse pass
+        (exact at even pixels, nearest-upsampled).  Returns (kept flags, estimated PSNRs)."""
         torch = self.torch
-        h, w = out.shape
+        h, w = outs[0].shape
         if w % 64 or h % 64:
             raise ValueError("judged frames need w and h multiples of 64")
         sh = stream.cuda_stream
-        self.render(slot, p, w, h, out.data_ptr(), sh)
-        sparse = self._buffer(("judge", w, h), (h // 8, w // 8))
+        o = JUDGE_OFFSET
+        n = len(plist)
+        sparse = self._buffer(("judge", n, w, h), (n, h // 8, w // 8))
         salpha = self._buffer(("judge_alpha", w, h), (h // 8, w // 8))
-        self.ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, w // 8, h // 8, sparse.data_ptr(),
-                             salpha.data_ptr(), stream=sh)
+        mse = self._buffer(("judge_mse", n), (n,))
         with torch.cuda.stream(stream):
-            mse = float(((out[::8, ::8] - sparse) ** 2).mean())
-            est = float("inf") if mse == 0 else 10. * math.log10(1. / mse)
-            if est >= self.threshold + margin:
+            for k, (p, out, alpha) in enumerate(zip(plist, outs, alphas)):
+                self.render(slot, p, w, h, out.data_ptr(), sh)
                 coarse_alpha = self._buffer(("alpha", w, h), (h // 2, w // 2))
                 alpha.copy_(coarse_alpha.repeat_interleave(2, 0).repeat_interleave(2, 1))
-                return True, est
-        self.ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, w, h, out.data_ptr(),
-                             alpha.data_ptr(), stream=sh)
-        return False, est
+                self.ctx.render_slot(slot, sparse_params(p, w, h), _lib.METHOD_MAX_PROJECT,
+                                     w // 8, h // 8, sparse[k].data_ptr(), salpha.data_ptr(),
+                                     stream=sh)
+                mse[k] = ((out[o::8, o::8] - sparse[k]) ** 2).mean()
+            m = mse.cpu().numpy()   # the one synchronisation
+        est = [float("inf") if v == 0 else 10. * math.log10(1. / float(v)) for v in m]
+        kept = [e >= self.threshold + margin for e in est]
+        for p, out, alpha, ok in zip(plist, outs, alphas, kept):
+            if not ok:
+                self.ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, w, h, out.data_ptr(),
+                                     alpha.data_ptr(), stream=sh)
+        return kept, est
 
     def render_or_exact(self, slot, p, w, h, d_out, d_alpha, stream):
         """Learned frame when the slot validated, else the exact render."""

@@ -198,8 +198,9 @@ def test_learned_render_path_shapes_and_fallback(torch):
 
 def test_sparse_judge_rays_are_the_full_frame_pixels(torch):
     """A (w/8) x (h/8) render with the same matrices traces the rays of pixels (8i, 8j) of
-    the w x h frame, bit for bit; render_judged falls back to the exact frame below its
-    threshold and keeps the learned frame above it."""
+    the w x h frame, bit for bit (the judge shifts that grid to (8i+1, 8j+1)); render_judged
+    falls back to the exact frame below its threshold and keeps the learned frame above
+    it."""
     vol = synth.blob_volume((64, 80, 72), n_blobs=14, sigma=(3., 6.), amp=(500., 3000.),
                             background=100., poisson=True, read_noise=5., dtype="uint16", seed=2)
     ctx = _lib.Context(0)
@@ -219,8 +220,8 @@ def test_sparse_judge_rays_are_the_full_frame_pixels(torch):
     sa = torch.empty_like(sparse)
     ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W, H, full.data_ptr(), fa.data_ptr(),
                     stream=s.cuda_stream)
-    ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W // 8, H // 8, sparse.data_ptr(), sa.data_ptr(),
-                    stream=s.cuda_stream)
+    ctx.render_slot(slot, learned.sparse_params(p, W, H, offset=0), _lib.METHOD_MAX_PROJECT, W // 8,
+                    H // 8, sparse.data_ptr(), sa.data_ptr(), stream=s.cuda_stream)
     s.synchronize()
     assert torch.equal(sparse, full[::8, ::8]) and torch.equal(sa, fa[::8, ::8])
     out = torch.empty_like(full)
@@ -237,8 +238,15 @@ def test_sparse_judge_rays_are_the_full_frame_pixels(torch):
     s.synchronize()
     assert kept is True and abs(est2 - est) < 1e-6
     assert torch.equal(out, ref)
-    mse = float(((ref[::8, ::8] - full[::8, ::8]) ** 2).mean())
+    # the judge's pixels: (8i+1, 8j+1), traced by the shifted sparse render
+    shifted = torch.empty_like(sparse)
+    ctx.render_slot(slot, learned.sparse_params(p, W, H), _lib.METHOD_MAX_PROJECT, W // 8, H // 8,
+                    shifted.data_ptr(), sa.data_ptr(), stream=s.cuda_stream)
+    s.synchronize()
+    mse = float(((ref[1::8, 1::8] - shifted) ** 2).mean())
     assert abs(est2 - 10 * np.log10(1. / mse)) < 1e-4
+    # and they are the full frame's rays at those pixels up to float rounding
+    assert float((shifted - full[1::8, 1::8]).abs().max()) < 1e-3
     # learned frames carry the coarse pass's alpha: exact at even pixels
     assert torch.equal(alpha[::2, ::2], fa[::2, ::2])
 
