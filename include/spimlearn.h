@@ -76,6 +76,17 @@ int spl_gradients(spl_net *net, const float *d_in, const float *d_target, int n,
 int spl_train_step(spl_net *net, const float *d_in, const float *d_target, int n, int h, int w,
                    float lr, double *sse, void *stream);
 
+/* The per-frame judge's error: *d_sse (one device float; set to 0 first unless
+   accumulate) += sum over i < h/stride, j < w/stride of
+   (d_frame[(stride i + offset) w + stride j + offset] - d_sparse[i (w/stride) + j])^2,
+   where d_sparse is an exact render of those pixels.  Enqueued on `stream`. */
+int spl_judge_sse(spl_net *net, const float *d_frame, int w, int h, const float *d_sparse,
+                  int stride, int offset, float *d_sse, int accumulate, void *stream);
+
+/* d_dst (2h x 2w) = nearest-neighbour x2 upsampling of d_src (h x w), on `stream` (the
+   learned frame's alpha from the coarse pass). */
+int spl_upsample2(spl_net *net, const float *d_src, int w, int h, float *d_dst, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -338,12 +338,15 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
         const uint32_t SX = (HX + 1u - lox + E - 1u) & ~(E - 1u);  // voxels, multiple of E
         const uint32_t SY = HY - LY + 1u, SZ = HZ - LZ + 1u;
         const uint32_t nu = SX / E;                                 // 16-byte units per row
-        const uint64_t bytes = (uint64_t)SX * SY * SZ * TS;
+        // slab row stride: SX voxels, plus 16 bytes when row_pad asks for an odd number of
+        // 16-byte units (rows a few apart then fall on different LDS banks)
+        const uint32_t rowb = (nu + (p.row_pad & ~nu & 1u)) * 16u;
+        const uint64_t bytes = (uint64_t)rowb * SY * SZ;
         const bool fits = bytes <= (uint64_t)SLAB_BYTES && nu <= NT;
         // rows addressed from plane LZ on (32-bit offsets whatever the volume's size)
         if (fits)
             staged_copy_rows<NT>(vol.rsrc_plane(LZ), slab, SHARED ? threadIdx.x : lane, LY, 0u,
-                                 lox * TS, nu, SX * TS, SY, SZ, PY, pitch);
+                                 lox * TS, nu, rowb, SY, SZ, PY, pitch);
         if constexpr (SHARED) {
             __syncthreads();
         } else {
@@ -352,8 +355,8 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
         }
         if (live) {
             if (fits)
-                staged_sample_slab<PaddedVolume<T, POW2, WIDE>, CH>(vol, slab, lox, LY, LZ, SX * TS,
-                                                                    SY, px, py, pz, dx, dy, dz, col);
+                staged_sample_slab<PaddedVolume<T, POW2, WIDE>, CH>(vol, slab, lox, LY, LZ, rowb, SY,
+                                                                    px, py, pz, dx, dy, dz, col);
             else
                 staged_sample_global<PaddedVolume<T, POW2, WIDE>, CH>(vol, px, py, pz, dx, dy, dz,
                                                                       col);

@@ -485,6 +485,42 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
     p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + 1e-8f);
 }
 
+// ------------------------------------------------------------------ judge helpers
+// sse[0] += sum over the sparse grid of (frame[stride i + off][stride j + off] - sparse[i][j])^2
+// (one block-reduced float atomic per workgroup)
+__global__ __launch_bounds__(256) void k_judge_sse(const float* __restrict__ frame, int w,
+                                                   const float* __restrict__ sparse, int sw, int sh,
+                                                   int stride, int off, float* __restrict__ sse) {
+    __shared__ float red[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    float v = 0.0f;
+    if (i < sw * sh) {
+        const int x = i % sw, y = i / sw;
+        const float d = frame[(size_t)(stride * y + off) * w + stride * x + off] - sparse[i];
+        v = d * d;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(sse, red[0]);
+}
+
+__global__ void k_fill(float* __restrict__ p, int n, float v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// dst (2h, 2w) = nearest-neighbour x2 of src (h, w)
+__global__ void k_upsample2(const float* __restrict__ src, int w, int h, float* __restrict__ dst) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)4 * w * h) return;
+    const int X = i % (2 * w), Y = i / (2 * w);
+    dst[i] = src[(size_t)(Y >> 1) * w + (X >> 1)];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------- host
@@ -844,6 +880,34 @@ int spl_train_step(spl_net* t, const float* d_in, const float* d_target, int n, 
     if (rc) return rc;
     HIPCALL(t, hipGetLastError());
     if (sse) return read_sse(t, sse, s);
+    return SPL_OK;
+}
+
+int spl_judge_sse(spl_net* t, const float* d_frame, int w, int h, const float* d_sparse, int stride,
+                  int offset, float* d_sse, int accumulate, void* stream) {
+    if (!t) return fail(nullptr, SPL_ERR_INVALID_ARG, "NULL network");
+    if (!d_frame || !d_sparse || !d_sse || w <= 0 || h <= 0 || stride < 1 || offset < 0 ||
+        offset >= stride)
+        return fail(t, SPL_ERR_INVALID_ARG, "invalid judge arguments");
+    DeviceGuard g(t->device);
+    hipStream_t s = stream ? (hipStream_t)stream : t->stream;
+    const int sw = w / stride, sh = h / stride;
+    if (!accumulate) hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, d_sse, 1, 0.0f);
+    if (sw > 0 && sh > 0)
+        hipLaunchKernelGGL(k_judge_sse, dim3((sw * sh + 255) / 256), dim3(256), 0, s, d_frame, w, d_sparse,
+                           sw, sh, stride, offset, d_sse);
+    HIPCALL(t, hipGetLastError());
+    return SPL_OK;
+}
+
+int spl_upsample2(spl_net* t, const float* d_src, int w, int h, float* d_dst, void* stream) {
+    if (!t) return fail(nullptr, SPL_ERR_INVALID_ARG, "NULL network");
+    if (!d_src || !d_dst || w <= 0 || h <= 0) return fail(t, SPL_ERR_INVALID_ARG, "invalid upsample arguments");
+    DeviceGuard g(t->device);
+    hipStream_t s = stream ? (hipStream_t)stream : t->stream;
+    const size_t n = (size_t)4 * w * h;
+    hipLaunchKernelGGL(k_upsample2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_src, w, h, d_dst);
+    HIPCALL(t, hipGetLastError());
     return SPL_OK;
 }
 

@@ -75,6 +75,7 @@ struct spr_ctx {
     int slab_kb = 16;  // per-wave LDS slab of the staged max projection, 0 = off
     int staging = 0;   // 0 per-wave boxes, 1 one box per workgroup
     int chunk = 0;     // samples per staged box, 0 = from the ray step
+    int row_pad = 0;   // pad staged slab rows to an odd number of 16-byte units
 };
 
 namespace {
@@ -199,6 +200,7 @@ RayParams make_params(const spr_params& prm, int tile_map, int w, int h) {
     p.nty = (uint32_t)((h + 15) / 16);
     p.ntiles = p.ntx * p.nty;
     p.tile_map = (uint32_t)tile_map;
+    p.row_pad = 0;
     return p;
 }
 
@@ -384,6 +386,7 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
                 int current_part, int w, int h, float* out, float* alpha, float* depth,
                 float* normals, float* occ, float* tmp, float* tmp_vec, hipStream_t s) {
     RayParams p = make_params(prm, c->tile_map, w, h);
+    p.row_pad = (uint32_t)c->row_pad;
     const dim3 grid(tile_grid(p.tile_map, p.ntx, p.nty)), block(256);
     const bool nearest = prm.filter == SPR_FILTER_NEAREST;
     int rc;
@@ -853,6 +856,10 @@ int spr_set_tuning(spr_ctx* c, int key, int value) {
                 return fail(c, SPR_ERR_INVALID_ARG, "chunk %d (0, 4, 8 or 16)", value);
             c->chunk = value;
             return SPR_OK;
+        case SPR_TUNE_ROW_PAD:
+            if (value != 0 && value != 1) return fail(c, SPR_ERR_INVALID_ARG, "row pad %d (0 or 1)", value);
+            c->row_pad = value;
+            return SPR_OK;
         default: return fail(c, SPR_ERR_INVALID_ARG, "unknown tuning key %d", key);
     }
 }
@@ -864,6 +871,7 @@ int spr_get_tuning(spr_ctx* c, int key, int* value) {
         case SPR_TUNE_SLAB_KB: *value = c->slab_kb; return SPR_OK;
         case SPR_TUNE_STAGING: *value = c->staging; return SPR_OK;
         case SPR_TUNE_CHUNK: *value = c->chunk; return SPR_OK;
+        case SPR_TUNE_ROW_PAD: *value = c->row_pad; return SPR_OK;
         default: return fail(c, SPR_ERR_INVALID_ARG, "unknown tuning key %d", key);
     }
 }
@@ -1142,6 +1150,7 @@ int spr_render_batch(spr_ctx* c, const spr_params* params, const spr_job* jobs, 
     uint32_t* d_counters = (uint32_t*)((char*)buf + jbytes);
     hipError_t e = hipMemcpyAsync(d_jobs, host.data(), jbytes, hipMemcpyHostToDevice, s);
     RayParams p = make_params(prm, c->tile_map, w, h);
+    p.row_pad = (uint32_t)c->row_pad;
     p.nouter = (prm.max_steps / num_parts) / kLoopUnroll;
     p.current_part = current_part;
     const BatchQueues q = make_queues(p, n);
@@ -1174,6 +1183,7 @@ int spr_render_coarse_pair(spr_ctx* c, int slot_a, int slot_b, const spr_params*
     DeviceGuard g(c->device);
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     RayParams p = make_params(*params, c->tile_map, w, h);
+    p.row_pad = (uint32_t)c->row_pad;
     p.nouter = params->max_steps / kLoopUnroll;
     p.current_part = 0;
     const dim3 grid((unsigned)(((w + 7) / 8) * ((h + 7) / 8)), 2), block(256);
@@ -1240,6 +1250,7 @@ int spr_batch_graph_create(spr_ctx* c, const spr_job* jobs, int n, int w, int h,
     rc = check_parts(c, c->params, 1);
     if (rc) return cleanup(rc);
     RayParams p = make_params(c, w, h);
+    p.row_pad = (uint32_t)c->row_pad;
     p.nouter = c->params.max_steps / kLoopUnroll;
     p.current_part = 0;
     const BatchQueues q = make_queues(p, n);
@@ -1327,6 +1338,7 @@ int spr_count_samples(spr_ctx* c, int num_parts, int w, int h, double* samples) 
     if (rc) return rc;
     DeviceGuard g(c->device);
     RayParams p = make_params(c, w, h);
+    p.row_pad = (uint32_t)c->row_pad;
     HIPCALL(c, hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_count_hits, dim3(tile_grid(p.tile_map, p.ntx, p.nty)), dim3(256), 0,
                        c->stream, p, c->d_count);

@@ -24,7 +24,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libspimlearn.so")
 WEIGHTS_PATH = os.path.join(HERE, "weights", "mip_x2_q4.npz")
 EXPORTS = ("spl_abi_version", "spl_param_count", "spl_create", "spl_destroy", "spl_last_error",
-           "spl_set_params", "spl_get_params", "spl_forward", "spl_gradients", "spl_train_step")
+           "spl_set_params", "spl_get_params", "spl_forward", "spl_gradients", "spl_train_step",
+           "spl_judge_sse", "spl_upsample2")
 C = 32
 LAYER_NAMES = ("e0", "e0b", "e1", "e1b", "e2a", "e2", "u1", "u0", "u0b")
 PSNR_THRESHOLD = 40.0
@@ -57,6 +58,9 @@ def load():
                                 ctypes.POINTER(ctypes.c_double), c_void_p]
     L.spl_train_step.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_float,
                                  ctypes.POINTER(ctypes.c_double), c_void_p]
+    L.spl_judge_sse.argtypes = [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_void_p,
+                                c_int, c_void_p]
+    L.spl_upsample2.argtypes = [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]
     _lib_learn = L
     return L
 
@@ -149,6 +153,14 @@ class Net(object):
         self._check(self.lib.spl_gradients(self.handle, d_in, d_target, n, h, w, g.ctypes.data,
                                            ctypes.byref(sse), stream))
         return g, sse.value
+
+    def judge_sse(self, d_frame, w, h, d_sparse, stride, offset, d_sse, accumulate=False,
+                  stream=None):
+        self._check(self.lib.spl_judge_sse(self.handle, d_frame, w, h, d_sparse, stride, offset,
+                                           d_sse, int(bool(accumulate)), stream))
+
+    def upsample2(self, d_src, w, h, d_dst, stream=None):
+        self._check(self.lib.spl_upsample2(self.handle, d_src, w, h, d_dst, stream))
 
     def train_step(self, d_in, d_target, n, h, w, lr, stream=None, want_loss=True):
         sse = ctypes.c_double()
@@ -273,26 +285,25 @@ class LearnedMIP(object):
         estimate is below threshold + margin.  outs / alphas: (h, w) float32 torch tensors;
         stream: a torch.cuda.Stream.  Learned frames take alpha from the coarse pass
         (exact at even pixels, nearest-upsampled).  Returns (kept flags, estimated PSNRs)."""
-        torch = self.torch
         h, w = outs[0].shape
         if w % 64 or h % 64:
             raise ValueError("judged frames need w and h multiples of 64")
         sh = stream.cuda_stream
-        o = JUDGE_OFFSET
+        S = JUDGE_STRIDE
         n = len(plist)
-        sparse = self._buffer(("judge", n, w, h), (n, h // 8, w // 8))
-        salpha = self._buffer(("judge_alpha", w, h), (h // 8, w // 8))
-        mse = self._buffer(("judge_mse", n), (n,))
-        with torch.cuda.stream(stream):
-            for k, (p, out, alpha) in enumerate(zip(plist, outs, alphas)):
-                self.render(slot, p, w, h, out.data_ptr(), sh)
-                coarse_alpha = self._buffer(("alpha", w, h), (h // 2, w // 2))
-                alpha.copy_(coarse_alpha.repeat_interleave(2, 0).repeat_interleave(2, 1))
-                self.ctx.render_slot(slot, sparse_params(p, w, h), _lib.METHOD_MAX_PROJECT,
-                                     w // 8, h // 8, sparse[k].data_ptr(), salpha.data_ptr(),
-                                     stream=sh)
-                mse[k] = ((out[o::8, o::8] - sparse[k]) ** 2).mean()
-            m = mse.cpu().numpy()   # the one synchronisation
+        sparse = self._buffer(("judge", w, h), (h // S, w // S))
+        salpha = self._buffer(("judge_alpha", w, h), (h // S, w // S))
+        sse = self._buffer(("judge_sse", n), (n,))
+        coarse_alpha = self._buffer(("alpha", w, h), (h // 2, w // 2))
+        for k, (p, out, alpha) in enumerate(zip(plist, outs, alphas)):
+            self.render(slot, p, w, h, out.data_ptr(), sh)
+            self.net.upsample2(coarse_alpha.data_ptr(), w // 2, h // 2, alpha.data_ptr(), sh)
+            self.ctx.render_slot(slot, sparse_params(p, w, h), _lib.METHOD_MAX_PROJECT, w // S,
+                                 h // S, sparse.data_ptr(), salpha.data_ptr(), stream=sh)
+            self.net.judge_sse(out.data_ptr(), w, h, sparse.data_ptr(), S, JUDGE_OFFSET,
+                               sse.data_ptr() + 4 * k, stream=sh)
+        stream.synchronize()        # the one synchronisation
+        m = sse.cpu().numpy() / float((h // S) * (w // S))
         est = [float("inf") if v == 0 else 10. * math.log10(1. / float(v)) for v in m]
         kept = [e >= self.threshold + margin for e in est]
         for p, out, alpha, ok in zip(plist, outs, alphas, kept):

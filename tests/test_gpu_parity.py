@@ -152,10 +152,12 @@ def test_staged_mip_matches_global(VR, dtype, shape, pose):
          np.dot(mat4_translate(0, 0, -1.2), mat4_rotation(.8, .2, 1, .1)),  # eye close to the box
          mat4_identity()][pose]                                          # eye inside the box
     frames = {}
-    for slab, staging in ((0, 0), (8, 0), (12, 0), (16, 0), (8, 1), (12, 1)):
+    for slab, staging, pad in ((0, 0, 0), (8, 0, 0), (12, 0, 0), (16, 0, 0), (8, 1, 0), (12, 1, 0),
+                               (16, 0, 1), (12, 1, 1)):
         rend = VR((130, 97))
         rend._ctx.set_tuning("slab_kb", slab)
         rend._ctx.set_tuning("staging", staging)
+        rend._ctx.set_tuning("row_pad", pad)
         rend.set_max_steps(160)
         rend.set_data(vol)
         rend.set_modelView(M)
@@ -164,11 +166,11 @@ def test_staged_mip_matches_global(VR, dtype, shape, pose):
         for part in range(3):
             rend.render(numParts=3, currentPart=part)
         res.append(rend.output.copy())
-        frames[slab, staging] = res
+        frames[slab, staging, pad] = res
     for key in frames:
-        for a, b in zip(frames[0, 0], frames[key]):
+        for a, b in zip(frames[0, 0, 0], frames[key]):
             np.testing.assert_array_equal(a, b)
-    assert (frames[0, 0][0] > 0).sum() > 200
+    assert (frames[0, 0, 0][0] > 0).sum() > 200
 
 
 @pytest.mark.parametrize("dtype", [np.uint16, np.uint8, np.float32])
