@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SPL_ABI_VERSION 1
+#define SPL_ABI_VERSION 2
 
 #define SPL_OK 0
 #define SPL_ERR_INVALID_ARG (-1)
@@ -76,12 +76,14 @@ int spl_gradients(spl_net *net, const float *d_in, const float *d_target, int n,
 int spl_train_step(spl_net *net, const float *d_in, const float *d_target, int n, int h, int w,
                    float lr, double *sse, void *stream);
 
-/* The per-frame judge's error: *d_sse (one device float; set to 0 first unless
-   accumulate) += sum over i < h/stride, j < w/stride of
-   (d_frame[(stride i + offset) w + stride j + offset] - d_sparse[i (w/stride) + j])^2,
-   where d_sparse is an exact render of those pixels.  Enqueued on `stream`. */
-int spl_judge_sse(spl_net *net, const float *d_frame, int w, int h, const float *d_sparse,
-                  int stride, int offset, float *d_sse, int accumulate, void *stream);
+/* The per-frame judge's error against gx x gy exact patches of patch x patch pixels (an
+   exact render of those pixels, e.g. one batched launch): patch (ty, tx) covers the frame
+   pixels from row ty (h / gy) + (h / gy - patch) / 2 and column tx (w / gx) +
+   (w / gx - patch) / 2 on; d_patches holds the patches one after another (tx fastest),
+   rows of `patch` floats.  *d_sse (one device float) = (accumulate ? *d_sse : 0) + the
+   summed squared difference, summed in a fixed order.  Enqueued on `stream`. */
+int spl_judge_sse(spl_net *net, const float *d_frame, int w, int h, const float *d_patches,
+                  int patch, int gx, int gy, float *d_sse, int accumulate, void *stream);
 
 /* d_dst (2h x 2w) = nearest-neighbour x2 upsampling of d_src (h x w), on `stream` (the
    learned frame's alpha from the coarse pass). */

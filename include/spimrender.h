@@ -77,7 +77,8 @@ extern "C" {
                       (default); 0 = sample every chunk from global memory
    SPR_TUNE_STAGING   0 = each wave stages its own box (default), 1 = the four waves of a
                       workgroup stage one union box into a slab of 4 x SLAB_KB
-   SPR_TUNE_CHUNK     samples per staged box: 0 = from the ray step (default), 4, 8, 16
+   SPR_TUNE_CHUNK     samples per staged box: 0 = from the ray step (default), 4, 8, 16,
+                      32 or 64 (boxes of 2 or 4 groups of 16 samples)
    SPR_TUNE_ROW_PAD   1 = pad staged rows to an odd number of 16-byte units (spreads the
                       LDS banks rows a few apart fall on), 0 = no padding (default) */
 #define SPR_TUNE_TILE_MAP 0

@@ -94,6 +94,31 @@ class SprJob(ctypes.Structure):
     ]
 
 
+# numpy view of struct spr_job, for building large job tables without a Python loop
+JOB_DTYPE = np.dtype({"names": ["slot", "inv_p", "inv_m", "d_out", "d_alpha"],
+                      "formats": [np.int32, (np.float32, 16), (np.float32, 16), np.uint64,
+                                  np.uint64],
+                      "offsets": [SprJob.slot.offset, SprJob.inv_p.offset, SprJob.inv_m.offset,
+                                  SprJob.d_out.offset, SprJob.d_alpha.offset],
+                      "itemsize": ctypes.sizeof(SprJob)})
+
+
+def jobs_from_arrays(slots, inv_p, inv_m, d_out, d_alpha):
+    """A ctypes array of n spr_jobs from arrays: slots (n,), inv_p and inv_m (n, 16) or
+    (n, 4, 4), d_out and d_alpha (n,) device addresses."""
+    slots = np.asarray(slots)
+    n = slots.shape[0]
+    buf = np.zeros(n, JOB_DTYPE)
+    buf["slot"] = slots
+    buf["inv_p"] = np.asarray(inv_p, np.float32).reshape(n, 16)
+    buf["inv_m"] = np.asarray(inv_m, np.float32).reshape(n, 16)
+    buf["d_out"] = np.asarray(d_out, np.uint64)
+    buf["d_alpha"] = np.asarray(d_alpha, np.uint64)
+    arr = (SprJob * n)()
+    ctypes.memmove(arr, buf.ctypes.data, buf.nbytes)
+    return arr
+
+
 class SprError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("spimrender error %d: %s" % (code, msg))

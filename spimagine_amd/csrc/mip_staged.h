@@ -130,20 +130,19 @@ __device__ __forceinline__ bool staged_ray_begin(const RayParams& p, uint32_t x,
     return true;
 }
 
-// Padded-voxel box of the CH samples starting at (px, py, pz), packed for a min-reduction:
-// b0 = lo.x | lo.y << 16, b1 = lo.z | (0xFFFF - hi.x) << 16, b2 = (0xFFFF - hi.y) |
-// (0xFFFF - hi.z) << 16 (hi exclusive).  Inactive lanes give all ones (no effect).  The
-// last sample's position is estimated with one fma; mg* covers its distance from the
-// sequentially added position.
-template <typename V, int CH>
-__device__ __forceinline__ void staged_chunk_box(const V& vol, bool active,
-                                                 float px, float py, float pz, float dx, float dy,
-                                                 float dz, float mgx, float mgy, float mgz,
-                                                 uint32_t& b0, uint32_t& b1, uint32_t& b2) {
+// Padded-voxel box of the n samples starting at (px, py, pz) (last = n - 1), packed for a
+// min-reduction: b0 = lo.x | lo.y << 16, b1 = lo.z | (0xFFFF - hi.x) << 16, b2 =
+// (0xFFFF - hi.y) | (0xFFFF - hi.z) << 16 (hi exclusive).  Inactive lanes give all ones (no
+// effect).  The last sample's position is estimated with one fma; mg* covers its distance
+// from the sequentially added position (at most 63 additions, each off by half an ulp).
+template <typename V>
+__device__ __forceinline__ void staged_box_n(const V& vol, bool active, float last,
+                                             float px, float py, float pz, float dx, float dy,
+                                             float dz, float mgx, float mgy, float mgz,
+                                             uint32_t& b0, uint32_t& b1, uint32_t& b2) {
     b0 = b1 = b2 = 0xFFFFFFFFu;
     if (!active) return;
-    const float ex = fmaf((float)(CH - 1), dx, px), ey = fmaf((float)(CH - 1), dy, py),
-                ez = fmaf((float)(CH - 1), dz, pz);
+    const float ex = fmaf(last, dx, px), ey = fmaf(last, dy, py), ez = fmaf(last, dz, pz);
     constexpr bool P2 = V::kPow2;
     const float u0x = texel<P2>(px, vol.fnx), u0y = texel<P2>(py, vol.fny),
                 u0z = texel<P2>(pz, vol.fnz);
@@ -161,6 +160,14 @@ __device__ __forceinline__ void staged_chunk_box(const V& vol, bool active,
     b0 = lx | (ly << 16);
     b1 = lz | ((0xFFFFu - hx) << 16);
     b2 = (0xFFFFu - hy) | ((0xFFFFu - hz) << 16);
+}
+
+template <typename V, int CH>
+__device__ __forceinline__ void staged_chunk_box(const V& vol, bool active,
+                                                 float px, float py, float pz, float dx, float dy,
+                                                 float dz, float mgx, float mgy, float mgz,
+                                                 uint32_t& b0, uint32_t& b1, uint32_t& b2) {
+    staged_box_n(vol, active, (float)(CH - 1), px, py, pz, dx, dy, dz, mgx, mgy, mgz, b0, b1, b2);
 }
 
 // Copies rows r = z SY + y of the box into slab[r SX TS ...] with NT threads (t = this
@@ -262,10 +269,14 @@ __device__ __forceinline__ void staged_ray_end(const RayParams& p, uint32_t o, f
     }
 }
 
-// CH: samples staged per box (16, 8 or 4).  The samples, their order and their positions
-// are those of the reference's 16-sample inner loop whatever CH is (a max does not depend
-// on how the sequence is cut); a smaller CH keeps a box small when the ray step is several
-// voxels long.
+// CH: samples per unrolled group (16, 8 or 4); a box holds p.box16 groups of 16 samples
+// when CH is 16 (one group otherwise).  The samples, their order and their positions are
+// those of the reference's 16-sample inner loop whatever the grouping is (a max does not
+// depend on how the sequence is cut); a smaller CH keeps a box small when the ray step is
+// several voxels long, a larger box spreads the box reduction and copy over more samples
+// when it is short.  With per-wave slabs and CH 16, a box larger than the slab is split
+// into halves of 8 samples and those into quarters of 4, each staged on its own; only a
+// quarter whose box still does not fit is sampled from global memory.
 // SHARED = false: each wave stages the box of its own 64 rays into its own slab
 // (SLAB_BYTES, wave-level ordering only).  SHARED = true: the workgroup's four waves
 // stage the union box of its 256 rays into one slab of SLAB_BYTES with all 256 threads,
@@ -309,12 +320,15 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
                 mgz = 1e-5f * vol.fnz + 0.01f;
     float col = 0.0f;
 
-    static_assert(kLoopUnroll % CH == 0, "a chunk is a whole number of boxes");
-    const int nchunks = (p.nouter + 1) * (kLoopUnroll / CH);
-    for (int i = 0; i < nchunks; ++i) {
+    using PV = PaddedVolume<T, POW2, WIDE>;
+    // Stages the box of the next n samples of every ray into the slab.  Returns false, with
+    // nothing staged, when the box does not fit; ends with the slab visible to the wave (the
+    // workgroup when SHARED).
+    auto stage = [&](int n, uint32_t& lox, uint32_t& LY, uint32_t& LZ, uint32_t& rowb,
+                     uint32_t& SY) -> bool {
         uint32_t b0, b1, b2;
-        staged_chunk_box<PaddedVolume<T, POW2, WIDE>, CH>(vol, active, px, py, pz, dx, dy, dz, mgx,
-                                                          mgy, mgz, b0, b1, b2);
+        staged_box_n<PV>(vol, active, (float)(n - 1), px, py, pz, dx, dy, dz, mgx, mgy, mgz, b0,
+                         b1, b2);
         uint32_t r0 = wave_min_u16x2(b0), r1 = wave_min_u16x2(b1), r2 = wave_min_u16x2(b2);
         if constexpr (SHARED) {
             const uint32_t w = threadIdx.x >> 6;
@@ -331,16 +345,19 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
                 r2 = pk_min_u16(r2, s_box[3 * k + 2]);
             }
         }
-        const uint32_t LX = r0 & 0xFFFFu, LY = r0 >> 16, LZ = r1 & 0xFFFFu;
+        const uint32_t LX = r0 & 0xFFFFu;
+        LY = r0 >> 16;
+        LZ = r1 & 0xFFFFu;
         const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
                        HZ = 0xFFFFu - (r2 >> 16);
-        const uint32_t lox = LX & ~(E - 1u);
+        lox = LX & ~(E - 1u);
         const uint32_t SX = (HX + 1u - lox + E - 1u) & ~(E - 1u);  // voxels, multiple of E
-        const uint32_t SY = HY - LY + 1u, SZ = HZ - LZ + 1u;
+        SY = HY - LY + 1u;
+        const uint32_t SZ = HZ - LZ + 1u;
         const uint32_t nu = SX / E;                                 // 16-byte units per row
         // slab row stride: SX voxels, plus 16 bytes when row_pad asks for an odd number of
         // 16-byte units (rows a few apart then fall on different LDS banks)
-        const uint32_t rowb = (nu + (p.row_pad & ~nu & 1u)) * 16u;
+        rowb = (nu + (p.row_pad & ~nu & 1u)) * 16u;
         const uint64_t bytes = (uint64_t)rowb * SY * SZ;
         const bool fits = bytes <= (uint64_t)SLAB_BYTES && nu <= NT;
         // rows addressed from plane LZ on (32-bit offsets whatever the volume's size)
@@ -353,17 +370,60 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
-        if (live) {
-            if (fits)
-                staged_sample_slab<PaddedVolume<T, POW2, WIDE>, CH>(vol, slab, lox, LY, LZ, rowb, SY,
-                                                                    px, py, pz, dx, dy, dz, col);
-            else
-                staged_sample_global<PaddedVolume<T, POW2, WIDE>, CH>(vol, px, py, pz, dx, dy, dz,
-                                                                      col);
-        }
+        return fits;
+    };
+    // the wave's reads of the slab complete before the next box overwrites it
+    auto release = [&]() {
         if constexpr (!SHARED) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
+        }
+    };
+
+    static_assert(kLoopUnroll % CH == 0, "a chunk is a whole number of boxes");
+    const int ngroups = (p.nouter + 1) * (kLoopUnroll / CH);
+    const int G = CH == 16 ? (int)p.box16 : 1;  // groups per box, uniform
+    for (int i = 0; i < ngroups; i += G) {
+        const int ng = min(G, ngroups - i);
+        uint32_t lox, LY, LZ, rowb, SY;
+        if (stage(ng * CH, lox, LY, LZ, rowb, SY)) {
+            if (live) {
+#pragma unroll 1
+                for (int k = 0; k < ng; ++k)
+                    staged_sample_slab<PV, CH>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx,
+                                               dy, dz, col);
+            }
+            release();
+            continue;
+        }
+        if constexpr (CH == 16 && !SHARED) {
+            // the box is larger than the slab: halves of 8 samples with a box each, then
+            // quarters of 4; quarters that still do not fit sample global memory
+#pragma unroll 1
+            for (int h = 0; h < 2 * ng; ++h) {
+                if (stage(8, lox, LY, LZ, rowb, SY)) {
+                    staged_sample_slab<PV, 8>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy,
+                                              dz, col);
+                    release();
+                    continue;
+                }
+#pragma unroll 1
+                for (int q = 0; q < 2; ++q) {
+                    if (stage(4, lox, LY, LZ, rowb, SY))
+                        staged_sample_slab<PV, 4>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx,
+                                                  dy, dz, col);
+                    else
+                        staged_sample_global<PV, 4>(vol, px, py, pz, dx, dy, dz, col);
+                    release();
+                }
+            }
+        } else {
+            if (live) {
+#pragma unroll 1
+                for (int k = 0; k < ng; ++k)
+                    staged_sample_global<PV, CH>(vol, px, py, pz, dx, dy, dz, col);
+            }
+            release();
         }
     }
     if (active) staged_ray_end<kShort>(p, x + p.W * y, col, tnear, out, alpha);

@@ -26,6 +26,7 @@ struct RayParams {
     uint32_t ntx, nty, ntiles;  // 16x16 tiles per row, tile rows, total tiles
     uint32_t tile_map;          // kTileMap* below
     uint32_t row_pad;           // staged kernels: pad LDS slab rows to an odd multiple of 16 B
+    uint32_t box16;             // staged kernels: 16-sample groups per staged box (1, 2, 4)
 };
 
 // Workgroup -> 16x16 tile assignment.  The dispatcher hands consecutive workgroups to

@@ -486,18 +486,26 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // ------------------------------------------------------------------ judge helpers
-// sse[0] += sum over the sparse grid of (frame[stride i + off][stride j + off] - sparse[i][j])^2
-// (one block-reduced float atomic per workgroup)
-__global__ __launch_bounds__(256) void k_judge_sse(const float* __restrict__ frame, int w,
-                                                   const float* __restrict__ sparse, int sw, int sh,
-                                                   int stride, int off, float* __restrict__ sse) {
+// Squared error of a frame (h x w) against gx x gy exact patches of c x c pixels: patch
+// (ty, tx) covers the frame pixels from row ty (h / gy) + (h / gy - c) / 2 and column
+// tx (w / gx) + (w / gx - c) / 2 on; the patches are stored one after another (tx
+// fastest), rows of c.  One workgroup sums everything in a fixed order (deterministic):
+// *sse = (accumulate ? *sse : 0) + the sum.
+__global__ __launch_bounds__(256) void k_judge_patches(const float* __restrict__ frame, int w,
+                                                       int h, const float* __restrict__ patches,
+                                                       int c, int gx, int gy, int accumulate,
+                                                       float* __restrict__ sse) {
     __shared__ float red[256];
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int cc = c * c, n = gx * gy * cc;
+    const int cw = w / gx, ch = h / gy, ox = (cw - c) / 2, oy = (ch - c) / 2;
     float v = 0.0f;
-    if (i < sw * sh) {
-        const int x = i % sw, y = i / sw;
-        const float d = frame[(size_t)(stride * y + off) * w + stride * x + off] - sparse[i];
-        v = d * d;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const int k = i / cc, r = i - k * cc;
+        const int ty = k / gx, tx = k - ty * gx;
+        const int py = r / c, px = r - py * c;
+        const int y = ty * ch + oy + py, x = tx * cw + ox + px;
+        const float d = frame[(size_t)y * w + x] - patches[i];
+        v = fmaf(d, d, v);
     }
     red[threadIdx.x] = v;
     __syncthreads();
@@ -505,7 +513,7 @@ __global__ __launch_bounds__(256) void k_judge_sse(const float* __restrict__ fra
         if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(sse, red[0]);
+    if (threadIdx.x == 0) sse[0] = (accumulate ? sse[0] : 0.0f) + red[0];
 }
 
 __global__ void k_fill(float* __restrict__ p, int n, float v) {
@@ -883,19 +891,16 @@ int spl_train_step(spl_net* t, const float* d_in, const float* d_target, int n, 
     return SPL_OK;
 }
 
-int spl_judge_sse(spl_net* t, const float* d_frame, int w, int h, const float* d_sparse, int stride,
-                  int offset, float* d_sse, int accumulate, void* stream) {
+int spl_judge_sse(spl_net* t, const float* d_frame, int w, int h, const float* d_patches,
+                  int patch, int gx, int gy, float* d_sse, int accumulate, void* stream) {
     if (!t) return fail(nullptr, SPL_ERR_INVALID_ARG, "NULL network");
-    if (!d_frame || !d_sparse || !d_sse || w <= 0 || h <= 0 || stride < 1 || offset < 0 ||
-        offset >= stride)
+    if (!d_frame || !d_patches || !d_sse || w <= 0 || h <= 0 || patch < 1 || gx < 1 || gy < 1 ||
+        w / gx < patch || h / gy < patch)
         return fail(t, SPL_ERR_INVALID_ARG, "invalid judge arguments");
     DeviceGuard g(t->device);
     hipStream_t s = stream ? (hipStream_t)stream : t->stream;
-    const int sw = w / stride, sh = h / stride;
-    if (!accumulate) hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, d_sse, 1, 0.0f);
-    if (sw > 0 && sh > 0)
-        hipLaunchKernelGGL(k_judge_sse, dim3((sw * sh + 255) / 256), dim3(256), 0, s, d_frame, w, d_sparse,
-                           sw, sh, stride, offset, d_sse);
+    hipLaunchKernelGGL(k_judge_patches, dim3(1), dim3(256), 0, s, d_frame, w, h, d_patches, patch,
+                       gx, gy, accumulate, d_sse);
     HIPCALL(t, hipGetLastError());
     return SPL_OK;
 }

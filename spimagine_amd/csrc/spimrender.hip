@@ -201,6 +201,7 @@ RayParams make_params(const spr_params& prm, int tile_map, int w, int h) {
     p.ntiles = p.ntx * p.nty;
     p.tile_map = (uint32_t)tile_map;
     p.row_pad = 0;
+    p.box16 = 1;
     return p;
 }
 
@@ -332,9 +333,16 @@ void launch_staged_ch(int slab_kb, bool shared, dim3 grid, hipStream_t s, Args..
         launch_staged_sh<T, P2, CH, false>(slab_kb, grid, s, args...);
 }
 
-// samples per staged box: the tuning value, else from the ray step
+// samples per unrolled group of the staged kernels: the tuning value (32 and 64 are boxes
+// of 2 and 4 groups of 16), else from the ray step
 int chunk_for(const spr_ctx* c, int nouter, int nmax) {
-    return c->chunk ? c->chunk : staged_chunk(nouter, nmax);
+    return c->chunk ? std::min(c->chunk, 16) : staged_chunk(nouter, nmax);
+}
+
+// the staged kernels' launch options carried in RayParams
+void staged_tuning(const spr_ctx* c, RayParams& p) {
+    p.row_pad = (uint32_t)c->row_pad;
+    p.box16 = c->chunk > 16 ? (uint32_t)(c->chunk / 16) : 1u;
 }
 
 // volumes above 4 GiB: per-wave 12 KiB slabs only
@@ -386,7 +394,7 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
                 int current_part, int w, int h, float* out, float* alpha, float* depth,
                 float* normals, float* occ, float* tmp, float* tmp_vec, hipStream_t s) {
     RayParams p = make_params(prm, c->tile_map, w, h);
-    p.row_pad = (uint32_t)c->row_pad;
+    staged_tuning(c, p);
     const dim3 grid(tile_grid(p.tile_map, p.ntx, p.nty)), block(256);
     const bool nearest = prm.filter == SPR_FILTER_NEAREST;
     int rc;
@@ -852,8 +860,8 @@ int spr_set_tuning(spr_ctx* c, int key, int value) {
             c->staging = value;
             return SPR_OK;
         case SPR_TUNE_CHUNK:
-            if (value != 0 && value != 4 && value != 8 && value != 16)
-                return fail(c, SPR_ERR_INVALID_ARG, "chunk %d (0, 4, 8 or 16)", value);
+            if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32 && value != 64)
+                return fail(c, SPR_ERR_INVALID_ARG, "chunk %d (0, 4, 8, 16, 32 or 64)", value);
             c->chunk = value;
             return SPR_OK;
         case SPR_TUNE_ROW_PAD:
@@ -1150,7 +1158,7 @@ int spr_render_batch(spr_ctx* c, const spr_params* params, const spr_job* jobs, 
     uint32_t* d_counters = (uint32_t*)((char*)buf + jbytes);
     hipError_t e = hipMemcpyAsync(d_jobs, host.data(), jbytes, hipMemcpyHostToDevice, s);
     RayParams p = make_params(prm, c->tile_map, w, h);
-    p.row_pad = (uint32_t)c->row_pad;
+    staged_tuning(c, p);
     p.nouter = (prm.max_steps / num_parts) / kLoopUnroll;
     p.current_part = current_part;
     const BatchQueues q = make_queues(p, n);
@@ -1183,7 +1191,7 @@ int spr_render_coarse_pair(spr_ctx* c, int slot_a, int slot_b, const spr_params*
     DeviceGuard g(c->device);
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     RayParams p = make_params(*params, c->tile_map, w, h);
-    p.row_pad = (uint32_t)c->row_pad;
+    staged_tuning(c, p);
     p.nouter = params->max_steps / kLoopUnroll;
     p.current_part = 0;
     const dim3 grid((unsigned)(((w + 7) / 8) * ((h + 7) / 8)), 2), block(256);
@@ -1250,7 +1258,7 @@ int spr_batch_graph_create(spr_ctx* c, const spr_job* jobs, int n, int w, int h,
     rc = check_parts(c, c->params, 1);
     if (rc) return cleanup(rc);
     RayParams p = make_params(c, w, h);
-    p.row_pad = (uint32_t)c->row_pad;
+    staged_tuning(c, p);
     p.nouter = c->params.max_steps / kLoopUnroll;
     p.current_part = 0;
     const BatchQueues q = make_queues(p, n);
@@ -1338,7 +1346,7 @@ int spr_count_samples(spr_ctx* c, int num_parts, int w, int h, double* samples) 
     if (rc) return rc;
     DeviceGuard g(c->device);
     RayParams p = make_params(c, w, h);
-    p.row_pad = (uint32_t)c->row_pad;
+    staged_tuning(c, p);
     HIPCALL(c, hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_count_hits, dim3(tile_grid(p.tile_map, p.ntx, p.nty)), dim3(256), 0,
                        c->stream, p, c->d_count);

@@ -73,7 +73,7 @@ def test_learn_library_exports_every_declared_symbol():
     L = learned.load()
     for name in declared:
         assert hasattr(L, name), name
-    assert L.spl_abi_version() == 1
+    assert L.spl_abi_version() == 2
     assert L.spl_param_count() == learned.param_count() == 84996
 
 

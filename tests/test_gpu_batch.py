@@ -66,18 +66,20 @@ def single(torch, vols, f, W, H, alpha_pow, filt, tuning=None):
     return o.cpu().numpy(), a.cpu().numpy()
 
 
-@pytest.mark.parametrize("dtype,alpha_pow,interp,staging", [(np.uint16, 0., "linear", 0),
-                                                             (np.uint8, 0., "linear", 0),
-                                                             (np.float32, 0., "linear", 0),
-                                                             (np.uint16, .5, "linear", 0),
-                                                             (np.float32, 0., "nearest", 0),
-                                                             (np.uint16, 0., "linear", 1),
-                                                             (np.float32, 0., "linear", 1)])
-def test_batch_equals_separate_renders(torch, dtype, alpha_pow, interp, staging):
+@pytest.mark.parametrize("dtype,alpha_pow,interp,staging,chunk", [
+    (np.uint16, 0., "linear", 0, 0), (np.uint8, 0., "linear", 0, 0),
+    (np.float32, 0., "linear", 0, 0), (np.uint16, .5, "linear", 0, 0),
+    (np.float32, 0., "nearest", 0, 0), (np.uint16, 0., "linear", 1, 0),
+    (np.float32, 0., "linear", 1, 0), (np.uint16, 0., "linear", 0, 32),
+    (np.uint8, 0., "linear", 0, 64)])
+def test_batch_equals_separate_renders(torch, dtype, alpha_pow, interp, staging, chunk):
+    """max_steps 128: 9 groups of 16 samples, so boxes of 2 or 4 groups (chunk 32, 64) end
+    with a shorter box."""
     W, H = 150, 117
     shapes = [(40, 64, 48), (33, 50, 61), (64, 64, 64)]
     ctx, vols, frames, filt = setup(torch, dtype, shapes, 11, W, H, alpha_pow, interp)
     ctx.set_tuning("staging", staging)
+    ctx.set_tuning("chunk", chunk)
     jobs = [(f["slot"], f["iP"], f["iM"], f["out"].data_ptr(), f["alpha"].data_ptr()) for f in frames]
     ctx.render_batch(jobs, W, H)
     torch.cuda.synchronize()

@@ -152,12 +152,14 @@ def test_staged_mip_matches_global(VR, dtype, shape, pose):
          np.dot(mat4_translate(0, 0, -1.2), mat4_rotation(.8, .2, 1, .1)),  # eye close to the box
          mat4_identity()][pose]                                          # eye inside the box
     frames = {}
-    for slab, staging, pad in ((0, 0, 0), (8, 0, 0), (12, 0, 0), (16, 0, 0), (8, 1, 0), (12, 1, 0),
-                               (16, 0, 1), (12, 1, 1)):
+    for slab, staging, pad, chunk in ((0, 0, 0, 0), (8, 0, 0, 0), (12, 0, 0, 0), (16, 0, 0, 0),
+                                      (8, 1, 0, 0), (12, 1, 0, 0), (16, 0, 1, 0), (12, 1, 1, 0),
+                                      (16, 0, 0, 32), (16, 0, 0, 64), (12, 1, 0, 32)):
         rend = VR((130, 97))
         rend._ctx.set_tuning("slab_kb", slab)
         rend._ctx.set_tuning("staging", staging)
         rend._ctx.set_tuning("row_pad", pad)
+        rend._ctx.set_tuning("chunk", chunk)
         rend.set_max_steps(160)
         rend.set_data(vol)
         rend.set_modelView(M)
@@ -166,11 +168,11 @@ def test_staged_mip_matches_global(VR, dtype, shape, pose):
         for part in range(3):
             rend.render(numParts=3, currentPart=part)
         res.append(rend.output.copy())
-        frames[slab, staging, pad] = res
+        frames[slab, staging, pad, chunk] = res
     for key in frames:
-        for a, b in zip(frames[0, 0, 0], frames[key]):
+        for a, b in zip(frames[0, 0, 0, 0], frames[key]):
             np.testing.assert_array_equal(a, b)
-    assert (frames[0, 0, 0][0] > 0).sum() > 200
+    assert (frames[0, 0, 0, 0][0] > 0).sum() > 200
 
 
 @pytest.mark.parametrize("dtype", [np.uint16, np.uint8, np.float32])
@@ -183,7 +185,7 @@ def test_staged_mip_long_steps(VR, dtype, max_steps):
     M = np.dot(mat4_translate(0, .1, -3.5), mat4_rotation(.7, .3, 1, .2))
     frames = {}
     for slab, staging, chunk in ((0, 0, 0), (12, 0, 0), (12, 1, 0), (12, 0, 16), (12, 1, 4),
-                                 (8, 0, 8)):
+                                 (8, 0, 8), (16, 0, 32), (16, 0, 64)):
         rend = VR((120, 104))
         rend._ctx.set_tuning("slab_kb", slab)
         rend._ctx.set_tuning("staging", staging)
