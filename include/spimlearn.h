@@ -77,13 +77,14 @@ int spl_train_step(spl_net *net, const float *d_in, const float *d_target, int n
                    float lr, double *sse, void *stream);
 
 /* The per-frame judge's error against gx x gy exact patches of patch x patch pixels (an
-   exact render of those pixels, e.g. one batched launch): patch (ty, tx) covers the frame
-   pixels from row ty (h / gy) + (h / gy - patch) / 2 and column tx (w / gx) +
-   (w / gx - patch) / 2 on; d_patches holds the patches one after another (tx fastest),
-   rows of `patch` floats.  *d_sse (one device float) = (accumulate ? *d_sse : 0) + the
+   exact render of those pixels): patch (ty, tx) covers the frame pixels from row
+   ty (h / gy) + oy and column tx (w / gx) + ox on; d_patches holds the patches one after
+   another (tx fastest), rows of `patch` floats (patch 1: a (w / gx) x (h / gy)-strided
+   sparse frame, row-major).  *d_sse (one device float) = (accumulate ? *d_sse : 0) + the
    summed squared difference, summed in a fixed order.  Enqueued on `stream`. */
 int spl_judge_sse(spl_net *net, const float *d_frame, int w, int h, const float *d_patches,
-                  int patch, int gx, int gy, float *d_sse, int accumulate, void *stream);
+                  int patch, int gx, int gy, int ox, int oy, float *d_sse, int accumulate,
+                  void *stream);
 
 /* d_dst (2h x 2w) = nearest-neighbour x2 upsampling of d_src (h x w), on `stream` (the
    learned frame's alpha from the coarse pass). */

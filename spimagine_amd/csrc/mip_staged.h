@@ -275,8 +275,8 @@ __device__ __forceinline__ void staged_ray_end(const RayParams& p, uint32_t o, f
 // depend on how the sequence is cut); a smaller CH keeps a box small when the ray step is
 // several voxels long, a larger box spreads the box reduction and copy over more samples
 // when it is short.  With per-wave slabs and CH 16, a box larger than the slab is split
-// into halves of 8 samples and those into quarters of 4, each staged on its own; only a
-// quarter whose box still does not fit is sampled from global memory.
+// into halves of 8 samples and those into quarters of 4, each staged on its own (when it
+// is at most 4x the slab); only what still does not fit is sampled from global memory.
 // SHARED = false: each wave stages the box of its own 64 rays into its own slab
 // (SLAB_BYTES, wave-level ordering only).  SHARED = true: the workgroup's four waves
 // stage the union box of its 256 rays into one slab of SLAB_BYTES with all 256 threads,
@@ -325,7 +325,7 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
     // nothing staged, when the box does not fit; ends with the slab visible to the wave (the
     // workgroup when SHARED).
     auto stage = [&](int n, uint32_t& lox, uint32_t& LY, uint32_t& LZ, uint32_t& rowb,
-                     uint32_t& SY) -> bool {
+                     uint32_t& SY, uint64_t& bytes) -> bool {
         uint32_t b0, b1, b2;
         staged_box_n<PV>(vol, active, (float)(n - 1), px, py, pz, dx, dy, dz, mgx, mgy, mgz, b0,
                          b1, b2);
@@ -358,8 +358,8 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
         // slab row stride: SX voxels, plus 16 bytes when row_pad asks for an odd number of
         // 16-byte units (rows a few apart then fall on different LDS banks)
         rowb = (nu + (p.row_pad & ~nu & 1u)) * 16u;
-        const uint64_t bytes = (uint64_t)rowb * SY * SZ;
-        const bool fits = bytes <= (uint64_t)SLAB_BYTES && nu <= NT;
+        bytes = nu <= NT ? (uint64_t)rowb * SY * SZ : ~0ull;
+        const bool fits = bytes <= (uint64_t)SLAB_BYTES;
         // rows addressed from plane LZ on (32-bit offsets whatever the volume's size)
         if (fits)
             staged_copy_rows<NT>(vol.rsrc_plane(LZ), slab, SHARED ? threadIdx.x : lane, LY, 0u,
@@ -386,7 +386,8 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
     for (int i = 0; i < ngroups; i += G) {
         const int ng = min(G, ngroups - i);
         uint32_t lox, LY, LZ, rowb, SY;
-        if (stage(ng * CH, lox, LY, LZ, rowb, SY)) {
+        uint64_t bytes;
+        if (stage(ng * CH, lox, LY, LZ, rowb, SY, bytes)) {
             if (live) {
 #pragma unroll 1
                 for (int k = 0; k < ng; ++k)
@@ -397,25 +398,34 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
             continue;
         }
         if constexpr (CH == 16 && !SHARED) {
-            // the box is larger than the slab: halves of 8 samples with a box each, then
-            // quarters of 4; quarters that still do not fit sample global memory
+            if (bytes <= (uint64_t)(4 * ng * SLAB_BYTES)) {
+                // a box up to 4x the slab: halves of 8 samples with a box each, then quarters
+                // of 4; a quarter that still does not fit samples global memory.  (Rays far
+                // apart, e.g. a sparse frame, give boxes that no split brings under the
+                // slab: those go to global memory directly.)
 #pragma unroll 1
-            for (int h = 0; h < 2 * ng; ++h) {
-                if (stage(8, lox, LY, LZ, rowb, SY)) {
-                    staged_sample_slab<PV, 8>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy,
-                                              dz, col);
-                    release();
-                    continue;
-                }
-#pragma unroll 1
-                for (int q = 0; q < 2; ++q) {
-                    if (stage(4, lox, LY, LZ, rowb, SY))
-                        staged_sample_slab<PV, 4>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx,
+                for (int h = 0; h < 2 * ng; ++h) {
+                    if (stage(8, lox, LY, LZ, rowb, SY, bytes)) {
+                        staged_sample_slab<PV, 8>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx,
                                                   dy, dz, col);
-                    else
-                        staged_sample_global<PV, 4>(vol, px, py, pz, dx, dy, dz, col);
-                    release();
+                        release();
+                        continue;
+                    }
+#pragma unroll 1
+                    for (int q = 0; q < 2; ++q) {
+                        if (stage(4, lox, LY, LZ, rowb, SY, bytes))
+                            staged_sample_slab<PV, 4>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz,
+                                                      dx, dy, dz, col);
+                        else
+                            staged_sample_global<PV, 4>(vol, px, py, pz, dx, dy, dz, col);
+                        release();
+                    }
                 }
+            } else {
+#pragma unroll 1
+                for (int k = 0; k < ng; ++k)
+                    staged_sample_global<PV, CH>(vol, px, py, pz, dx, dy, dz, col);
+                release();
             }
         } else {
             if (live) {

@@ -487,17 +487,16 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 
 // ------------------------------------------------------------------ judge helpers
 // Squared error of a frame (h x w) against gx x gy exact patches of c x c pixels: patch
-// (ty, tx) covers the frame pixels from row ty (h / gy) + (h / gy - c) / 2 and column
-// tx (w / gx) + (w / gx - c) / 2 on; the patches are stored one after another (tx
-// fastest), rows of c.  One workgroup sums everything in a fixed order (deterministic):
-// *sse = (accumulate ? *sse : 0) + the sum.
+// (ty, tx) covers the frame pixels from row ty (h / gy) + oy and column tx (w / gx) + ox
+// on; the patches are stored one after another (tx fastest), rows of c.  One workgroup
+// sums everything in a fixed order (deterministic): *sse = (accumulate ? *sse : 0) + sum.
 __global__ __launch_bounds__(256) void k_judge_patches(const float* __restrict__ frame, int w,
                                                        int h, const float* __restrict__ patches,
-                                                       int c, int gx, int gy, int accumulate,
-                                                       float* __restrict__ sse) {
+                                                       int c, int gx, int gy, int ox, int oy,
+                                                       int accumulate, float* __restrict__ sse) {
     __shared__ float red[256];
     const int cc = c * c, n = gx * gy * cc;
-    const int cw = w / gx, ch = h / gy, ox = (cw - c) / 2, oy = (ch - c) / 2;
+    const int cw = w / gx, ch = h / gy;
     float v = 0.0f;
     for (int i = threadIdx.x; i < n; i += 256) {
         const int k = i / cc, r = i - k * cc;
@@ -892,15 +891,16 @@ int spl_train_step(spl_net* t, const float* d_in, const float* d_target, int n, 
 }
 
 int spl_judge_sse(spl_net* t, const float* d_frame, int w, int h, const float* d_patches,
-                  int patch, int gx, int gy, float* d_sse, int accumulate, void* stream) {
+                  int patch, int gx, int gy, int ox, int oy, float* d_sse, int accumulate,
+                  void* stream) {
     if (!t) return fail(nullptr, SPL_ERR_INVALID_ARG, "NULL network");
     if (!d_frame || !d_patches || !d_sse || w <= 0 || h <= 0 || patch < 1 || gx < 1 || gy < 1 ||
-        w / gx < patch || h / gy < patch)
+        ox < 0 || oy < 0 || ox + patch > w / gx || oy + patch > h / gy)
         return fail(t, SPL_ERR_INVALID_ARG, "invalid judge arguments");
     DeviceGuard g(t->device);
     hipStream_t s = stream ? (hipStream_t)stream : t->stream;
     hipLaunchKernelGGL(k_judge_patches, dim3(1), dim3(256), 0, s, d_frame, w, h, d_patches, patch,
-                       gx, gy, accumulate, d_sse);
+                       gx, gy, ox, oy, accumulate, d_sse);
     HIPCALL(t, hipGetLastError());
     return SPL_OK;
 }

@@ -5,8 +5,8 @@ A frame is first rendered coarse -- half the resolution and a quarter of the ray
 bounds the samples the coarse step skips.  A small convolutional network on the matrix
 cores turns those two planes into the full-resolution frame.  The exact renderer is the
 reference: the network is trained on coarse/fine pairs the renderer produces, and
-``LearnedMIP.render_judged`` checks every frame against the exact renderer on a grid
-of pixel patches and falls back to the exact frame below the PSNR threshold (``validate``
+``LearnedMIP.render_judged`` checks every frame against the exact renderer on a sparse
+pixel grid and falls back to the exact frame below the PSNR threshold (``validate``
 measures full-frame PSNR on a set of poses).  No reference counterpart: the nearest analogue is
 progressive multipass rendering (spimagine/volumerender/kernels/volume_kernel.cl:100-106,
 driven from spimagine/gui/glwidget.py:633-635).
@@ -59,7 +59,7 @@ def load():
     L.spl_train_step.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_float,
                                  ctypes.POINTER(ctypes.c_double), c_void_p]
     L.spl_judge_sse.argtypes = [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int,
-                                c_void_p, c_int, c_void_p]
+                                c_int, c_int, c_void_p, c_int, c_void_p]
     L.spl_upsample2.argtypes = [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]
     _lib_learn = L
     return L
@@ -154,11 +154,11 @@ class Net(object):
                                            ctypes.byref(sse), stream))
         return g, sse.value
 
-    def judge_sse(self, d_frame, w, h, d_patches, patch, grid, d_sse, accumulate=False,
-                  stream=None):
-        """Squared error of a frame against grid x grid exact patches (spl_judge_sse)."""
-        self._check(self.lib.spl_judge_sse(self.handle, d_frame, w, h, d_patches, patch, grid,
-                                           grid, d_sse, int(bool(accumulate)), stream))
+    def judge_sse(self, d_frame, w, h, d_patches, patch, gx, gy, ox, oy, d_sse,
+                  accumulate=False, stream=None):
+        """Squared error of a frame against gx x gy exact patches (spl_judge_sse)."""
+        self._check(self.lib.spl_judge_sse(self.handle, d_frame, w, h, d_patches, patch, gx, gy,
+                                           ox, oy, d_sse, int(bool(accumulate)), stream))
 
     def upsample2(self, d_src, w, h, d_dst, stream=None):
         self._check(self.lib.spl_upsample2(self.handle, d_src, w, h, d_dst, stream))
@@ -190,34 +190,25 @@ def psnr(a, b):
     return float("inf") if mse == 0 else 10. * math.log10(1. / mse)
 
 
-JUDGE_PATCH = 16   # the judge traces 16 x 16 pixel patches of every frame ...
-JUDGE_GRID = 8     # ... one centred in each cell of an 8 x 8 grid (1/64 of a 1024^2 frame)
+JUDGE_STRIDE = 8
+JUDGE_OFFSET = 1   # judge pixels (8i+1, 8j+1): between the coarse rays, where errors are largest
 
 
-def judge_patch_origins(w, h, patch=JUDGE_PATCH, grid=JUDGE_GRID):
-    """Top-left pixels (x0, y0) of the judge's patches, tx fastest (spl_judge_sse's order)."""
-    cw, ch = w // grid, h // grid
-    return [(tx * cw + (cw - patch) // 2, ty * ch + (ch - patch) // 2)
-            for ty in range(grid) for tx in range(grid)]
-
-
-def patch_inv_p(inv_p, w, h, origins, patch=JUDGE_PATCH):
-    """Inverse projections (n, 16) of patch x patch renders whose pixel (i, j) traces the ray
-    of pixel (x0 + j, y0 + i) of the w x h frame.  A frame maps pixel x to clip
-    u = x / W * 2 - 1; the patch's u_c = j / patch * 2 - 1 becomes the frame's
-    u = (patch / w) (u_c + 1) - 1 + 2 x0 / w (likewise v), so inv_p is multiplied by that
-    clip-space scale and translation.  The rays agree with the frame's up to float32
-    rounding of the composed matrix."""
-    iP = np.asarray(list(inv_p), np.float64).reshape(4, 4)
-    o = np.asarray(origins, np.float64)
-    sx, sy = patch / float(w), patch / float(h)
-    tx = sx - 1. + 2. * o[:, 0] / w
-    ty = sy - 1. + 2. * o[:, 1] / h
-    out = np.repeat(iP[None], len(o), 0)
-    out[:, :, 0] = iP[:, 0] * sx
-    out[:, :, 1] = iP[:, 1] * sy
-    out[:, :, 3] = iP[:, 3] + iP[None, :, 0] * tx[:, None] + iP[None, :, 1] * ty[:, None]
-    return out.reshape(len(o), 16).astype(np.float32)
+def sparse_params(p, w, h, stride=JUDGE_STRIDE, offset=JUDGE_OFFSET):
+    """Parameters of a (w/stride) x (h/stride) exact render whose pixel (i, j) traces the ray
+    of pixel (stride i + offset, stride j + offset) of the w x h frame: the pixel mapping
+    u = x / W * 2 - 1 of the small frame is shifted by 2 offset / w in clip space, i.e.
+    inv_p is multiplied by a clip-space translation (offset 0: the rays of pixels
+    (stride i, stride j) exactly, bit for bit)."""
+    q = _lib.SprParams()
+    ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(q))
+    if offset:
+        T = np.eye(4)
+        T[0, 3] = 2. * offset / w
+        T[1, 3] = 2. * offset / h
+        iP = np.asarray(list(p.inv_p), np.float64).reshape(4, 4).dot(T)
+        q.inv_p[:] = [float(v) for v in iP.astype(np.float32).ravel()]
+    return q
 
 
 def coarse_params(p):
@@ -288,48 +279,46 @@ class LearnedMIP(object):
         return kept[0], est[0]
 
     def render_judged_many(self, slot, plist, outs, alphas, stream, margin=0.5):
-        """Frames with a per-frame judge: the learned frames, then the exact renderer on
-        JUDGE_GRID^2 patches of JUDGE_PATCH^2 pixels of every frame (1/64 of a 1024^2
-        frame's rays; all frames' patches in one batched launch per group of frames that
-        share their non-pose parameters) and each learned frame's PSNR on those pixels,
-        computed on the device; the exact full frame for every frame whose estimate is below
-        threshold + margin.  Patches hold whole pixel blocks, so every phase of the x2
-        upsampling counts.  outs / alphas: (h, w) float32 torch tensors; stream: a
-        torch.cuda.Stream.  Learned frames take alpha from the coarse pass (exact at even
-        pixels, nearest-upsampled).  Returns (kept flags, estimated PSNRs).  The slot must
-        be batchable (padded layout, below 4 GiB)."""
+        """Frames with a per-frame judge: the learned frames, then the exact renderer on one
+        pixel in 8 x 8 of every frame (a (w/8) x (h/8) render, sparse_params; the sparse
+        frames of all frames that share their non-pose parameters in one batched launch)
+        and each learned frame's PSNR on those pixels, computed on the device; the exact
+        full frame for every frame whose estimate is below threshold + margin.  A scattered
+        grid: the error of a learned frame sits in few pixels, and 1/64 of the pixels
+        estimates the PSNR well only when they are spread over the whole frame (pixel
+        blocks of the same count miss or over-weight the few bright structures).
+        outs / alphas: (h, w) float32 torch tensors; stream: a torch.cuda.Stream.  Learned
+        frames take alpha from the coarse pass (exact at even pixels, nearest-upsampled).
+        Returns (kept flags, estimated PSNRs).  The slot must be batchable (padded layout,
+        below 4 GiB)."""
         h, w = outs[0].shape
-        c, g = JUDGE_PATCH, JUDGE_GRID
-        if w % 8 or h % 8 or w < g * c or h < g * c:
-            raise ValueError("judged frames need w and h multiples of 8 and at least %d" % (g * c))
+        S = JUDGE_STRIDE
+        if w % 64 or h % 64:
+            raise ValueError("judged frames need w and h multiples of 64")
         sh = stream.cuda_stream
         n = len(plist)
-        origins = judge_patch_origins(w, h)
-        npat = len(origins)
-        patches = self._buffer(("judge", n, w, h), (n, npat, c, c))
-        palpha = self._buffer(("judge_alpha", n, w, h), (n, npat, c, c))
+        sw, sh_ = w // S, h // S
+        sparse = self._buffer(("judge", n, w, h), (n, sh_, sw))
+        salpha = self._buffer(("judge_alpha", n, w, h), (n, sh_, sw))
         sse = self._buffer(("judge_sse", n), (n,))
         coarse_alpha = self._buffer(("alpha", w, h), (h // 2, w // 2))
         for p, out, alpha in zip(plist, outs, alphas):
             self.render(slot, p, w, h, out.data_ptr(), sh)
             self.net.upsample2(coarse_alpha.data_ptr(), w // 2, h // 2, alpha.data_ptr(), sh)
-        # the exact patches (spr_render_batch returns once its launch has completed)
+        # the sparse exact frames (spr_render_batch returns once its launch has completed)
         groups = {}
         for k, p in enumerate(plist):
             groups.setdefault(bytes(p)[_lib.SprParams.box.offset:], []).append(k)
-        pb, ab, step = patches.data_ptr(), palpha.data_ptr(), 4 * c * c
         for ks in groups.values():
-            ip = np.concatenate([patch_inv_p(plist[k].inv_p, w, h, origins) for k in ks])
-            im = np.repeat(np.asarray([list(plist[k].inv_m) for k in ks], np.float32), npat, 0)
-            idx = np.asarray([k * npat + t for k in ks for t in range(npat)], np.uint64)
-            jobs = _lib.jobs_from_arrays(np.full(len(idx), slot, np.int32), ip, im,
-                                         pb + idx * step, ab + idx * step)
-            self.ctx.render_batch(jobs, c, c, params=plist[ks[0]], stream=sh)
+            qs = [sparse_params(plist[k], w, h) for k in ks]
+            jobs = [(slot, list(q.inv_p), list(q.inv_m), sparse[k].data_ptr(),
+                     salpha[k].data_ptr()) for k, q in zip(ks, qs)]
+            self.ctx.render_batch(jobs, sw, sh_, params=qs[0], stream=sh)
         for k, out in enumerate(outs):
-            self.net.judge_sse(out.data_ptr(), w, h, patches[k].data_ptr(), c, g,
-                               sse.data_ptr() + 4 * k, stream=sh)
+            self.net.judge_sse(out.data_ptr(), w, h, sparse[k].data_ptr(), 1, sw, sh_,
+                               JUDGE_OFFSET, JUDGE_OFFSET, sse.data_ptr() + 4 * k, stream=sh)
         stream.synchronize()
-        m = sse.cpu().numpy() / float(npat * c * c)
+        m = sse.cpu().numpy() / float(sw * sh_)
         est = [float("inf") if v == 0 else 10. * math.log10(1. / float(v)) for v in m]
         kept = [e >= self.threshold + margin for e in est]
         for p, out, alpha, ok in zip(plist, outs, alphas, kept):

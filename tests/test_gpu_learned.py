@@ -197,12 +197,12 @@ def test_learned_render_path_shapes_and_fallback(torch):
 
 
 @pytest.mark.parametrize("W,H", [(128, 128), (256, 192)])
-def test_judge_patches_are_the_full_frame_pixels(torch, W, H):
-    """The judge's exact patches (patch_inv_p: a clip-space scale and shift of inv_p, all
-    patches of a frame in one batched launch) trace the full frame's rays at their pixels
-    up to float rounding; its PSNR estimate is the learned frame's PSNR on exactly those
-    pixels; render_judged falls back to the exact frame below its threshold and keeps the
-    learned frame above it."""
+def test_sparse_judge_rays_are_the_full_frame_pixels(torch, W, H):
+    """A (w/8) x (h/8) render with the same matrices traces the rays of pixels (8i, 8j) of
+    the w x h frame, bit for bit (the judge shifts that grid to (8i+1, 8j+1), up to float
+    rounding); the judge's estimate is the learned frame's PSNR on those pixels;
+    render_judged falls back to the exact frame below its threshold and keeps the learned
+    frame above it."""
     vol = synth.blob_volume((64, 80, 72), n_blobs=14, sigma=(3., 6.), amp=(500., 3000.),
                             background=100., poisson=True, read_noise=5., dtype="uint16", seed=2)
     ctx = _lib.Context(0)
@@ -217,20 +217,14 @@ def test_judge_patches_are_the_full_frame_pixels(torch, W, H):
     s = torch.cuda.Stream()
     full = torch.empty((H, W), device="cuda")
     fa = torch.empty_like(full)
+    sparse = torch.empty((H // 8, W // 8), device="cuda")
+    sa = torch.empty_like(sparse)
     ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W, H, full.data_ptr(), fa.data_ptr(),
                     stream=s.cuda_stream)
-    c, g = learned.JUDGE_PATCH, learned.JUDGE_GRID
-    origins = learned.judge_patch_origins(W, H)
-    assert len(origins) == g * g
-    patches = torch.empty((g * g, c, c), device="cuda")
-    pa = torch.empty_like(patches)
-    ip = learned.patch_inv_p(p.inv_p, W, H, origins)
-    jobs = [(slot, ip[t], iM, patches[t].data_ptr(), pa[t].data_ptr()) for t in range(g * g)]
-    ctx.render_batch(jobs, c, c, params=p, stream=s.cuda_stream)
+    ctx.render_slot(slot, learned.sparse_params(p, W, H, offset=0), _lib.METHOD_MAX_PROJECT, W // 8,
+                    H // 8, sparse.data_ptr(), sa.data_ptr(), stream=s.cuda_stream)
     s.synchronize()
-    for t, (x0, y0) in enumerate(origins):
-        crop = full[y0:y0 + c, x0:x0 + c]
-        assert float((patches[t] - crop).abs().max()) < 1e-3, t
+    assert torch.equal(sparse, full[::8, ::8]) and torch.equal(sa, fa[::8, ::8])
     out = torch.empty_like(full)
     alpha = torch.empty_like(full)
     strict = learned.LearnedMIP(ctx, params=learned.init_params(0), threshold=float("inf"))
@@ -245,13 +239,42 @@ def test_judge_patches_are_the_full_frame_pixels(torch, W, H):
     s.synchronize()
     assert kept is True and abs(est2 - est) < 1e-6
     assert torch.equal(out, ref)
-    # the estimate is the PSNR on the patches' pixels (against the patches' values)
-    sq = sum(float(((ref[y0:y0 + c, x0:x0 + c].double() - patches[t].double()) ** 2).sum())
-             for t, (x0, y0) in enumerate(origins))
-    mse = sq / (g * g * c * c)
+    # the judge's pixels: (8i+1, 8j+1), traced by the shifted sparse render
+    shifted = torch.empty_like(sparse)
+    ctx.render_slot(slot, learned.sparse_params(p, W, H), _lib.METHOD_MAX_PROJECT, W // 8, H // 8,
+                    shifted.data_ptr(), sa.data_ptr(), stream=s.cuda_stream)
+    s.synchronize()
+    mse = float(((ref[1::8, 1::8].double() - shifted.double()) ** 2).mean())
     assert abs(est2 - 10 * np.log10(1. / mse)) < 1e-3
+    # and they are the full frame's rays at those pixels up to float rounding
+    assert float((shifted - full[1::8, 1::8]).abs().max()) < 1e-3
     # learned frames carry the coarse pass's alpha: exact at even pixels
     assert torch.equal(alpha[::2, ::2], fa[::2, ::2])
+
+
+def test_judge_sse_patch_layout(torch):
+    """spl_judge_sse on patches of 3 x 3 pixels at offset (2, 1) of 5 x 4 cells of a frame
+    of arange values, against the same sum on the host."""
+    net = learned.Net(0, learned.init_params(0))
+    w, h, c, gx, gy, ox, oy = 40, 24, 3, 5, 4, 2, 1
+    frame = torch.arange(w * h, device="cuda", dtype=torch.float32).reshape(h, w) / (w * h)
+    pat = torch.rand((gy * gx, c, c), device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+    sse = torch.full((2,), 5., device="cuda")
+    torch.cuda.synchronize()
+    net.judge_sse(frame.data_ptr(), w, h, pat.data_ptr(), c, gx, gy, ox, oy, sse.data_ptr())
+    net.judge_sse(frame.data_ptr(), w, h, pat.data_ptr(), c, gx, gy, ox, oy, sse.data_ptr() + 4,
+                  accumulate=True)
+    torch.cuda.synchronize()
+    F, Pt = frame.cpu().double().numpy(), pat.cpu().double().numpy()
+    want = 0.
+    for ty in range(gy):
+        for tx in range(gx):
+            y0, x0 = ty * (h // gy) + oy, tx * (w // gx) + ox
+            want += ((F[y0:y0 + c, x0:x0 + c] - Pt[ty * gx + tx]) ** 2).sum()
+    got = sse.cpu().double().numpy()
+    assert abs(got[0] - want) < 1e-4 * want and abs(got[1] - 5. - want) < 1e-4 * want
+    with pytest.raises(learned.SplError):
+        net.judge_sse(frame.data_ptr(), w, h, pat.data_ptr(), c, gx, gy, 6, oy, sse.data_ptr())
 
 
 @pytest.mark.parametrize("dtype,shape", [(np.uint16, (64, 80, 72)), (np.float32, (48, 64, 64)),
