@@ -548,8 +548,10 @@ def bench_learned(args):
         "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
         "config": {"workload": "MIP 512^3 uint16 (2000 blobs, Poisson + read noise, held-out seed) "
                                "-> 1024^2, max_steps %d, 8 seeded poses; learned = two coarse "
-                               "512^2 renders at max_steps %d (inputs: %s) + conv-net"
-                               % (S, S // 4, lm.inputs),
+                               "512^2 planes at max_steps %d (inputs: %s%s) + conv-net"
+                               % (S, S // 4, lm.inputs,
+                                  ", one pair render of the interleaved 2x mean/max pooled volume"
+                                  if lm.attach(slot)[0] == lm.attach(slot)[1] else ""),
                    "training_pairs": int(w["pairs"]) if "pairs" in w else None},
         "exact_ms": round(exact_ms, 4), "exact_frames_per_s": round(1e3 / exact_ms, 2),
         "learned_ms": round(learned_ms, 4), "speedup": round(exact_ms / learned_ms, 3),

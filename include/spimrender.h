@@ -47,6 +47,10 @@ extern "C" {
 #define SPR_DTYPE_F32 0
 #define SPR_DTYPE_U16 1
 #define SPR_DTYPE_U8 2
+/* Two uint16 channels per voxel, channel 0 in the low half (4 bytes a voxel): only
+   spr_volume_add, spr_render_pair, spr_volume_info, spr_volume_read and spr_volume_release
+   accept such a slot (spr_volume_add_pooled_pair makes one). */
+#define SPR_DTYPE_U16X2 3
 
 /* sampler filter: interpolation_defines "linear" / "nearest" (volumerender.py:69-70) */
 #define SPR_FILTER_LINEAR 0
@@ -187,6 +191,10 @@ int spr_volume_add_dilated(spr_ctx *ctx, int src_slot, int radius, int *slot);
    1 = max (edges clamped); one level of a mean / max pyramid, computed on the device (the
    coarse inputs of the learned reconstruction, spimlearn.h). */
 int spr_volume_add_pooled(spr_ctx *ctx, int src_slot, int mode, int *slot);
+/* A new channel-pair slot (SPR_DTYPE_U16X2, padded layout) holding both 2x poolings of
+   the uint16 volume of slot src_slot: channel 0 = mode 0 (mean), channel 1 = mode 1 (max),
+   each equal to spr_volume_add_pooled's volume.  spr_render_pair draws both at once. */
+int spr_volume_add_pooled_pair(spr_ctx *ctx, int src_slot, int *slot);
 /* Copy a slot's volume back to host memory as a (nz,ny,nx) C-order array (bytes must be
    nx ny nz x voxel size); the readback of update_data's image (volumerender.py:294). */
 int spr_volume_read(spr_ctx *ctx, int slot, void *host, size_t bytes);
@@ -221,6 +229,15 @@ int spr_render_batch(spr_ctx *ctx, const spr_params *params, const spr_job *jobs
 int spr_render_slot(spr_ctx *ctx, int slot, const spr_params *params, int method, int num_parts,
                     int current_part, int w, int h, float *d_out, float *d_alpha, float *d_depth,
                     float *d_normals, float *d_occlusion, void *stream);
+
+/* Max projections (linear filter, alpha_pow 0) of both channels of a channel-pair slot
+   with one set of rays: d_out holds two w x h planes (channel 0's, then channel 1's),
+   d_alpha the alpha plane.  Each plane is bit-identical to spr_render_slot's max
+   projection of a one-channel volume holding that channel (same staged sampling, same
+   operation order); one box, one address and one LDS read per corner pair serve both.
+   Enqueued on `stream`.  The coarse input of the learned reconstruction (spimlearn.h). */
+int spr_render_pair(spr_ctx *ctx, int slot, const spr_params *params, int num_parts,
+                    int current_part, int w, int h, float *d_out, float *d_alpha, void *stream);
 
 /* Coarse max projections of two slots of one voxel type and shape (a volume and its
    spr_volume_add_dilated copy) with the same params, in one launch: d_out holds two w x h

@@ -19,7 +19,7 @@ SPR_ERR_OUT_OF_MEMORY = -4
 SPR_ERR_HIP = -5
 SPR_ERR_NO_DEVICE = -6
 
-DTYPE_F32, DTYPE_U16, DTYPE_U8 = 0, 1, 2
+DTYPE_F32, DTYPE_U16, DTYPE_U8, DTYPE_U16X2 = 0, 1, 2, 3
 DTYPE_CODES = {np.dtype(np.float32): DTYPE_F32, np.dtype(np.uint16): DTYPE_U16,
                np.dtype(np.uint8): DTYPE_U8}
 FILTER_LINEAR, FILTER_NEAREST = 0, 1
@@ -37,9 +37,10 @@ EXPORTS = (
     "spr_set_volume", "spr_set_volume_device", "spr_volume_minmax",
     "spr_set_params", "spr_render", "spr_render_device", "spr_synchronize", "spr_get_stream",
     "spr_count_samples", "spr_volume_add", "spr_volume_add_dilated", "spr_volume_add_pooled",
-    "spr_volume_read",
+    "spr_volume_add_pooled_pair", "spr_volume_read",
     "spr_volume_release", "spr_volume_info",
-    "spr_render_batch", "spr_render_slot", "spr_render_coarse_pair", "spr_batch_graph_create",
+    "spr_render_batch", "spr_render_slot", "spr_render_pair", "spr_render_coarse_pair",
+    "spr_batch_graph_create",
     "spr_batch_graph_update",
     "spr_batch_graph_launch", "spr_batch_graph_destroy",
 )
@@ -175,6 +176,7 @@ def load():
                                  ctypes.POINTER(c_int)]
     L.spr_volume_add_dilated.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(c_int)]
     L.spr_volume_add_pooled.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(c_int)]
+    L.spr_volume_add_pooled_pair.argtypes = [c_void_p, c_int, ctypes.POINTER(c_int)]
     L.spr_volume_read.argtypes = [c_void_p, c_int, c_void_p, ctypes.c_size_t]
     L.spr_volume_release.argtypes = [c_void_p, c_int]
     L.spr_volume_info.argtypes = [c_void_p, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
@@ -186,6 +188,8 @@ def load():
     L.spr_render_slot.argtypes = [c_void_p, c_int, ctypes.POINTER(SprParams), c_int, c_int, c_int,
                                   c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p]
+    L.spr_render_pair.argtypes = [c_void_p, c_int, ctypes.POINTER(SprParams), c_int, c_int, c_int,
+                                  c_int, c_void_p, c_void_p, c_void_p]
     L.spr_render_coarse_pair.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(SprParams), c_int,
                                          c_int, c_void_p, c_void_p, c_void_p]
     L.spr_batch_graph_create.argtypes = [c_void_p, jp, c_int, c_int, c_int,
@@ -343,10 +347,19 @@ class Context:
                                                    ctypes.byref(out)))
         return out.value
 
+    def volume_add_pooled_pair(self, slot):
+        """A new channel-pair slot: the 2x mean (channel 0) and max (channel 1) poolings of
+        the uint16 volume in `slot`, interleaved (spr_volume_add_pooled_pair)."""
+        out = ctypes.c_int()
+        self._check(self.lib.spr_volume_add_pooled_pair(self.handle, int(slot), ctypes.byref(out)))
+        return out.value
+
     def volume_read(self, slot):
-        """The slot's volume as a host (nz, ny, nx) array."""
+        """The slot's volume as a host (nz, ny, nx) array (uint32 for a channel pair: channel 0
+        in the low 16 bits)."""
         info = self.volume_info(slot)
-        dt = {DTYPE_F32: np.float32, DTYPE_U16: np.uint16, DTYPE_U8: np.uint8}[info["dtype"]]
+        dt = {DTYPE_F32: np.float32, DTYPE_U16: np.uint16, DTYPE_U8: np.uint8,
+              DTYPE_U16X2: np.uint32}[info["dtype"]]
         out = np.empty(info["shape"], dt)
         self._check(self.lib.spr_volume_read(self.handle, int(slot), out.ctypes.data, out.nbytes))
         return out
@@ -386,6 +399,12 @@ class Context:
         self._check(self.lib.spr_render_slot(self.handle, int(slot), ctypes.byref(params), method,
                                              num_parts, current_part, w, h, d_out, d_alpha, d_depth,
                                              d_normals, d_occlusion, stream))
+
+    def render_pair(self, slot, params, w, h, d_out, d_alpha, num_parts=1, current_part=0,
+                    stream=None):
+        """Both channels of a channel-pair slot into d_out (2, h, w); see spr_render_pair."""
+        self._check(self.lib.spr_render_pair(self.handle, int(slot), ctypes.byref(params),
+                                             num_parts, current_part, w, h, d_out, d_alpha, stream))
 
     def render_coarse_pair(self, slot_a, slot_b, params, w, h, d_out, d_alpha, stream=None):
         """Two coarse planes (slot_a, slot_b) into d_out (2, h, w); see spr_render_coarse_pair."""

@@ -253,6 +253,92 @@ __device__ __forceinline__ void staged_sample_global(const V& vol, float& px,
     }
 }
 
+// Two-channel (U16Pair) forms of the two samplers: one locate and one address per sample,
+// both channels interpolated with the single-channel operation order, so each plane is
+// bit-identical to a single-channel render of that channel.
+template <typename V, int CH>
+__device__ __forceinline__ void staged_sample_slab_pair(const V& vol, const uint8_t* slab,
+                                                        uint32_t lox, uint32_t LY, uint32_t LZ,
+                                                        uint32_t rowb, uint32_t SY, float& px,
+                                                        float& py, float& pz, float dx, float dy,
+                                                        float dz, float& col, float& col2) {
+    const uint32_t sy = rowb, sz = rowb * SY;
+    const float fsy = (float)sy, fsz = (float)sz;
+    const float oy = 1.0f - (float)LY, oz = 1.0f - (float)LZ;
+    const float ox = (1.0f - (float)lox) * 4.0f;
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+        float cx, cy, cz, ax, ay, az;
+        vol.locate_f(px, py, pz, &cx, &cy, &cz, &ax, &ay, &az);
+        // a multiple of 4: voxels are 4-byte aligned in the slab
+        const uint32_t q = (uint32_t)fmaf(cz + oz, fsz, fmaf(cy + oy, fsy, fmaf(cx, 4.0f, ox)));
+        const uint32_t* a = reinterpret_cast<const uint32_t*>(slab + q);
+        const uint32_t* b = reinterpret_cast<const uint32_t*>(slab + q + sy);
+        const uint32_t* c = reinterpret_cast<const uint32_t*>(slab + q + sz);
+        const uint32_t* d = reinterpret_cast<const uint32_t*>(slab + q + sy + sz);
+        const uint32_t a0 = a[0], a1 = a[1], b0 = b[0], b1 = b[1];
+        const uint32_t c0 = c[0], c1 = c[1], d0 = d[0], d1 = d[1];
+        col = fmaxf(col, trilerp((float)(a0 & 0xFFFFu), (float)(a1 & 0xFFFFu), (float)(b0 & 0xFFFFu),
+                                 (float)(b1 & 0xFFFFu), (float)(c0 & 0xFFFFu), (float)(c1 & 0xFFFFu),
+                                 (float)(d0 & 0xFFFFu), (float)(d1 & 0xFFFFu), ax, ay, az));
+        col2 = fmaxf(col2, trilerp((float)(a0 >> 16), (float)(a1 >> 16), (float)(b0 >> 16),
+                                   (float)(b1 >> 16), (float)(c0 >> 16), (float)(c1 >> 16),
+                                   (float)(d0 >> 16), (float)(d1 >> 16), ax, ay, az));
+        px = px + dx;
+        py = py + dy;
+        pz = pz + dz;
+    }
+}
+
+template <typename V, int CH>
+__device__ __forceinline__ void staged_sample_global_pair(const V& vol, float& px, float& py,
+                                                          float& pz, float dx, float dy, float dz,
+                                                          float& col, float& col2) {
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+        float cx, cy, cz, ax, ay, az;
+        vol.locate_f(px, py, pz, &cx, &cy, &cz, &ax, &ay, &az);
+        const uint32_t off = vol.byte_offset_f(cx, cy, cz);
+        const __amdgpu_buffer_rsrc_t r = vol.rsrc();
+        float l00[2], h00[2], l10[2], h10[2], l01[2], h01[2], l11[2], h11[2];
+        PairBits<U16Pair>::load2(r, off, 0, l00, h00);
+        PairBits<U16Pair>::load2(r, off, vol.soff_y, l10, h10);
+        PairBits<U16Pair>::load2(r, off, vol.soff_z, l01, h01);
+        PairBits<U16Pair>::load2(r, off, vol.soff_yz, l11, h11);
+        col = fmaxf(col, trilerp(l00[0], h00[0], l10[0], h10[0], l01[0], h01[0], l11[0], h11[0],
+                                 ax, ay, az));
+        col2 = fmaxf(col2, trilerp(l00[1], h00[1], l10[1], h10[1], l01[1], h01[1], l11[1], h11[1],
+                                   ax, ay, az));
+        px = px + dx;
+        py = py + dy;
+        pz = pz + dz;
+    }
+}
+
+// the sampler matching the voxel type (col2: the second channel of a U16Pair volume)
+template <typename V, int N>
+__device__ __forceinline__ void sample_slab_any(const V& vol, const uint8_t* slab, uint32_t lox,
+                                                uint32_t LY, uint32_t LZ, uint32_t rowb,
+                                                uint32_t SY, float& px, float& py, float& pz,
+                                                float dx, float dy, float dz, float& col,
+                                                float& col2) {
+    if constexpr (std::is_same<typename V::value_type, U16Pair>::value)
+        staged_sample_slab_pair<V, N>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy, dz, col,
+                                      col2);
+    else
+        staged_sample_slab<V, N>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy, dz, col);
+}
+
+template <typename V, int N>
+__device__ __forceinline__ void sample_global_any(const V& vol, float& px, float& py, float& pz,
+                                                  float dx, float dy, float dz, float& col,
+                                                  float& col2) {
+    if constexpr (std::is_same<typename V::value_type, U16Pair>::value)
+        staged_sample_global_pair<V, N>(vol, px, py, pz, dx, dy, dz, col, col2);
+    else
+        staged_sample_global<V, N>(vol, px, py, pz, dx, dy, dz, col);
+}
+
 template <bool kShort>
 __device__ __forceinline__ void staged_ray_end(const RayParams& p, uint32_t o, float col,
                                                float tnear, float* __restrict__ out,
@@ -267,6 +353,14 @@ __device__ __forceinline__ void staged_ray_end(const RayParams& p, uint32_t o, f
         out[o] = fmaxf(col, out[o]);
         alpha[o] = fmaxf(a, alpha[o]);
     }
+}
+
+// the output value alone (the second plane of a pair render; alpha is the first plane's)
+__device__ __forceinline__ void staged_plane_end(const RayParams& p, uint32_t o, float col,
+                                                 float* __restrict__ out) {
+    col = (p.max_val == 0.0f) ? col : (col - p.min_val) / (p.max_val - p.min_val);
+    col = apply_gamma(col, p.gamma);
+    out[o] = p.current_part == 0 ? col : fmaxf(col, out[o]);
 }
 
 // CH: samples per unrolled group (16, 8 or 4); a box holds p.box16 groups of 16 samples
@@ -287,6 +381,7 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
                                                 uint32_t* s_box, float* __restrict__ out,
                                                 float* __restrict__ alpha) {
     constexpr bool kShort = !std::is_same<T, float>::value;
+    constexpr bool kPair = std::is_same<T, U16Pair>::value;  // out: two planes of W x H
     constexpr uint32_t E = 16u / sizeof(T);  // voxels per 16-byte unit
     constexpr uint32_t TS = sizeof(T);
     constexpr uint32_t NT = SHARED ? 256u : 64u;
@@ -294,6 +389,9 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
     float px = 0.5f, py = 0.5f, pz = 0.5f, dx = 0.0f, dy = 0.0f, dz = 0.0f, tnear = 0.0f;
     const bool active = staged_ray_begin<kShort>(p, x, y, inside, out, alpha, px, py, pz, dx, dy,
                                                  dz, tnear);
+    if constexpr (kPair) {
+        if (inside && !active) out[x + p.W * y + p.W * p.H] = 0.0f;  // the miss, second plane
+    }
     const uint64_t act = __ballot(active);
     if constexpr (SHARED) {
         if (__syncthreads_or(active ? 1 : 0) == 0) return;  // uniform over the workgroup
@@ -318,7 +416,7 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
     const uint32_t PY = vol.py, pitch = vol.px_bytes;
     const float mgx = 1e-5f * vol.fnx + 0.01f, mgy = 1e-5f * vol.fny + 0.01f,
                 mgz = 1e-5f * vol.fnz + 0.01f;
-    float col = 0.0f;
+    float col = 0.0f, col2 = 0.0f;
 
     using PV = PaddedVolume<T, POW2, WIDE>;
     // Stages the box of the next n samples of every ray into the slab.  Returns false, with
@@ -391,8 +489,8 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
             if (live) {
 #pragma unroll 1
                 for (int k = 0; k < ng; ++k)
-                    staged_sample_slab<PV, CH>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx,
-                                               dy, dz, col);
+                    sample_slab_any<PV, CH>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy,
+                                            dz, col, col2);
             }
             release();
             continue;
@@ -406,37 +504,40 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
 #pragma unroll 1
                 for (int h = 0; h < 2 * ng; ++h) {
                     if (stage(8, lox, LY, LZ, rowb, SY, bytes)) {
-                        staged_sample_slab<PV, 8>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx,
-                                                  dy, dz, col);
+                        sample_slab_any<PV, 8>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy,
+                                               dz, col, col2);
                         release();
                         continue;
                     }
 #pragma unroll 1
                     for (int q = 0; q < 2; ++q) {
                         if (stage(4, lox, LY, LZ, rowb, SY, bytes))
-                            staged_sample_slab<PV, 4>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz,
-                                                      dx, dy, dz, col);
+                            sample_slab_any<PV, 4>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz,
+                                                   dx, dy, dz, col, col2);
                         else
-                            staged_sample_global<PV, 4>(vol, px, py, pz, dx, dy, dz, col);
+                            sample_global_any<PV, 4>(vol, px, py, pz, dx, dy, dz, col, col2);
                         release();
                     }
                 }
             } else {
 #pragma unroll 1
                 for (int k = 0; k < ng; ++k)
-                    staged_sample_global<PV, CH>(vol, px, py, pz, dx, dy, dz, col);
+                    sample_global_any<PV, CH>(vol, px, py, pz, dx, dy, dz, col, col2);
                 release();
             }
         } else {
             if (live) {
 #pragma unroll 1
                 for (int k = 0; k < ng; ++k)
-                    staged_sample_global<PV, CH>(vol, px, py, pz, dx, dy, dz, col);
+                    sample_global_any<PV, CH>(vol, px, py, pz, dx, dy, dz, col, col2);
             }
             release();
         }
     }
-    if (active) staged_ray_end<kShort>(p, x + p.W * y, col, tnear, out, alpha);
+    if (active) {
+        staged_ray_end<kShort>(p, x + p.W * y, col, tnear, out, alpha);
+        if constexpr (kPair) staged_plane_end(p, x + p.W * y + p.W * p.H, col2, out);
+    }
 }
 
 // one frame: one workgroup per 16x16 tile (tile map as k_max_project)

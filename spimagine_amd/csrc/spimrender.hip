@@ -110,6 +110,7 @@ size_t dtype_size(int dtype) {
         case SPR_DTYPE_F32: return 4;
         case SPR_DTYPE_U16: return 2;
         case SPR_DTYPE_U8: return 1;
+        case SPR_DTYPE_U16X2: return 4;
         default: return 0;
     }
 }
@@ -148,9 +149,22 @@ void free_slot(VolSlot& v) {
     v = VolSlot();
 }
 
-const VolSlot* slot_of(const spr_ctx* c, int slot) {
+// any filled slot
+const VolSlot* any_slot_of(const spr_ctx* c, int slot) {
     if (slot < 0 || slot >= (int)c->vols.size() || !c->vols[slot].used) return nullptr;
     return &c->vols[slot];
+}
+
+// a filled one-channel slot (what every entry point but spr_render_pair samples)
+const VolSlot* slot_of(const spr_ctx* c, int slot) {
+    const VolSlot* v = any_slot_of(c, slot);
+    return v && v->dtype != SPR_DTYPE_U16X2 ? v : nullptr;
+}
+
+int no_volume(spr_ctx* c, int slot) {
+    const VolSlot* v = any_slot_of(c, slot);
+    if (v) return fail(c, SPR_ERR_INVALID_ARG, "slot %d holds a channel pair (spr_render_pair only)", slot);
+    return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d", slot);
 }
 
 int ensure_outputs(spr_ctx* c, int w, int h) {
@@ -387,6 +401,25 @@ void launch_staged(int slab_kb, bool shared, int ch, dim3 grid, hipStream_t s, A
         launch_staged_ch<T, P2, 4>(slab_kb, shared, grid, s, args...);
 }
 
+// both channels of a padded channel-pair slot (16 KiB per-wave slabs)
+template <bool P2, int CH>
+void launch_pair(const VolSlot& vs, dim3 grid, hipStream_t s, const RayParams& p, float* out,
+                 float* alpha) {
+    hipLaunchKernelGGL((k_max_project_staged<U16Pair, P2, 16384, CH, false>), grid, dim3(256), 0, s,
+                       padded_volume<U16Pair, P2>(vs), p, out, alpha);
+}
+
+template <bool P2>
+void launch_pair_ch(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
+                    float* out, float* alpha) {
+    if (ch == 16)
+        launch_pair<P2, 16>(vs, grid, s, p, out, alpha);
+    else if (ch == 8)
+        launch_pair<P2, 8>(vs, grid, s, p, out, alpha);
+    else
+        launch_pair<P2, 4>(vs, grid, s, p, out, alpha);
+}
+
 // render(method, numParts, currentPart) into device buffers; the reference's launch
 // order (volumerender.py:366-386 for max_project, :451-497 for iso_surface)
 // tmp (w*h floats) and tmp_vec (3*w*h floats) are the iso post-pass scratch
@@ -536,6 +569,29 @@ void launch_pool2(const VolSlot& vs, void* dst, int mode, hipStream_t s) {
                            px, py, pad ? 1 : 0);
 }
 
+// out (ceil(nz/2), ceil(ny/2), ceil(nx/2)) pairs: channel 0 = k_pool2_volume's mean,
+// channel 1 = its max, of a uint16 volume (the same arithmetic, so each channel equals the
+// one-channel pooled volume)
+__global__ __launch_bounds__(256) void k_pool2_pair(const uint16_t* __restrict__ in,
+                                                    U16Pair* __restrict__ out, int nx, int ny,
+                                                    int nz, int px, int py, int o) {
+    const int ox = (nx + 1) / 2, oy = (ny + 1) / 2;
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= ox) return;
+    float sum = 0.0f, mx = -INFINITY;
+    for (int k = 0; k < 8; ++k) {
+        const int xx = min(2 * x + (k & 1), nx - 1), yy = min(2 * y + ((k >> 1) & 1), ny - 1),
+                  zz = min(2 * z + (k >> 2), nz - 1);
+        const float v = (float)in[((size_t)(zz + o) * py + yy + o) * px + xx + o];
+        sum = sum + v;
+        mx = fmaxf(mx, v);
+    }
+    sum *= 0.125f;
+    U16Pair r;
+    r.bits = (uint32_t)__float2uint_rn(sum) | ((uint32_t)__float2uint_rn(mx) << 16);
+    out[((size_t)z * oy + y) * ox + x] = r;
+}
+
 template <typename T>
 void launch_max_filter3(const VolSlot& vs, void* t1, void* t2, int r, hipStream_t s) {
     const dim3 grid((vs.nx + 255) / 256, vs.ny, vs.nz), block(256);
@@ -613,6 +669,7 @@ int fill_slot(spr_ctx* c, VolSlot& vs, const void* src, bool src_on_device, int 
         switch (dtype) {
             case SPR_DTYPE_F32: launch_pad<float>(from, vs.d_pad, nx, ny, nz, c->stream); break;
             case SPR_DTYPE_U16: launch_pad<uint16_t>(from, vs.d_pad, nx, ny, nz, c->stream); break;
+            case SPR_DTYPE_U16X2: launch_pad<U16Pair>(from, vs.d_pad, nx, ny, nz, c->stream); break;
             default: launch_pad<uint8_t>(from, vs.d_pad, nx, ny, nz, c->stream); break;
         }
         HIPCALL(c, hipGetLastError());
@@ -650,7 +707,12 @@ int build_jobs(spr_ctx* c, const spr_job* jobs, int n, std::vector<BatchJob>& ou
     *pow2 = true;
     for (int i = 0; i < n; ++i) {
         const VolSlot* vs = slot_of(c, jobs[i].slot);
-        if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "job %d: no volume in slot %d", i, jobs[i].slot);
+        if (!vs) {
+            if (any_slot_of(c, jobs[i].slot))
+                return fail(c, SPR_ERR_INVALID_ARG, "job %d: slot %d holds a channel pair", i,
+                            jobs[i].slot);
+            return fail(c, SPR_ERR_NO_VOLUME, "job %d: no volume in slot %d", i, jobs[i].slot);
+        }
         if (vs->layout != SPR_LAYOUT_PADDED)
             return fail(c, SPR_ERR_INVALID_ARG, "job %d: batched rendering needs the padded layout", i);
         if (*dtype >= 0 && vs->dtype != *dtype)
@@ -886,11 +948,15 @@ int spr_get_tuning(spr_ctx* c, int key, int* value) {
 
 int spr_set_volume(spr_ctx* c, const void* host, int dtype, int nx, int ny, int nz) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (dtype == SPR_DTYPE_U16X2)
+        return fail(c, SPR_ERR_UNSUPPORTED_DTYPE, "the renderer's volume has one channel");
     return fill_slot(c, c->vols[0], host, false, dtype, nx, ny, nz);
 }
 
 int spr_set_volume_device(spr_ctx* c, const void* dev, int dtype, int nx, int ny, int nz) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (dtype == SPR_DTYPE_U16X2)
+        return fail(c, SPR_ERR_UNSUPPORTED_DTYPE, "the renderer's volume has one channel");
     return fill_slot(c, c->vols[0], dev, true, dtype, nx, ny, nz);
 }
 
@@ -918,7 +984,7 @@ int spr_volume_add_dilated(spr_ctx* c, int src_slot, int radius, int* slot) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
     if (!slot) return fail(c, SPR_ERR_INVALID_ARG, "NULL slot output");
     const VolSlot* vs = slot_of(c, src_slot);
-    if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d", src_slot);
+    if (!vs) return no_volume(c, src_slot);
     if (radius < 1 || radius > 16) return fail(c, SPR_ERR_INVALID_ARG, "radius %d not in [1, 16]", radius);
     DeviceGuard g(c->device);
     const size_t bytes = (size_t)vs->nx * vs->ny * vs->nz * dtype_size(vs->dtype);
@@ -949,7 +1015,7 @@ int spr_volume_add_pooled(spr_ctx* c, int src_slot, int mode, int* slot) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
     if (!slot) return fail(c, SPR_ERR_INVALID_ARG, "NULL slot output");
     const VolSlot* vs = slot_of(c, src_slot);
-    if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d", src_slot);
+    if (!vs) return no_volume(c, src_slot);
     if (mode != 0 && mode != 1) return fail(c, SPR_ERR_INVALID_ARG, "pooling mode %d (0 mean, 1 max)", mode);
     DeviceGuard g(c->device);
     const int ox = (vs->nx + 1) / 2, oy = (vs->ny + 1) / 2, oz = (vs->nz + 1) / 2;
@@ -975,9 +1041,45 @@ int spr_volume_add_pooled(spr_ctx* c, int src_slot, int mode, int* slot) {
     return rc;
 }
 
+int spr_volume_add_pooled_pair(spr_ctx* c, int src_slot, int* slot) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (!slot) return fail(c, SPR_ERR_INVALID_ARG, "NULL slot output");
+    const VolSlot* vs = slot_of(c, src_slot);
+    if (!vs) return no_volume(c, src_slot);
+    if (vs->dtype != SPR_DTYPE_U16)
+        return fail(c, SPR_ERR_UNSUPPORTED_DTYPE, "pooled pairs need a uint16 volume");
+    DeviceGuard g(c->device);
+    const int ox = (vs->nx + 1) / 2, oy = (vs->ny + 1) / 2, oz = (vs->nz + 1) / 2;
+    void* t = nullptr;
+    hipError_t e = hipMalloc(&t, (size_t)ox * oy * oz * sizeof(U16Pair));
+    if (e == hipSuccess) {
+        const dim3 grid((ox + 255) / 256, oy, oz), block(256);
+        const bool pad = vs->layout == SPR_LAYOUT_PADDED;
+        const int px = pad ? (int)padded_pitch<uint16_t>(vs->nx) : vs->nx, py = pad ? vs->ny + 2 : vs->ny;
+        hipLaunchKernelGGL(k_pool2_pair, grid, block, 0, c->stream,
+                           (const uint16_t*)(pad ? vs->d_pad : vs->d_vol), (U16Pair*)t, vs->nx, vs->ny,
+                           vs->nz, px, py, pad ? 1 : 0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    int rc = SPR_OK;
+    if (e != hipSuccess) {
+        rc = fail(c, e == hipErrorOutOfMemory ? SPR_ERR_OUT_OF_MEMORY : SPR_ERR_HIP, "pooling failed: %s",
+                  hipGetErrorString(e));
+    } else {
+        // padded whatever the layout request: spr_render_pair samples the padded layout
+        const int req = c->layout_req;
+        c->layout_req = SPR_LAYOUT_PADDED;
+        rc = spr_volume_add(c, t, 1, SPR_DTYPE_U16X2, ox, oy, oz, slot);
+        c->layout_req = req;
+    }
+    if (t) (void)hipFree(t);
+    return rc;
+}
+
 int spr_volume_read(spr_ctx* c, int slot, void* host, size_t bytes) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
-    const VolSlot* vs = slot_of(c, slot);
+    const VolSlot* vs = any_slot_of(c, slot);
     if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d", slot);
     const size_t es = dtype_size(vs->dtype), row = (size_t)vs->nx * es;
     const size_t need = row * vs->ny * vs->nz;
@@ -985,9 +1087,10 @@ int spr_volume_read(spr_ctx* c, int slot, void* host, size_t bytes) {
         return fail(c, SPR_ERR_INVALID_ARG, "host buffer of %zu bytes, the volume has %zu", bytes, need);
     DeviceGuard g(c->device);
     if (vs->layout == SPR_LAYOUT_PADDED) {
-        const size_t pitch = (size_t)(vs->dtype == SPR_DTYPE_F32   ? padded_pitch<float>(vs->nx)
-                                      : vs->dtype == SPR_DTYPE_U16 ? padded_pitch<uint16_t>(vs->nx)
-                                                                   : padded_pitch<uint8_t>(vs->nx)) *
+        const size_t pitch = (size_t)(vs->dtype == SPR_DTYPE_F32     ? padded_pitch<float>(vs->nx)
+                                      : vs->dtype == SPR_DTYPE_U16   ? padded_pitch<uint16_t>(vs->nx)
+                                      : vs->dtype == SPR_DTYPE_U16X2 ? padded_pitch<U16Pair>(vs->nx)
+                                                                     : padded_pitch<uint8_t>(vs->nx)) *
                              es;
         for (int z = 0; z < vs->nz; ++z) {  // interior rows of each padded plane
             const char* src = (const char*)vs->d_pad + (((size_t)z + 1) * (vs->ny + 2) + 1) * pitch + es;
@@ -1003,7 +1106,7 @@ int spr_volume_read(spr_ctx* c, int slot, void* host, size_t bytes) {
 
 int spr_volume_release(spr_ctx* c, int slot) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
-    if (slot <= 0 || !slot_of(c, slot))
+    if (slot <= 0 || !any_slot_of(c, slot))
         return fail(c, SPR_ERR_INVALID_ARG, "slot %d is not an added volume", slot);
     std::lock_guard<std::mutex> lock(c->table_mu);
     DeviceGuard g(c->device);
@@ -1015,7 +1118,7 @@ int spr_volume_release(spr_ctx* c, int slot) {
 int spr_volume_info(spr_ctx* c, int slot, int* dtype, int* nx, int* ny, int* nz,
                     size_t* device_bytes) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
-    const VolSlot* vs = slot_of(c, slot);
+    const VolSlot* vs = any_slot_of(c, slot);
     if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d", slot);
     if (dtype) *dtype = vs->dtype;
     if (nx) *nx = vs->nx;
@@ -1114,7 +1217,7 @@ int spr_render_slot(spr_ctx* c, int slot, const spr_params* params, int method, 
                     float* d_normals, float* d_occlusion, void* stream) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
     const VolSlot* vs = slot_of(c, slot);
-    if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d", slot);
+    if (!vs) return no_volume(c, slot);
     if (!params) return fail(c, SPR_ERR_INVALID_ARG, "NULL params");
     int rc = check_params(c, *params);
     if (rc) return rc;
@@ -1222,6 +1325,42 @@ int spr_render_coarse_pair(spr_ctx* c, int slot_a, int slot_b, const spr_params*
                                    padded_volume<uint8_t, false>(*a), padded_volume<uint8_t, false>(*b), p, d_out, d_alpha);
             break;
     }
+    HIPCALL(c, hipGetLastError());
+    return SPR_OK;
+}
+
+int spr_render_pair(spr_ctx* c, int slot, const spr_params* params, int num_parts,
+                    int current_part, int w, int h, float* d_out, float* d_alpha, void* stream) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    const VolSlot* vs = any_slot_of(c, slot);
+    if (!vs) return fail(c, SPR_ERR_NO_VOLUME, "no volume in slot %d", slot);
+    if (vs->dtype != SPR_DTYPE_U16X2 || vs->layout != SPR_LAYOUT_PADDED)
+        return fail(c, SPR_ERR_INVALID_ARG, "slot %d is not a padded channel-pair volume", slot);
+    if (!params) return fail(c, SPR_ERR_INVALID_ARG, "NULL params");
+    int rc = check_params(c, *params);
+    if (rc) return rc;
+    if (params->filter != SPR_FILTER_LINEAR || params->alpha_pow != 0.0f)
+        return fail(c, SPR_ERR_INVALID_ARG, "pair renders are plain max projections (linear filter, alpha_pow 0)");
+    rc = check_parts(c, *params, num_parts);
+    if (rc) return rc;
+    if (current_part < 0 || current_part >= num_parts)
+        return fail(c, SPR_ERR_INVALID_ARG, "current part %d of %d", current_part, num_parts);
+    if (w <= 0 || h <= 0 || w > 32768 || h > 32768)
+        return fail(c, SPR_ERR_INVALID_ARG, "invalid output size %d x %d", w, h);
+    if (!d_out || !d_alpha) return fail(c, SPR_ERR_INVALID_ARG, "NULL output");
+    DeviceGuard g(c->device);
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    RayParams p = make_params(*params, c->tile_map, w, h);
+    staged_tuning(c, p);
+    p.nouter = (params->max_steps / num_parts) / kLoopUnroll;
+    p.current_part = current_part;
+    const dim3 grid(tile_grid(p.tile_map, p.ntx, p.nty));
+    const int nmax = std::max(vs->nx, std::max(vs->ny, vs->nz));
+    const int ch = chunk_for(c, p.nouter, nmax);
+    if (slot_pow2(*vs))
+        launch_pair_ch<true>(*vs, ch, grid, s, p, d_out, d_alpha);
+    else
+        launch_pair_ch<false>(*vs, ch, grid, s, p, d_out, d_alpha);
     HIPCALL(c, hipGetLastError());
     return SPR_OK;
 }

@@ -138,6 +138,26 @@ struct PairBits<float> {
     }
 };
 
+// Two uint16 channels per voxel, channel 0 in the low half: the 2x mean- and max-pooled
+// copies of a volume interleaved, so that one box, one address and one read per corner
+// pair serve both planes of a coarse pair render.
+struct U16Pair {
+    uint32_t bits;
+};
+template <>
+struct PairBits<U16Pair> {
+    // the corner pair (x, x+1) of both channels: lo[c] = channel c at x, hi[c] at x+1
+    __device__ __forceinline__ static void load2(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                 uint32_t soff, float* lo, float* hi) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
+        const uint32_t a = (uint32_t)v[0], b = (uint32_t)v[1];
+        lo[0] = (float)(a & 0xFFFFu);
+        lo[1] = (float)(a >> 16);
+        hi[0] = (float)(b & 0xFFFFu);
+        hi[1] = (float)(b >> 16);
+    }
+};
+
 // word 3 of a gfx9-family buffer resource (raw buffer, 32-bit data format)
 constexpr int kBufferWord3 = 0x00020000;
 

@@ -243,11 +243,17 @@ class LearnedMIP(object):
         self._bufs = {}
 
     def attach(self, slot):
-        """The two slots whose coarse projections feed the network (created once)."""
+        """The slots whose coarse projections feed the network (created once): for pooled
+        inputs of a uint16 volume one channel-pair slot (both poolings interleaved, rendered
+        by spr_render_pair), else two one-channel slots."""
         if slot not in self.dilated:
             if self.inputs == "pooled":
-                self.dilated[slot] = (self.ctx.volume_add_pooled(slot, 0),
-                                      self.ctx.volume_add_pooled(slot, 1))
+                if self.ctx.volume_info(slot)["dtype"] == _lib.DTYPE_U16:
+                    pair = self.ctx.volume_add_pooled_pair(slot)
+                    self.dilated[slot] = (pair, pair)
+                else:
+                    self.dilated[slot] = (self.ctx.volume_add_pooled(slot, 0),
+                                          self.ctx.volume_add_pooled(slot, 1))
             else:
                 self.dilated[slot] = (slot, self.ctx.volume_add_dilated(slot, self.radius))
         return self.dilated[slot]
@@ -259,11 +265,16 @@ class LearnedMIP(object):
 
     def coarse(self, slot, p, w, h, planes, k=0, stream=None):
         """Renders the two coarse planes of frame k of `planes` (n, 2, h/2, w/2) in one
-        launch (spr_render_coarse_pair; the coarse alpha lands in the "alpha" buffer)."""
+        launch (spr_render_pair on a channel-pair slot, else spr_render_coarse_pair; the
+        coarse alpha lands in the "alpha" buffer)."""
         a, b = self.attach(slot)
         scratch = self._buffer(("alpha", w, h), (h // 2, w // 2))
-        self.ctx.render_coarse_pair(a, b, coarse_params(p), w // 2, h // 2,
-                                    planes[k].data_ptr(), scratch.data_ptr(), stream=stream)
+        if a == b:
+            self.ctx.render_pair(a, coarse_params(p), w // 2, h // 2, planes[k].data_ptr(),
+                                 scratch.data_ptr(), stream=stream)
+        else:
+            self.ctx.render_coarse_pair(a, b, coarse_params(p), w // 2, h // 2,
+                                        planes[k].data_ptr(), scratch.data_ptr(), stream=stream)
 
     def render(self, slot, p, w, h, d_out, stream):
         """The learned w x h frame of `slot` with parameters p into device buffer d_out."""

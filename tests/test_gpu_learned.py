@@ -344,3 +344,77 @@ def test_pooled_volumes(torch, dtype):
             np.testing.assert_allclose(mn, mean, rtol=1e-6, atol=1e-6)
         else:
             np.testing.assert_array_equal(mn, np.rint(mean).astype(dtype))
+
+
+def test_pooled_pair_volume_channels():
+    """spr_volume_add_pooled_pair: channel 0 = the mean pooling, channel 1 = the max pooling,
+    each equal to spr_volume_add_pooled's volume; the pair slot is refused by the
+    one-channel entry points."""
+    rng = np.random.default_rng(9)
+    vol = rng.integers(0, 65535, (17, 22, 35)).astype(np.uint16)
+    ctx = _lib.Context(0)
+    for layout in (_lib.LAYOUT_PADDED, _lib.LAYOUT_LINEAR):
+        ctx.set_layout(layout)
+        s = ctx.volume_add(vol)
+        pair = ctx.volume_add_pooled_pair(s)
+        info = ctx.volume_info(pair)
+        assert info["dtype"] == _lib.DTYPE_U16X2 and info["shape"] == (9, 11, 18)
+        bits = ctx.volume_read(pair)
+        np.testing.assert_array_equal((bits & 0xFFFF).astype(np.uint16),
+                                      ctx.volume_read(ctx.volume_add_pooled(s, 0)))
+        np.testing.assert_array_equal((bits >> 16).astype(np.uint16),
+                                      ctx.volume_read(ctx.volume_add_pooled(s, 1)))
+        eye = np.eye(4, dtype=np.float32)
+        with pytest.raises(_lib.SprError):
+            ctx.volume_add_pooled(pair, 0)
+        with pytest.raises(_lib.SprError):
+            ctx.render_slot(pair, _lib.make_params(eye, eye, max_steps=64), _lib.METHOD_MAX_PROJECT,
+                            16, 16, 0, 0)
+        ctx.volume_release(pair)
+    with pytest.raises(_lib.SprError):
+        ctx.volume_add_pooled_pair(ctx.volume_add(vol.astype(np.float32)))
+
+
+@pytest.mark.parametrize("shape,steps,parts", [((64, 80, 72), 224, 1), ((128, 128, 128), 224, 1),
+                                               ((40, 70, 50), 160, 2)])
+def test_pair_render_planes_equal_single_renders(torch, shape, steps, parts):
+    """spr_render_pair on a pooled pair: plane 0 is bit-identical to spr_render_slot on the
+    mean-pooled slot, plane 1 to the max-pooled slot, alpha to both (multipass included);
+    rendered with the staged kernel's chunk of 16, 8 and 4 samples."""
+    vol = synth.blob_volume(shape, n_blobs=20, sigma=(2., 6.), amp=(500., 3000.), background=100.,
+                            poisson=True, read_noise=5., dtype="uint16", seed=4)
+    ctx = _lib.Context(0)
+    s0 = ctx.volume_add(vol)
+    pair = ctx.volume_add_pooled_pair(s0)
+    mean, mx = ctx.volume_add_pooled(s0, 0), ctx.volume_add_pooled(s0, 1)
+    from spimagine_amd.utils import mat4_perspective
+    from spimagine_amd.volumerender.volumerender import render_matrices
+    nz, ny, nx = ctx.volume_info(pair)["shape"]
+    W, H = 150, 118
+    P = mat4_perspective(45, W / float(H), .1, 10)
+    s = torch.cuda.Stream()
+    for k, M in enumerate(synth.random_poses(3, seed=21)):
+        iP, iM = render_matrices(M, P, (nx, ny, nz))
+        p = _lib.make_params(iP, iM, (-1, 1, -1, 1, -1, 1), 100., 3000., 1., 0., .1, 21, 30, steps,
+                             _lib.FILTER_LINEAR)
+        for chunk in (0, 16, 8, 4):
+            ctx.set_tuning("chunk", chunk)
+            planes = torch.empty((2, H, W), device="cuda")
+            pa = torch.empty((H, W), device="cuda")
+            refs = [(torch.empty((H, W), device="cuda"), torch.empty((H, W), device="cuda"))
+                    for _ in range(2)]
+            for part in range(parts):
+                ctx.render_pair(pair, p, W, H, planes.data_ptr(), pa.data_ptr(), num_parts=parts,
+                                current_part=part, stream=s.cuda_stream)
+                for (o, a), slot in zip(refs, (mean, mx)):
+                    ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W, H, o.data_ptr(), a.data_ptr(),
+                                    num_parts=parts, current_part=part, stream=s.cuda_stream)
+            s.synchronize()
+            assert torch.equal(planes[0], refs[0][0]), (k, chunk)
+            assert torch.equal(planes[1], refs[1][0]), (k, chunk)
+            assert torch.equal(pa, refs[0][1]) and torch.equal(pa, refs[1][1])
+            assert (refs[1][0] > 0).sum().item() > 1000
+    ctx.set_tuning("chunk", 0)
+    with pytest.raises(_lib.SprError):  # plain max projections only
+        ctx.render_pair(pair, _lib.make_params(iP, iM, alpha_pow=.5, max_steps=64), W, H,
+                        planes.data_ptr(), pa.data_ptr(), stream=s.cuda_stream)
