@@ -29,16 +29,30 @@ __device__ __forceinline__ uint32_t pk_min_u16(uint32_t a, uint32_t b) {
                                                                   __builtin_bit_cast(spr_us2, b)));
 }
 
-// wave64 minimum of each 16-bit half (row_shr 1/2/4/8, row_bcast 15/31), uniform result
-__device__ __forceinline__ uint32_t wave_min_u16x2(uint32_t v) {
-    v = pk_min_u16(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xf, 0xf, false));
-    v = pk_min_u16(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xf, 0xf, false));
-    v = pk_min_u16(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xf, 0xf, false));
-    v = pk_min_u16(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xf, 0xf, false));
-    v = pk_min_u16(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xa, 0xf, false));
-    v = pk_min_u16(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xc, 0xf, false));
-    return __builtin_amdgcn_readlane(v, 63);
+// wave64 minima of three packed u16 pairs at once (row_shr 1/2/4/8, row_bcast 15/31):
+// the three chains interleave, so no chain waits on its own DPP read; uniform results
+__device__ __forceinline__ void wave_min3_u16x2(uint32_t& a, uint32_t& b, uint32_t& c) {
+#define SPR_MIN3_STEP(CTRL, RM)                                                                \
+    {                                                                                          \
+        const uint32_t da = __builtin_amdgcn_update_dpp(a, a, CTRL, RM, 0xf, false);           \
+        const uint32_t db = __builtin_amdgcn_update_dpp(b, b, CTRL, RM, 0xf, false);           \
+        const uint32_t dc = __builtin_amdgcn_update_dpp(c, c, CTRL, RM, 0xf, false);           \
+        a = pk_min_u16(a, da);                                                                 \
+        b = pk_min_u16(b, db);                                                                 \
+        c = pk_min_u16(c, dc);                                                                 \
+    }
+    SPR_MIN3_STEP(0x111, 0xf)
+    SPR_MIN3_STEP(0x112, 0xf)
+    SPR_MIN3_STEP(0x114, 0xf)
+    SPR_MIN3_STEP(0x118, 0xf)
+    SPR_MIN3_STEP(0x142, 0xa)
+    SPR_MIN3_STEP(0x143, 0xc)
+#undef SPR_MIN3_STEP
+    a = __builtin_amdgcn_readlane(a, 63);
+    b = __builtin_amdgcn_readlane(b, 63);
+    c = __builtin_amdgcn_readlane(c, 63);
 }
+
 
 template <typename T>
 __device__ __forceinline__ float lds_voxel(const uint8_t* p) {
@@ -172,36 +186,78 @@ __device__ __forceinline__ void staged_chunk_box(const V& vol, bool active,
 
 // Copies rows r = z SY + y of the box into slab[r SX TS ...] with NT threads (t = this
 // thread's index among them): 16-byte buffer loads, nu units per row, NT / nu rows a pass.
+// q = n / d and n - q d for n, d < 2^16 (d > 0): a float quotient off by at most one,
+// corrected (no integer division)
+__device__ __forceinline__ void divmod16(uint32_t n, uint32_t d, float rd, uint32_t& q,
+                                         uint32_t& m) {
+    q = (uint32_t)((float)n * rd);
+    int32_t r = (int32_t)n - (int32_t)__umul24(q, d);
+    if (r < 0) {
+        q -= 1u;
+        r += (int32_t)d;
+    } else if (r >= (int32_t)d) {
+        q += 1u;
+        r -= (int32_t)d;
+    }
+    m = (uint32_t)r;
+}
+
+// Copies rows r = z SY + y of the box into slab[r rowb ...] with NT threads (t = this
+// thread's index among them): 16-byte buffer loads, nu units per row, R = NT / nu rows a
+// pass.  Thread t copies unit t mod nu of rows t / nu + k R; its row's (z, y) and byte
+// offsets advance by fixed steps (no division or wide multiply in the loop), and four rows
+// are loaded before the first is stored, so four loads are in flight per thread.  Loads of
+// rows past the box read inside the buffer or return zero, and are not stored.
 template <uint32_t NT>
 __device__ __forceinline__ void staged_copy_rows(__amdgpu_buffer_rsrc_t rsrc, uint8_t* slab,
                                                  uint32_t t, uint32_t LY, uint32_t LZ,
                                                  uint32_t gx0, uint32_t nu, uint32_t rowb,
                                                  uint32_t SY, uint32_t SZ, uint32_t PY,
                                                  uint32_t pitch) {
-    const uint32_t R = NT / nu;
-    const uint32_t lr = t / nu, lu = t - lr * nu;
-    const uint32_t nrows = SY * SZ;
-    const uint32_t gx = gx0 + lu * 16u;
+    const uint32_t R = NT / nu;                 // uniform
+    uint32_t lr, lu;
+    divmod16(t, nu, __builtin_amdgcn_rcpf((float)nu), lr, lu);
     if (lr >= R) return;
-    const float inv_sy = 1.0f / (float)SY;
-    for (uint32_t r = lr; r < nrows; r += R) {
-        uint32_t zr = (uint32_t)((float)r * inv_sy);
-        int32_t yr = (int32_t)(r - zr * SY);
-        if (yr < 0) {
-            zr -= 1u;
-            yr += (int32_t)SY;
-        } else if (yr >= (int32_t)SY) {
-            zr += 1u;
-            yr -= (int32_t)SY;
+    const uint32_t nrows = SY * SZ;
+    const uint32_t dz = R / SY, dy = R - dz * SY;  // uniform row step in (z, y)
+    uint32_t zr, yr;
+    divmod16(lr, SY, __builtin_amdgcn_rcpf((float)SY), zr, yr);
+    uint32_t row = __umul24(LZ + zr, PY) + LY + yr;  // padded row index of row lr
+    const uint32_t drow = dz * PY + dy, wrap = PY - SY;
+    const uint32_t gx = gx0 + lu * 16u;
+    uint32_t dst = __umul24(lr, rowb) + lu * 16u;   // slab byte offset of row lr's unit
+    const uint32_t ddst = R * rowb;
+    for (uint32_t r = lr; r < nrows; r += 4u * R) {
+        uint32_t off[4], at[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            off[k] = __umul24(row, pitch) + gx;
+            at[k] = dst;
+            dst += ddst;
+            row += drow;
+            yr += dy;
+            if (yr >= SY) {  // next plane of the box
+                yr -= SY;
+                row += wrap;
+            }
         }
-        const uint32_t goff = __umul24(__umul24(LZ + zr, PY) + LY + (uint32_t)yr, pitch) + gx;
-        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, goff, 0, 0);
-        Unit16 u;
-        u.w[0] = v[0];
-        u.w[1] = v[1];
-        u.w[2] = v[2];
-        u.w[3] = v[3];
-        *reinterpret_cast<Unit16*>(slab + r * rowb + lu * 16u) = u;
+        __attribute__((ext_vector_type(4))) uint32_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, off[k], 0, 0);
+        // all four loads issue before any store (keeps the compiler from sinking the
+        // conditional rows' loads into their branches, one round trip each)
+        __asm__ volatile("" ::"v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(v[3]));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (r + (uint32_t)k * R < nrows) {
+                Unit16 u;
+                u.w[0] = v[k][0];
+                u.w[1] = v[k][1];
+                u.w[2] = v[k][2];
+                u.w[3] = v[k][3];
+                *reinterpret_cast<Unit16*>(slab + at[k]) = u;
+            }
+        }
     }
 }
 
@@ -427,7 +483,8 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
         uint32_t b0, b1, b2;
         staged_box_n<PV>(vol, active, (float)(n - 1), px, py, pz, dx, dy, dz, mgx, mgy, mgz, b0,
                          b1, b2);
-        uint32_t r0 = wave_min_u16x2(b0), r1 = wave_min_u16x2(b1), r2 = wave_min_u16x2(b2);
+        uint32_t r0 = b0, r1 = b1, r2 = b2;
+        wave_min3_u16x2(r0, r1, r2);
         if constexpr (SHARED) {
             const uint32_t w = threadIdx.x >> 6;
             if (lane == 0) {
@@ -744,7 +801,8 @@ __global__ __launch_bounds__(256) void k_coarse_mip_pair(PaddedVolume<T, POW2> v
             // the margins cover fma(k + CH - 1, d, p0) against fma(CH - 1, d, fma(k, d, p0))
             staged_chunk_box<PaddedVolume<T, POW2>, CH>(vol, active, px, py, pz, dx, dy, dz, mgx,
                                                         mgy, mgz, b0, b1, b2);
-            const uint32_t r0 = wave_min_u16x2(b0), r1 = wave_min_u16x2(b1), r2 = wave_min_u16x2(b2);
+            uint32_t r0 = b0, r1 = b1, r2 = b2;
+            wave_min3_u16x2(r0, r1, r2);
             const uint32_t LX = r0 & 0xFFFFu, LY = r0 >> 16, LZ = r1 & 0xFFFFu;
             const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
                            HZ = 0xFFFFu - (r2 >> 16);
