@@ -144,13 +144,14 @@ __device__ __forceinline__ bool staged_ray_begin(const RayParams& p, uint32_t x,
     return true;
 }
 
-// Padded-voxel box of the n samples starting at (px, py, pz) (last = n - 1), packed for a
-// min-reduction: b0 = lo.x | lo.y << 16, b1 = lo.z | (0xFFFF - hi.x) << 16, b2 =
-// (0xFFFF - hi.y) | (0xFFFF - hi.z) << 16 (hi exclusive).  Inactive lanes give all ones (no
-// effect).  The last sample's position is estimated with one fma; mg* covers its distance
-// from the sequentially added position (at most 63 additions, each off by half an ulp).
-template <typename V>
-__device__ __forceinline__ void staged_box_n(const V& vol, bool active, float last,
+// Padded-voxel box of samples first..last of a ray whose sample 0 is at (px, py, pz),
+// packed for a min-reduction: b0 = lo.x | lo.y << 16, b1 = lo.z | (0xFFFF - hi.x) << 16,
+// b2 = (0xFFFF - hi.y) | (0xFFFF - hi.z) << 16 (hi exclusive).  Inactive lanes give all
+// ones (no effect).  Positions other than sample 0 are estimated with one fma; mg* covers
+// their distance from the sequentially added positions (at most 63 additions, each off by
+// half an ulp).  EST = false: first is 0 and sample 0's corner is exact.
+template <typename V, bool EST = false>
+__device__ __forceinline__ void staged_box_n(const V& vol, bool active, float first, float last,
                                              float px, float py, float pz, float dx, float dy,
                                              float dz, float mgx, float mgy, float mgz,
                                              uint32_t& b0, uint32_t& b1, uint32_t& b2) {
@@ -158,19 +159,31 @@ __device__ __forceinline__ void staged_box_n(const V& vol, bool active, float la
     if (!active) return;
     const float ex = fmaf(last, dx, px), ey = fmaf(last, dy, py), ez = fmaf(last, dz, pz);
     constexpr bool P2 = V::kPow2;
-    const float u0x = texel<P2>(px, vol.fnx), u0y = texel<P2>(py, vol.fny),
-                u0z = texel<P2>(pz, vol.fnz);
+    const float sx = EST ? fmaf(first, dx, px) : px, sy = EST ? fmaf(first, dy, py) : py,
+                sz = EST ? fmaf(first, dz, pz) : pz;
+    const float u0x = texel<P2>(sx, vol.fnx), u0y = texel<P2>(sy, vol.fny),
+                u0z = texel<P2>(sz, vol.fnz);
     const float u1x = texel<P2>(ex, vol.fnx), u1y = texel<P2>(ey, vol.fny),
                 u1z = texel<P2>(ez, vol.fnz);
-    const uint32_t a0x = vol.corner1(floorf(u0x), vol.mx);
-    const uint32_t a0y = vol.corner1(floorf(u0y), vol.my);
-    const uint32_t a0z = vol.corner1(floorf(u0z), vol.mz);
-    const uint32_t lx = min(a0x, vol.corner1(floorf(u1x - mgx), vol.mx));
-    const uint32_t ly = min(a0y, vol.corner1(floorf(u1y - mgy), vol.my));
-    const uint32_t lz = min(a0z, vol.corner1(floorf(u1z - mgz), vol.mz));
-    const uint32_t hx = max(a0x, vol.corner1(floorf(u1x + mgx), vol.mx)) + 1u;
-    const uint32_t hy = max(a0y, vol.corner1(floorf(u1y + mgy), vol.my)) + 1u;
-    const uint32_t hz = max(a0z, vol.corner1(floorf(u1z + mgz), vol.mz)) + 1u;
+    uint32_t l0x, l0y, l0z, h0x, h0y, h0z;
+    if constexpr (EST) {
+        l0x = vol.corner1(floorf(u0x - mgx), vol.mx);
+        l0y = vol.corner1(floorf(u0y - mgy), vol.my);
+        l0z = vol.corner1(floorf(u0z - mgz), vol.mz);
+        h0x = vol.corner1(floorf(u0x + mgx), vol.mx);
+        h0y = vol.corner1(floorf(u0y + mgy), vol.my);
+        h0z = vol.corner1(floorf(u0z + mgz), vol.mz);
+    } else {
+        l0x = h0x = vol.corner1(floorf(u0x), vol.mx);
+        l0y = h0y = vol.corner1(floorf(u0y), vol.my);
+        l0z = h0z = vol.corner1(floorf(u0z), vol.mz);
+    }
+    const uint32_t lx = min(l0x, vol.corner1(floorf(u1x - mgx), vol.mx));
+    const uint32_t ly = min(l0y, vol.corner1(floorf(u1y - mgy), vol.my));
+    const uint32_t lz = min(l0z, vol.corner1(floorf(u1z - mgz), vol.mz));
+    const uint32_t hx = max(h0x, vol.corner1(floorf(u1x + mgx), vol.mx)) + 1u;
+    const uint32_t hy = max(h0y, vol.corner1(floorf(u1y + mgy), vol.my)) + 1u;
+    const uint32_t hz = max(h0z, vol.corner1(floorf(u1z + mgz), vol.mz)) + 1u;
     b0 = lx | (ly << 16);
     b1 = lz | ((0xFFFFu - hx) << 16);
     b2 = (0xFFFFu - hy) | ((0xFFFFu - hz) << 16);
@@ -181,7 +194,8 @@ __device__ __forceinline__ void staged_chunk_box(const V& vol, bool active,
                                                  float px, float py, float pz, float dx, float dy,
                                                  float dz, float mgx, float mgy, float mgz,
                                                  uint32_t& b0, uint32_t& b1, uint32_t& b2) {
-    staged_box_n(vol, active, (float)(CH - 1), px, py, pz, dx, dy, dz, mgx, mgy, mgz, b0, b1, b2);
+    staged_box_n(vol, active, 0.0f, (float)(CH - 1), px, py, pz, dx, dy, dz, mgx, mgy, mgz, b0, b1,
+                 b2);
 }
 
 // Copies rows r = z SY + y of the box into slab[r SX TS ...] with NT threads (t = this
@@ -258,6 +272,62 @@ __device__ __forceinline__ void staged_copy_rows(__amdgpu_buffer_rsrc_t rsrc, ui
                 *reinterpret_cast<Unit16*>(slab + at[k]) = u;
             }
         }
+    }
+}
+
+// The next box's rows in flight while the current box is sampled: staged_prefetch_rows
+// issues the loads of staged_copy_rows (one wave, NT = 64) into registers, at most
+// kPrefetchRows rows per lane (npass = ceil(rows / R), uniform); staged_commit_rows stores
+// them into the slab once the wave is done with the previous box.
+constexpr int kPrefetchRows = 8;
+using U32x4 = __attribute__((ext_vector_type(4))) uint32_t;
+
+__device__ __forceinline__ void staged_prefetch_rows(__amdgpu_buffer_rsrc_t rsrc, uint32_t t,
+                                                     uint32_t LY, uint32_t gx0, uint32_t nu,
+                                                     uint32_t SY, uint32_t PY, uint32_t pitch,
+                                                     int npass, U32x4 (&v)[kPrefetchRows]) {
+    const uint32_t R = 64u / nu;
+    uint32_t lr, lu;
+    divmod16(t, nu, __builtin_amdgcn_rcpf((float)nu), lr, lu);
+    if (lr >= R) return;
+    const uint32_t dz = R / SY, dy = R - dz * SY;
+    uint32_t zr, yr;
+    divmod16(lr, SY, __builtin_amdgcn_rcpf((float)SY), zr, yr);
+    uint32_t row = __umul24(zr, PY) + LY + yr;
+    const uint32_t drow = dz * PY + dy, wrap = PY - SY;
+    const uint32_t gx = gx0 + lu * 16u;
+#pragma unroll
+    for (int k = 0; k < kPrefetchRows; ++k) {
+        if (k < npass) v[k] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, __umul24(row, pitch) + gx, 0, 0);
+        row += drow;
+        yr += dy;
+        if (yr >= SY) {
+            yr -= SY;
+            row += wrap;
+        }
+    }
+}
+
+__device__ __forceinline__ void staged_commit_rows(uint8_t* slab, uint32_t t, uint32_t nu,
+                                                   uint32_t rowb, uint32_t nrows, int npass,
+                                                   const U32x4 (&v)[kPrefetchRows]) {
+    const uint32_t R = 64u / nu;
+    uint32_t lr, lu;
+    divmod16(t, nu, __builtin_amdgcn_rcpf((float)nu), lr, lu);
+    if (lr >= R) return;
+    uint32_t dst = __umul24(lr, rowb) + lu * 16u;
+    const uint32_t ddst = R * rowb;
+#pragma unroll
+    for (int k = 0; k < kPrefetchRows; ++k) {
+        if (k < npass && lr + (uint32_t)k * R < nrows) {
+            Unit16 u;
+            u.w[0] = v[k][0];
+            u.w[1] = v[k][1];
+            u.w[2] = v[k][2];
+            u.w[3] = v[k][3];
+            *reinterpret_cast<Unit16*>(slab + dst) = u;
+        }
+        dst += ddst;
     }
 }
 
@@ -475,14 +545,16 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
     float col = 0.0f, col2 = 0.0f;
 
     using PV = PaddedVolume<T, POW2, WIDE>;
-    // Stages the box of the next n samples of every ray into the slab.  Returns false, with
-    // nothing staged, when the box does not fit; ends with the slab visible to the wave (the
-    // workgroup when SHARED).
-    auto stage = [&](int n, uint32_t& lox, uint32_t& LY, uint32_t& LZ, uint32_t& rowb,
-                     uint32_t& SY, uint64_t& bytes) -> bool {
+    // The wave's (workgroup's when SHARED) box of samples first..last of every ray, from the
+    // reduced per-lane boxes; EST: sample `first` is estimated (a box ahead of the rays).
+    struct Box {
+        uint32_t lox, LY, LZ, rowb, SY, SZ, nu;
+        uint64_t bytes;
+    };
+    auto box_of = [&](auto est, float first, float last, Box& bx) -> bool {
         uint32_t b0, b1, b2;
-        staged_box_n<PV>(vol, active, (float)(n - 1), px, py, pz, dx, dy, dz, mgx, mgy, mgz, b0,
-                         b1, b2);
+        staged_box_n<PV, decltype(est)::value>(vol, active, first, last, px, py, pz, dx, dy, dz,
+                                               mgx, mgy, mgz, b0, b1, b2);
         uint32_t r0 = b0, r1 = b1, r2 = b2;
         wave_min3_u16x2(r0, r1, r2);
         if constexpr (SHARED) {
@@ -501,30 +573,39 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
             }
         }
         const uint32_t LX = r0 & 0xFFFFu;
-        LY = r0 >> 16;
-        LZ = r1 & 0xFFFFu;
+        bx.LY = r0 >> 16;
+        bx.LZ = r1 & 0xFFFFu;
         const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
                        HZ = 0xFFFFu - (r2 >> 16);
-        lox = LX & ~(E - 1u);
-        const uint32_t SX = (HX + 1u - lox + E - 1u) & ~(E - 1u);  // voxels, multiple of E
-        SY = HY - LY + 1u;
-        const uint32_t SZ = HZ - LZ + 1u;
-        const uint32_t nu = SX / E;                                 // 16-byte units per row
+        bx.lox = LX & ~(E - 1u);
+        const uint32_t SX = (HX + 1u - bx.lox + E - 1u) & ~(E - 1u);  // voxels, multiple of E
+        bx.SY = HY - bx.LY + 1u;
+        bx.SZ = HZ - bx.LZ + 1u;
+        bx.nu = SX / E;                                  // 16-byte units per row
         // slab row stride: SX voxels, plus 16 bytes when row_pad asks for an odd number of
         // 16-byte units (rows a few apart then fall on different LDS banks)
-        rowb = (nu + (p.row_pad & ~nu & 1u)) * 16u;
-        bytes = nu <= NT ? (uint64_t)rowb * SY * SZ : ~0ull;
-        const bool fits = bytes <= (uint64_t)SLAB_BYTES;
-        // rows addressed from plane LZ on (32-bit offsets whatever the volume's size)
-        if (fits)
-            staged_copy_rows<NT>(vol.rsrc_plane(LZ), slab, SHARED ? threadIdx.x : lane, LY, 0u,
-                                 lox * TS, nu, rowb, SY, SZ, PY, pitch);
+        bx.rowb = (bx.nu + (p.row_pad & ~bx.nu & 1u)) * 16u;
+        bx.bytes = bx.nu <= NT ? (uint64_t)bx.rowb * bx.SY * bx.SZ : ~0ull;
+        return bx.bytes <= (uint64_t)SLAB_BYTES;
+    };
+    // the slab's contents visible to the wave (workgroup when SHARED)
+    auto visible = [&]() {
         if constexpr (SHARED) {
             __syncthreads();
         } else {
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
+    };
+    // Stages the box of the next n samples of every ray into the slab.  Returns false, with
+    // nothing staged, when the box does not fit.
+    auto stage = [&](int n, Box& bx) -> bool {
+        const bool fits = box_of(std::false_type(), 0.0f, (float)(n - 1), bx);
+        // rows addressed from plane LZ on (32-bit offsets whatever the volume's size)
+        if (fits)
+            staged_copy_rows<NT>(vol.rsrc_plane(bx.LZ), slab, SHARED ? threadIdx.x : lane, bx.LY,
+                                 0u, bx.lox * TS, bx.nu, bx.rowb, bx.SY, bx.SZ, PY, pitch);
+        visible();
         return fits;
     };
     // the wave's reads of the slab complete before the next box overwrites it
@@ -538,39 +619,71 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
     static_assert(kLoopUnroll % CH == 0, "a chunk is a whole number of boxes");
     const int ngroups = (p.nouter + 1) * (kLoopUnroll / CH);
     const int G = CH == 16 ? (int)p.box16 : 1;  // groups per box, uniform
+    // per-wave slabs: the next box's rows are loaded into registers while this box is
+    // sampled (when the next box fits and needs at most kPrefetchRows rows per lane)
+    Box nb{};
+    bool pf = false;
+    int pf_n = 0;
+    U32x4 pv[kPrefetchRows];
     for (int i = 0; i < ngroups; i += G) {
         const int ng = min(G, ngroups - i);
-        uint32_t lox, LY, LZ, rowb, SY;
-        uint64_t bytes;
-        if (stage(ng * CH, lox, LY, LZ, rowb, SY, bytes)) {
+        Box b;
+        bool fits;
+        if (!SHARED && pf) {
+            staged_commit_rows(slab, lane, nb.nu, nb.rowb, nb.SY * nb.SZ, pf_n, pv);
+            visible();
+            b = nb;
+            fits = true;
+        } else {
+            fits = stage(ng * CH, b);
+        }
+        pf = false;
+        if constexpr (!SHARED) {
+            const int i2 = i + G;
+            if (i2 < ngroups) {
+                const int ng2 = min(G, ngroups - i2);
+                if (box_of(std::true_type(), (float)(ng * CH), (float)((ng + ng2) * CH - 1), nb)) {
+                    const uint32_t R = 64u / nb.nu;
+                    pf_n = (int)((nb.SY * nb.SZ + R - 1u) / R);
+                    if (pf_n <= kPrefetchRows) {
+                        staged_prefetch_rows(vol.rsrc_plane(nb.LZ), lane, nb.LY, nb.lox * TS, nb.nu,
+                                             nb.SY, PY, pitch, pf_n, pv);
+                        pf = true;
+                    }
+                }
+            }
+        }
+        if (fits) {
             if (live) {
 #pragma unroll 1
                 for (int k = 0; k < ng; ++k)
-                    sample_slab_any<PV, CH>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy,
-                                            dz, col, col2);
+                    sample_slab_any<PV, CH>(vol, slab, b.lox, b.LY, b.LZ, b.rowb, b.SY, px, py, pz,
+                                            dx, dy, dz, col, col2);
             }
             release();
             continue;
         }
         if constexpr (CH == 16 && !SHARED) {
-            if (bytes <= (uint64_t)(4 * ng * SLAB_BYTES)) {
+            if (b.bytes <= (uint64_t)(4 * ng * SLAB_BYTES)) {
                 // a box up to 4x the slab: halves of 8 samples with a box each, then quarters
                 // of 4; a quarter that still does not fit samples global memory.  (Rays far
                 // apart, e.g. a sparse frame, give boxes that no split brings under the
                 // slab: those go to global memory directly.)
 #pragma unroll 1
                 for (int h = 0; h < 2 * ng; ++h) {
-                    if (stage(8, lox, LY, LZ, rowb, SY, bytes)) {
-                        sample_slab_any<PV, 8>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy,
-                                               dz, col, col2);
+                    Box hb;
+                    if (stage(8, hb)) {
+                        sample_slab_any<PV, 8>(vol, slab, hb.lox, hb.LY, hb.LZ, hb.rowb, hb.SY, px,
+                                               py, pz, dx, dy, dz, col, col2);
                         release();
                         continue;
                     }
 #pragma unroll 1
                     for (int q = 0; q < 2; ++q) {
-                        if (stage(4, lox, LY, LZ, rowb, SY, bytes))
-                            sample_slab_any<PV, 4>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz,
-                                                   dx, dy, dz, col, col2);
+                        Box qb;
+                        if (stage(4, qb))
+                            sample_slab_any<PV, 4>(vol, slab, qb.lox, qb.LY, qb.LZ, qb.rowb, qb.SY,
+                                                   px, py, pz, dx, dy, dz, col, col2);
                         else
                             sample_global_any<PV, 4>(vol, px, py, pz, dx, dy, dz, col, col2);
                         release();
