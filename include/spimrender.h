@@ -77,8 +77,9 @@ extern "C" {
    SPR_TUNE_TILE_MAP  how 16x16 pixel tiles are dealt to workgroups and so to the 8 XCDs:
                       0 = tile rows interleaved over XCDs (default), 1 = one contiguous
                       band per XCD, 2 = row-major
-   SPR_TUNE_SLAB_KB   per-wave LDS slab (KiB) of the staged max projection: 8, 12 or 16
-                      (default); 0 = sample every chunk from global memory
+   SPR_TUNE_SLAB_KB   per-wave LDS slab (KiB) of the staged max projection: 8, 12, 16 or
+                      19 (default; per-wave staging only, 16 for workgroup staging); 0 =
+                      sample every chunk from global memory
    SPR_TUNE_STAGING   0 = each wave stages its own box (default), 1 = the four waves of a
                       workgroup stage one union box into a slab of 4 x SLAB_KB
    SPR_TUNE_CHUNK     samples per staged box: 0 = from the ray step (default), 4, 8, 16,

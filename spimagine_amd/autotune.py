@@ -3,7 +3,7 @@
 The kernels' launch knobs (spr_set_tuning) change speed, never results:
   tile_map  how 16x16 pixel tiles are dealt to workgroups and so to the 8 XCDs
             (0 interleaved rows, 1 one band per XCD, 2 row-major)
-  slab_kb   per-wave LDS slab of the staged sampler (0 = global gathers only, 8, 12, 16)
+  slab_kb   per-wave LDS slab of the staged sampler (0 = global gathers only, 12, 16, 19)
 The best pair depends on voxel type, volume and frame size, step count and the poses
 (how far the rays' footprints spread across a slab), so the tuner times every candidate
 on a sample of the actual batch and keeps the fastest for that workload key.  The
@@ -16,7 +16,7 @@ import time
 
 from . import _lib
 
-CANDIDATES = tuple(dict(tile_map=m, slab_kb=s) for m in (0, 1, 2) for s in (0, 8, 12, 16))
+CANDIDATES = tuple(dict(tile_map=m, slab_kb=s) for m in (0, 1, 2) for s in (0, 12, 16, 19))
 
 _cache = {}
 

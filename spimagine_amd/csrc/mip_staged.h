@@ -200,6 +200,8 @@ __device__ __forceinline__ void staged_chunk_box(const V& vol, bool active,
 
 // Copies rows r = z SY + y of the box into slab[r SX TS ...] with NT threads (t = this
 // thread's index among them): 16-byte buffer loads, nu units per row, NT / nu rows a pass.
+using U32x4 = __attribute__((ext_vector_type(4))) uint32_t;
+
 // q = n / d and n - q d for n, d < 2^16 (d > 0): a float quotient off by at most one,
 // corrected (no integer division)
 __device__ __forceinline__ void divmod16(uint32_t n, uint32_t d, float rd, uint32_t& q,
@@ -220,8 +222,7 @@ __device__ __forceinline__ void divmod16(uint32_t n, uint32_t d, float rd, uint3
 // thread's index among them): 16-byte buffer loads, nu units per row, R = NT / nu rows a
 // pass.  Thread t copies unit t mod nu of rows t / nu + k R; its row's (z, y) and byte
 // offsets advance by fixed steps (no division or wide multiply in the loop), and four rows
-// are loaded before the first is stored, so four loads are in flight per thread.  Loads of
-// rows past the box read inside the buffer or return zero, and are not stored.
+// are loaded before the first is stored, so four loads are in flight per thread.
 template <uint32_t NT>
 __device__ __forceinline__ void staged_copy_rows(__amdgpu_buffer_rsrc_t rsrc, uint8_t* slab,
                                                  uint32_t t, uint32_t LY, uint32_t LZ,
@@ -255,9 +256,11 @@ __device__ __forceinline__ void staged_copy_rows(__amdgpu_buffer_rsrc_t rsrc, ui
                 row += wrap;
             }
         }
-        __attribute__((ext_vector_type(4))) uint32_t v[4];
+        U32x4 v[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, off[k], 0, 0);
+        for (int k = 0; k < 4; ++k)
+            v[k] = r + (uint32_t)k * R < nrows ? __builtin_amdgcn_raw_buffer_load_b128(rsrc, off[k], 0, 0)
+                                               : U32x4{0u, 0u, 0u, 0u};
         // all four loads issue before any store (keeps the compiler from sinking the
         // conditional rows' loads into their branches, one round trip each)
         __asm__ volatile("" ::"v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(v[3]));
@@ -280,12 +283,12 @@ __device__ __forceinline__ void staged_copy_rows(__amdgpu_buffer_rsrc_t rsrc, ui
 // kPrefetchRows rows per lane (npass = ceil(rows / R), uniform); staged_commit_rows stores
 // them into the slab once the wave is done with the previous box.
 constexpr int kPrefetchRows = 8;
-using U32x4 = __attribute__((ext_vector_type(4))) uint32_t;
 
 __device__ __forceinline__ void staged_prefetch_rows(__amdgpu_buffer_rsrc_t rsrc, uint32_t t,
                                                      uint32_t LY, uint32_t gx0, uint32_t nu,
                                                      uint32_t SY, uint32_t PY, uint32_t pitch,
-                                                     int npass, U32x4 (&v)[kPrefetchRows]) {
+                                                     uint32_t nrows, int npass,
+                                                     U32x4 (&v)[kPrefetchRows]) {
     const uint32_t R = 64u / nu;
     uint32_t lr, lu;
     divmod16(t, nu, __builtin_amdgcn_rcpf((float)nu), lr, lu);
@@ -298,7 +301,8 @@ __device__ __forceinline__ void staged_prefetch_rows(__amdgpu_buffer_rsrc_t rsrc
     const uint32_t gx = gx0 + lu * 16u;
 #pragma unroll
     for (int k = 0; k < kPrefetchRows; ++k) {
-        if (k < npass) v[k] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, __umul24(row, pitch) + gx, 0, 0);
+        if (k < npass && lr + (uint32_t)k * R < nrows)
+            v[k] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, __umul24(row, pitch) + gx, 0, 0);
         row += drow;
         yr += dy;
         if (yr >= SY) {
@@ -647,7 +651,7 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
                     pf_n = (int)((nb.SY * nb.SZ + R - 1u) / R);
                     if (pf_n <= kPrefetchRows) {
                         staged_prefetch_rows(vol.rsrc_plane(nb.LZ), lane, nb.LY, nb.lox * TS, nb.nu,
-                                             nb.SY, PY, pitch, pf_n, pv);
+                                             nb.SY, PY, pitch, nb.SY * nb.SZ, pf_n, pv);
                         pf = true;
                     }
                 }

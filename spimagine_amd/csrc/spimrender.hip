@@ -72,7 +72,7 @@ struct spr_ctx {
     float* d_partial = nullptr;
     // launch tuning (spr_set_tuning)
     int tile_map = (int)kTileMapInterleaved;
-    int slab_kb = 16;  // per-wave LDS slab of the staged max projection, 0 = off
+    int slab_kb = 19;  // per-wave LDS slab of the staged max projection, 0 = off
     int staging = 0;   // 0 per-wave boxes, 1 one box per workgroup
     int chunk = 0;     // samples per staged box, 0 = from the ray step
     int row_pad = 0;   // pad staged slab rows to an odd number of 16-byte units
@@ -333,8 +333,10 @@ void launch_staged_sh(int slab_kb, dim3 grid, hipStream_t s, Args... args) {
     constexpr int F = SH ? 4 : 1;  // a workgroup slab holds the four waves' share
     if (slab_kb == 8)
         hipLaunchKernelGGL((k_max_project_staged<T, P2, 8192 * F, CH, SH>), grid, dim3(256), 0, s, args...);
-    else if (slab_kb == 16)
+    else if (slab_kb == 16 || (SH && slab_kb == 19))
         hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384 * F, CH, SH>), grid, dim3(256), 0, s, args...);
+    else if (slab_kb == 19)  // per-wave only: two workgroups of 4 x 19 KiB fill 152 of 160 KiB
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, CH, false>), grid, dim3(256), 0, s, args...);
     else
         hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288 * F, CH, SH>), grid, dim3(256), 0, s, args...);
 }
@@ -359,17 +361,17 @@ void staged_tuning(const spr_ctx* c, RayParams& p) {
     p.box16 = c->chunk > 16 ? (uint32_t)(c->chunk / 16) : 1u;
 }
 
-// volumes above 4 GiB: per-wave 12 KiB slabs only
+// volumes above 4 GiB: per-wave 16 KiB slabs only
 template <typename T, bool P2>
 void launch_staged_wide_t(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
                           float* out, float* alpha) {
     const PaddedVolume<T, P2, true> v = padded_volume<T, P2, true>(vs);
     if (ch == 16)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288, 16, false, true>), grid, dim3(256), 0, s, v, p, out, alpha);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, 16, false, true>), grid, dim3(256), 0, s, v, p, out, alpha);
     else if (ch == 8)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288, 8, false, true>), grid, dim3(256), 0, s, v, p, out, alpha);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, 8, false, true>), grid, dim3(256), 0, s, v, p, out, alpha);
     else
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288, 4, false, true>), grid, dim3(256), 0, s, v, p, out, alpha);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, 4, false, true>), grid, dim3(256), 0, s, v, p, out, alpha);
 }
 
 void launch_staged_wide(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
@@ -789,7 +791,9 @@ hipError_t dispatch_batch_typed(const spr_ctx* c, int mode, int ch, const BatchJ
     if (mode == 2) return launch_batch_kernel<T, P2, 16, 2, 16, false>(c, d_jobs, p, q, counters, s);
     const bool sh = c->staging == 1;
     if (c->slab_kb == 8) return launch_batch_slab<T, P2, 8192>(sh, ch, c, d_jobs, p, q, counters, s);
-    if (c->slab_kb == 16) return launch_batch_slab<T, P2, 16384>(sh, ch, c, d_jobs, p, q, counters, s);
+    if (c->slab_kb == 16 || (sh && c->slab_kb == 19))
+        return launch_batch_slab<T, P2, 16384>(sh, ch, c, d_jobs, p, q, counters, s);
+    if (c->slab_kb == 19) return launch_batch_staged<T, P2, 19456, false>(ch, c, d_jobs, p, q, counters, s);
     return launch_batch_slab<T, P2, 12288>(sh, ch, c, d_jobs, p, q, counters, s);
 }
 
@@ -913,8 +917,8 @@ int spr_set_tuning(spr_ctx* c, int key, int value) {
             c->tile_map = value;
             return SPR_OK;
         case SPR_TUNE_SLAB_KB:
-            if (value != 0 && value != 8 && value != 12 && value != 16)
-                return fail(c, SPR_ERR_INVALID_ARG, "slab size %d KiB (0, 8, 12 or 16)", value);
+            if (value != 0 && value != 8 && value != 12 && value != 16 && value != 19)
+                return fail(c, SPR_ERR_INVALID_ARG, "slab size %d KiB (0, 8, 12, 16 or 19)", value);
             c->slab_kb = value;
             return SPR_OK;
         case SPR_TUNE_STAGING:
