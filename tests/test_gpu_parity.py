@@ -153,8 +153,9 @@ def test_staged_mip_matches_global(VR, dtype, shape, pose):
          mat4_identity()][pose]                                          # eye inside the box
     frames = {}
     for slab, staging, pad, chunk in ((0, 0, 0, 0), (8, 0, 0, 0), (12, 0, 0, 0), (16, 0, 0, 0),
-                                      (8, 1, 0, 0), (12, 1, 0, 0), (16, 0, 1, 0), (12, 1, 1, 0),
-                                      (16, 0, 0, 32), (16, 0, 0, 64), (12, 1, 0, 32)):
+                                      (19, 0, 0, 0), (8, 1, 0, 0), (12, 1, 0, 0), (16, 0, 1, 0),
+                                      (12, 1, 1, 0), (16, 0, 0, 32), (16, 0, 0, 64), (12, 1, 0, 32),
+                                      (19, 1, 0, 0)):
         rend = VR((130, 97))
         rend._ctx.set_tuning("slab_kb", slab)
         rend._ctx.set_tuning("staging", staging)
@@ -185,7 +186,7 @@ def test_staged_mip_long_steps(VR, dtype, max_steps):
     M = np.dot(mat4_translate(0, .1, -3.5), mat4_rotation(.7, .3, 1, .2))
     frames = {}
     for slab, staging, chunk in ((0, 0, 0), (12, 0, 0), (12, 1, 0), (12, 0, 16), (12, 1, 4),
-                                 (8, 0, 8), (16, 0, 32), (16, 0, 64)):
+                                 (8, 0, 8), (16, 0, 32), (16, 0, 64), (19, 0, 0), (19, 0, 4)):
         rend = VR((120, 104))
         rend._ctx.set_tuning("slab_kb", slab)
         rend._ctx.set_tuning("staging", staging)
