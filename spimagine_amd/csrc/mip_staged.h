@@ -335,15 +335,47 @@ __device__ __forceinline__ void staged_commit_rows(uint8_t* slab, uint32_t t, ui
     }
 }
 
+// Opacity mode (alpha_pow > 0; volume_kernel.cl:120-150, the plain kernel's mip_rays): each
+// sample v is normalised by the transfer range, col = max(col, cumsum v), cumsum *= 1 - a v
+// (v clamped to [0, 1] for float volumes), and the reference's inner 16-sample loop breaks
+// once cumsum <= 0.01 — after the position advanced past the breaking sample; the next
+// 16-sample group starts over, so a spent ray takes one sample per group.  A lane whose
+// group has broken keeps its col, cumsum and position (brk, reset at each group start).
+struct Opacity {
+    float cumsum;
+    bool brk;
+    bool norm;       // max_val != 0
+    float mn, den;   // min_val, max_val - min_val
+    float a;         // 0.1 alpha_pow^2 (integer voxels) or alpha_pow^2 (float voxels)
+};
+
+template <bool kShort>
+__device__ __forceinline__ void opacity_step(Opacity& o, float v, float& col, float& px,
+                                             float& py, float& pz, float dx, float dy,
+                                             float dz) {
+    v = o.norm ? (v - o.mn) / o.den : v;
+    const float c = fmaxf(col, o.cumsum * v);
+    const float cs = kShort ? o.cumsum * (1.0f - o.a * v)
+                            : o.cumsum * (1.0f - o.a * fminf(fmaxf(v, 0.0f), 1.0f));
+    if (!o.brk) {
+        col = c;
+        o.cumsum = cs;
+        px = px + dx;
+        py = py + dy;
+        pz = pz + dz;
+        o.brk = cs <= 0.01f;
+    }
+}
+
 // CH samples from the slab holding the box (lox, LY, LZ, row bytes rowb, SY rows a plane);
 // slab offset of corner (c+1) relative to the box computed exactly in float (every term is
 // an integer below 2^24)
-template <typename V, int CH>
+template <typename V, int CH, bool OPAC = false>
 __device__ __forceinline__ void staged_sample_slab(const V& vol,
                                                    const uint8_t* slab, uint32_t lox, uint32_t LY,
                                                    uint32_t LZ, uint32_t rowb, uint32_t SY,
                                                    float& px, float& py, float& pz, float dx,
-                                                   float dy, float dz, float& col) {
+                                                   float dy, float dz, float& col, Opacity& op) {
     using T = typename V::value_type;
     constexpr uint32_t TS = sizeof(T);
     const uint32_t sy = rowb, sz = rowb * SY;
@@ -363,23 +395,34 @@ __device__ __forceinline__ void staged_sample_slab(const V& vol,
         lds_pair_at(a + sy, sh, v010, v110, T());
         lds_pair_at(a + sz, sh, v001, v101, T());
         lds_pair_at(a + sy + sz, sh, v011, v111, T());
-        col = fmaxf(col, trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az));
-        px = px + dx;
-        py = py + dy;
-        pz = pz + dz;
+        const float v = trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az);
+        if constexpr (OPAC) {
+            opacity_step<!std::is_same<T, float>::value>(op, v, col, px, py, pz, dx, dy, dz);
+        } else {
+            col = fmaxf(col, v);
+            px = px + dx;
+            py = py + dy;
+            pz = pz + dz;
+        }
     }
 }
 
-template <typename V, int CH>
+template <typename V, int CH, bool OPAC = false>
 __device__ __forceinline__ void staged_sample_global(const V& vol, float& px,
                                                      float& py, float& pz, float dx, float dy,
-                                                     float dz, float& col) {
+                                                     float dz, float& col, Opacity& op) {
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
-        col = fmaxf(col, vol.linear(px, py, pz));
-        px = px + dx;
-        py = py + dy;
-        pz = pz + dz;
+        const float v = vol.linear(px, py, pz);
+        if constexpr (OPAC) {
+            opacity_step<!std::is_same<typename V::value_type, float>::value>(op, v, col, px, py,
+                                                                             pz, dx, dy, dz);
+        } else {
+            col = fmaxf(col, v);
+            px = px + dx;
+            py = py + dy;
+            pz = pz + dz;
+        }
     }
 }
 
@@ -446,34 +489,36 @@ __device__ __forceinline__ void staged_sample_global_pair(const V& vol, float& p
 }
 
 // the sampler matching the voxel type (col2: the second channel of a U16Pair volume)
-template <typename V, int N>
+template <typename V, int N, bool OPAC>
 __device__ __forceinline__ void sample_slab_any(const V& vol, const uint8_t* slab, uint32_t lox,
                                                 uint32_t LY, uint32_t LZ, uint32_t rowb,
                                                 uint32_t SY, float& px, float& py, float& pz,
                                                 float dx, float dy, float dz, float& col,
-                                                float& col2) {
+                                                float& col2, Opacity& op) {
     if constexpr (std::is_same<typename V::value_type, U16Pair>::value)
         staged_sample_slab_pair<V, N>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy, dz, col,
                                       col2);
     else
-        staged_sample_slab<V, N>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy, dz, col);
+        staged_sample_slab<V, N, OPAC>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy, dz, col,
+                                       op);
 }
 
-template <typename V, int N>
+template <typename V, int N, bool OPAC>
 __device__ __forceinline__ void sample_global_any(const V& vol, float& px, float& py, float& pz,
                                                   float dx, float dy, float dz, float& col,
-                                                  float& col2) {
+                                                  float& col2, Opacity& op) {
     if constexpr (std::is_same<typename V::value_type, U16Pair>::value)
         staged_sample_global_pair<V, N>(vol, px, py, pz, dx, dy, dz, col, col2);
     else
-        staged_sample_global<V, N>(vol, px, py, pz, dx, dy, dz, col);
+        staged_sample_global<V, N, OPAC>(vol, px, py, pz, dx, dy, dz, col, op);
 }
 
-template <bool kShort>
+// the transfer and the outputs (volume_kernel.cl:159-183); OPAC: col is already normalised
+template <bool kShort, bool OPAC = false>
 __device__ __forceinline__ void staged_ray_end(const RayParams& p, uint32_t o, float col,
                                                float tnear, float* __restrict__ out,
                                                float* __restrict__ alpha) {
-    col = (p.max_val == 0.0f) ? col : (col - p.min_val) / (p.max_val - p.min_val);
+    if constexpr (!OPAC) col = (p.max_val == 0.0f) ? col : (col - p.min_val) / (p.max_val - p.min_val);
     col = apply_gamma(col, p.gamma);
     const float a = kShort ? tnear : 1.0f;
     if (p.current_part == 0) {
@@ -505,7 +550,10 @@ __device__ __forceinline__ void staged_plane_end(const RayParams& p, uint32_t o,
 // (SLAB_BYTES, wave-level ordering only).  SHARED = true: the workgroup's four waves
 // stage the union box of its 256 rays into one slab of SLAB_BYTES with all 256 threads,
 // two workgroup barriers per box; s_box holds the four waves' partial boxes.
-template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE = false>
+// OPAC: the opacity mode (alpha_pow > 0, Opacity); boxes are not loaded ahead (a spent
+// lane's position does not advance with the group).
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE = false,
+          bool OPAC = false>
 __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE>& vol, const RayParams& p,
                                                 uint32_t x, uint32_t y, bool inside, uint8_t* slab,
                                                 uint32_t* s_box, float* __restrict__ out,
@@ -621,8 +669,24 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
     };
 
     static_assert(kLoopUnroll % CH == 0, "a chunk is a whole number of boxes");
+    static_assert(!(OPAC && kPair), "pair renders are plain max projections");
     const int ngroups = (p.nouter + 1) * (kLoopUnroll / CH);
     const int G = CH == 16 ? (int)p.box16 : 1;  // groups per box, uniform
+    Opacity op;
+    op.cumsum = 1.0f;
+    op.brk = false;
+    op.norm = p.max_val != 0.0f;
+    op.mn = p.min_val;
+    op.den = p.max_val - p.min_val;
+    op.a = kShort ? 0.1f * p.alpha_pow * p.alpha_pow : p.alpha_pow * p.alpha_pow;
+    int done = 0;  // samples of the current ray position sequence taken so far (uniform)
+    // before n samples: a new reference 16-sample group starts the break state afresh
+    auto group = [&](int n) {
+        if constexpr (OPAC) {
+            if ((done & 15) == 0) op.brk = false;
+        }
+        done += n;
+    };
     // per-wave slabs: the next box's rows are loaded into registers while this box is
     // sampled (when the next box fits and needs at most kPrefetchRows rows per lane)
     Box nb{};
@@ -642,7 +706,7 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
             fits = stage(ng * CH, b);
         }
         pf = false;
-        if constexpr (!SHARED) {
+        if constexpr (!SHARED && !OPAC) {
             const int i2 = i + G;
             if (i2 < ngroups) {
                 const int ng2 = min(G, ngroups - i2);
@@ -660,9 +724,11 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
         if (fits) {
             if (live) {
 #pragma unroll 1
-                for (int k = 0; k < ng; ++k)
-                    sample_slab_any<PV, CH>(vol, slab, b.lox, b.LY, b.LZ, b.rowb, b.SY, px, py, pz,
-                                            dx, dy, dz, col, col2);
+                for (int k = 0; k < ng; ++k) {
+                    group(CH);
+                    sample_slab_any<PV, CH, OPAC>(vol, slab, b.lox, b.LY, b.LZ, b.rowb, b.SY, px, py,
+                                                  pz, dx, dy, dz, col, col2, op);
+                }
             }
             release();
             continue;
@@ -677,46 +743,54 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
                 for (int h = 0; h < 2 * ng; ++h) {
                     Box hb;
                     if (stage(8, hb)) {
-                        sample_slab_any<PV, 8>(vol, slab, hb.lox, hb.LY, hb.LZ, hb.rowb, hb.SY, px,
-                                               py, pz, dx, dy, dz, col, col2);
+                        group(8);
+                        sample_slab_any<PV, 8, OPAC>(vol, slab, hb.lox, hb.LY, hb.LZ, hb.rowb, hb.SY,
+                                                     px, py, pz, dx, dy, dz, col, col2, op);
                         release();
                         continue;
                     }
 #pragma unroll 1
                     for (int q = 0; q < 2; ++q) {
                         Box qb;
-                        if (stage(4, qb))
-                            sample_slab_any<PV, 4>(vol, slab, qb.lox, qb.LY, qb.LZ, qb.rowb, qb.SY,
-                                                   px, py, pz, dx, dy, dz, col, col2);
+                        const bool qfits = stage(4, qb);
+                        group(4);
+                        if (qfits)
+                            sample_slab_any<PV, 4, OPAC>(vol, slab, qb.lox, qb.LY, qb.LZ, qb.rowb,
+                                                         qb.SY, px, py, pz, dx, dy, dz, col, col2, op);
                         else
-                            sample_global_any<PV, 4>(vol, px, py, pz, dx, dy, dz, col, col2);
+                            sample_global_any<PV, 4, OPAC>(vol, px, py, pz, dx, dy, dz, col, col2, op);
                         release();
                     }
                 }
             } else {
 #pragma unroll 1
-                for (int k = 0; k < ng; ++k)
-                    sample_global_any<PV, CH>(vol, px, py, pz, dx, dy, dz, col, col2);
+                for (int k = 0; k < ng; ++k) {
+                    group(CH);
+                    sample_global_any<PV, CH, OPAC>(vol, px, py, pz, dx, dy, dz, col, col2, op);
+                }
                 release();
             }
         } else {
             if (live) {
 #pragma unroll 1
-                for (int k = 0; k < ng; ++k)
-                    sample_global_any<PV, CH>(vol, px, py, pz, dx, dy, dz, col, col2);
+                for (int k = 0; k < ng; ++k) {
+                    group(CH);
+                    sample_global_any<PV, CH, OPAC>(vol, px, py, pz, dx, dy, dz, col, col2, op);
+                }
             }
             release();
         }
     }
     if (active) {
-        staged_ray_end<kShort>(p, x + p.W * y, col, tnear, out, alpha);
+        staged_ray_end<kShort, OPAC>(p, x + p.W * y, col, tnear, out, alpha);
         if constexpr (kPair) staged_plane_end(p, x + p.W * y + p.W * p.H, col2, out);
     }
 }
 
 // one frame: one workgroup per 16x16 tile (tile map as k_max_project)
 // SLAB_BYTES: per wave (SHARED = false) or per workgroup (SHARED = true)
-template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE = false>
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE = false,
+          bool OPAC = false>
 __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2, WIDE> vol, RayParams p,
                                                             float* __restrict__ out,
                                                             float* __restrict__ alpha) {
@@ -726,8 +800,8 @@ __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2
     uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (SHARED ? 0u : (threadIdx.x >> 6) * SLAB_BYTES);
     uint32_t x = 0, y = 0;
     const bool inside = tile_pixel(p, &x, &y);
-    mip_staged_rays<T, POW2, SLAB_BYTES, CH, SHARED, WIDE>(vol, p, x, y, inside, slab, s_box, out,
-                                                           alpha);
+    mip_staged_rays<T, POW2, SLAB_BYTES, CH, SHARED, WIDE, OPAC>(vol, p, x, y, inside, slab, s_box,
+                                                                 out, alpha);
 }
 
 // Samples per staged box for a frame: the ray advances at most N_a / (16 nouter) voxels

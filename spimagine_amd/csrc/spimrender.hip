@@ -403,6 +403,17 @@ void launch_staged(int slab_kb, bool shared, int ch, dim3 grid, hipStream_t s, A
         launch_staged_ch<T, P2, 4>(slab_kb, shared, grid, s, args...);
 }
 
+// the opacity mode (alpha_pow > 0) staged: per-wave 19 KiB slabs
+template <typename T, bool P2, typename... Args>
+void launch_staged_opac(int ch, dim3 grid, hipStream_t s, Args... args) {
+    if (ch == 16)
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 16, false, false, true>), grid, dim3(256), 0, s, args...);
+    else if (ch == 8)
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 8, false, false, true>), grid, dim3(256), 0, s, args...);
+    else
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 4, false, false, true>), grid, dim3(256), 0, s, args...);
+}
+
 // both channels of a padded channel-pair slot (16 KiB per-wave slabs)
 template <bool P2, int CH>
 void launch_pair(const VolSlot& vs, dim3 grid, hipStream_t s, const RayParams& p, float* out,
@@ -440,6 +451,9 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
         p.nouter = (prm.max_steps / num_parts) / kLoopUnroll;
         p.current_part = current_part;
         const bool staged = !nearest && prm.alpha_pow == 0.0f && c->slab_kb > 0;
+        // the opacity mode: per-wave staging, 19 KiB slabs, not for wide volumes
+        const bool staged_opac = !nearest && prm.alpha_pow != 0.0f && c->slab_kb > 0 &&
+                                 c->staging == 0 && !vs.pad_wide;
         if (staged && vs.pad_wide) {
             const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
             launch_staged_wide(vs, chunk_for(c, p.nouter, nmax), grid, s, p, out, alpha);
@@ -449,6 +463,12 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
         hipError_t e = visit_volume(vs, [&](auto v) {
             using V = decltype(v);
             if constexpr (is_padded<V>::value) {
+                if (staged_opac) {
+                    const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
+                    launch_staged_opac<typename V::value_type, V::kPow2>(
+                        chunk_for(c, p.nouter, nmax), grid, s, v, p, out, alpha);
+                    return hipGetLastError();
+                }
                 if (staged) {
                     const int nmax = vs.nx > vs.ny ? (vs.nx > vs.nz ? vs.nx : vs.nz)
                                                    : (vs.ny > vs.nz ? vs.ny : vs.nz);

@@ -202,6 +202,38 @@ def test_staged_mip_long_steps(VR, dtype, max_steps):
     assert (frames[0, 0, 0][0] > 0).sum() > 500
 
 
+@pytest.mark.parametrize("dtype", [np.uint16, np.uint8, np.float32])
+@pytest.mark.parametrize("max_steps,pose", [(160, 0), (64, 1), (224, 2)])
+def test_staged_opacity_matches_global(VR, dtype, max_steps, pose):
+    """The opacity mode (alpha_pow > 0) staged in LDS (per-wave slabs; boxes split when
+    large; rays whose cumulative transparency falls to 0.01 break their 16-sample group
+    and take one sample per group after that) is bit-identical to the unstaged kernel,
+    multipass included, with bright volumes that make most rays break."""
+    vol = synth.blob_volume((64, 96, 80), n_blobs=25, sigma=(3., 8.), amp=(150., 250.),
+                            background=20., poisson=False, read_noise=0., dtype=dtype, seed=pose)
+    M = [np.dot(mat4_translate(0, 0, -4), mat4_rotation(np.pi / 6, 0, 1, 0)),
+         np.dot(mat4_translate(.3, -.2, -2.6), mat4_rotation(2.1, 1, .4, -.3)),
+         np.dot(mat4_translate(0, 0, -1.2), mat4_rotation(.8, .2, 1, .1))][pose]
+    frames = {}
+    for slab, ap in ((0, .6), (19, .6), (16, .6), (0, 3.), (19, 3.)):
+        rend = VR((130, 97))
+        rend._ctx.set_tuning("slab_kb", slab)
+        rend.set_max_steps(max_steps)
+        rend.set_data(vol)
+        rend.set_modelView(M)
+        rend.set_alpha_pow(ap)
+        rend.render(minVal=float(vol.min()), maxVal=float(vol.max()))
+        res = [rend.output.copy(), rend.output_alpha.copy()]
+        for part in range(2):
+            rend.render(numParts=2, currentPart=part)
+        res.append(rend.output.copy())
+        frames[slab, ap] = res
+    for key in frames:
+        for a, b in zip(frames[0, key[1]], frames[key]):
+            np.testing.assert_array_equal(a, b)
+    assert (frames[0, .6][0] > 0).sum() > 200
+
+
 def test_tuning_does_not_change_results(VR):
     vol = synth.blob_volume((40, 48, 56), n_blobs=10, sigma=(2., 5.), amp=(500., 3000.),
                             background=100., poisson=True, read_noise=5., dtype="uint16", seed=4)
