@@ -804,6 +804,150 @@ __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2
                                                                  out, alpha);
 }
 
+// Iso-surface march through the per-wave LDS slabs (iso_kernel.cl:83-131, the plain
+// kernel's k_iso_surface): sample k of a ray at pos0 + the running sum of k deltas, k = 0
+// .. max_steps-1; sample 0 sets isGreater, the first later sample on the other side of
+// the iso value is the crossing.  A wave stages the box of its still-marching rays' next
+// CH samples; rays that crossed (and lanes without a ray) follow a marching lane's
+// positions, so every LDS read stays in the box, and their samples are ignored.  The
+// wave stops marching when no lane marches; the crossing's refinement, gradient and
+// shading then run per lane from global memory (iso_finish), as in k_iso_surface.
+template <typename V, int CH, bool SLAB>
+__device__ __forceinline__ void iso_march_chunk(const V& vol, const uint8_t* slab, uint32_t lox,
+                                                uint32_t LY, uint32_t LZ, uint32_t rowb,
+                                                uint32_t SY, float& px, float& py, float& pz,
+                                                float dx, float dy, float dz, int k0, int nsteps,
+                                                float iso, bool march, int& gt, bool& found,
+                                                int& ihit) {
+    using T = typename V::value_type;
+    constexpr uint32_t TS = sizeof(T);
+    const uint32_t sy = rowb, sz = rowb * SY;
+    const float fsy = (float)sy, fsz = (float)sz;
+    const float oy = 1.0f - (float)LY, oz = 1.0f - (float)LZ;
+    const float ox = (1.0f - (float)lox) * (float)TS;
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+        float v;
+        if constexpr (SLAB) {
+            float cx, cy, cz, ax, ay, az;
+            vol.locate_f(px, py, pz, &cx, &cy, &cz, &ax, &ay, &az);
+            const float bf = fmaf(cz + oz, fsz, fmaf(cy + oy, fsy, fmaf(cx, (float)TS, ox)));
+            const uint32_t q = (uint32_t)bf;
+            const uint8_t* a = slab + (q & ~3u);
+            const uint32_t sh = q & 3u;
+            float v000, v100, v010, v110, v001, v101, v011, v111;
+            lds_pair_at(a, sh, v000, v100, T());
+            lds_pair_at(a + sy, sh, v010, v110, T());
+            lds_pair_at(a + sz, sh, v001, v101, T());
+            lds_pair_at(a + sy + sz, sh, v011, v111, T());
+            v = trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az);
+        } else {
+            v = vol.linear(px, py, pz);
+        }
+        const int k = k0 + j;
+        const int g = (int)(v > iso);
+        if (k == 0) {
+            gt = g;
+        } else if (march && !found && k < nsteps && g != gt) {
+            found = true;
+            ihit = k;
+        }
+        px = px + dx;
+        py = py + dy;
+        pz = pz + dz;
+    }
+}
+
+template <typename T, bool POW2, int SLAB_BYTES, int CH>
+__global__ __launch_bounds__(256) void k_iso_staged(PaddedVolume<T, POW2> vol, RayParams p,
+                                                    float* __restrict__ out,
+                                                    float* __restrict__ alpha,
+                                                    float* __restrict__ depth,
+                                                    float* __restrict__ normals) {
+    using PV = PaddedVolume<T, POW2>;
+    constexpr uint32_t E = 16u / sizeof(T);
+    constexpr uint32_t TS = sizeof(T);
+    __shared__ Unit16 lds[4 * SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
+    const uint32_t lane = threadIdx.x & 63u;
+    uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * SLAB_BYTES;
+    uint32_t x = 0, y = 0;
+    const bool inside = tile_pixel(p, &x, &y);
+    const uint32_t o = x + p.W * y;
+    F4 orig = mk4(0, 0, 0, 0), direc = mk4(0, 0, 0, 0), pos0 = mk4(0.5f, 0.5f, 0.5f, 0.0f),
+       delta = mk4(0, 0, 0, 0);
+    float tnear = 0.0f, tfar = 0.0f, dt = 0.0f;
+    bool active = false;
+    if (inside) {
+        ray_setup(p.invP, p.invM, x, y, p.W, p.H, &orig, &direc);
+        active = intersect_box(orig, direc, p.box, &tnear, &tfar);
+        if (active) {
+            if (tnear < 0.0f) tnear = 0.0f;
+            dt = 1.0f * (tfar - tnear) / ((float)p.max_steps - 1.0f);
+            delta = scl4(0.5f * dt, direc);
+            pos0 = scl4(0.5f, add4(mk4(1.0f + orig.x, 1.0f + orig.y, 1.0f + orig.z, 1.0f + orig.w),
+                                   scl4(tnear, direc)));
+        }
+    }
+    float px = pos0.x, py = pos0.y, pz = pos0.z;
+    const float dx = delta.x, dy = delta.y, dz = delta.z;
+    const float mgx = 1e-5f * vol.fnx + 0.01f, mgy = 1e-5f * vol.fny + 0.01f,
+                mgz = 1e-5f * vol.fnz + 0.01f;
+    const uint32_t PY = vol.py, pitch = vol.px_bytes;
+    int gt = 0, ihit = 0;
+    bool found = false;
+    const int nsteps = p.max_steps;
+    for (int k0 = 0; k0 < nsteps; k0 += CH) {
+        const bool march = active && !found;
+        const uint64_t live = __ballot(march);
+        if (live == 0) break;  // uniform
+        // lanes not marching take a marching lane's position and step for this chunk
+        const int src = __builtin_ctzll(live);
+        float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
+        const float ex = __shfl(dx, src), ey = __shfl(dy, src), ez = __shfl(dz, src);
+        float sx = march ? px : qx, sy_ = march ? py : qy, sz_ = march ? pz : qz;
+        const float tx = march ? dx : ex, ty = march ? dy : ey, tz = march ? dz : ez;
+        uint32_t b0, b1, b2;
+        staged_box_n<PV>(vol, march, 0.0f, (float)(CH - 1), sx, sy_, sz_, tx, ty, tz, mgx, mgy,
+                         mgz, b0, b1, b2);
+        uint32_t r0 = b0, r1 = b1, r2 = b2;
+        wave_min3_u16x2(r0, r1, r2);
+        const uint32_t LX = r0 & 0xFFFFu, LY = r0 >> 16, LZ = r1 & 0xFFFFu;
+        const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
+                       HZ = 0xFFFFu - (r2 >> 16);
+        const uint32_t lox = LX & ~(E - 1u);
+        const uint32_t SX = (HX + 1u - lox + E - 1u) & ~(E - 1u);
+        const uint32_t SY = HY - LY + 1u, SZ = HZ - LZ + 1u;
+        const uint32_t nu = SX / E;
+        const uint32_t rowb = nu * 16u;
+        const uint64_t bytes = nu <= 64u ? (uint64_t)rowb * SY * SZ : ~0ull;
+        const bool fits = bytes <= (uint64_t)SLAB_BYTES;
+        if (fits)
+            staged_copy_rows<64>(vol.rsrc_plane(LZ), slab, lane, LY, 0u, lox * TS, nu, rowb, SY, SZ,
+                                 PY, pitch);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (fits)
+            iso_march_chunk<PV, CH, true>(vol, slab, lox, LY, LZ, rowb, SY, sx, sy_, sz_, tx, ty, tz,
+                                          k0, nsteps, p.iso_val, march, gt, found, ihit);
+        else
+            iso_march_chunk<PV, CH, false>(vol, slab, lox, LY, LZ, rowb, SY, sx, sy_, sz_, tx, ty,
+                                           tz, k0, nsteps, p.iso_val, march, gt, found, ihit);
+        if (march) {  // a marching lane's own position moves on
+            px = sx;
+            py = sy_;
+            pz = sz_;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (!inside) return;
+    if (!found) {
+        iso_miss(o, out, alpha, depth, normals);
+        return;
+    }
+    iso_finish<0>(vol, p, o, tnear, dt, delta, pos0, ihit, gt, direc, out, alpha, depth, normals);
+}
+
 // Samples per staged box for a frame: the ray advances at most N_a / (16 nouter) voxels
 // per sample along axis a (its whole path inside the box spans at most the volume), so
 // the largest CH whose CH - 1 steps stay within ~12 voxels keeps boxes inside a slab.
@@ -870,8 +1014,9 @@ __global__ void k_reset_counters(uint32_t* __restrict__ counters) {
 // queues with one atomic per item until every queue is drained; every workgroup leaves
 // after one failed claim on each queue.
 // MODE 0: staged plain max projection (linear filter, alpha_pow == 0; CH samples per
-// staged box); MODE 1 / 2: every sample from global memory with the linear / nearest
-// filter (any alpha_pow; CH and SHARED unused)
+// staged box); MODE 3: staged opacity mode (linear filter, alpha_pow > 0, per-wave slabs);
+// MODE 1 / 2: every sample from global memory with the linear / nearest filter (any
+// alpha_pow; CH and SHARED unused)
 template <typename T, bool POW2, int SLAB_BYTES, int MODE, int CH, bool SHARED>
 __global__ __launch_bounds__(256) void k_max_project_batch(const BatchJob* __restrict__ jobs,
                                                            RayParams shared, BatchQueues qs,
@@ -906,9 +1051,9 @@ __global__ __launch_bounds__(256) void k_max_project_batch(const BatchJob* __res
         tile_thread_pixel(tx, ty, &x, &y);
         const bool inside = x < p.W && y < p.H;
         const PaddedVolume<T, POW2> vol = job_volume<T, POW2>(j);
-        if constexpr (MODE == 0)
-            mip_staged_rays<T, POW2, SLAB_BYTES, CH, SHARED>(vol, p, x, y, inside, slab, s_box,
-                                                             j.out, j.alpha);
+        if constexpr (MODE == 0 || MODE == 3)
+            mip_staged_rays<T, POW2, SLAB_BYTES, CH, SHARED, false, MODE == 3>(
+                vol, p, x, y, inside, slab, s_box, j.out, j.alpha);
         else
             mip_rays<PaddedVolume<T, POW2>, MODE == 2 ? 1 : 0>(vol, p, x, y, inside, j.out,
                                                                 j.alpha);

@@ -154,56 +154,29 @@ __global__ __launch_bounds__(256) void k_max_project(Vol vol, RayParams p,
     mip_rays<Vol, FILTER>(vol, p, x, y, inside, out, alpha);
 }
 
-// iso_kernel.cl:17-225
-template <class Vol, int FILTER>
-__global__ __launch_bounds__(256) void k_iso_surface(Vol vol, RayParams p,
-                                                     float* __restrict__ out,
-                                                     float* __restrict__ alpha,
-                                                     float* __restrict__ depth,
-                                                     float* __restrict__ normals) {
-    uint32_t x, y;
-    if (!tile_pixel(p, &x, &y)) return;
-    const uint32_t o = x + p.W * y;
-    F4 orig, direc;
-    ray_setup(p.invP, p.invM, x, y, p.W, p.H, &orig, &direc);
-    float tnear, tfar;
-    bool hit = intersect_box(orig, direc, p.box, &tnear, &tfar);
-    int isGreater = 0;
-    int i = 1;
-    float dt = 0.0f, t_hit = INFINITY;
-    F4 pos0 = mk4(0, 0, 0, 0), delta = mk4(0, 0, 0, 0);
+// outputs of a ray without an iso crossing (iso_kernel.cl:133-141)
+__device__ __forceinline__ void iso_miss(uint32_t o, float* __restrict__ out,
+                                         float* __restrict__ alpha, float* __restrict__ depth,
+                                         float* __restrict__ normals) {
+    out[o] = 0.0f;
+    alpha[o] = 0.0f;
+    depth[o] = INFINITY;
+    normals[3 * o] = 0.0f;
+    normals[3 * o + 1] = 0.0f;
+    normals[3 * o + 2] = 0.0f;
+}
+
+// A ray whose march crossed the iso value between samples i-1 and i (iso_kernel.cl:
+// 143-225): 10-substep refinement from sample i-1, the 4-tap gradient at the refined
+// point, Phong shading; t_hit enters as tnear + i dt.
+template <int FILTER, class Vol>
+__device__ __forceinline__ void iso_finish(const Vol& vol, const RayParams& p, uint32_t o,
+                                           float tnear, float dt, F4 delta, F4 pos0, int i,
+                                           int isGreater, F4 direc, float* __restrict__ out,
+                                           float* __restrict__ alpha, float* __restrict__ depth,
+                                           float* __restrict__ normals) {
     const float iso = p.iso_val;
-    if (hit) {
-        if (tnear < 0.0f) tnear = 0.0f;
-        dt = 1.0f * (tfar - tnear) / ((float)p.max_steps - 1.0f);
-        delta = scl4(0.5f * dt, direc);
-        pos0 = scl4(0.5f, add4(mk4(1.0f + orig.x, 1.0f + orig.y, 1.0f + orig.z, 1.0f + orig.w),
-                               scl4(tnear, direc)));
-        float px = pos0.x, py = pos0.y, pz = pos0.z;
-        float v = vol.template sample<FILTER>(px, py, pz);
-        isGreater = v > iso;
-        hit = false;
-        for (i = 1; i < p.max_steps; i++) {
-            px = px + delta.x;
-            py = py + delta.y;
-            pz = pz + delta.z;
-            t_hit = tnear + (float)i * dt;
-            v = vol.template sample<FILTER>(px, py, pz);
-            if ((int)(v > iso) != isGreater) {
-                hit = true;
-                break;
-            }
-        }
-    }
-    if (!hit) {
-        out[o] = 0.0f;
-        alpha[o] = 0.0f;
-        depth[o] = INFINITY;
-        normals[3 * o] = 0.0f;
-        normals[3 * o + 1] = 0.0f;
-        normals[3 * o + 2] = 0.0f;
-        return;
-    }
+    float t_hit = tnear + (float)i * dt;
     // 10-substep refinement from step i-1 (iso_kernel.cl:145-162)
     const float fb = 10.0f;
     F4 delta2 = mk4(delta.x / fb, delta.y / fb, delta.z / fb, delta.w / fb);
@@ -246,6 +219,54 @@ __global__ __launch_bounds__(256) void k_iso_surface(Vol vol, RayParams p,
     normals[3 * o] = n.x;
     normals[3 * o + 1] = n.y;
     normals[3 * o + 2] = n.z;
+}
+
+// iso_kernel.cl:17-225
+template <class Vol, int FILTER>
+__global__ __launch_bounds__(256) void k_iso_surface(Vol vol, RayParams p,
+                                                     float* __restrict__ out,
+                                                     float* __restrict__ alpha,
+                                                     float* __restrict__ depth,
+                                                     float* __restrict__ normals) {
+    uint32_t x, y;
+    if (!tile_pixel(p, &x, &y)) return;
+    const uint32_t o = x + p.W * y;
+    F4 orig, direc;
+    ray_setup(p.invP, p.invM, x, y, p.W, p.H, &orig, &direc);
+    float tnear, tfar;
+    bool hit = intersect_box(orig, direc, p.box, &tnear, &tfar);
+    int isGreater = 0;
+    int i = 1;
+    float dt = 0.0f;
+    F4 pos0 = mk4(0, 0, 0, 0), delta = mk4(0, 0, 0, 0);
+    const float iso = p.iso_val;
+    if (hit) {
+        if (tnear < 0.0f) tnear = 0.0f;
+        dt = 1.0f * (tfar - tnear) / ((float)p.max_steps - 1.0f);
+        delta = scl4(0.5f * dt, direc);
+        pos0 = scl4(0.5f, add4(mk4(1.0f + orig.x, 1.0f + orig.y, 1.0f + orig.z, 1.0f + orig.w),
+                               scl4(tnear, direc)));
+        float px = pos0.x, py = pos0.y, pz = pos0.z;
+        float v = vol.template sample<FILTER>(px, py, pz);
+        isGreater = v > iso;
+        hit = false;
+        for (i = 1; i < p.max_steps; i++) {
+            px = px + delta.x;
+            py = py + delta.y;
+            pz = pz + delta.z;
+            v = vol.template sample<FILTER>(px, py, pz);
+            if ((int)(v > iso) != isGreater) {
+                hit = true;
+                break;
+            }
+        }
+    }
+    if (!hit) {
+        iso_miss(o, out, alpha, depth, normals);
+        return;
+    }
+    iso_finish<FILTER>(vol, p, o, tnear, dt, delta, pos0, i, isGreater, direc, out, alpha, depth,
+                       normals);
 }
 
 struct ConvWeights {

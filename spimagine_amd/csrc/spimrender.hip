@@ -414,6 +414,17 @@ void launch_staged_opac(int ch, dim3 grid, hipStream_t s, Args... args) {
         hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 4, false, false, true>), grid, dim3(256), 0, s, args...);
 }
 
+// the iso-surface march staged: per-wave 19 KiB slabs
+template <typename T, bool P2, typename... Args>
+void launch_iso_staged(int ch, dim3 grid, hipStream_t s, Args... args) {
+    if (ch == 16)
+        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 16>), grid, dim3(256), 0, s, args...);
+    else if (ch == 8)
+        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 8>), grid, dim3(256), 0, s, args...);
+    else
+        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 4>), grid, dim3(256), 0, s, args...);
+}
+
 // both channels of a padded channel-pair slot (16 KiB per-wave slabs)
 template <bool P2, int CH>
 void launch_pair(const VolSlot& vs, dim3 grid, hipStream_t s, const RayParams& p, float* out,
@@ -494,8 +505,19 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
             return fail(c, SPR_ERR_INVALID_ARG, "iso_surface needs all five output buffers");
         if (prm.occ_n_points < 1 || prm.occ_radius < 0)
             return fail(c, SPR_ERR_INVALID_ARG, "invalid occlusion parameters");
+        // the march staged in LDS: linear filter, per-wave staging, padded (not wide)
+        const bool staged_iso = !nearest && c->slab_kb > 0 && c->staging == 0 && !vs.pad_wide;
         hipError_t e = visit_volume(vs, [&](auto v) {
             using V = decltype(v);
+            if constexpr (is_padded<V>::value) {
+                if (staged_iso) {
+                    const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
+                    launch_iso_staged<typename V::value_type, V::kPow2>(
+                        chunk_for(c, prm.max_steps / kLoopUnroll, nmax), grid, s, v, p, out, alpha,
+                        depth, normals);
+                    return hipGetLastError();
+                }
+            }
             if (nearest)
                 hipLaunchKernelGGL((k_iso_surface<V, 1>), grid, block, 0, s, v, p, out, alpha,
                                    depth, normals);
@@ -809,6 +831,11 @@ hipError_t dispatch_batch_typed(const spr_ctx* c, int mode, int ch, const BatchJ
                                 hipStream_t s) {
     if (mode == 1) return launch_batch_kernel<T, P2, 16, 1, 16, false>(c, d_jobs, p, q, counters, s);
     if (mode == 2) return launch_batch_kernel<T, P2, 16, 2, 16, false>(c, d_jobs, p, q, counters, s);
+    if (mode == 3) {
+        if (ch == 16) return launch_batch_kernel<T, P2, 19456, 3, 16, false>(c, d_jobs, p, q, counters, s);
+        if (ch == 8) return launch_batch_kernel<T, P2, 19456, 3, 8, false>(c, d_jobs, p, q, counters, s);
+        return launch_batch_kernel<T, P2, 19456, 3, 4, false>(c, d_jobs, p, q, counters, s);
+    }
     const bool sh = c->staging == 1;
     if (c->slab_kb == 8) return launch_batch_slab<T, P2, 8192>(sh, ch, c, d_jobs, p, q, counters, s);
     if (c->slab_kb == 16 || (sh && c->slab_kb == 19))
@@ -822,7 +849,12 @@ hipError_t dispatch_batch(const spr_ctx* c, const spr_params& prm, int dtype, bo
                           const BatchJob* d_jobs, const RayParams& p, const BatchQueues& q,
                           uint32_t* counters, hipStream_t s) {
     const bool nearest = prm.filter == SPR_FILTER_NEAREST;
-    const int mode = nearest ? 2 : ((prm.alpha_pow == 0.0f && c->slab_kb > 0) ? 0 : 1);
+    // 0: staged max projection, 3: staged opacity mode (per-wave 19 KiB slabs), 1 / 2: every
+    // sample from global memory (linear / nearest)
+    const int mode = nearest ? 2
+                     : c->slab_kb <= 0 ? 1
+                     : prm.alpha_pow == 0.0f ? 0
+                     : c->staging == 0 ? 3 : 1;
     const int ch = chunk_for(c, p.nouter, nmax);
     switch (dtype) {
         case SPR_DTYPE_F32:

@@ -71,7 +71,8 @@ def single(torch, vols, f, W, H, alpha_pow, filt, tuning=None):
     (np.float32, 0., "linear", 0, 0), (np.uint16, .5, "linear", 0, 0),
     (np.float32, 0., "nearest", 0, 0), (np.uint16, 0., "linear", 1, 0),
     (np.float32, 0., "linear", 1, 0), (np.uint16, 0., "linear", 0, 32),
-    (np.uint8, 0., "linear", 0, 64)])
+    (np.uint8, 0., "linear", 0, 64), (np.float32, .5, "linear", 0, 0),
+    (np.uint8, 2., "linear", 0, 0), (np.uint16, .5, "linear", 1, 0)])
 def test_batch_equals_separate_renders(torch, dtype, alpha_pow, interp, staging, chunk):
     """max_steps 128: 9 groups of 16 samples, so boxes of 2 or 4 groups (chunk 32, 64) end
     with a shorter box."""

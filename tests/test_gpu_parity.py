@@ -429,3 +429,34 @@ def test_headline_size_rows_parity():
         assert d <= TOL, (r, d)
         np.testing.assert_array_equal(alpha[r], a[r])
     say("headline rows parity ok")
+
+
+@pytest.mark.parametrize("dtype", [np.uint16, np.uint8, np.float32])
+@pytest.mark.parametrize("pose,steps", [(0, 160), (1, 96), (2, 224)])
+def test_staged_iso_march_matches_global(VR, dtype, pose, steps):
+    """The iso march staged in LDS (crossed rays leave the box; the wave stops when no ray
+    marches) gives every output of the unstaged march bit for bit: colour, alpha, depth,
+    normals, occlusion."""
+    shape = (48, 64, 56)
+    amp, bg = ((500., 3000.), 100.) if dtype != np.float32 else ((.3, 1.), .05)
+    if dtype == np.uint8:
+        amp, bg = (60., 200.), 10.
+    vol = synth.blob_volume(shape, n_blobs=16, sigma=(3., 7.), amp=amp, background=bg,
+                            poisson=False, read_noise=0., dtype=dtype, seed=pose)
+    M = [np.dot(mat4_translate(0, 0, -3.3), mat4_rotation(.8, -.2, 1, .3)),
+         np.dot(mat4_translate(.3, -.2, -2.6), mat4_rotation(2.1, 1, .4, -.3)),
+         np.dot(mat4_translate(0, 0, -1.2), mat4_rotation(.8, .2, 1, .1))][pose]
+    lo, hi = float(vol.min()), float(vol.max())
+    res = {}
+    for slab in (0, 19):
+        rend = VR((120, 100))
+        rend._ctx.set_tuning("slab_kb", slab)
+        rend.set_max_steps(steps)
+        rend.set_data(vol)
+        rend.set_modelView(M)
+        rend.render(maxVal=2 * (lo + .3 * (hi - lo)), method="iso_surface")
+        res[slab] = [rend.output.copy(), rend.output_alpha.copy(), rend.output_depth.copy(),
+                     rend.output_normals.copy(), rend.output_occlusion.copy()]
+    for a, b in zip(res[0], res[19]):
+        np.testing.assert_array_equal(a, b)
+    assert np.isfinite(res[0][2]).sum() > 300
