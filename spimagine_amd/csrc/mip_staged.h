@@ -858,13 +858,13 @@ __device__ __forceinline__ void iso_march_chunk(const V& vol, const uint8_t* sla
     }
 }
 
-template <typename T, bool POW2, int SLAB_BYTES, int CH>
-__global__ __launch_bounds__(256) void k_iso_staged(PaddedVolume<T, POW2> vol, RayParams p,
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool WIDE = false>
+__global__ __launch_bounds__(256) void k_iso_staged(PaddedVolume<T, POW2, WIDE> vol, RayParams p,
                                                     float* __restrict__ out,
                                                     float* __restrict__ alpha,
                                                     float* __restrict__ depth,
                                                     float* __restrict__ normals) {
-    using PV = PaddedVolume<T, POW2>;
+    using PV = PaddedVolume<T, POW2, WIDE>;
     constexpr uint32_t E = 16u / sizeof(T);
     constexpr uint32_t TS = sizeof(T);
     __shared__ Unit16 lds[4 * SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair

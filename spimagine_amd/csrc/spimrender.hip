@@ -361,34 +361,67 @@ void staged_tuning(const spr_ctx* c, RayParams& p) {
     p.box16 = c->chunk > 16 ? (uint32_t)(c->chunk / 16) : 1u;
 }
 
-// volumes above 4 GiB: per-wave 16 KiB slabs only
-template <typename T, bool P2>
+// volumes above 4 GiB: per-wave 16 KiB slabs only; OPAC: the opacity mode
+template <typename T, bool P2, bool OPAC>
 void launch_staged_wide_t(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
                           float* out, float* alpha) {
     const PaddedVolume<T, P2, true> v = padded_volume<T, P2, true>(vs);
     if (ch == 16)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, 16, false, true>), grid, dim3(256), 0, s, v, p, out, alpha);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, 16, false, true, OPAC>), grid, dim3(256), 0, s, v, p, out, alpha);
     else if (ch == 8)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, 8, false, true>), grid, dim3(256), 0, s, v, p, out, alpha);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, 8, false, true, OPAC>), grid, dim3(256), 0, s, v, p, out, alpha);
     else
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, 4, false, true>), grid, dim3(256), 0, s, v, p, out, alpha);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, 4, false, true, OPAC>), grid, dim3(256), 0, s, v, p, out, alpha);
 }
 
+template <bool OPAC>
 void launch_staged_wide(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
                         float* out, float* alpha) {
     const bool p2 = slot_pow2(vs);
     switch (vs.dtype) {
         case SPR_DTYPE_F32:
-            p2 ? launch_staged_wide_t<float, true>(vs, ch, grid, s, p, out, alpha)
-               : launch_staged_wide_t<float, false>(vs, ch, grid, s, p, out, alpha);
+            p2 ? launch_staged_wide_t<float, true, OPAC>(vs, ch, grid, s, p, out, alpha)
+               : launch_staged_wide_t<float, false, OPAC>(vs, ch, grid, s, p, out, alpha);
             break;
         case SPR_DTYPE_U16:
-            p2 ? launch_staged_wide_t<uint16_t, true>(vs, ch, grid, s, p, out, alpha)
-               : launch_staged_wide_t<uint16_t, false>(vs, ch, grid, s, p, out, alpha);
+            p2 ? launch_staged_wide_t<uint16_t, true, OPAC>(vs, ch, grid, s, p, out, alpha)
+               : launch_staged_wide_t<uint16_t, false, OPAC>(vs, ch, grid, s, p, out, alpha);
             break;
         default:
-            p2 ? launch_staged_wide_t<uint8_t, true>(vs, ch, grid, s, p, out, alpha)
-               : launch_staged_wide_t<uint8_t, false>(vs, ch, grid, s, p, out, alpha);
+            p2 ? launch_staged_wide_t<uint8_t, true, OPAC>(vs, ch, grid, s, p, out, alpha)
+               : launch_staged_wide_t<uint8_t, false, OPAC>(vs, ch, grid, s, p, out, alpha);
+            break;
+    }
+}
+
+// the iso-surface march of a volume above 4 GiB staged (16 KiB slabs)
+template <typename T, bool P2>
+void launch_iso_wide_t(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
+                       float* out, float* alpha, float* depth, float* normals) {
+    const PaddedVolume<T, P2, true> v = padded_volume<T, P2, true>(vs);
+    if (ch == 16)
+        hipLaunchKernelGGL((k_iso_staged<T, P2, 16384, 16, true>), grid, dim3(256), 0, s, v, p, out, alpha, depth, normals);
+    else if (ch == 8)
+        hipLaunchKernelGGL((k_iso_staged<T, P2, 16384, 8, true>), grid, dim3(256), 0, s, v, p, out, alpha, depth, normals);
+    else
+        hipLaunchKernelGGL((k_iso_staged<T, P2, 16384, 4, true>), grid, dim3(256), 0, s, v, p, out, alpha, depth, normals);
+}
+
+void launch_iso_wide(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
+                     float* out, float* alpha, float* depth, float* normals) {
+    const bool p2 = slot_pow2(vs);
+    switch (vs.dtype) {
+        case SPR_DTYPE_F32:
+            p2 ? launch_iso_wide_t<float, true>(vs, ch, grid, s, p, out, alpha, depth, normals)
+               : launch_iso_wide_t<float, false>(vs, ch, grid, s, p, out, alpha, depth, normals);
+            break;
+        case SPR_DTYPE_U16:
+            p2 ? launch_iso_wide_t<uint16_t, true>(vs, ch, grid, s, p, out, alpha, depth, normals)
+               : launch_iso_wide_t<uint16_t, false>(vs, ch, grid, s, p, out, alpha, depth, normals);
+            break;
+        default:
+            p2 ? launch_iso_wide_t<uint8_t, true>(vs, ch, grid, s, p, out, alpha, depth, normals)
+               : launch_iso_wide_t<uint8_t, false>(vs, ch, grid, s, p, out, alpha, depth, normals);
             break;
     }
 }
@@ -462,12 +495,15 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
         p.nouter = (prm.max_steps / num_parts) / kLoopUnroll;
         p.current_part = current_part;
         const bool staged = !nearest && prm.alpha_pow == 0.0f && c->slab_kb > 0;
-        // the opacity mode: per-wave staging, 19 KiB slabs, not for wide volumes
+        // the opacity mode: per-wave staging (19 KiB slabs; 16 KiB for wide volumes)
         const bool staged_opac = !nearest && prm.alpha_pow != 0.0f && c->slab_kb > 0 &&
-                                 c->staging == 0 && !vs.pad_wide;
-        if (staged && vs.pad_wide) {
+                                 c->staging == 0;
+        if ((staged || staged_opac) && vs.pad_wide) {
             const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
-            launch_staged_wide(vs, chunk_for(c, p.nouter, nmax), grid, s, p, out, alpha);
+            if (staged)
+                launch_staged_wide<false>(vs, chunk_for(c, p.nouter, nmax), grid, s, p, out, alpha);
+            else
+                launch_staged_wide<true>(vs, chunk_for(c, p.nouter, nmax), grid, s, p, out, alpha);
             HIPCALL(c, hipGetLastError());
             return SPR_OK;
         }
@@ -505,9 +541,15 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
             return fail(c, SPR_ERR_INVALID_ARG, "iso_surface needs all five output buffers");
         if (prm.occ_n_points < 1 || prm.occ_radius < 0)
             return fail(c, SPR_ERR_INVALID_ARG, "invalid occlusion parameters");
-        // the march staged in LDS: linear filter, per-wave staging, padded (not wide)
-        const bool staged_iso = !nearest && c->slab_kb > 0 && c->staging == 0 && !vs.pad_wide;
-        hipError_t e = visit_volume(vs, [&](auto v) {
+        // the march staged in LDS: linear filter, per-wave staging, padded or wide
+        const bool staged_iso = !nearest && c->slab_kb > 0 && c->staging == 0;
+        hipError_t e = hipSuccess;
+        if (staged_iso && vs.pad_wide) {
+            const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
+            launch_iso_wide(vs, chunk_for(c, prm.max_steps / kLoopUnroll, nmax), grid, s, p, out,
+                            alpha, depth, normals);
+            e = hipGetLastError();
+        } else e = visit_volume(vs, [&](auto v) {
             using V = decltype(v);
             if constexpr (is_padded<V>::value) {
                 if (staged_iso) {

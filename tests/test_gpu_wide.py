@@ -21,17 +21,11 @@ def torch():
 
 def test_wide_padded_volume_matches_linear(torch):
     shape = (512, 2048, 2048)
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(3)
-    # smooth-ish structure: a coarse random field upsampled, plus noise
-    coarse = torch.rand((1, 1, 32, 64, 64), generator=gen, device="cuda") * 3000.
-    vol = torch.nn.functional.interpolate(coarse, size=shape, mode="trilinear",
-                                          align_corners=False)[0, 0]
-    vol += torch.rand(shape, generator=gen, device="cuda") * 200.
-    v16 = torch.clamp(vol, 0, 65535).to(torch.int32)
-    del vol, coarse
-    v16 = torch.where(v16 >= 32768, v16 - 65536, v16).to(torch.int16)
+    # 3000 seeded blobs over a background of 100 plus read noise (uint16 bit pattern)
+    v16 = next(synth.blob_series_device(shape, 3000, (3., 10.), (500., 3000.), 100., False, 5.,
+                                        1, seed=3))
     torch.cuda.synchronize()
+    assert int(v16.max()) > 2000
     wide = _lib.Context(0)
     sw = wide.volume_add(dev_ptr=v16.data_ptr(), dtype=np.uint16, shape=shape)
     nbytes = 2048 * 2048 * 512 * 2
@@ -61,18 +55,36 @@ def test_wide_padded_volume_matches_linear(torch):
             assert torch.equal(outs[0][0], outs[1][0]), (k, steps)
             assert torch.equal(outs[0][1], outs[1][1])
             assert (outs[0][0] > 0).sum().item() > 5000
-    # the other kernels run on the slot's linear volume
-    p = _lib.make_params(iP, iM, (-1, 1, -1, 1, -1, 1), 0., 3300., 1., .5, .1, 21, 30, 96,
+    # the opacity mode (staged on the wide copy) and the nearest filter (the slot's linear
+    # volume) equal the linear layout's unstaged kernels
+    for ap, filt in ((.5, _lib.FILTER_LINEAR), (3., _lib.FILTER_LINEAR), (0., _lib.FILTER_NEAREST)):
+        p = _lib.make_params(iP, iM, (-1, 1, -1, 1, -1, 1), 0., 3300., 1., ap, .1, 21, 30, 96, filt)
+        outs = []
+        for ctx, slot in ((wide, sw), (lin, sl)):
+            o = torch.empty((H, W), device="cuda")
+            a = torch.empty_like(o)
+            ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W, H, o.data_ptr(), a.data_ptr(),
+                            stream=s.cuda_stream)
+            outs.append((o, a))
+        s.synchronize()
+        assert torch.equal(outs[0][0], outs[1][0]), (ap, filt)
+        assert torch.equal(outs[0][1], outs[1][1])
+    # the iso-surface march staged on the wide copy: all five outputs as the linear layout's
+    p = _lib.make_params(iP, iM, (-1, 1, -1, 1, -1, 1), 0., 2 * 1500., 1., 0., .1, 21, 30, 160,
                          _lib.FILTER_LINEAR)
-    outs = []
+    res = []
     for ctx, slot in ((wide, sw), (lin, sl)):
-        o = torch.empty((H, W), device="cuda")
-        a = torch.empty_like(o)
-        ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W, H, o.data_ptr(), a.data_ptr(),
+        bufs = [torch.empty((H, W), device="cuda") for _ in range(3)]
+        nrm = torch.empty((H, W, 3), device="cuda")
+        occ = torch.empty((H, W), device="cuda")
+        ctx.render_slot(slot, p, _lib.METHOD_ISO_SURFACE, W, H, bufs[0].data_ptr(),
+                        bufs[1].data_ptr(), bufs[2].data_ptr(), nrm.data_ptr(), occ.data_ptr(),
                         stream=s.cuda_stream)
-        outs.append(o)
+        res.append(bufs + [nrm, occ])
     s.synchronize()
-    assert torch.equal(outs[0], outs[1])
+    for a, b in zip(*res):
+        assert torch.equal(a, b)
+    assert torch.isfinite(res[0][2]).sum().item() > 200
     # a wide slot is not batchable (the batch kernels address 32-bit offsets)
     o = torch.empty((H, W), device="cuda")
     with pytest.raises(_lib.SprError):
