@@ -53,39 +53,11 @@ __device__ __forceinline__ void wave_min3_u16x2(uint32_t& a, uint32_t& b, uint32
     c = __builtin_amdgcn_readlane(c, 63);
 }
 
-
-template <typename T>
-__device__ __forceinline__ float lds_voxel(const uint8_t* p) {
-    return (float)*reinterpret_cast<const T*>(p);
-}
-
-// The voxel pair (x, x+1) at byte offset b of a 4-byte aligned slab, read as two aligned
-// dwords (one ds_read2_b32) and shifted into place with v_alignbyte: LDS accesses that
-// are not dword aligned stall the LDS pipeline.  Reads the 8 bytes from the dword holding
-// byte b, so up to 6 bytes past the pair (the slab array carries 16 bytes of slack).
-__device__ __forceinline__ void lds_pair(const uint8_t* slab, uint32_t b, float& v0, float& v1,
-                                         uint16_t) {
-    const uint32_t* d = reinterpret_cast<const uint32_t*>(slab + (b & ~3u));
-    const uint32_t w = __builtin_amdgcn_alignbyte(d[1], d[0], b & 3u);
-    v0 = (float)(w & 0xFFFFu);
-    v1 = (float)(w >> 16);
-}
-__device__ __forceinline__ void lds_pair(const uint8_t* slab, uint32_t b, float& v0, float& v1,
-                                         uint8_t) {
-    const uint32_t* d = reinterpret_cast<const uint32_t*>(slab + (b & ~3u));
-    const uint32_t w = __builtin_amdgcn_alignbyte(d[1], d[0], b & 3u);
-    v0 = (float)(w & 0xFFu);
-    v1 = (float)((w >> 8) & 0xFFu);
-}
-__device__ __forceinline__ void lds_pair(const uint8_t* slab, uint32_t b, float& v0, float& v1,
-                                         float) {
-    const float* d = reinterpret_cast<const float*>(slab + b);  // b is a multiple of 4
-    v0 = d[0];
-    v1 = d[1];
-}
-
-// The same from a dword-aligned address a and the byte shift of the pair within it (the
-// four corner rows of a sample share the shift: row and plane strides are multiples of 16)
+// The voxel pair (x, x+1) from a dword-aligned slab address a and the byte shift sh of the
+// pair within that dword, read as two aligned dwords (one ds_read2_b32) and shifted into
+// place with v_alignbyte: LDS accesses that are not dword aligned stall the LDS pipeline.
+// The four corner rows of a sample share the shift (row and plane strides are multiples
+// of 16).  Reads up to 6 bytes past the pair (the slab arrays carry 16 bytes of slack).
 __device__ __forceinline__ void lds_pair_at(const uint8_t* a, uint32_t sh, float& v0, float& v1,
                                             uint16_t) {
     const uint32_t* d = reinterpret_cast<const uint32_t*>(a);

@@ -429,8 +429,9 @@ def train(ctx, slots, ranges, steps=2500, batch=8, frame=1024, max_steps=896, lr
 
 
 def main():
-    """python -m spimagine_amd.learned [--steps N] [--out PATH]: train on seeded 512^3
-    uint16 blob volumes and save the parameters."""
+    """python -m spimagine_amd.learned [--steps N] [--volumes N] [--lr LR] [--init PATH]
+    [--seed S] [--out PATH]: train on seeded 512^3 uint16 blob volumes (seeds 1..N) and
+    save the parameters; --init continues from saved parameters (their input kind)."""
     import argparse
     import sys
     import torch
@@ -439,6 +440,8 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--volumes", type=int, default=4)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--init", default=None)
     ap.add_argument("--out", default=WEIGHTS_PATH)
     ap.add_argument("--inputs", choices=INPUTS, default=INPUTS_DEFAULT)
     args = ap.parse_args()
@@ -446,6 +449,11 @@ def main():
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
 
+    params0, inputs, pairs0 = None, args.inputs, 0
+    if args.init:
+        params0, inputs = load_weights(args.init)
+        with np.load(args.init) as w:
+            pairs0 = int(w["pairs"]) if "pairs" in w.files else 0
     ctx = _lib.Context(0)
     slots, ranges = [], []
     for v in training_volumes(args.volumes):
@@ -456,10 +464,10 @@ def main():
         del v
     log("training on %d volumes, %d steps x %d pairs" % (len(slots), args.steps, args.batch))
     params, hist = train(ctx, slots, ranges, steps=args.steps, batch=args.batch, lr=args.lr, log=log,
-                         inputs=args.inputs)
+                         inputs=inputs, params=params0, seed=args.seed)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.savez(args.out, params=params, history=np.array(hist, np.float64),
-             pairs=args.steps * args.batch, inputs=args.inputs)
+             pairs=pairs0 + args.steps * args.batch, inputs=inputs)
     log("saved %s" % args.out)
 
 
