@@ -76,7 +76,10 @@ extern "C" {
    depend on them.
    SPR_TUNE_TILE_MAP  how 16x16 pixel tiles are dealt to workgroups and so to the 8 XCDs:
                       0 = tile rows interleaved over XCDs (default), 1 = one contiguous
-                      band per XCD, 2 = row-major
+                      band per XCD, 2 = row-major, 3 = as 0 with each XCD's tiles in
+                      descending estimated cost (per-wave staged max projection and
+                      opacity mode; two small kernels order the tiles before the frame,
+                      on its stream)
    SPR_TUNE_SLAB_KB   per-wave LDS slab (KiB) of the staged max projection: 8, 12, 16 or
                       19 (default; per-wave staging only, 16 for workgroup staging); 0 =
                       sample every chunk from global memory
@@ -85,12 +88,17 @@ extern "C" {
    SPR_TUNE_CHUNK     samples per staged box: 0 = from the ray step (default), 4, 8, 16,
                       32 or 64 (boxes of 2 or 4 groups of 16 samples)
    SPR_TUNE_ROW_PAD   1 = pad staged rows to an odd number of 16-byte units (spreads the
-                      LDS banks rows a few apart fall on), 0 = no padding (default) */
+                      LDS banks rows a few apart fall on), 0 = no padding (default)
+   SPR_TUNE_WG_WAVES  waves per workgroup of the per-wave staged max projection (slab 16
+                      or 19) and opacity mode: 1 (an 8x8 quarter of a tile, default: a CU
+                      starts new work as soon as any of its waves ends) or 4 (a 16x16
+                      tile) */
 #define SPR_TUNE_TILE_MAP 0
 #define SPR_TUNE_SLAB_KB 1
 #define SPR_TUNE_STAGING 2
 #define SPR_TUNE_CHUNK 3
 #define SPR_TUNE_ROW_PAD 4
+#define SPR_TUNE_WG_WAVES 5
 
 typedef struct spr_ctx spr_ctx;
 

@@ -26,9 +26,9 @@ FILTER_LINEAR, FILTER_NEAREST = 0, 1
 METHOD_MAX_PROJECT, METHOD_ISO_SURFACE = 0, 1
 LAYOUT_AUTO, LAYOUT_LINEAR, LAYOUT_PADDED = 0, 1, 2
 TUNE_TILE_MAP, TUNE_SLAB_KB = 0, 1
-TUNE_STAGING, TUNE_CHUNK, TUNE_ROW_PAD = 2, 3, 4
+TUNE_STAGING, TUNE_CHUNK, TUNE_ROW_PAD, TUNE_WG_WAVES = 2, 3, 4, 5
 TUNING_KEYS = {"tile_map": TUNE_TILE_MAP, "slab_kb": TUNE_SLAB_KB, "staging": TUNE_STAGING,
-               "chunk": TUNE_CHUNK, "row_pad": TUNE_ROW_PAD}
+               "chunk": TUNE_CHUNK, "row_pad": TUNE_ROW_PAD, "wg_waves": TUNE_WG_WAVES}
 
 # every symbol include/spimrender.h declares
 EXPORTS = (

@@ -1,12 +1,13 @@
 """Launch autotuner for batched max projections.
 
-The kernels' launch knobs (spr_set_tuning) change speed, never results:
-  tile_map  how 16x16 pixel tiles are dealt to workgroups and so to the 8 XCDs
-            (0 interleaved rows, 1 one band per XCD, 2 row-major)
+The kernels' launch knobs (spr_set_tuning) change speed, never results.  The batch
+kernel's knob:
   slab_kb   per-wave LDS slab of the staged sampler (0 = global gathers only, 12, 16, 19)
-The best pair depends on voxel type, volume and frame size, step count and the poses
-(how far the rays' footprints spread across a slab), so the tuner times every candidate
-on a sample of the actual batch and keeps the fastest for that workload key.  The
+(its persistent workgroups take tiles from per-XCD queues, so the single-frame knobs
+tile_map and wg_waves do not apply).  The best value depends on voxel type, volume and
+frame size, step count and the poses (how far the rays' footprints spread across a
+slab), so the tuner times every candidate on a sample of the actual batch and keeps the
+fastest for that workload key.  The
 reference has no counterpart: its OpenCL launch is a fixed global size of one work item
 per pixel (spimagine/volumerender/volumerender.py:366-386).
 """
@@ -16,7 +17,7 @@ import time
 
 from . import _lib
 
-CANDIDATES = tuple(dict(tile_map=m, slab_kb=s) for m in (0, 1, 2) for s in (0, 12, 16, 19))
+CANDIDATES = tuple(dict(slab_kb=s) for s in (0, 12, 16, 19))
 
 _cache = {}
 

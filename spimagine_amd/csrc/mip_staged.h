@@ -83,24 +83,15 @@ struct alignas(16) Unit16 {
     uint32_t w[4];
 };
 
-// Ray set-up shared by the staged kernels (volume_kernel.cl:45-106): writes the miss
-// outputs and returns whether the ray hits the box; position and step in texture units.
-template <bool kShort>
-__device__ __forceinline__ bool staged_ray_begin(const RayParams& p, uint32_t x, uint32_t y,
-                                                 bool inside, float* __restrict__ out,
-                                                 float* __restrict__ alpha, float& px, float& py,
-                                                 float& pz, float& dx, float& dy, float& dz,
-                                                 float& tnear) {
-    if (!inside) return false;
-    const uint32_t o = x + p.W * y;
+// Ray set-up of the staged kernels (volume_kernel.cl:45-106) without outputs: whether the
+// ray hits the box, and its sample 0 position and step in texture units.
+__device__ __forceinline__ bool staged_ray_pos(const RayParams& p, uint32_t x, uint32_t y,
+                                               float& px, float& py, float& pz, float& dx,
+                                               float& dy, float& dz, float& tnear) {
     F4 orig, direc;
     ray_setup(p.invP, p.invM, x, y, p.W, p.H, &orig, &direc);
     float tfar;
-    if (!intersect_box(orig, direc, p.box, &tnear, &tfar)) {
-        out[o] = 0.0f;
-        alpha[o] = kShort ? 0.0f : -1.0f;
-        return false;
-    }
+    if (!intersect_box(orig, direc, p.box, &tnear, &tfar)) return false;
     if (tnear < 0.0f) tnear = 0.0f;
     const float dt = fabsf(tfar - tnear) / (float)(p.nouter * kLoopUnroll);
     orig = add4(orig, scl4((float)p.current_part * dt, direc));
@@ -114,6 +105,22 @@ __device__ __forceinline__ bool staged_ray_begin(const RayParams& p, uint32_t x,
     dy = delta.y;
     dz = delta.z;
     return true;
+}
+
+// staged_ray_pos, writing a missed ray's outputs (volume_kernel.cl:85-91; the short kernel's
+// alpha 0, volume_kernel.cl:258-264)
+template <bool kShort>
+__device__ __forceinline__ bool staged_ray_begin(const RayParams& p, uint32_t x, uint32_t y,
+                                                 bool inside, float* __restrict__ out,
+                                                 float* __restrict__ alpha, float& px, float& py,
+                                                 float& pz, float& dx, float& dy, float& dz,
+                                                 float& tnear) {
+    if (!inside) return false;
+    if (staged_ray_pos(p, x, y, px, py, pz, dx, dy, dz, tnear)) return true;
+    const uint32_t o = x + p.W * y;
+    out[o] = 0.0f;
+    alpha[o] = kShort ? 0.0f : -1.0f;
+    return false;
 }
 
 // Padded-voxel box of samples first..last of a ray whose sample 0 is at (px, py, pz),
@@ -510,63 +517,44 @@ __device__ __forceinline__ void staged_plane_end(const RayParams& p, uint32_t o,
     out[o] = p.current_part == 0 ? col : fmaxf(col, out[o]);
 }
 
-// CH: samples per unrolled group (16, 8 or 4); a box holds p.box16 groups of 16 samples
-// when CH is 16 (one group otherwise).  The samples, their order and their positions are
-// those of the reference's 16-sample inner loop whatever the grouping is (a max does not
-// depend on how the sequence is cut); a smaller CH keeps a box small when the ray step is
-// several voxels long, a larger box spreads the box reduction and copy over more samples
-// when it is short.  With per-wave slabs and CH 16, a box larger than the slab is split
-// into halves of 8 samples and those into quarters of 4, each staged on its own (when it
-// is at most 4x the slab); only what still does not fit is sampled from global memory.
-// SHARED = false: each wave stages the box of its own 64 rays into its own slab
-// (SLAB_BYTES, wave-level ordering only).  SHARED = true: the workgroup's four waves
-// stage the union box of its 256 rays into one slab of SLAB_BYTES with all 256 threads,
-// two workgroup barriers per box; s_box holds the four waves' partial boxes.
-// OPAC: the opacity mode (alpha_pow > 0, Opacity); boxes are not loaded ahead (a spent
-// lane's position does not advance with the group).
-template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE = false,
-          bool OPAC = false>
-__device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE>& vol, const RayParams& p,
-                                                uint32_t x, uint32_t y, bool inside, uint8_t* slab,
-                                                uint32_t* s_box, float* __restrict__ out,
-                                                float* __restrict__ alpha) {
+// Lanes without a ray take the first active lane's ray (act: the wave's ballot of active,
+// nonzero), so every lane's LDS address is valid; the shuffles run with every lane of the
+// wave enabled.
+__device__ __forceinline__ void shadow_first_active(uint64_t act, bool active, float& px,
+                                                    float& py, float& pz, float& dx, float& dy,
+                                                    float& dz) {
+    const int src = __builtin_ctzll(act);
+    const float spx = __shfl(px, src), spy = __shfl(py, src), spz = __shfl(pz, src);
+    const float sdx = __shfl(dx, src), sdy = __shfl(dy, src), sdz = __shfl(dz, src);
+    px = active ? px : spx;
+    py = active ? py : spy;
+    pz = active ? pz : spz;
+    dx = active ? dx : sdx;
+    dy = active ? dy : sdy;
+    dz = active ? dz : sdz;
+}
+
+// The sampling loop of mip_staged_rays over groups g_begin .. g_end-1 of CH samples of
+// every ray of the wave (workgroup when SHARED), from positions (px, py, pz) on: stages
+// the boxes, samples them and folds the samples into col (col2: a pair's second channel).
+// active: this lane has a ray; live: the wave has one (lanes without a ray follow an
+// active lane's positions).
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE, bool OPAC>
+__device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WIDE>& vol,
+                                                 const RayParams& p, bool active, bool live,
+                                                 uint8_t* slab, uint32_t* s_box, float& px,
+                                                 float& py, float& pz, float dx, float dy,
+                                                 float dz, int g_begin, int g_end, float& col,
+                                                 float& col2) {
     constexpr bool kShort = !std::is_same<T, float>::value;
-    constexpr bool kPair = std::is_same<T, U16Pair>::value;  // out: two planes of W x H
+    constexpr bool kPair = std::is_same<T, U16Pair>::value;
     constexpr uint32_t E = 16u / sizeof(T);  // voxels per 16-byte unit
     constexpr uint32_t TS = sizeof(T);
     constexpr uint32_t NT = SHARED ? 256u : 64u;
     const uint32_t lane = threadIdx.x & 63u;
-    float px = 0.5f, py = 0.5f, pz = 0.5f, dx = 0.0f, dy = 0.0f, dz = 0.0f, tnear = 0.0f;
-    const bool active = staged_ray_begin<kShort>(p, x, y, inside, out, alpha, px, py, pz, dx, dy,
-                                                 dz, tnear);
-    if constexpr (kPair) {
-        if (inside && !active) out[x + p.W * y + p.W * p.H] = 0.0f;  // the miss, second plane
-    }
-    const uint64_t act = __ballot(active);
-    if constexpr (SHARED) {
-        if (__syncthreads_or(active ? 1 : 0) == 0) return;  // uniform over the workgroup
-    } else {
-        if (act == 0) return;
-    }
-    const bool live = act != 0;  // uniform over the wave
-    // lanes without a ray shadow the first active lane of their wave, so every lane's LDS
-    // address is valid (the shuffles run with every lane of the wave enabled)
-    if (live) {
-        const int src = __builtin_ctzll(act);
-        const float spx = __shfl(px, src), spy = __shfl(py, src), spz = __shfl(pz, src);
-        const float sdx = __shfl(dx, src), sdy = __shfl(dy, src), sdz = __shfl(dz, src);
-        px = active ? px : spx;
-        py = active ? py : spy;
-        pz = active ? pz : spz;
-        dx = active ? dx : sdx;
-        dy = active ? dy : sdy;
-        dz = active ? dz : sdz;
-    }
-
     const uint32_t PY = vol.py, pitch = vol.px_bytes;
     const float mgx = 1e-5f * vol.fnx + 0.01f, mgy = 1e-5f * vol.fny + 0.01f,
                 mgz = 1e-5f * vol.fnz + 0.01f;
-    float col = 0.0f, col2 = 0.0f;
 
     using PV = PaddedVolume<T, POW2, WIDE>;
     // The wave's (workgroup's when SHARED) box of samples first..last of every ray, from the
@@ -642,7 +630,6 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
 
     static_assert(kLoopUnroll % CH == 0, "a chunk is a whole number of boxes");
     static_assert(!(OPAC && kPair), "pair renders are plain max projections");
-    const int ngroups = (p.nouter + 1) * (kLoopUnroll / CH);
     const int G = CH == 16 ? (int)p.box16 : 1;  // groups per box, uniform
     Opacity op;
     op.cumsum = 1.0f;
@@ -665,8 +652,8 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
     bool pf = false;
     int pf_n = 0;
     U32x4 pv[kPrefetchRows];
-    for (int i = 0; i < ngroups; i += G) {
-        const int ng = min(G, ngroups - i);
+    for (int i = g_begin; i < g_end; i += G) {
+        const int ng = min(G, g_end - i);
         Box b;
         bool fits;
         if (!SHARED && pf) {
@@ -680,8 +667,8 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
         pf = false;
         if constexpr (!SHARED && !OPAC) {
             const int i2 = i + G;
-            if (i2 < ngroups) {
-                const int ng2 = min(G, ngroups - i2);
+            if (i2 < g_end) {
+                const int ng2 = min(G, g_end - i2);
                 if (box_of(std::true_type(), (float)(ng * CH), (float)((ng + ng2) * CH - 1), nb)) {
                     const uint32_t R = 64u / nb.nu;
                     pf_n = (int)((nb.SY * nb.SZ + R - 1u) / R);
@@ -753,6 +740,49 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
             release();
         }
     }
+}
+
+// A wave's (workgroup's when SHARED) rays of a max projection, from set-up to outputs.
+// CH: samples per unrolled group (16, 8 or 4); a box holds p.box16 groups of 16 samples
+// when CH is 16 (one group otherwise).  The samples, their order and their positions are
+// those of the reference's 16-sample inner loop whatever the grouping is (a max does not
+// depend on how the sequence is cut); a smaller CH keeps a box small when the ray step is
+// several voxels long, a larger box spreads the box reduction and copy over more samples
+// when it is short.  With per-wave slabs and CH 16, a box larger than the slab is split
+// into halves of 8 samples and those into quarters of 4, each staged on its own (when it
+// is at most 4x the slab); only what still does not fit is sampled from global memory.
+// SHARED = false: each wave stages the box of its own 64 rays into its own slab
+// (SLAB_BYTES, wave-level ordering only).  SHARED = true: the workgroup's four waves
+// stage the union box of its 256 rays into one slab of SLAB_BYTES with all 256 threads,
+// two workgroup barriers per box; s_box holds the four waves' partial boxes.
+// OPAC: the opacity mode (alpha_pow > 0, Opacity); boxes are not loaded ahead (a spent
+// lane's position does not advance with the group).
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE = false,
+          bool OPAC = false>
+__device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE>& vol, const RayParams& p,
+                                                uint32_t x, uint32_t y, bool inside, uint8_t* slab,
+                                                uint32_t* s_box, float* __restrict__ out,
+                                                float* __restrict__ alpha) {
+    constexpr bool kShort = !std::is_same<T, float>::value;
+    constexpr bool kPair = std::is_same<T, U16Pair>::value;  // out: two planes of W x H
+    float px = 0.5f, py = 0.5f, pz = 0.5f, dx = 0.0f, dy = 0.0f, dz = 0.0f, tnear = 0.0f;
+    const bool active = staged_ray_begin<kShort>(p, x, y, inside, out, alpha, px, py, pz, dx, dy,
+                                                 dz, tnear);
+    if constexpr (kPair) {
+        if (inside && !active) out[x + p.W * y + p.W * p.H] = 0.0f;  // the miss, second plane
+    }
+    const uint64_t act = __ballot(active);
+    if constexpr (SHARED) {
+        if (__syncthreads_or(active ? 1 : 0) == 0) return;  // uniform over the workgroup
+    } else {
+        if (act == 0) return;
+    }
+    const bool live = act != 0;  // uniform over the wave
+    if (live) shadow_first_active(act, active, px, py, pz, dx, dy, dz);
+    float col = 0.0f, col2 = 0.0f;
+    mip_staged_march<T, POW2, SLAB_BYTES, CH, SHARED, WIDE, OPAC>(
+        vol, p, active, live, slab, s_box, px, py, pz, dx, dy, dz, 0,
+        (p.nouter + 1) * (kLoopUnroll / CH), col, col2);
     if (active) {
         staged_ray_end<kShort, OPAC>(p, x + p.W * y, col, tnear, out, alpha);
         if constexpr (kPair) staged_plane_end(p, x + p.W * y + p.W * p.H, col2, out);
@@ -774,6 +804,161 @@ __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2
     const bool inside = tile_pixel(p, &x, &y);
     mip_staged_rays<T, POW2, SLAB_BYTES, CH, SHARED, WIDE, OPAC>(vol, p, x, y, inside, slab, s_box,
                                                                  out, alpha);
+}
+
+// Estimated cost of each wave of a frame of the per-wave staged kernels, for the
+// cost-ordered tile map.  A wave's rays take their 16-sample boxes from a slab when the
+// box fits (the rays run close together) and otherwise split the box or gather from
+// global memory, at ~3.4x the time per sample (measured).  The estimate: 0 for a wave
+// without a ray, else 1 + the number of kCostProbes 16-sample groups, spread along the
+// rays, whose box does not fit slab_bytes (0 .. kCostProbes + 1).  One workgroup per
+// 16x16 tile, row-major (grid p.ntiles), 256 threads; no outputs but cost[4 tile + wave]
+// (wave: the 8x8 quarter of tile_thread_pixel).
+constexpr int kCostProbes = 6;
+
+template <typename T, bool POW2, bool WIDE = false>
+__global__ __launch_bounds__(256) void k_tile_cost(PaddedVolume<T, POW2, WIDE> vol, RayParams p,
+                                                   uint32_t slab_bytes,
+                                                   uint32_t* __restrict__ cost) {
+    using PV = PaddedVolume<T, POW2, WIDE>;
+    constexpr uint32_t E = 16u / sizeof(T);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, t = blockIdx.x;
+    uint32_t x, y;
+    tile_thread_pixel(t % p.ntx, t / p.ntx, &x, &y);
+    float px = 0.5f, py = 0.5f, pz = 0.5f, dx = 0.0f, dy = 0.0f, dz = 0.0f, tnear = 0.0f;
+    const bool active = x < p.W && y < p.H &&
+                        staged_ray_pos(p, x, y, px, py, pz, dx, dy, dz, tnear);
+    const uint64_t act = __ballot(active);
+    uint32_t c = 0;
+    if (act != 0) {
+        c = 1;
+        // lanes without a ray take the first active lane's ray, as in mip_staged_rays
+        const int src = __builtin_ctzll(act);
+        const float spx = __shfl(px, src), spy = __shfl(py, src), spz = __shfl(pz, src);
+        const float sdx = __shfl(dx, src), sdy = __shfl(dy, src), sdz = __shfl(dz, src);
+        px = active ? px : spx;
+        py = active ? py : spy;
+        pz = active ? pz : spz;
+        dx = active ? dx : sdx;
+        dy = active ? dy : sdy;
+        dz = active ? dz : sdz;
+        const float mgx = 1e-5f * vol.fnx + 0.01f, mgy = 1e-5f * vol.fny + 0.01f,
+                    mgz = 1e-5f * vol.fnz + 0.01f;
+        const int last_group = p.nouter;  // groups 0 .. nouter of 16 samples
+#pragma unroll 1
+        for (int m = 0; m < kCostProbes; ++m) {
+            const float first = (float)(kLoopUnroll * ((m * last_group) / (kCostProbes - 1)));
+            uint32_t b0, b1, b2;
+            staged_box_n<PV, true>(vol, active, first, first + (float)(kLoopUnroll - 1), px, py,
+                                   pz, dx, dy, dz, mgx, mgy, mgz, b0, b1, b2);
+            wave_min3_u16x2(b0, b1, b2);
+            // the box's slab bytes as mip_staged_rays' box_of (rows of whole 16-byte units)
+            const uint32_t LX = b0 & 0xFFFFu, LY = b0 >> 16, LZ = b1 & 0xFFFFu;
+            const uint32_t HX = 0xFFFFu - (b1 >> 16), HY = 0xFFFFu - (b2 & 0xFFFFu),
+                           HZ = 0xFFFFu - (b2 >> 16);
+            const uint32_t lox = LX & ~(E - 1u);
+            const uint32_t nu = ((HX + 1u - lox + E - 1u) & ~(E - 1u)) / E;
+            const uint64_t bytes = (uint64_t)nu * 16u * (HY - LY + 1u) * (HZ - LZ + 1u);
+            c += nu <= 64u && bytes <= (uint64_t)slab_bytes ? 0u : 1u;
+        }
+    }
+    if (lane == 0) cost[4u * t + wave] = c;
+}
+
+// Workgroup -> work item order for kTileMapCostXcd: XCD x takes the items of tile rows
+// x, x+8, x+16, ... in descending cost, row-major among equal costs (neighbouring items
+// run close in time and share the XCD's L2).  Items: 16x16 tiles (per_wave = 0; a tile
+// costs its slowest wave, as a workgroup holds its LDS until its last wave ends) or the
+// tiles' 8x8 quarters, item 4 tile + quarter (per_wave = 1, one wave per workgroup).
+// order[8 k + x] = XCD x's k-th item; slots left over (nty not a multiple of 8) get the
+// item past the last (below the frame).  One workgroup of 8 waves, wave x orders XCD x's
+// items: a count per cost value, then a stable scatter by ballots over 64 items at a time.
+constexpr uint32_t kCostValues = kCostProbes + 2;
+
+__global__ __launch_bounds__(512) void k_tile_order(const uint32_t* __restrict__ cost,
+                                                    RayParams p, uint32_t per_wave,
+                                                    uint32_t* __restrict__ order) {
+    const uint32_t lane = threadIdx.x & 63u, x = threadIdx.x >> 6;
+    const uint32_t Q = per_wave ? 4u : 1u;                // items per tile
+    const uint32_t row = p.ntx * Q;                       // items per tile row
+    const uint32_t rows = x < p.nty ? (p.nty - x + 7u) / 8u : 0u;  // tile rows of XCD x
+    const uint32_t cnt = rows * row;
+    const uint32_t slots = (p.nty + 7u) / 8u * row;       // per XCD
+    const uint64_t below = (1ull << lane) - 1ull;
+    auto item = [&](uint32_t j) { return (x + 8u * (j / row)) * row + j % row; };
+    auto value = [&](uint32_t e) -> uint32_t {
+        const uint32_t c = per_wave ? cost[e]
+                                    : max(max(cost[4 * e], cost[4 * e + 1]),
+                                          max(cost[4 * e + 2], cost[4 * e + 3]));
+        return min(c, kCostValues - 1u);
+    };
+    uint32_t start[kCostValues];  // wave-uniform
+#pragma unroll
+    for (uint32_t v = 0; v < kCostValues; ++v) start[v] = 0;
+    for (uint32_t j0 = 0; j0 < cnt; j0 += 64u) {
+        const uint32_t j = j0 + lane;
+        const uint32_t v = j < cnt ? value(item(j)) : kCostValues;
+#pragma unroll
+        for (uint32_t k = 0; k < kCostValues; ++k) start[k] += (uint32_t)__popcll(__ballot(v == k));
+    }
+    // descending: the costliest value first
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = (int)kCostValues - 1; k >= 0; --k) {
+        const uint32_t h = start[k];
+        start[k] = run;
+        run += h;
+    }
+    for (uint32_t j0 = 0; j0 < cnt; j0 += 64u) {
+        const uint32_t j = j0 + lane;
+        const uint32_t e = j < cnt ? item(j) : 0u;
+        const uint32_t v = j < cnt ? value(e) : kCostValues;
+        uint32_t pos = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kCostValues; ++k) {
+            const uint64_t m = __ballot(v == k);
+            if (v == k) pos = start[k] + (uint32_t)__popcll(m & below);
+            start[k] += (uint32_t)__popcll(m);
+        }
+        if (j < cnt) order[8u * pos + x] = e;
+    }
+    for (uint32_t k = cnt + lane; k < slots; k += 64u) order[8u * k + x] = p.ntiles * Q;
+}
+
+// Pixel of this thread in a one-wave workgroup: an 8x8 quarter of a 16x16 tile.  Tile
+// rows are dealt to the XCDs as kTileMapInterleaved, the four quarters of a tile to
+// consecutive workgroups of its XCD; kTileMapCostXcd: item p.order[b] = 4 tile + quarter.
+__device__ __forceinline__ bool wave_tile_pixel(const RayParams& p, uint32_t* x, uint32_t* y) {
+    const uint32_t b = blockIdx.x, lane = threadIdx.x & 63u;
+    uint32_t tx, ty, q;
+    if (p.tile_map == kTileMapCostXcd) {
+        const uint32_t e = p.order[b];  // >= 4 ntiles: a surplus workgroup (ty >= nty)
+        q = e & 3u;
+        tx = (e >> 2) % p.ntx;
+        ty = (e >> 2) / p.ntx;
+    } else {
+        const uint32_t xcd = b & 7u, k = b >> 3;
+        q = k & 3u;
+        ty = xcd + 8u * ((k >> 2) / p.ntx);
+        tx = (k >> 2) % p.ntx;
+    }
+    *x = tx * 16u + (q & 1u) * 8u + (lane & 7u);
+    *y = ty * 16u + (q >> 1) * 8u + (lane >> 3);
+    return ty < p.nty && *x < p.W && *y < p.H;
+}
+
+// one frame with one wave per workgroup (64 threads, one slab of SLAB_BYTES): a CU takes a
+// new wave's work as soon as any of its waves ends, instead of when a workgroup's slowest
+// of four waves does.  Grid: 4 x tile_grid (interleaved or cost-ordered maps).
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool WIDE = false, bool OPAC = false>
+__global__ __launch_bounds__(64) void k_max_project_wave(PaddedVolume<T, POW2, WIDE> vol,
+                                                         RayParams p, float* __restrict__ out,
+                                                         float* __restrict__ alpha) {
+    __shared__ Unit16 lds[SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
+    uint32_t x = 0, y = 0;
+    const bool inside = wave_tile_pixel(p, &x, &y);
+    mip_staged_rays<T, POW2, SLAB_BYTES, CH, false, WIDE, OPAC>(
+        vol, p, x, y, inside, reinterpret_cast<uint8_t*>(lds), nullptr, out, alpha);
 }
 
 // Iso-surface march through the per-wave LDS slabs (iso_kernel.cl:83-131, the plain

@@ -27,6 +27,7 @@ struct RayParams {
     uint32_t tile_map;          // kTileMap* below
     uint32_t row_pad;           // staged kernels: pad LDS slab rows to an odd multiple of 16 B
     uint32_t box16;             // staged kernels: 16-sample groups per staged box (1, 2, 4)
+    const uint32_t* order;      // kTileMapCostXcd: workgroup -> work item (k_tile_order)
 };
 
 // Workgroup -> 16x16 tile assignment.  The dispatcher hands consecutive workgroups to
@@ -37,10 +38,19 @@ struct RayParams {
 //                        8 * ceil(nty/8) * ntx workgroups; the surplus ones exit at once.
 //   kTileMapBands        XCD x renders one contiguous horizontal band of the frame.
 //   kTileMapLinear       tile = workgroup id (row-major).
-constexpr uint32_t kTileMapInterleaved = 0, kTileMapBands = 1, kTileMapLinear = 2;
+//   kTileMapCostXcd      as kTileMapInterleaved, but each XCD takes its tile rows' work
+//                        in descending estimated cost (p.order, from k_tile_cost and
+//                        k_tile_order), row-major among equal costs: the dispatcher hands
+//                        the slowest work out first, so the frame does not end on a few
+//                        long tiles started late.  The order changes only which
+//                        workgroup renders a tile and when: every pixel is the same.
+constexpr uint32_t kTileMapInterleaved = 0, kTileMapBands = 1, kTileMapLinear = 2,
+                   kTileMapCostXcd = 3;
 
 __host__ __device__ inline uint32_t tile_grid(uint32_t tile_map, uint32_t ntx, uint32_t nty) {
-    return tile_map == kTileMapInterleaved ? 8u * ((nty + 7u) / 8u) * ntx : ntx * nty;
+    return tile_map == kTileMapInterleaved || tile_map == kTileMapCostXcd
+               ? 8u * ((nty + 7u) / 8u) * ntx
+               : ntx * nty;
 }
 
 // pixel of this thread inside 16x16 tile (tx, ty): each wave64 takes an 8x8 quarter
@@ -57,6 +67,10 @@ __device__ __forceinline__ bool tile_pixel(const RayParams& p, uint32_t* x, uint
         const uint32_t xcd = b & 7u, k = b >> 3;
         ty = xcd + 8u * (k / p.ntx);
         tx = k % p.ntx;
+    } else if (p.tile_map == kTileMapCostXcd) {
+        const uint32_t t = p.order[b];  // >= ntiles: a surplus workgroup (ty >= nty)
+        tx = t % p.ntx;
+        ty = t / p.ntx;
     } else if (p.tile_map == kTileMapBands) {
         const uint32_t q = p.ntiles >> 3, r = p.ntiles & 7u;
         const uint32_t xcd = b & 7u, k = b >> 3;

@@ -76,6 +76,7 @@ struct spr_ctx {
     int staging = 0;   // 0 per-wave boxes, 1 one box per workgroup
     int chunk = 0;     // samples per staged box, 0 = from the ray step
     int row_pad = 0;   // pad staged slab rows to an odd number of 16-byte units
+    int wg_waves = 1;  // per-wave staged kernels: waves per workgroup (1 or 4)
 };
 
 namespace {
@@ -213,9 +214,12 @@ RayParams make_params(const spr_params& prm, int tile_map, int w, int h) {
     p.ntx = (uint32_t)((w + 15) / 16);
     p.nty = (uint32_t)((h + 15) / 16);
     p.ntiles = p.ntx * p.nty;
-    p.tile_map = (uint32_t)tile_map;
+    // the cost-ordered map needs an order computed for the frame (order_tiles); every
+    // other launch deals tiles interleaved
+    p.tile_map = tile_map == (int)kTileMapCostXcd ? kTileMapInterleaved : (uint32_t)tile_map;
     p.row_pad = 0;
     p.box16 = 1;
+    p.order = nullptr;
     return p;
 }
 
@@ -360,6 +364,55 @@ void staged_tuning(const spr_ctx* c, RayParams& p) {
     p.row_pad = (uint32_t)c->row_pad;
     p.box16 = c->chunk > 16 ? (uint32_t)(c->chunk / 16) : 1u;
 }
+
+// The cost-ordered tile map (kTileMapCostXcd) for the per-wave staged kernels: the waves'
+// estimated costs (k_tile_cost) and the workgroups' order (k_tile_order), computed on
+// stream s into *buf from the stream-ordered allocator (the caller frees it after the
+// frame's launch).  per_wave: one wave per workgroup (k_max_project_wave) instead of a
+// 16x16 tile.  Sets p.tile_map and p.order and the frame's grid; other tile maps, and
+// frames of more than kMaxOrderedTiles tiles, are left as they are.
+constexpr uint32_t kMaxOrderedTiles = 65536;
+
+template <typename T, bool P2, bool WIDE>
+hipError_t order_tiles(int tile_map, bool per_wave, const PaddedVolume<T, P2, WIDE>& v,
+                       uint32_t slab_bytes, RayParams& p, hipStream_t s, void** buf, dim3* grid) {
+    if (tile_map != (int)kTileMapCostXcd || p.ntiles > kMaxOrderedTiles) return hipSuccess;
+    const uint32_t slots = tile_grid((uint32_t)tile_map, p.ntx, p.nty) << (per_wave ? 2 : 0);
+    hipError_t e = hipMallocAsync(buf, (size_t)(4 * p.ntiles + slots) * sizeof(uint32_t), s);
+    if (e != hipSuccess) return e;
+    uint32_t* cost = (uint32_t*)*buf;
+    uint32_t* order = cost + 4 * p.ntiles;
+    hipLaunchKernelGGL((k_tile_cost<T, P2, WIDE>), dim3(p.ntiles), dim3(256), 0, s, v, p,
+                       slab_bytes, cost);
+    p.tile_map = (uint32_t)tile_map;
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(512), 0, s, (const uint32_t*)cost, p,
+                       per_wave ? 1u : 0u, order);
+    p.order = order;
+    *grid = dim3(slots);
+    return hipGetLastError();
+}
+
+// one wave per workgroup: the plain max projection (slab 16 or 19 KiB) or the opacity mode
+template <typename T, bool P2, bool OPAC, int CH, typename... Args>
+void launch_wave_ch(int slab_kb, dim3 grid, hipStream_t s, Args... args) {
+    if (slab_kb == 16 && !OPAC)
+        hipLaunchKernelGGL((k_max_project_wave<T, P2, 16384, CH, false, OPAC>), grid, dim3(64), 0, s, args...);
+    else
+        hipLaunchKernelGGL((k_max_project_wave<T, P2, 19456, CH, false, OPAC>), grid, dim3(64), 0, s, args...);
+}
+
+template <typename T, bool P2, bool OPAC, typename... Args>
+void launch_wave(int slab_kb, int ch, dim3 grid, hipStream_t s, Args... args) {
+    if (ch == 16)
+        launch_wave_ch<T, P2, OPAC, 16>(slab_kb, grid, s, args...);
+    else if (ch == 8)
+        launch_wave_ch<T, P2, OPAC, 8>(slab_kb, grid, s, args...);
+    else
+        launch_wave_ch<T, P2, OPAC, 4>(slab_kb, grid, s, args...);
+}
+
+// a per-wave slab's bytes for the slab_kb tuning value
+uint32_t slab_bytes_of(int slab_kb) { return slab_kb == 19 ? 19456u : (uint32_t)slab_kb * 1024u; }
 
 // volumes above 4 GiB: per-wave 16 KiB slabs only; OPAC: the opacity mode
 template <typename T, bool P2, bool OPAC>
@@ -510,19 +563,37 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
         hipError_t e = visit_volume(vs, [&](auto v) {
             using V = decltype(v);
             if constexpr (is_padded<V>::value) {
-                if (staged_opac) {
+                if (staged_opac || staged) {
                     const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
-                    launch_staged_opac<typename V::value_type, V::kPow2>(
-                        chunk_for(c, p.nouter, nmax), grid, s, v, p, out, alpha);
-                    return hipGetLastError();
-                }
-                if (staged) {
-                    const int nmax = vs.nx > vs.ny ? (vs.nx > vs.nz ? vs.nx : vs.nz)
-                                                   : (vs.ny > vs.nz ? vs.ny : vs.nz);
-                    launch_staged<typename V::value_type, V::kPow2>(
-                        c->slab_kb, c->staging == 1, chunk_for(c, p.nouter, nmax), grid, s, v, p,
-                        out, alpha);
-                    return hipGetLastError();
+                    const int ch = chunk_for(c, p.nouter, nmax);
+                    // one wave per workgroup: per-wave slabs of 16 or 19 KiB
+                    const bool one_wave = c->wg_waves == 1 &&
+                                          (staged_opac || (c->staging == 0 && c->slab_kb >= 16));
+                    // (one wave: tile rows interleaved over the XCDs unless cost-ordered)
+                    dim3 g = one_wave ? dim3(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty)) : grid;
+                    void* obuf = nullptr;
+                    hipError_t e = hipSuccess;
+                    if (staged_opac || c->staging == 0)  // per-wave slabs
+                        e = order_tiles(c->tile_map, one_wave, v,
+                                        staged_opac ? 19456u : slab_bytes_of(c->slab_kb), p, s, &obuf,
+                                        &g);
+                    if (e == hipSuccess) {
+                        if (one_wave && staged_opac)
+                            launch_wave<typename V::value_type, V::kPow2, true>(19, ch, g, s, v, p,
+                                                                                out, alpha);
+                        else if (one_wave)
+                            launch_wave<typename V::value_type, V::kPow2, false>(c->slab_kb, ch, g, s,
+                                                                                 v, p, out, alpha);
+                        else if (staged_opac)
+                            launch_staged_opac<typename V::value_type, V::kPow2>(ch, g, s, v, p, out,
+                                                                                 alpha);
+                        else
+                            launch_staged<typename V::value_type, V::kPow2>(
+                                c->slab_kb, c->staging == 1, ch, g, s, v, p, out, alpha);
+                        e = hipGetLastError();
+                    }
+                    if (obuf) (void)hipFreeAsync(obuf, s);
+                    return e;
                 }
             }
             if (nearest)
@@ -1007,7 +1078,8 @@ int spr_set_tuning(spr_ctx* c, int key, int value) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
     switch (key) {
         case SPR_TUNE_TILE_MAP:
-            if (value < 0 || value > 2) return fail(c, SPR_ERR_INVALID_ARG, "tile map %d", value);
+            if (value < 0 || value > (int)kTileMapCostXcd)
+                return fail(c, SPR_ERR_INVALID_ARG, "tile map %d (0 .. 3)", value);
             c->tile_map = value;
             return SPR_OK;
         case SPR_TUNE_SLAB_KB:
@@ -1028,6 +1100,10 @@ int spr_set_tuning(spr_ctx* c, int key, int value) {
             if (value != 0 && value != 1) return fail(c, SPR_ERR_INVALID_ARG, "row pad %d (0 or 1)", value);
             c->row_pad = value;
             return SPR_OK;
+        case SPR_TUNE_WG_WAVES:
+            if (value != 1 && value != 4) return fail(c, SPR_ERR_INVALID_ARG, "waves per workgroup %d (1 or 4)", value);
+            c->wg_waves = value;
+            return SPR_OK;
         default: return fail(c, SPR_ERR_INVALID_ARG, "unknown tuning key %d", key);
     }
 }
@@ -1040,6 +1116,7 @@ int spr_get_tuning(spr_ctx* c, int key, int* value) {
         case SPR_TUNE_STAGING: *value = c->staging; return SPR_OK;
         case SPR_TUNE_CHUNK: *value = c->chunk; return SPR_OK;
         case SPR_TUNE_ROW_PAD: *value = c->row_pad; return SPR_OK;
+        case SPR_TUNE_WG_WAVES: *value = c->wg_waves; return SPR_OK;
         default: return fail(c, SPR_ERR_INVALID_ARG, "unknown tuning key %d", key);
     }
 }

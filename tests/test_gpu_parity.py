@@ -152,15 +152,21 @@ def test_staged_mip_matches_global(VR, dtype, shape, pose):
          np.dot(mat4_translate(0, 0, -1.2), mat4_rotation(.8, .2, 1, .1)),  # eye close to the box
          mat4_identity()][pose]                                          # eye inside the box
     frames = {}
-    for slab, staging, pad, chunk in ((0, 0, 0, 0), (8, 0, 0, 0), (12, 0, 0, 0), (16, 0, 0, 0),
-                                      (19, 0, 0, 0), (8, 1, 0, 0), (12, 1, 0, 0), (16, 0, 1, 0),
-                                      (12, 1, 1, 0), (16, 0, 0, 32), (16, 0, 0, 64), (12, 1, 0, 32),
-                                      (19, 1, 0, 0)):
+    # (slab KiB, workgroup staging, row pad, chunk, waves per workgroup, tile map)
+    for key in ((0, 0, 0, 0, 4, 0), (8, 0, 0, 0, 4, 0), (12, 0, 0, 0, 4, 0), (16, 0, 0, 0, 4, 0),
+                (19, 0, 0, 0, 4, 0), (8, 1, 0, 0, 4, 0), (12, 1, 0, 0, 4, 0), (16, 0, 1, 0, 4, 0),
+                (12, 1, 1, 0, 4, 0), (16, 0, 0, 32, 4, 0), (16, 0, 0, 64, 4, 0),
+                (12, 1, 0, 32, 4, 0), (19, 1, 0, 0, 4, 0), (19, 0, 0, 0, 1, 0), (16, 0, 0, 0, 1, 0),
+                (19, 0, 0, 32, 1, 0), (19, 0, 0, 0, 4, 3), (19, 0, 0, 0, 1, 3), (16, 0, 0, 64, 1, 3),
+                (12, 0, 0, 0, 4, 3), (12, 1, 0, 0, 1, 3)):
+        slab, staging, pad, chunk, waves, tile_map = key
         rend = VR((130, 97))
         rend._ctx.set_tuning("slab_kb", slab)
         rend._ctx.set_tuning("staging", staging)
         rend._ctx.set_tuning("row_pad", pad)
         rend._ctx.set_tuning("chunk", chunk)
+        rend._ctx.set_tuning("wg_waves", waves)
+        rend._ctx.set_tuning("tile_map", tile_map)
         rend.set_max_steps(160)
         rend.set_data(vol)
         rend.set_modelView(M)
@@ -169,11 +175,12 @@ def test_staged_mip_matches_global(VR, dtype, shape, pose):
         for part in range(3):
             rend.render(numParts=3, currentPart=part)
         res.append(rend.output.copy())
-        frames[slab, staging, pad, chunk] = res
+        frames[key] = res
+    ref = frames[0, 0, 0, 0, 4, 0]
     for key in frames:
-        for a, b in zip(frames[0, 0, 0, 0], frames[key]):
-            np.testing.assert_array_equal(a, b)
-    assert (frames[0, 0, 0, 0][0] > 0).sum() > 200
+        for a, b in zip(ref, frames[key]):
+            np.testing.assert_array_equal(a, b, err_msg=str(key))
+    assert (ref[0] > 0).sum() > 200
 
 
 @pytest.mark.parametrize("dtype", [np.uint16, np.uint8, np.float32])
@@ -215,9 +222,14 @@ def test_staged_opacity_matches_global(VR, dtype, max_steps, pose):
          np.dot(mat4_translate(.3, -.2, -2.6), mat4_rotation(2.1, 1, .4, -.3)),
          np.dot(mat4_translate(0, 0, -1.2), mat4_rotation(.8, .2, 1, .1))][pose]
     frames = {}
-    for slab, ap in ((0, .6), (19, .6), (16, .6), (0, 3.), (19, 3.)):
+    # (slab KiB, alpha_pow, waves per workgroup, tile map)
+    for slab, ap, waves, tile_map in ((0, .6, 4, 0), (19, .6, 4, 0), (16, .6, 4, 0), (0, 3., 4, 0),
+                                      (19, 3., 4, 0), (19, .6, 1, 0), (19, 3., 1, 3),
+                                      (19, .6, 4, 3)):
         rend = VR((130, 97))
         rend._ctx.set_tuning("slab_kb", slab)
+        rend._ctx.set_tuning("wg_waves", waves)
+        rend._ctx.set_tuning("tile_map", tile_map)
         rend.set_max_steps(max_steps)
         rend.set_data(vol)
         rend.set_modelView(M)
@@ -227,33 +239,42 @@ def test_staged_opacity_matches_global(VR, dtype, max_steps, pose):
         for part in range(2):
             rend.render(numParts=2, currentPart=part)
         res.append(rend.output.copy())
-        frames[slab, ap] = res
+        frames[slab, ap, waves, tile_map] = res
     for key in frames:
-        for a, b in zip(frames[0, key[1]], frames[key]):
-            np.testing.assert_array_equal(a, b)
-    assert (frames[0, .6][0] > 0).sum() > 200
+        for a, b in zip(frames[0, key[1], 4, 0], frames[key]):
+            np.testing.assert_array_equal(a, b, err_msg=str(key))
+    assert (frames[0, .6, 4, 0][0] > 0).sum() > 200
 
 
 def test_tuning_does_not_change_results(VR):
     vol = synth.blob_volume((40, 48, 56), n_blobs=10, sigma=(2., 5.), amp=(500., 3000.),
                             background=100., poisson=True, read_noise=5., dtype="uint16", seed=4)
-    frames = []
-    for tile_map in (0, 1, 2):
-        rend = VR((203, 150))
-        rend._ctx.set_tuning("tile_map", tile_map)
-        assert rend._ctx.get_tuning("tile_map") == tile_map
-        rend.set_data(vol)
-        rend.set_modelView(np.dot(mat4_translate(0, 0, -3.), mat4_rotation(.6, 0, 1, 0)))
-        rend.render(minVal=float(vol.min()), maxVal=float(vol.max()))
-        frames.append(rend.output.copy())
-        rend.render(maxVal=1500., method="iso_surface")
-        frames.append(rend.output.copy())
-    np.testing.assert_array_equal(frames[0], frames[2])
-    np.testing.assert_array_equal(frames[0], frames[4])
-    np.testing.assert_array_equal(frames[1], frames[3])
-    np.testing.assert_array_equal(frames[1], frames[5])
+    frames = {}
+    # 203 x 150: 13 x 10 tiles, tile rows not a multiple of the 8 XCDs
+    for tile_map in (0, 1, 2, 3):
+        for waves in (4, 1):
+            rend = VR((203, 150))
+            rend._ctx.set_tuning("tile_map", tile_map)
+            rend._ctx.set_tuning("wg_waves", waves)
+            assert rend._ctx.get_tuning("tile_map") == tile_map
+            assert rend._ctx.get_tuning("wg_waves") == waves
+            rend.set_data(vol)
+            rend.set_modelView(np.dot(mat4_translate(0, 0, -3.), mat4_rotation(.6, 0, 1, 0)))
+            res = []
+            for ap in (0., .7):
+                rend.set_alpha_pow(ap)
+                rend.render(minVal=float(vol.min()), maxVal=float(vol.max()))
+                res += [rend.output.copy(), rend.output_alpha.copy()]
+            rend.render(maxVal=1500., method="iso_surface")
+            res.append(rend.output.copy())
+            frames[tile_map, waves] = res
+    for key in frames:
+        for a, b in zip(frames[0, 4], frames[key]):
+            np.testing.assert_array_equal(a, b, err_msg=str(key))
     with pytest.raises(_lib.SprError):
         rend._ctx.set_tuning("tile_map", 9)
+    with pytest.raises(_lib.SprError):
+        rend._ctx.set_tuning("wg_waves", 2)
 
 
 def test_multipass_parts_merge(VR):
