@@ -53,6 +53,33 @@ __device__ __forceinline__ void wave_min3_u16x2(uint32_t& a, uint32_t& b, uint32
     c = __builtin_amdgcn_readlane(c, 63);
 }
 
+// The same minima over each half of the wave: (a, b, c) of lanes 0-31 and (a2, b2, c2) of
+// lanes 32-63 (the reduction without its last step), uniform results
+__device__ __forceinline__ void wave_min3_u16x2_halves(uint32_t& a, uint32_t& b, uint32_t& c,
+                                                       uint32_t& a2, uint32_t& b2, uint32_t& c2) {
+#define SPR_MIN3_STEP(CTRL, RM)                                                                \
+    {                                                                                          \
+        const uint32_t da = __builtin_amdgcn_update_dpp(a, a, CTRL, RM, 0xf, false);           \
+        const uint32_t db = __builtin_amdgcn_update_dpp(b, b, CTRL, RM, 0xf, false);           \
+        const uint32_t dc = __builtin_amdgcn_update_dpp(c, c, CTRL, RM, 0xf, false);           \
+        a = pk_min_u16(a, da);                                                                 \
+        b = pk_min_u16(b, db);                                                                 \
+        c = pk_min_u16(c, dc);                                                                 \
+    }
+    SPR_MIN3_STEP(0x111, 0xf)
+    SPR_MIN3_STEP(0x112, 0xf)
+    SPR_MIN3_STEP(0x114, 0xf)
+    SPR_MIN3_STEP(0x118, 0xf)
+    SPR_MIN3_STEP(0x142, 0xa)
+#undef SPR_MIN3_STEP
+    a2 = __builtin_amdgcn_readlane(a, 63);
+    b2 = __builtin_amdgcn_readlane(b, 63);
+    c2 = __builtin_amdgcn_readlane(c, 63);
+    a = __builtin_amdgcn_readlane(a, 31);
+    b = __builtin_amdgcn_readlane(b, 31);
+    c = __builtin_amdgcn_readlane(c, 31);
+}
+
 // The voxel pair (x, x+1) from a dword-aligned slab address a and the byte shift sh of the
 // pair within that dword, read as two aligned dwords (one ds_read2_b32) and shifted into
 // place with v_alignbyte: LDS accesses that are not dword aligned stall the LDS pipeline.
@@ -563,6 +590,24 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
         uint32_t lox, LY, LZ, rowb, SY, SZ, nu;
         uint64_t bytes;
     };
+    // a reduced box (packed as staged_box_n) in slab terms; true when it fits the slab
+    auto decode = [&](uint32_t r0, uint32_t r1, uint32_t r2, Box& bx) -> bool {
+        const uint32_t LX = r0 & 0xFFFFu;
+        bx.LY = r0 >> 16;
+        bx.LZ = r1 & 0xFFFFu;
+        const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
+                       HZ = 0xFFFFu - (r2 >> 16);
+        bx.lox = LX & ~(E - 1u);
+        const uint32_t SX = (HX + 1u - bx.lox + E - 1u) & ~(E - 1u);  // voxels, multiple of E
+        bx.SY = HY - bx.LY + 1u;
+        bx.SZ = HZ - bx.LZ + 1u;
+        bx.nu = SX / E;                                  // 16-byte units per row
+        // slab row stride: SX voxels, plus 16 bytes when row_pad asks for an odd number of
+        // 16-byte units (rows a few apart then fall on different LDS banks)
+        bx.rowb = (bx.nu + (p.row_pad & ~bx.nu & 1u)) * 16u;
+        bx.bytes = bx.nu <= NT ? (uint64_t)bx.rowb * bx.SY * bx.SZ : ~0ull;
+        return bx.bytes <= (uint64_t)SLAB_BYTES;
+    };
     auto box_of = [&](auto est, float first, float last, Box& bx) -> bool {
         uint32_t b0, b1, b2;
         staged_box_n<PV, decltype(est)::value>(vol, active, first, last, px, py, pz, dx, dy, dz,
@@ -584,21 +629,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
                 r2 = pk_min_u16(r2, s_box[3 * k + 2]);
             }
         }
-        const uint32_t LX = r0 & 0xFFFFu;
-        bx.LY = r0 >> 16;
-        bx.LZ = r1 & 0xFFFFu;
-        const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
-                       HZ = 0xFFFFu - (r2 >> 16);
-        bx.lox = LX & ~(E - 1u);
-        const uint32_t SX = (HX + 1u - bx.lox + E - 1u) & ~(E - 1u);  // voxels, multiple of E
-        bx.SY = HY - bx.LY + 1u;
-        bx.SZ = HZ - bx.LZ + 1u;
-        bx.nu = SX / E;                                  // 16-byte units per row
-        // slab row stride: SX voxels, plus 16 bytes when row_pad asks for an odd number of
-        // 16-byte units (rows a few apart then fall on different LDS banks)
-        bx.rowb = (bx.nu + (p.row_pad & ~bx.nu & 1u)) * 16u;
-        bx.bytes = bx.nu <= NT ? (uint64_t)bx.rowb * bx.SY * bx.SZ : ~0ull;
-        return bx.bytes <= (uint64_t)SLAB_BYTES;
+        return decode(r0, r1, r2, bx);
     };
     // the slab's contents visible to the wave (workgroup when SHARED)
     auto visible = [&]() {
@@ -626,6 +657,25 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
         }
+    };
+    // stages the boxes of the next CH samples of each half of the wave side by side in the
+    // slab (ha at its start, hb after it) when both fit; false, with nothing staged, if not
+    auto stage_halves = [&](Box& ha, Box& hb) -> bool {
+        uint32_t a0, a1, a2, b0, b1, b2;
+        staged_box_n<PV, false>(vol, active, 0.0f, (float)(CH - 1), px, py, pz, dx, dy, dz,
+                                mgx, mgy, mgz, a0, a1, a2);
+        wave_min3_u16x2_halves(a0, a1, a2, b0, b1, b2);
+        // a half without an active lane has no box (all ones): then the other half's box is
+        // the wave's, which did not fit
+        if (!decode(a0, a1, a2, ha) || !decode(b0, b1, b2, hb) ||
+            ha.bytes + hb.bytes > (uint64_t)SLAB_BYTES)
+            return false;
+        staged_copy_rows<NT>(vol.rsrc_plane(ha.LZ), slab, lane, ha.LY, 0u, ha.lox * TS, ha.nu,
+                             ha.rowb, ha.SY, ha.SZ, PY, pitch);
+        staged_copy_rows<NT>(vol.rsrc_plane(hb.LZ), slab + ha.bytes, lane, hb.LY, 0u,
+                             hb.lox * TS, hb.nu, hb.rowb, hb.SY, hb.SZ, PY, pitch);
+        visible();
+        return true;
     };
 
     static_assert(kLoopUnroll % CH == 0, "a chunk is a whole number of boxes");
@@ -691,6 +741,26 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
             }
             release();
             continue;
+        }
+        if constexpr (CH == 16 && !SHARED && !kPair) {
+            // a box too large for the slab because the wave's two halves (rows 0-3 and 4-7
+            // of its 8x8 rays) run apart, e.g. along the volume's silhouette where the rays'
+            // lengths change from row to row: one box per half, side by side in the slab.
+            // Lanes without a ray follow the wave's first active lane, in the first half.
+            Box ha, hb;
+            if (ng == 1 && stage_halves(ha, hb)) {
+                if (live) {
+                    const bool second = active && lane >= 32u;
+                    group(CH);
+                    sample_slab_any<PV, CH, OPAC>(
+                        vol, second ? slab + ha.bytes : slab,
+                        second ? hb.lox : ha.lox, second ? hb.LY : ha.LY, second ? hb.LZ : ha.LZ,
+                        second ? hb.rowb : ha.rowb, second ? hb.SY : ha.SY, px, py, pz, dx, dy, dz,
+                        col, col2, op);
+                }
+                release();
+                continue;
+            }
         }
         if constexpr (CH == 16 && !SHARED) {
             if (b.bytes <= (uint64_t)(4 * ng * SLAB_BYTES)) {
