@@ -90,9 +90,10 @@ extern "C" {
    SPR_TUNE_ROW_PAD   1 = pad staged rows to an odd number of 16-byte units (spreads the
                       LDS banks rows a few apart fall on), 0 = no padding (default)
    SPR_TUNE_WG_WAVES  waves per workgroup of the per-wave staged max projection (slab 16
-                      or 19) and opacity mode: 1 (an 8x8 quarter of a tile, default: a CU
-                      starts new work as soon as any of its waves ends) or 4 (a 16x16
-                      tile) */
+                      or 19, and volumes above 4 GiB) and opacity mode: 1 (an 8x8 quarter
+                      of a tile: a CU starts new work as soon as any of its waves ends),
+                      4 (a 16x16 tile), 0 = by frame size (default: 1 for frames of at
+                      most 24 tiles per CU, e.g. 1024^2 on 256 CUs, else 4) */
 #define SPR_TUNE_TILE_MAP 0
 #define SPR_TUNE_SLAB_KB 1
 #define SPR_TUNE_STAGING 2

@@ -38,6 +38,8 @@ def test_wide_padded_volume_matches_linear(torch):
     del v16
     P = mat4_perspective(45, 1., .1, 10)
     s = torch.cuda.Stream()
+    # the wide slot with one and with four waves per workgroup, then the linear layout
+    runs = ((wide, sw, 1), (wide, sw, 4), (lin, sl, 1))
     W, H = 192, 160
     for k, M in enumerate(synth.random_poses(3, seed=11)):
         iP, iM = render_matrices(M, P, shape[::-1])
@@ -45,35 +47,40 @@ def test_wide_padded_volume_matches_linear(torch):
             p = _lib.make_params(iP, iM, (-1, 1, -1, 1, -1, 1), 0., 3300., 1., 0., .1, 21, 30, steps,
                                  _lib.FILTER_LINEAR)
             outs = []
-            for ctx, slot in ((wide, sw), (lin, sl)):
+            for ctx, slot, waves in runs:
+                ctx.set_tuning("wg_waves", waves)
                 o = torch.empty((H, W), device="cuda")
                 a = torch.empty_like(o)
                 ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W, H, o.data_ptr(), a.data_ptr(),
                                 stream=s.cuda_stream)
                 outs.append((o, a))
             s.synchronize()
-            assert torch.equal(outs[0][0], outs[1][0]), (k, steps)
-            assert torch.equal(outs[0][1], outs[1][1])
+            for o, a in outs[:2]:
+                assert torch.equal(o, outs[2][0]), (k, steps)
+                assert torch.equal(a, outs[2][1])
             assert (outs[0][0] > 0).sum().item() > 5000
     # the opacity mode (staged on the wide copy) and the nearest filter (the slot's linear
     # volume) equal the linear layout's unstaged kernels
     for ap, filt in ((.5, _lib.FILTER_LINEAR), (3., _lib.FILTER_LINEAR), (0., _lib.FILTER_NEAREST)):
         p = _lib.make_params(iP, iM, (-1, 1, -1, 1, -1, 1), 0., 3300., 1., ap, .1, 21, 30, 96, filt)
         outs = []
-        for ctx, slot in ((wide, sw), (lin, sl)):
+        for ctx, slot, waves in runs:
+            ctx.set_tuning("wg_waves", waves)
             o = torch.empty((H, W), device="cuda")
             a = torch.empty_like(o)
             ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W, H, o.data_ptr(), a.data_ptr(),
                             stream=s.cuda_stream)
             outs.append((o, a))
         s.synchronize()
-        assert torch.equal(outs[0][0], outs[1][0]), (ap, filt)
-        assert torch.equal(outs[0][1], outs[1][1])
+        for o, a in outs[:2]:
+            assert torch.equal(o, outs[2][0]), (ap, filt)
+            assert torch.equal(a, outs[2][1])
     # the iso-surface march staged on the wide copy: all five outputs as the linear layout's
     p = _lib.make_params(iP, iM, (-1, 1, -1, 1, -1, 1), 0., 2 * 1500., 1., 0., .1, 21, 30, 160,
                          _lib.FILTER_LINEAR)
     res = []
-    for ctx, slot in ((wide, sw), (lin, sl)):
+    for ctx, slot, waves in runs:
+        ctx.set_tuning("wg_waves", waves)
         bufs = [torch.empty((H, W), device="cuda") for _ in range(3)]
         nrm = torch.empty((H, W, 3), device="cuda")
         occ = torch.empty((H, W), device="cuda")
@@ -82,8 +89,9 @@ def test_wide_padded_volume_matches_linear(torch):
                         stream=s.cuda_stream)
         res.append(bufs + [nrm, occ])
     s.synchronize()
-    for a, b in zip(*res):
-        assert torch.equal(a, b)
+    for r in res[:2]:
+        for a, b in zip(r, res[2]):
+            assert torch.equal(a, b)
     assert torch.isfinite(res[0][2]).sum().item() > 200
     # a wide slot is not batchable (the batch kernels address 32-bit offsets)
     o = torch.empty((H, W), device="cuda")
