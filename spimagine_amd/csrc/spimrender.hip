@@ -427,7 +427,8 @@ bool one_wave_for(const spr_ctx* c, const RayParams& p) {
 // a per-wave slab's bytes for the slab_kb tuning value
 uint32_t slab_bytes_of(int slab_kb) { return slab_kb == 19 ? 19456u : (uint32_t)slab_kb * 1024u; }
 
-// volumes above 4 GiB: per-wave 16 KiB slabs only; OPAC: the opacity mode
+// volumes above 4 GiB: per-wave 16 KiB slabs (19 KiB measured 2% slower on 2048^2
+// frames); OPAC: the opacity mode
 template <typename T, bool P2, bool OPAC>
 void launch_staged_wide_t(const VolSlot& vs, int ch, bool one_wave, dim3 grid, hipStream_t s,
                           const RayParams& p, float* out, float* alpha) {
@@ -535,11 +536,11 @@ void launch_iso_staged(int ch, dim3 grid, hipStream_t s, Args... args) {
         hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 4>), grid, dim3(256), 0, s, args...);
 }
 
-// both channels of a padded channel-pair slot (16 KiB per-wave slabs)
+// both channels of a padded channel-pair slot (19 KiB per-wave slabs)
 template <bool P2, int CH>
 void launch_pair(const VolSlot& vs, dim3 grid, hipStream_t s, const RayParams& p, float* out,
                  float* alpha) {
-    hipLaunchKernelGGL((k_max_project_staged<U16Pair, P2, 16384, CH, false>), grid, dim3(256), 0, s,
+    hipLaunchKernelGGL((k_max_project_staged<U16Pair, P2, 19456, CH, false>), grid, dim3(256), 0, s,
                        padded_volume<U16Pair, P2>(vs), p, out, alpha);
 }
 
