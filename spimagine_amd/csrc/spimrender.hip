@@ -392,13 +392,18 @@ hipError_t order_tiles(int tile_map, bool per_wave, const PaddedVolume<T, P2, WI
     return hipGetLastError();
 }
 
+// The slab of a one-wave workgroup for slab_kb 19: LDS is allocated in 512-byte granules,
+// so eight one-wave workgroups per CU fit 19,968 B each (the slab plus its 16 bytes of
+// slack), 2.5% more than 19 KiB (measured ~0.5-1% faster).
+constexpr uint32_t kWaveSlab19 = 19952;
+
 // one wave per workgroup: the plain max projection (slab 16 or 19 KiB) or the opacity mode
 template <typename T, bool P2, bool OPAC, int CH, typename... Args>
 void launch_wave_ch(int slab_kb, dim3 grid, hipStream_t s, Args... args) {
     if (slab_kb == 16 && !OPAC)
         hipLaunchKernelGGL((k_max_project_wave<T, P2, 16384, CH, false, OPAC>), grid, dim3(64), 0, s, args...);
     else
-        hipLaunchKernelGGL((k_max_project_wave<T, P2, 19456, CH, false, OPAC>), grid, dim3(64), 0, s, args...);
+        hipLaunchKernelGGL((k_max_project_wave<T, P2, kWaveSlab19, CH, false, OPAC>), grid, dim3(64), 0, s, args...);
 }
 
 template <typename T, bool P2, bool OPAC, typename... Args>
@@ -603,8 +608,9 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
                     hipError_t e = hipSuccess;
                     if (staged_opac || c->staging == 0)  // per-wave slabs
                         e = order_tiles(c->tile_map, one_wave, v,
-                                        staged_opac ? 19456u : slab_bytes_of(c->slab_kb), p, s, &obuf,
-                                        &g);
+                                        one_wave ? (staged_opac || c->slab_kb != 16 ? kWaveSlab19 : 16384u)
+                                                 : staged_opac ? 19456u : slab_bytes_of(c->slab_kb),
+                                        p, s, &obuf, &g);
                     if (e == hipSuccess) {
                         if (one_wave && staged_opac)
                             launch_wave<typename V::value_type, V::kPow2, true>(19, ch, g, s, v, p,
