@@ -81,8 +81,9 @@ extern "C" {
                       opacity mode; two small kernels order the tiles before the frame,
                       on its stream)
    SPR_TUNE_SLAB_KB   per-wave LDS slab (KiB) of the staged max projection: 8, 12, 16 or
-                      19 (default; per-wave staging only, 16 for workgroup staging); 0 =
-                      sample every chunk from global memory
+                      19 (default: 19,456 B, or 19,952 B with one wave per workgroup;
+                      per-wave staging only, 16 for workgroup staging); 0 = sample every
+                      chunk from global memory
    SPR_TUNE_STAGING   0 = each wave stages its own box (default), 1 = the four waves of a
                       workgroup stage one union box into a slab of 4 x SLAB_KB
    SPR_TUNE_CHUNK     samples per staged box: 0 = from the ray step (default), 4, 8, 16,
