@@ -95,7 +95,7 @@ def cpu_baseline(vol, invP, invM, lo, hi, S, W, H, seconds, gpu_out, gpu_alpha, 
     hits = int((a[r0:r1] > 0).sum())
     cpu_samples = hits * per_ray
     cpu = {"value": round(cpu_samples / tcpu / 1e9, 4), "unit": "Gsamples/s", "cores": cores,
-           "kind": "port",
+           "kind": "port", "cpu": cpu_model(),
            "sample": "rows %d-%d of %s (%d hit rays, %d samples), %.1f s"
                      % (r0, r1 - 1, what, hits, cpu_samples, tcpu)}
     parity = float(np.abs(gpu_out[r0:r1] - o[r0:r1]).max())
@@ -201,6 +201,10 @@ def bench_headline(args):
     pose_gsps = float(np.mean([samples_pose[i % 8] / (lat[i] * 1e6) for i in range(len(lat_ev))]
                               or [float("nan")]))
     set_pose(invP, invM)
+    # empty-space skipping off and on (SPR_TUNE_SKIP), the headline pose and the 8 poses
+    skipping = skip_table(ctx, stream, frame, lambda k: set_pose(*([(invP, invM)] + mats)[k]),
+                          [n_samp] + samples_pose, args.skip_frames)
+    set_pose(invP, invM)
 
     if args.calibrate_copy_mib:
         n = args.calibrate_copy_mib << 19  # int16 elements
@@ -229,8 +233,40 @@ def bench_headline(args):
         frames_per_s=round(1e3 / ms_per_step, 2),
         latency_ms={"p50": round(pct(lat, 50), 4), "p99": round(pct(lat, 99), 4),
                     "frames": len(lat_ev), "poses": 8, "gsamples_per_s": round(pose_gsps, 3)},
+        skipping=skipping,
         roofline=roofline(b_alg, kernel_ms, traffic_from_profile("headline")),
         cpu_baseline=cpu, parity_max_abs_diff_vs_oracle=parity, upload=upload)
+
+
+def skip_table(ctx, stream, frame, set_pose, samples, reps):
+    """Per-pose frame times (HIP events on the frame stream) with SPR_TUNE_SKIP 0 and 1:
+    pose 0 is the headline pose, the rest the seeded random poses.  Restores the context's
+    skip setting.  Samples are the reference count either way: skipping raises the
+    effective rate, not the work."""
+    import torch
+    keep = ctx.get_tuning("skip")
+    out = {}
+    for skip in (0, 1):
+        ctx.set_tuning("skip", skip)
+        ms = []
+        for k in range(len(samples)):
+            set_pose(k)
+            frame()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                frame()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1) / reps)
+        out["on" if skip else "off"] = {
+            "headline_ms": round(ms[0], 4),
+            "headline_gsamples_per_s": round(samples[0] / (ms[0] * 1e6), 2),
+            "poses_mean_ms": round(float(np.mean(ms[1:])), 4),
+            "poses_gsamples_per_s": round(float(np.mean([n / (t * 1e6) for n, t in
+                                                         zip(samples[1:], ms[1:])])), 2)}
+    ctx.set_tuning("skip", keep)
+    return out
 
 
 def measure_upload(ctx, vol):
@@ -248,6 +284,216 @@ def measure_upload(ctx, vol):
     return {"bytes": int(vol.nbytes), "host_to_padded_ms": round(1e3 * (t1 - t0), 2),
             "host_to_padded_GBps": round(vol.nbytes / (t1 - t0) / 1e9, 2),
             "minmax_ms": round(1e3 * (t3 - t2), 3), "minmax": [lo, hi]}
+
+
+# ------------------------------------------------------------------------------ iso
+def iso_reference_samples(depth, tnear, invP, invM, W, H, S, rows, n_box_hits):
+    """Samples the reference's iso march takes for the frame rows [r0, r1) (iso_kernel.cl:
+    83-196): a ray that crosses the iso value at step i takes i + 1 march samples, j <= 10
+    refinement samples and 24 gradient samples; a ray inside the box that never crosses
+    takes S.  i and j come from the outputs, t_hit = tnear + i dt + j dt/10 with dt from
+    the float64 box intersection (exact to within one sample per ray); n_box_hits: the rays
+    of those rows that enter the box."""
+    r0, r1 = rows
+    iP = np.asarray(invP, np.float64).reshape(4, 4)
+    iM = np.asarray(invM, np.float64).reshape(4, 4)
+    ys, xs = np.mgrid[r0:r1, 0:W]
+    hit = np.isfinite(depth[r0:r1])
+    u, v = xs[hit] / W * 2 - 1, ys[hit] / H * 2 - 1
+    one = np.ones_like(u)
+    front = np.stack([u, v, -one, one], -1) @ iP.T
+    front /= front[:, 3:]
+    back = np.stack([u, v, one, one], -1) @ iP.T
+    back /= back[:, 3:]
+    orig = front @ iM.T
+    orig /= orig[:, 3:]
+    d = back - front
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    direc = (d @ iM.T)[:, :3]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t0 = (-1 - orig[:, :3]) / direc
+        t1 = (1 - orig[:, :3]) / direc
+    tfar = np.nanmin(np.maximum(t0, t1), axis=1)
+    tn = tnear[r0:r1][hit].astype(np.float64)
+    dt = (tfar - tn) / (S - 1)
+    x = (depth[r0:r1][hit].astype(np.float64) - tn) / dt
+    i = np.clip(np.ceil(x - 1e-6) - 1, 1, S - 1)
+    j = np.clip(np.rint((x - i) * 10), 1, 10)
+    n_hit = int(hit.sum())
+    return float(np.sum(i + 1 + j + 24) + (n_box_hits - n_hit) * S)
+
+
+def bench_iso(args):
+    """Iso surface (full pipeline) and opacity-mode frames of the 512^3 uint16 volume into
+    1024^2 over 8 seeded poses, max_steps 896, skipping off and on."""
+    import torch
+    from spimagine_amd import _lib, synth
+    from spimagine_amd.utils import mat4_perspective
+    from spimagine_amd.volumerender.volumerender import render_matrices
+
+    cfg = synth.CONFIGS["iso"]
+    nz, ny, nx = cfg["shape"]
+    W, H = cfg["out"]
+    S = cfg["max_steps"]
+    log("bench: generating 512^3 uint16 blob volume on cuda:0")
+    vt = synth.config_volume("iso", device="cuda")
+    ctx = _lib.Context(0)
+    ctx.set_volume_device(vt.data_ptr(), np.uint16, cfg["shape"])
+    lo, hi = ctx.volume_minmax()
+    iso_max = 2 * (lo + .3 * (hi - lo))
+    P = mat4_perspective(45, 1., .1, 10)
+    mats = [render_matrices(M, P, (nx, ny, nz)) for M in synth.random_poses(8, seed=0)]
+    bufs = {k: torch.empty((H, W), device="cuda") for k in ("out", "alpha", "depth", "occ")}
+    normals = torch.empty((H, W, 3), device="cuda")
+    stream = torch.cuda.Stream()
+    sptr = stream.cuda_stream
+
+    def set_pose(k, mode):
+        iP, iM = mats[k]
+        if mode == "iso":
+            ctx.set_params(iP, iM, BOX, lo, iso_max, 1., 0., .1, 21, 30, S, _lib.FILTER_LINEAR)
+        else:
+            ctx.set_params(iP, iM, BOX, lo, hi, 1., .6, .1, 21, 30, S, _lib.FILTER_LINEAR)
+
+    def frame(mode):
+        if mode == "iso":
+            ctx.render_device(_lib.METHOD_ISO_SURFACE, 1, 0, W, H, bufs["out"].data_ptr(),
+                              bufs["alpha"].data_ptr(), bufs["depth"].data_ptr(),
+                              normals.data_ptr(), bufs["occ"].data_ptr(), stream=sptr)
+        else:
+            ctx.render_device(_lib.METHOD_MAX_PROJECT, 1, 0, W, H, bufs["out"].data_ptr(),
+                              bufs["alpha"].data_ptr(), stream=sptr)
+
+    def pose_ms(mode, k, reps):
+        set_pose(k, mode)
+        frame(mode)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            frame(mode)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    # reference iso samples per pose (from the outputs) and rays entering the box
+    iso_samples, box_hits = [], []
+    for k in range(8):
+        set_pose(k, "iso")
+        rays = ctx.count_samples(1, W, H) // ((S // 16 + 1) * 16)
+        frame("iso")
+        torch.cuda.synchronize()
+        d, a = bufs["depth"].cpu().numpy(), bufs["alpha"].cpu().numpy()
+        box_hits.append(rays)
+        iso_samples.append(iso_reference_samples(d, a, mats[k][0], mats[k][1], W, H, S, (0, H),
+                                                 rays))
+    table = {}
+    for skip in (0, 1):
+        ctx.set_tuning("skip", skip)
+        for mode in ("iso", "alpha"):
+            ms = [pose_ms(mode, k, args.skip_frames) for k in range(8)]
+            table.setdefault(mode, {})["on" if skip else "off"] = {
+                "poses_ms": [round(t, 4) for t in ms], "mean_ms": round(float(np.mean(ms)), 4)}
+    ctx.set_tuning("skip", 1)
+    log("bench: timing %d iso frames (8 poses, skipping on)" % args.steps)
+    for k in range(8):
+        set_pose(k, "iso")
+        frame("iso")
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for i in range(args.steps):
+        set_pose(i % 8, "iso")
+        frame("iso")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = e0.elapsed_time(e1) / args.steps
+    ms_per_step = 1e3 * wall / args.steps
+    samp = float(np.mean([iso_samples[i % 8] for i in range(args.steps)]))
+    b_alg = samp * 8 * 2 + W * H * 4 * (3 + 3)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_iso(synth.device_volume_to_numpy(vt, np.uint16), mats[0], iso_max / 2.,
+                               S, W, H, args.cpu_seconds)
+    print(json.dumps({
+        "metric": "iso-surface + opacity frames/s, 512^3 u16 -> 1024^2, 8 poses",
+        "value": round(1e3 / ms_per_step, 2), "unit": "frames/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 8, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "iso-surface (Phong + normal blur + occlusion + shading) of a "
+                               "512^3 uint16 volume (2000 seeded blobs, Poisson + read noise) "
+                               "-> 1024x1024, iso at 0.3 of the data range, max_steps 896, "
+                               "8 seeded poses; opacity mode at alpha_pow 0.6 timed beside it",
+                   "voxel": "u16", "frame": [W, H], "max_steps": S,
+                   "tuning": {k: ctx.get_tuning(k) for k in _lib.TUNING_KEYS}},
+        "frames_per_s": round(1e3 / ms_per_step, 2),
+        "iso_reference_samples_per_frame": [int(v) for v in iso_samples],
+        "gsamples_per_s": round(samp / (kernel_ms * 1e6), 2),
+        "skipping_ms": table,
+        "roofline": dict(roofline(b_alg, kernel_ms, None),
+                         kernel="iso pipeline per frame (march k_iso_staged + 6 post-pass "
+                                "kernels; the march dominates, see profiles/)"),
+        "cpu_baseline": cpu}), flush=True)
+
+
+def cpu_baseline_iso(vol, mat, iso_val, S, W, H, seconds):
+    """The oracle's iso march on a block of rows around the middle of the frame, in
+    reference samples per second (counted as iso_reference_samples does)."""
+    import oracle
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    invP, invM = mat
+    mid = H // 2
+    t = time.perf_counter()
+    oracle.iso_surface(vol, invP, invM, iso_val=np.float32(iso_val), max_steps=S, width=W,
+                       height=H, rows=(mid, mid + cores), nthreads=cores)
+    per_row = (time.perf_counter() - t) / cores
+    n = int(max(cores, min(H, seconds / max(per_row, 1e-6) // cores * cores)))
+    r0 = max(0, mid - n // 2)
+    r1 = min(H, r0 + n)
+    t = time.perf_counter()
+    _, a, d, _ = oracle.iso_surface(vol, invP, invM, iso_val=np.float32(iso_val), max_steps=S,
+                                    width=W, height=H, rows=(r0, r1), nthreads=cores)
+    tcpu = time.perf_counter() - t
+    # rays of those rows that enter the box: iso misses and box misses both read alpha 0, so
+    # count the box hits with the max projection's sample counter geometry on the host
+    iP = np.asarray(invP, np.float64).reshape(4, 4)
+    iM = np.asarray(invM, np.float64).reshape(4, 4)
+    ys, xs = np.mgrid[r0:r1, 0:W]
+    u, v = xs / W * 2 - 1, ys / H * 2 - 1
+    one = np.ones_like(u)
+    front = np.stack([u, v, -one, one], -1) @ iP.T
+    front /= front[..., 3:]
+    back = np.stack([u, v, one, one], -1) @ iP.T
+    back /= back[..., 3:]
+    orig = (front @ iM.T)
+    orig = orig[..., :3] / orig[..., 3:]
+    dd = back - front
+    dd /= np.linalg.norm(dd, axis=-1, keepdims=True)
+    direc = (dd @ iM.T)[..., :3]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t0 = (-1 - orig) / direc
+        t1 = (1 - orig) / direc
+    tmin = np.nanmax(np.minimum(t0, t1), axis=-1)
+    tmax = np.nanmin(np.maximum(t0, t1), axis=-1)
+    rays = int((tmax > tmin).sum())
+    samples = iso_reference_samples(d, a, invP, invM, W, H, S, (r0, r1), rays)
+    return {"value": round(samples / tcpu / 1e9, 4), "unit": "Gsamples/s", "cores": cores,
+            "kind": "port", "cpu": cpu_model(),
+            "sample": "iso march (oracle) of rows %d-%d of pose 0 (%d rays in the box, ~%d "
+                      "reference samples), %.1f s" % (r0, r1 - 1, rays, samples, tcpu)}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 # ------------------------------------------------------------------------------ batch
@@ -367,34 +613,6 @@ def bench_batch(args):
 
 
 # ------------------------------------------------------------------------------ serve
-SERVE_VOLUMES = [  # name, numpy shape (z, y, x), dtype, blobs
-    ("f32_256", (256, 256, 256), np.float32, 500),
-    ("u8_256x512x512", (256, 512, 512), np.uint8, 1000),
-    ("u16_512", (512, 512, 512), np.uint16, 2000),
-    ("u16_512x1024x1024", (512, 1024, 1024), np.uint16, 8000),
-    ("u16_512x2048x2048", (512, 2048, 2048), np.uint16, 16000),
-]
-
-
-def serve_volume(shape, dtype, n_blobs, seed):
-    import torch
-    from spimagine_amd import synth
-    dt = np.dtype(dtype)
-    amp = (.3, 1.) if dt == np.float32 else (60., 200.) if dt == np.uint8 else (500., 3000.)
-    bg = amp[0] / 10 if dt != np.uint16 else 100.
-    _, c, s, a = synth.blob_params(shape, n_blobs, (3., 10.), amp, seed)
-    vol = torch.full(shape, bg, dtype=torch.float32, device="cuda")
-    synth._splat_blobs_device(vol, c, s, a)
-    if dt == np.float32:
-        return vol.cpu().numpy()
-    top = 255 if dt == np.uint8 else 65535
-    v = torch.clamp(torch.round(vol), 0, top).to(torch.int32)
-    del vol
-    if dt == np.uint8:
-        return v.to(torch.uint8).cpu().numpy()
-    return torch.where(v >= 32768, v - 65536, v).to(torch.int16).cpu().numpy().view(np.uint16)
-
-
 def bench_serve(args):
     import torch
     from spimagine_amd import _lib, synth
@@ -405,8 +623,8 @@ def bench_serve(args):
     server = RenderServer(0, hbm_budget=budget, max_batch=args.max_batch, workers=args.workers)
     info = {}
     t = time.time()
-    for k, (name, shape, dt, nb) in enumerate(SERVE_VOLUMES):
-        vol = serve_volume(shape, dt, nb, k)
+    for k, (name, shape, dt, nb) in enumerate(synth.SERVING_VOLUMES):
+        vol = synth.serving_volume(shape, dt, nb, k)
         if dt == np.float32:
             rng = (0., float(vol.max()))
         else:
@@ -447,6 +665,24 @@ def bench_serve(args):
     batch = [r.batch_size for r in res]
     px = sum(r.size[0] * r.size[1] for r in reqs)
     server.close()
+    # algorithmic bytes of the max-projection requests (reference samples of the rays that
+    # enter the box, from a float64 box test, x 8 corners x voxel bytes, plus the two output
+    # planes), over the whole run's wall time: idle gaps between arrivals included
+    from spimagine_amd.volumerender.volumerender import render_matrices
+    b_alg, mip = 0., []
+    for r in reqs:
+        if r.method != "max_project":
+            continue
+        d = info[r.volume]
+        nz, ny, nx = d["shape"]
+        iP, iM = render_matrices(r.model_view, r.projection, (nx, ny, nz))
+        rays = box_hit_rays(iP, iM, r.size[0], r.size[1])
+        per_ray = (r.settings["max_steps"] // 16 + 1) * 16
+        b_alg += rays * per_ray * 8 * np.dtype(d["dtype"]).itemsize + r.size[0] * r.size[1] * 8
+        mip.append((r, iP, iM))
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_requests(server.volumes, mip[:4], args.cpu_seconds)
     print(json.dumps({
         "metric": "serving: p50/p99 frame latency and frames/s, Poisson arrivals over mixed "
                   "resident volumes", "value": round(len(res) / wall, 2), "unit": "frames/s",
@@ -463,7 +699,76 @@ def bench_serve(args):
         "latency_ms": {"p50": round(pct(lat, 50), 3), "p99": round(pct(lat, 99), 3),
                        "mean": round(float(np.mean(lat)), 3)},
         "mean_batch": round(float(np.mean(batch)), 2), "gpixels_per_s": round(px / wall / 1e9, 4),
-        "batches": server.batches}), flush=True)
+        "batches": server.batches,
+        "roofline": dict(roofline(b_alg, 1e3 * wall, None),
+                         kernel="all launches of the run: max-projection requests' algorithmic "
+                                "bytes over the run's wall time (arrival gaps included)"),
+        "cpu_baseline": cpu}), flush=True)
+
+
+def box_hit_rays(invP, invM, W, H, box=BOX):
+    """Rays of a W x H frame that enter the box (ray set-up of volume_kernel.cl:48-93 in
+    float64)."""
+    iP = np.asarray(invP, np.float64).reshape(4, 4)
+    iM = np.asarray(invM, np.float64).reshape(4, 4)
+    ys, xs = np.mgrid[0:H, 0:W]
+    u, v = xs / W * 2 - 1, ys / H * 2 - 1
+    one = np.ones_like(u)
+    front = np.stack([u, v, -one, one], -1) @ iP.T
+    front /= front[..., 3:]
+    back = np.stack([u, v, one, one], -1) @ iP.T
+    back /= back[..., 3:]
+    orig = front @ iM.T
+    orig = orig[..., :3] / orig[..., 3:]
+    d = back - front
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    direc = (d @ iM.T)[..., :3]
+    lo = np.array(box[0::2], np.float64)
+    hi = np.array(box[1::2], np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t0 = (lo - orig) / direc
+        t1 = (hi - orig) / direc
+    tmin = np.nanmax(np.minimum(t0, t1), axis=-1)
+    tmax = np.nanmin(np.maximum(t0, t1), axis=-1)
+    return int((tmax > tmin).sum())
+
+
+def cpu_baseline_requests(volumes, mip, seconds):
+    """The oracle on a bounded block of rows of each of a few max-projection requests of the
+    run; Gsamples/s over all of them."""
+    import oracle
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    per = seconds / max(1, len(mip))
+    total_s = total_t = 0.
+    parts = []
+    for r, iP, iM in mip:
+        vol = volumes[r.volume].data
+        W, H = r.size
+        st = r.settings
+        per_ray = (st["max_steps"] // 16 + 1) * 16
+        mid = H // 2
+        t = time.perf_counter()
+        oracle.max_project(vol, iP, iM, box=st["box"], min_val=st["min_val"],
+                           max_val=st["max_val"], gamma=st["gamma"], alpha_pow=st["alpha_pow"],
+                           max_steps=st["max_steps"], width=W, height=H, rows=(mid, mid + cores),
+                           nthreads=cores)
+        per_row = (time.perf_counter() - t) / cores
+        n = int(max(cores, min(H, per / max(per_row, 1e-6) // cores * cores)))
+        r0 = max(0, mid - n // 2)
+        r1 = min(H, r0 + n)
+        t = time.perf_counter()
+        _, a = oracle.max_project(vol, iP, iM, box=st["box"], min_val=st["min_val"],
+                                  max_val=st["max_val"], gamma=st["gamma"],
+                                  alpha_pow=st["alpha_pow"], max_steps=st["max_steps"], width=W,
+                                  height=H, rows=(r0, r1), nthreads=cores)
+        dt = time.perf_counter() - t
+        hits = int((a[r0:r1] > 0).sum())
+        total_s += hits * per_ray
+        total_t += dt
+        parts.append("%s %dx%d rows %d-%d" % (r.volume, W, H, r0, r1 - 1))
+    return {"value": round(total_s / max(total_t, 1e-9) / 1e9, 4), "unit": "Gsamples/s",
+            "cores": cores, "kind": "port", "cpu": cpu_model(),
+            "sample": "oracle max projection of %s (%.1f s)" % ("; ".join(parts), total_t)}
 
 
 # ------------------------------------------------------------------------------ learned
@@ -574,12 +879,14 @@ def bench_learned(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("headline", "batch", "serve", "learned"),
+    ap.add_argument("--workload", choices=("headline", "iso", "batch", "serve", "learned"),
                     default="headline")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--latency-frames", type=int, default=104)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--skip-frames", type=int, default=5,
+                    help="frames per pose timed with skipping off and on")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="launch tuning passed to spr_set_tuning, e.g. tile_map=1 "
@@ -604,13 +911,13 @@ def main():
               "(multi-GPU sharding is a design note, not a measured path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
-    defaults = {"headline": (50, 5), "batch": (3, 1), "serve": (None, 10),
+    defaults = {"headline": (50, 5), "iso": (40, 8), "batch": (3, 1), "serve": (None, 10),
                 "learned": (48, 8)}[args.workload]
     if args.steps is None:
         args.steps = defaults[0]
     if args.warmup is None:
         args.warmup = defaults[1]
-    {"headline": bench_headline, "batch": bench_batch, "serve": bench_serve,
+    {"headline": bench_headline, "iso": bench_iso, "batch": bench_batch, "serve": bench_serve,
      "learned": bench_learned}[args.workload](args)
 
 

@@ -94,13 +94,26 @@ extern "C" {
                       or 19, and volumes above 4 GiB) and opacity mode: 1 (an 8x8 quarter
                       of a tile: a CU starts new work as soon as any of its waves ends),
                       4 (a 16x16 tile), 0 = by frame size (default: 1 for frames of at
-                      most 24 tiles per CU, e.g. 1024^2 on 256 CUs, else 4) */
+                      most 24 tiles per CU, e.g. 1024^2 on 256 CUs, else 4)
+   SPR_TUNE_SKIP      1 = empty-space skipping and early ray termination by the volume's
+                      brick bounds (default), 0 = off.  Every padded one-channel volume
+                      keeps the minimum and maximum of each 8x8x8 brick (built on upload,
+                      8 bytes a brick).  With per-wave 19 KiB slabs (the default; 16 KiB
+                      for volumes above 4 GiB) the max projection skips a 16-sample box
+                      whose bricks' maximum cannot exceed any of the wave's running maxima,
+                      the opacity mode stops a wave once no later sample can raise a ray's
+                      colour (cumsum x the volume's normalised maximum <= colour, with a
+                      transparency factor provably in [0, 1]), and the iso march skips a
+                      box whose bricks lie wholly on each marching ray's starting side of
+                      the iso value.  Only skipped samples' positions are still advanced,
+                      by the same additions: results are bit-identical either way. */
 #define SPR_TUNE_TILE_MAP 0
 #define SPR_TUNE_SLAB_KB 1
 #define SPR_TUNE_STAGING 2
 #define SPR_TUNE_CHUNK 3
 #define SPR_TUNE_ROW_PAD 4
 #define SPR_TUNE_WG_WAVES 5
+#define SPR_TUNE_SKIP 6
 
 typedef struct spr_ctx spr_ctx;
 
@@ -234,6 +247,29 @@ typedef struct spr_job {
 int spr_render_batch(spr_ctx *ctx, const spr_params *params, const spr_job *jobs, int n, int w,
                      int h, int num_parts, int current_part, void *stream);
 
+/* One frame of a mixed batch (spr_render_frames): any padded slot, its own frame size,
+   box, transfer range, gamma and max_steps, and two w x h float32 device planes. */
+typedef struct spr_frame {
+    int slot;
+    int w, h;
+    int max_steps;
+    float inv_p[16];
+    float inv_m[16];
+    float box[6];
+    float min_val, max_val, gamma;
+    float *d_out;
+    float *d_alpha;
+} spr_frame;
+
+/* Max projections (alpha_pow 0) or opacity-mode projections (alpha_pow > 0) of n frames of
+   any sizes and settings in one persistent launch: the continuous batching of a render
+   server, which takes whatever requests are waiting.  The frames share the voxel type,
+   alpha_pow and the filter; every slot uses the padded layout.  Each frame equals
+   spr_render_slot with its own parameters (num_parts 1).  Blocks until the batch
+   completes; calls from several host threads on different streams run concurrently. */
+int spr_render_frames(spr_ctx *ctx, const spr_frame *frames, int n, float alpha_pow, int filter,
+                      void *stream);
+
 /* One frame of any method from any slot into device buffers on `stream`, with explicit
    `params` (matrices included): the stateless form of spr_render_device for callers that
    keep many volumes resident and render with per-request settings from several threads. */
@@ -261,7 +297,12 @@ int spr_render_coarse_pair(spr_ctx *ctx, int slot_a, int slot_b, const spr_param
 
 /* The same batch (num_parts = 1, the spr_set_params state) captured once in a HIP graph: replay it any number of
    times on any stream, rewrite its jobs (same count; waits for the last replay first) and
-   destroy it.  Launches are asynchronous. */
+   destroy it.  Launches are asynchronous.  A graph records the slots its jobs read:
+   releasing or refilling one of them (spr_volume_release, a spr_volume_add that reuses the
+   slot, a spr_set_volume* that reallocates slot 0 or changes its shape, type or layout)
+   makes spr_batch_graph_launch fail with SPR_ERR_INVALID_ARG until spr_batch_graph_update
+   names live slots again.  An update must keep the voxel type and whether every volume's
+   dimensions are powers of two (the captured kernel's texel rule). */
 typedef struct spr_batch_graph spr_batch_graph;
 int spr_batch_graph_create(spr_ctx *ctx, const spr_job *jobs, int n, int w, int h,
                            spr_batch_graph **graph);

@@ -64,6 +64,9 @@ def lib():
         L.spimref_render_iso.argtypes = [vp, i, i, i, i, i, fp, fp, fp, f, f, i, f, i, i, i, i,
                                          fp, fp, fp, fp, fp, fp, fp, i]
         L.spimref_render_iso.restype = i
+        L.spimref_render_iso_rows.argtypes = [vp, i, i, i, i, i, fp, fp, fp, f, f, i, f, i, i, i,
+                                              i, i, i, fp, fp, fp, fp, fp, fp, fp, i]
+        L.spimref_render_iso_rows.restype = i
         for n in ("spimref_conv_x", "spimref_conv_y", "spimref_conv_vec_x", "spimref_conv_vec_y"):
             getattr(L, n).argtypes = [fp, fp, i, i, i]
             getattr(L, n).restype = None
@@ -128,17 +131,20 @@ def max_project(data, invP, invM, box=(-1, 1, -1, 1, -1, 1), min_val=0.0, max_va
 
 
 def iso_surface(data, invP, invM, box=(-1, 1, -1, 1, -1, 1), iso_val=0.0, gamma=1.0,
-                max_steps=200, width=64, height=64, interpolation="linear", nthreads=0):
-    """iso_kernel.cl:17-225 alone (no post passes)."""
+                max_steps=200, width=64, height=64, interpolation="linear", rows=None,
+                nthreads=0):
+    """iso_kernel.cl:17-225 alone (no post passes); rows=(y0, y1) marches only those rows
+    (the others stay 0)."""
     data, dt, nx, ny, nz = _vol(data)
     invP, invM, box = _f32(invP, 16), _f32(invM, 16), _f32(box, 6)
     out = np.zeros((height, width), np.float32)
     alpha = np.zeros_like(out)
     depth = np.zeros_like(out)
     normals = np.zeros((height, width, 3), np.float32)
+    y0, y1 = (0, height) if rows is None else rows
     rc = lib().spimref_iso_surface(data.ctypes.data, dt, nx, ny, nz, FILTERS[interpolation],
                                    _fp(invP), _fp(invM), _fp(box), iso_val, gamma, max_steps,
-                                   width, height, 0, height, _fp(out), _fp(alpha), _fp(depth),
+                                   width, height, y0, y1, _fp(out), _fp(alpha), _fp(depth),
                                    _fp(normals), nthreads)
     if rc:
         raise ValueError("invalid iso_surface arguments")
@@ -164,6 +170,33 @@ def render_iso(data, invP, invM, box=(-1, 1, -1, 1, -1, 1), iso_val=0.0, gamma=1
                                   occ_strength, occ_radius, occ_n_points, width, height, _fp(out),
                                   _fp(alpha), _fp(depth), _fp(normals), _fp(occ), _fp(tmpv),
                                   _fp(tmp), nthreads)
+    if rc:
+        raise ValueError("invalid iso arguments")
+    return out, alpha, depth, normals, occ
+
+
+def render_iso_rows(data, invP, invM, rows, box=(-1, 1, -1, 1, -1, 1), iso_val=0.0,
+                    gamma=1.0, max_steps=200, occ_strength=0.1, occ_radius=21,
+                    occ_n_points=30, width=64, height=64, interpolation="linear", nthreads=0):
+    """render_iso's five outputs on rows=(y0, y1) only: the iso march runs on the rows
+    those rows depend on (within max(3, occ_radius + 3)); the rest of each frame holds
+    misses.  Returns full-size (output, alpha, depth, normals, occlusion), valid on the
+    rows asked for."""
+    data, dt, nx, ny, nz = _vol(data)
+    invP, invM, box = _f32(invP, 16), _f32(invM, 16), _f32(box, 6)
+    out = np.zeros((height, width), np.float32)
+    alpha = np.zeros_like(out)
+    depth = np.full_like(out, np.inf)
+    occ = np.zeros_like(out)
+    tmp = np.zeros_like(out)
+    normals = np.zeros((height, width, 3), np.float32)
+    tmpv = np.zeros_like(normals)
+    rc = lib().spimref_render_iso_rows(data.ctypes.data, dt, nx, ny, nz, FILTERS[interpolation],
+                                       _fp(invP), _fp(invM), _fp(box), iso_val, gamma, max_steps,
+                                       occ_strength, occ_radius, occ_n_points, width, height,
+                                       int(rows[0]), int(rows[1]), _fp(out), _fp(alpha),
+                                       _fp(depth), _fp(normals), _fp(occ), _fp(tmpv), _fp(tmp),
+                                       nthreads)
     if rc:
         raise ValueError("invalid iso arguments")
     return out, alpha, depth, normals, occ

@@ -69,7 +69,7 @@ static inline f4 normalize4(f4 v) {
     return mk4(v.x / l, v.y / l, v.z / l, v.w / l);
 }
 
-/* utils.cl:92-99  res.i = dot(v, row i of M) */
+/* utils.cl:63-70  res.i = dot(v, row i of M) */
 static inline f4 mult(const float *M, f4 v) {
     f4 r;
     r.x = dot4(v, mk4(M[0], M[1], M[2], M[3]));
@@ -82,7 +82,7 @@ static inline f4 mult(const float *M, f4 v) {
 static inline float cl_min(float x, float y) { return (y < x) ? y : x; }
 static inline float cl_max(float x, float y) { return (x < y) ? y : x; }
 
-/* utils.cl:70-89 slab test; only x,y,z enter tnear/tfar */
+/* utils.cl:41-60 slab test; only x,y,z enter tnear/tfar */
 static inline int intersect_box(f4 ro, f4 rd, f4 bmin, f4 bmax, float *tnear, float *tfar) {
     float ix = 1.0f / rd.x, iy = 1.0f / rd.y, iz = 1.0f / rd.z;
     float tbx = ix * (bmin.x - ro.x), tby = iy * (bmin.y - ro.y), tbz = iz * (bmin.z - ro.z);
@@ -399,11 +399,11 @@ static inline void tap_range(int i, int N, int Nh, int *hs, int *he) {
 }
 
 /* convolve_2d.cl:6-61 (scalar, k=10) */
-void spimref_conv_x(const float *in, float *out, int W, int H, int Nh) {
+static void conv_x_rows(const float *in, float *out, int W, int H, int Nh, int j0, int j1) {
     float w[256];
     if (Nh > 256) return;
     spimref_conv_weights(Nh, 10.0f, w);
-    for (int j = 0; j < H; ++j)
+    for (int j = j0; j < j1; ++j)
         for (int i = 0; i < W; ++i) {
             int hs, he;
             tap_range(i, W, Nh, &hs, &he);
@@ -417,11 +417,11 @@ void spimref_conv_x(const float *in, float *out, int W, int H, int Nh) {
         }
 }
 
-void spimref_conv_y(const float *in, float *out, int W, int H, int Nh) {
+static void conv_y_rows(const float *in, float *out, int W, int H, int Nh, int j0, int j1) {
     float w[256];
     if (Nh > 256) return;
     spimref_conv_weights(Nh, 10.0f, w);
-    for (int j = 0; j < H; ++j)
+    for (int j = j0; j < j1; ++j)
         for (int i = 0; i < W; ++i) {
             int hs, he;
             tap_range(j, H, Nh, &hs, &he);
@@ -436,11 +436,11 @@ void spimref_conv_y(const float *in, float *out, int W, int H, int Nh) {
 }
 
 /* convolve_2d.cl:65-137 (3-vector, k=5) */
-void spimref_conv_vec_x(const float *in, float *out, int W, int H, int Nh) {
+static void conv_vec_x_rows(const float *in, float *out, int W, int H, int Nh, int j0, int j1) {
     float w[256];
     if (Nh > 256) return;
     spimref_conv_weights(Nh, 5.0f, w);
-    for (int j = 0; j < H; ++j)
+    for (int j = j0; j < j1; ++j)
         for (int i = 0; i < W; ++i) {
             int hs, he;
             tap_range(i, W, Nh, &hs, &he);
@@ -460,11 +460,11 @@ void spimref_conv_vec_x(const float *in, float *out, int W, int H, int Nh) {
         }
 }
 
-void spimref_conv_vec_y(const float *in, float *out, int W, int H, int Nh) {
+static void conv_vec_y_rows(const float *in, float *out, int W, int H, int Nh, int j0, int j1) {
     float w[256];
     if (Nh > 256) return;
     spimref_conv_weights(Nh, 5.0f, w);
-    for (int j = 0; j < H; ++j)
+    for (int j = j0; j < j1; ++j)
         for (int i = 0; i < W; ++i) {
             int hs, he;
             tap_range(j, H, Nh, &hs, &he);
@@ -484,14 +484,19 @@ void spimref_conv_vec_y(const float *in, float *out, int W, int H, int Nh) {
         }
 }
 
-/* utils.cl:39-52: uint32 LCG, 10 rounds, mod 79197919 */
+void spimref_conv_x(const float *in, float *out, int W, int H, int Nh) { conv_x_rows(in, out, W, H, Nh, 0, H); }
+void spimref_conv_y(const float *in, float *out, int W, int H, int Nh) { conv_y_rows(in, out, W, H, Nh, 0, H); }
+void spimref_conv_vec_x(const float *in, float *out, int W, int H, int Nh) { conv_vec_x_rows(in, out, W, H, Nh, 0, H); }
+void spimref_conv_vec_y(const float *in, float *out, int W, int H, int Nh) { conv_vec_y_rows(in, out, W, H, Nh, 0, H); }
+
+/* utils.cl:10-23: uint32 LCG, 10 rounds, mod 79197919 */
 float spimref_random(uint32_t x, uint32_t y) {
     uint32_t a = 4421u + (1u + x) * (1u + y) + x + y;
     for (int i = 0; i < 10; i++) a = (1664525u * a + 1013904223u) % 79197919u;
     return ((float)a * 1.0f) / (float)79197919;
 }
 
-/* utils.cl:54-67 (declared float, returns an int value) */
+/* utils.cl:25-38 (declared float, returns an int value) */
 float spimref_rand_int(uint32_t x, uint32_t y, int start, int end) {
     float rnd = spimref_random(x, y);
     return (float)(int)((float)start + rnd * (float)(end - start));
@@ -522,9 +527,13 @@ static inline void spr_sincosf(float x, float *s, float *c) {
 void spimref_sincos(float x, float *s, float *c) { spr_sincosf(x, s, c); }
 
 /* occlusion.cl:41-82 */
-void spimref_occlusion(float *out, int W, int H, int radius, int n_points, const float *depth) {
+static void occlusion_rows(float *out, int W, int H, int radius, int n_points, const float *depth,
+                           int y0, int y1) {
     const float MPI_2 = 6.2831853071795f;
-    for (int y = 0; y < H; ++y)
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int y = y0; y < y1; ++y)
         for (int x = 0; x < W; ++x) {
             float depth0 = depth[x + y * W];
             float occ = 0.0f;
@@ -547,11 +556,15 @@ void spimref_occlusion(float *out, int W, int H, int radius, int n_points, const
         }
 }
 
+void spimref_occlusion(float *out, int W, int H, int radius, int n_points, const float *depth) {
+    occlusion_rows(out, W, H, radius, n_points, depth, 0, H);
+}
+
 /* iso_kernel.cl:505-588 */
-void spimref_shading(float *out, int W, int H, const float *invP, const float *invM,
-                     float occ_strength, const float *normals, const float *depth,
-                     const float *occl) {
-    for (int y = 0; y < H; ++y)
+static void shading_rows(float *out, int W, int H, const float *invP, const float *invM,
+                         float occ_strength, const float *normals, const float *depth,
+                         const float *occl, int y0, int y1) {
+    for (int y = y0; y < y1; ++y)
         for (int x = 0; x < W; ++x) {
             f4 orig, direc;
             ray_setup(invP, invM, (uint32_t)x, (uint32_t)y, (uint32_t)W, (uint32_t)H, &orig, &direc);
@@ -575,6 +588,12 @@ void spimref_shading(float *out, int W, int H, const float *invP, const float *i
         }
 }
 
+void spimref_shading(float *out, int W, int H, const float *invP, const float *invM,
+                     float occ_strength, const float *normals, const float *depth,
+                     const float *occl) {
+    shading_rows(out, W, H, invP, invM, occ_strength, normals, depth, occl, 0, H);
+}
+
 /* volumerender.py:446-506 pipeline: iso_surface -> conv_vec(7) -> occlusion -> conv(5) -> shading */
 int spimref_render_iso(const void *data, int dtype, int nx, int ny, int nz, int filter,
                        const float *invP, const float *invM, const float *box, float iso_val,
@@ -590,5 +609,33 @@ int spimref_render_iso(const void *data, int dtype, int nx, int ny, int nz, int 
     spimref_conv_x(occl, tmp, W, H, 5);
     spimref_conv_y(tmp, occl, W, H, 5);
     spimref_shading(out, W, H, invP, invM, occ_strength, normals, depth, occl);
+    return 0;
+}
+
+/* The same pipeline's output, alpha, depth, normals and occlusion on rows [y_begin, y_end)
+   only.  A row's result depends on the iso march of the rows within max(3, occ_radius + 3)
+   of it (normal blur +-3; occlusion samples within occ_radius, blurred +-2), so only
+   those rows are marched; the arrays' other rows must hold a miss (depth +inf, normals 0)
+   or anything else: they are read but cannot reach rows [y_begin, y_end). */
+int spimref_render_iso_rows(const void *data, int dtype, int nx, int ny, int nz, int filter,
+                            const float *invP, const float *invM, const float *box,
+                            float iso_val, float gamma, int max_steps, float occ_strength,
+                            int occ_radius, int occ_n_points, int W, int H, int y_begin,
+                            int y_end, float *out, float *alpha, float *depth, float *normals,
+                            float *occl, float *tmp_vec, float *tmp, int nthreads) {
+    if (y_begin < 0 || y_end > H || y_begin >= y_end || occ_radius < 0) return 1;
+    const int m = occ_radius + 3 > 3 ? occ_radius + 3 : 3;
+    const int ma = y_begin - m < 0 ? 0 : y_begin - m, mb = y_end + m > H ? H : y_end + m;
+    int rc = spimref_iso_surface(data, dtype, nx, ny, nz, filter, invP, invM, box, iso_val, gamma,
+                                 max_steps, W, H, ma, mb, out, alpha, depth, normals, nthreads);
+    if (rc) return rc;
+    const int na = y_begin - 3 < 0 ? 0 : y_begin - 3, nb = y_end + 3 > H ? H : y_end + 3;
+    const int oa = y_begin - 2 < 0 ? 0 : y_begin - 2, ob = y_end + 2 > H ? H : y_end + 2;
+    conv_vec_x_rows(normals, tmp_vec, W, H, 7, na, nb);
+    conv_vec_y_rows(tmp_vec, normals, W, H, 7, y_begin, y_end);
+    occlusion_rows(occl, W, H, occ_radius, occ_n_points, depth, oa, ob);
+    conv_x_rows(occl, tmp, W, H, 5, oa, ob);
+    conv_y_rows(tmp, occl, W, H, 5, y_begin, y_end);
+    shading_rows(out, W, H, invP, invM, occ_strength, normals, depth, occl, y_begin, y_end);
     return 0;
 }

@@ -26,9 +26,10 @@ FILTER_LINEAR, FILTER_NEAREST = 0, 1
 METHOD_MAX_PROJECT, METHOD_ISO_SURFACE = 0, 1
 LAYOUT_AUTO, LAYOUT_LINEAR, LAYOUT_PADDED = 0, 1, 2
 TUNE_TILE_MAP, TUNE_SLAB_KB = 0, 1
-TUNE_STAGING, TUNE_CHUNK, TUNE_ROW_PAD, TUNE_WG_WAVES = 2, 3, 4, 5
+TUNE_STAGING, TUNE_CHUNK, TUNE_ROW_PAD, TUNE_WG_WAVES, TUNE_SKIP = 2, 3, 4, 5, 6
 TUNING_KEYS = {"tile_map": TUNE_TILE_MAP, "slab_kb": TUNE_SLAB_KB, "staging": TUNE_STAGING,
-               "chunk": TUNE_CHUNK, "row_pad": TUNE_ROW_PAD, "wg_waves": TUNE_WG_WAVES}
+               "chunk": TUNE_CHUNK, "row_pad": TUNE_ROW_PAD, "wg_waves": TUNE_WG_WAVES,
+               "skip": TUNE_SKIP}
 
 # every symbol include/spimrender.h declares
 EXPORTS = (
@@ -39,7 +40,7 @@ EXPORTS = (
     "spr_count_samples", "spr_volume_add", "spr_volume_add_dilated", "spr_volume_add_pooled",
     "spr_volume_add_pooled_pair", "spr_volume_read",
     "spr_volume_release", "spr_volume_info",
-    "spr_render_batch", "spr_render_slot", "spr_render_pair", "spr_render_coarse_pair",
+    "spr_render_batch", "spr_render_frames", "spr_render_slot", "spr_render_pair", "spr_render_coarse_pair",
     "spr_batch_graph_create",
     "spr_batch_graph_update",
     "spr_batch_graph_launch", "spr_batch_graph_destroy",
@@ -93,6 +94,40 @@ class SprJob(ctypes.Structure):
         ("d_out", ctypes.c_void_p),
         ("d_alpha", ctypes.c_void_p),
     ]
+
+
+class SprFrame(ctypes.Structure):
+    """struct spr_frame (include/spimrender.h)."""
+    _fields_ = [
+        ("slot", ctypes.c_int),
+        ("w", ctypes.c_int),
+        ("h", ctypes.c_int),
+        ("max_steps", ctypes.c_int),
+        ("inv_p", ctypes.c_float * 16),
+        ("inv_m", ctypes.c_float * 16),
+        ("box", ctypes.c_float * 6),
+        ("min_val", ctypes.c_float),
+        ("max_val", ctypes.c_float),
+        ("gamma", ctypes.c_float),
+        ("d_out", ctypes.c_void_p),
+        ("d_alpha", ctypes.c_void_p),
+    ]
+
+
+def make_frame(slot, params, w, h, d_out, d_alpha):
+    """An SprFrame of `slot` rendered with SprParams `params` (its matrices, box, transfer
+    range, gamma and max_steps) into w x h device planes."""
+    f = SprFrame()
+    f.slot = int(slot)
+    f.w, f.h = int(w), int(h)
+    f.max_steps = params.max_steps
+    f.inv_p[:] = list(params.inv_p)
+    f.inv_m[:] = list(params.inv_m)
+    f.box[:] = list(params.box)
+    f.min_val, f.max_val, f.gamma = params.min_val, params.max_val, params.gamma
+    f.d_out = d_out
+    f.d_alpha = d_alpha
+    return f
 
 
 # numpy view of struct spr_job, for building large job tables without a Python loop
@@ -185,6 +220,8 @@ def load():
     jp = ctypes.POINTER(SprJob)
     L.spr_render_batch.argtypes = [c_void_p, ctypes.POINTER(SprParams), jp, c_int, c_int, c_int,
                                    c_int, c_int, c_void_p]
+    L.spr_render_frames.argtypes = [c_void_p, ctypes.POINTER(SprFrame), c_int, c_float, c_int,
+                                    c_void_p]
     L.spr_render_slot.argtypes = [c_void_p, c_int, ctypes.POINTER(SprParams), c_int, c_int, c_int,
                                   c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p]
@@ -393,6 +430,12 @@ class Context:
         pp = ctypes.byref(params) if params is not None else None
         self._check(self.lib.spr_render_batch(self.handle, pp, arr, len(arr), w, h, num_parts,
                                               current_part, stream))
+
+    def render_frames(self, frames, alpha_pow=0., filter=FILTER_LINEAR, stream=None):
+        """frames: a list of SprFrame (make_frame); one persistent launch (spr_render_frames)."""
+        arr = (SprFrame * len(frames))(*frames)
+        self._check(self.lib.spr_render_frames(self.handle, arr, len(frames), float(alpha_pow),
+                                               int(filter), stream))
 
     def render_slot(self, slot, params, method, w, h, d_out, d_alpha, d_depth=None,
                     d_normals=None, d_occlusion=None, num_parts=1, current_part=0, stream=None):

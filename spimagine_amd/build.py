@@ -4,19 +4,28 @@ gfx950 in-tree with hipcc.
     python -m spimagine_amd.build
 
 The libraries land next to this file (spimagine_amd/lib*.so) so they travel with the
-source tree; nothing is installed into site-packages.
+source tree; nothing is installed into site-packages.  The ray caster is compiled as
+several objects in parallel: the C ABI (spimrender.hip) and launch_typed.hip once per
+voxel type and power-of-two rule (the kernel templates' instantiations), then linked.
 """
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspimrender.so")
 LEARN_LIB = os.path.join(HERE, "libspimlearn.so")
-LIBRARIES = {LIB: [os.path.join(CSRC, "spimrender.hip")],
-             LEARN_LIB: [os.path.join(CSRC, "spimlearn.hip")]}
+
+# (object name, source, extra defines)
+RENDER_OBJECTS = [("spimrender", "spimrender.hip", [])] + [
+    ("launch_v%d_p%d" % (v, p2), "launch_typed.hip", ["-DSPR_VOXEL=%d" % v, "-DSPR_POW2=%d" % p2])
+    for v in (0, 1, 2) for p2 in (0, 1)]
+LEARN_OBJECTS = [("spimlearn", "spimlearn.hip", [])]
+LIBRARIES = {LIB: RENDER_OBJECTS, LEARN_LIB: LEARN_OBJECTS}
 
 # -ffp-contract=off + correctly rounded fp32 div/sqrt: the kernels' float32 operation
 # sequence is fixed (every FMA is an explicit fmaf).  Packed FP32 instructions are
@@ -26,7 +35,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -40,24 +48,37 @@ def hipcc():
     return p if os.path.exists(p) else "hipcc"
 
 
-def build_one(lib, sources, verbose=False):
-    tmp = lib + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-I", CSRC,
-                                     "-o", tmp] + sources
+def _run(cmd, what, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building %s" % lib)
-    os.replace(tmp, lib)
-    return lib
+        raise RuntimeError("hipcc failed building %s" % what)
 
 
-def build_library(verbose=False):
-    """Builds every library; returns the ray caster's path."""
-    for lib, sources in LIBRARIES.items():
-        build_one(lib, sources, verbose)
+def compile_object(name, source, defines, verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    obj = os.path.join(OBJ, name + ".o")
+    cmd = [hipcc()] + HIPCC_FLAGS + defines + ["-I", os.path.join(ROOT, "include"), "-I", CSRC,
+                                                "-c", "-o", obj, os.path.join(CSRC, source)]
+    _run(cmd, obj, verbose)
+    return obj
+
+
+def build_library(verbose=False, jobs=None):
+    """Builds every library (objects compiled in parallel); returns the ray caster's path."""
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    work = [(lib, o) for lib, objs in LIBRARIES.items() for o in objs]
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = {(lib, o[0]): ex.submit(compile_object, o[0], o[1], o[2], verbose) for lib, o in work}
+        objs = {k: f.result() for k, f in futs.items()}
+    for lib, specs in LIBRARIES.items():
+        tmp = lib + ".tmp"
+        cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + \
+              [objs[(lib, o[0])] for o in specs]
+        _run(cmd, lib, verbose)
+        os.replace(tmp, lib)
     return LIB
 
 

@@ -80,6 +80,79 @@ __device__ __forceinline__ void wave_min3_u16x2_halves(uint32_t& a, uint32_t& b,
     c = __builtin_amdgcn_readlane(c, 31);
 }
 
+// q = n / d and n - q d for n, d < 2^16 (d > 0): a float quotient off by at most one,
+// corrected (no integer division)
+__device__ __forceinline__ void divmod16(uint32_t n, uint32_t d, float rd, uint32_t& q,
+                                         uint32_t& m) {
+    q = (uint32_t)((float)n * rd);
+    int32_t r = (int32_t)n - (int32_t)__umul24(q, d);
+    if (r < 0) {
+        q -= 1u;
+        r += (int32_t)d;
+    } else if (r >= (int32_t)d) {
+        q += 1u;
+        r -= (int32_t)d;
+    }
+    m = (uint32_t)r;
+}
+
+// wave64 minima of three unsigned 32-bit values (the same DPP chain as wave_min3_u16x2);
+// uniform results
+__device__ __forceinline__ void wave_min3_u32(uint32_t& a, uint32_t& b, uint32_t& c) {
+#define SPR_MIN3_STEP(CTRL, RM)                                                                \
+    {                                                                                          \
+        const uint32_t da = __builtin_amdgcn_update_dpp(a, a, CTRL, RM, 0xf, false);           \
+        const uint32_t db = __builtin_amdgcn_update_dpp(b, b, CTRL, RM, 0xf, false);           \
+        const uint32_t dc = __builtin_amdgcn_update_dpp(c, c, CTRL, RM, 0xf, false);           \
+        a = min(a, da);                                                                        \
+        b = min(b, db);                                                                        \
+        c = min(c, dc);                                                                        \
+    }
+    SPR_MIN3_STEP(0x111, 0xf)
+    SPR_MIN3_STEP(0x112, 0xf)
+    SPR_MIN3_STEP(0x114, 0xf)
+    SPR_MIN3_STEP(0x118, 0xf)
+    SPR_MIN3_STEP(0x142, 0xa)
+    SPR_MIN3_STEP(0x143, 0xc)
+#undef SPR_MIN3_STEP
+    a = __builtin_amdgcn_readlane(a, 63);
+    b = __builtin_amdgcn_readlane(b, 63);
+    c = __builtin_amdgcn_readlane(c, 63);
+}
+
+// The bricks (brick_bounds.h) covering the padded-voxel box [LX, HX] x [LY, HY] x [LZ, HZ]
+// (inclusive; every corner a sample of the box reads lies inside): each lane loads up to
+// two of them (uniform box; lanes past the count load nothing).  Returns false, loading
+// nothing, when the box covers more than kBrickMaxQuery bricks.  keys: this lane's
+// (min key, ~max key) so far (all ones: none).
+template <typename V>
+__device__ __forceinline__ bool brick_load(const V& vol, uint32_t LX, uint32_t LY, uint32_t LZ,
+                                           uint32_t HX, uint32_t HY, uint32_t HZ, uint32_t lane,
+                                           BrickKey& k) {
+    k.kmin = k.nkmax = 0xFFFFFFFFu;
+    const uint32_t bx0 = LX >> kBrickShift, by0 = LY >> kBrickShift, bz0 = LZ >> kBrickShift;
+    const uint32_t sx = (HX >> kBrickShift) - bx0 + 1u, sy = (HY >> kBrickShift) - by0 + 1u,
+                   sz = (HZ >> kBrickShift) - bz0 + 1u;
+    const uint32_t n = sx * sy * sz;
+    if (n > kBrickMaxQuery || vol.bricks.keys == nullptr) return false;
+    const float rsx = __builtin_amdgcn_rcpf((float)sx), rsy = __builtin_amdgcn_rcpf((float)sy);
+#pragma unroll
+    for (uint32_t r = 0; r < kBrickMaxQuery / 64u; ++r) {
+        const uint32_t j = lane + 64u * r;
+        if (j < n) {
+            uint32_t q, dx, dy, dz;
+            divmod16(j, sx, rsx, q, dx);
+            divmod16(q, sy, rsy, dz, dy);
+            const uint32_t idx =
+                (__umul24(bz0 + dz, vol.bricks.nby) + by0 + dy) * vol.bricks.nbx + bx0 + dx;
+            const BrickKey b = vol.bricks.keys[idx];
+            k.kmin = min(k.kmin, b.kmin);
+            k.nkmax = min(k.nkmax, b.nkmax);
+        }
+    }
+    return true;
+}
+
 // The voxel pair (x, x+1) from a dword-aligned slab address a and the byte shift sh of the
 // pair within that dword, read as two aligned dwords (one ds_read2_b32) and shifted into
 // place with v_alignbyte: LDS accesses that are not dword aligned stall the LDS pipeline.
@@ -207,22 +280,6 @@ __device__ __forceinline__ void staged_chunk_box(const V& vol, bool active,
 // Copies rows r = z SY + y of the box into slab[r SX TS ...] with NT threads (t = this
 // thread's index among them): 16-byte buffer loads, nu units per row, NT / nu rows a pass.
 using U32x4 = __attribute__((ext_vector_type(4))) uint32_t;
-
-// q = n / d and n - q d for n, d < 2^16 (d > 0): a float quotient off by at most one,
-// corrected (no integer division)
-__device__ __forceinline__ void divmod16(uint32_t n, uint32_t d, float rd, uint32_t& q,
-                                         uint32_t& m) {
-    q = (uint32_t)((float)n * rd);
-    int32_t r = (int32_t)n - (int32_t)__umul24(q, d);
-    if (r < 0) {
-        q -= 1u;
-        r += (int32_t)d;
-    } else if (r >= (int32_t)d) {
-        q += 1u;
-        r -= (int32_t)d;
-    }
-    m = (uint32_t)r;
-}
 
 // Copies rows r = z SY + y of the box into slab[r rowb ...] with NT threads (t = this
 // thread's index among them): 16-byte buffer loads, nu units per row, R = NT / nu rows a
@@ -354,6 +411,35 @@ struct Opacity {
     float mn, den;   // min_val, max_val - min_val
     float a;         // 0.1 alpha_pow^2 (integer voxels) or alpha_pow^2 (float voxels)
 };
+
+// Whether a ray of the opacity mode may stop before its last sample (uniform).  With every
+// sample of the volume in [vmin, vmax] (brick bounds widened for float voxels), normalised
+// as the samples are to [lo, hi]: when every step multiplies cumsum by a factor in [0, 1]
+// -- integer voxels: lo >= 0 and fl(a hi) <= 1; float voxels (v clamped to [0, 1] in the
+// transparency update): a <= 1 -- cumsum never grows and stays >= 0, so a later sample
+// adds fl(cumsum' v) <= fl(cumsum hi) to col (rounding is monotone).  *hi_out = max(hi, 0).
+template <typename T>
+__device__ __forceinline__ bool opacity_can_stop(const Opacity& o, float vmin, float vmax,
+                                                 float& hi_out) {
+    constexpr bool kShort = !std::is_same<T, float>::value;
+    hi_out = 0.0f;
+    if (!(vmin <= vmax)) return false;  // a NaN voxel: never stop
+    const float m = bound_margin<T>(vmin, vmax);
+    float lo = vmin - m, hi = vmax + m;
+    if (o.norm) {
+        if (!(o.den > 0.0f)) return false;
+        lo = (lo - o.mn) / o.den;
+        hi = (hi - o.mn) / o.den;
+    }
+    hi_out = fmaxf(hi, 0.0f);
+    if (kShort) return lo >= 0.0f && o.a * hi_out <= 1.0f;
+    return o.a <= 1.0f;
+}
+
+// a ray whose later samples can no longer raise col (opacity_can_stop held)
+__device__ __forceinline__ bool opacity_spent(const Opacity& o, float col, float hi) {
+    return o.cumsum * hi <= col;
+}
 
 template <bool kShort>
 __device__ __forceinline__ void opacity_step(Opacity& o, float v, float& col, float& px,
@@ -566,9 +652,13 @@ __device__ __forceinline__ void shadow_first_active(uint64_t act, bool active, f
 // the boxes, samples them and folds the samples into col (col2: a pair's second channel).
 // active: this lane has a ray; live: the wave has one (lanes without a ray follow an
 // active lane's positions).  HALVES: a box too large for the slab is first tried as two
-// half-wave boxes.
+// half-wave boxes.  SKIP (per-wave slabs, volumes with brick bounds): the plain max
+// projection skips a box whose bricks' maximum cannot exceed any active ray's running
+// maximum (its samples cannot change col; only the positions advance, by the same
+// additions) and stops once every ray holds the volume's maximum; the opacity mode stops
+// once no later sample can raise any ray's colour (opacity_spent).
 template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE, bool OPAC,
-          bool HALVES = true>
+          bool HALVES = true, bool SKIP = false>
 __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WIDE>& vol,
                                                  const RayParams& p, bool active, bool live,
                                                  uint8_t* slab, uint32_t* s_box, float& px,
@@ -590,6 +680,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
     // reduced per-lane boxes; EST: sample `first` is estimated (a box ahead of the rays).
     struct Box {
         uint32_t lox, LY, LZ, rowb, SY, SZ, nu;
+        uint32_t hx;  // last padded x the box's samples read
         uint64_t bytes;
     };
     // a reduced box (packed as staged_box_n) in slab terms; true when it fits the slab
@@ -600,6 +691,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
         const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
                        HZ = 0xFFFFu - (r2 >> 16);
         bx.lox = LX & ~(E - 1u);
+        bx.hx = HX;
         const uint32_t SX = (HX + 1u - bx.lox + E - 1u) & ~(E - 1u);  // voxels, multiple of E
         bx.SY = HY - bx.LY + 1u;
         bx.SZ = HZ - bx.LZ + 1u;
@@ -642,16 +734,48 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
     };
+    // copies a box that fits into the slab (rows addressed from plane LZ on: 32-bit offsets
+    // whatever the volume's size) and makes it visible
+    auto copy_box = [&](const Box& bx) {
+        staged_copy_rows<NT>(vol.rsrc_plane(bx.LZ), slab, SHARED ? threadIdx.x : lane, bx.LY, 0u,
+                             bx.lox * TS, bx.nu, bx.rowb, bx.SY, bx.SZ, PY, pitch);
+        visible();
+    };
     // Stages the box of the next n samples of every ray into the slab.  Returns false, with
     // nothing staged, when the box does not fit.
     auto stage = [&](int n, Box& bx) -> bool {
         const bool fits = box_of(std::false_type(), 0.0f, (float)(n - 1), bx);
-        // rows addressed from plane LZ on (32-bit offsets whatever the volume's size)
-        if (fits)
-            staged_copy_rows<NT>(vol.rsrc_plane(bx.LZ), slab, SHARED ? threadIdx.x : lane, bx.LY,
-                                 0u, bx.lox * TS, bx.nu, bx.rowb, bx.SY, bx.SZ, PY, pitch);
-        visible();
+        if (fits) {
+            copy_box(bx);
+        } else {
+            visible();
+        }
         return fits;
+    };
+    // this lane's share of the brick bounds of box bx (false: too many bricks to ask)
+    auto box_keys = [&](const Box& bx, BrickKey& k) -> bool {
+        return brick_load(vol, bx.lox, bx.LY, bx.LZ, bx.hx, bx.LY + bx.SY - 1u, bx.LZ + bx.SZ - 1u,
+                          lane, k);
+    };
+    // SKIP, plain max projection: 0 = sample the box, 1 = skip it (its bricks' maximum is at
+    // most every active ray's running maximum), 2 = stop (every active ray holds the
+    // volume's maximum).  Uniform.
+    auto skip_test = [&](BrickKey k) -> int {
+        uint32_t c = active ? order_key(col) : 0xFFFFFFFFu;  // col >= 0, never NaN
+        wave_min3_u32(k.kmin, k.nkmax, c);
+        const float bmin = key_value<T>(k.kmin), bmax = key_value<T>(~k.nkmax);
+        const float cmin = key_value<float>(c);
+        if (vol.vmax + bound_margin<T>(vol.vmin, vol.vmax) <= cmin) return 2;
+        return bmax + bound_margin<T>(bmin, bmax) <= cmin ? 1 : 0;
+    };
+    // the positions of n samples, as the sampling loops advance them
+    auto advance = [&](int n) {
+#pragma unroll 1
+        for (int j = 0; j < n; ++j) {
+            px = px + dx;
+            py = py + dy;
+            pz = pz + dz;
+        }
     };
     // the wave's reads of the slab complete before the next box overwrites it
     auto release = [&]() {
@@ -698,23 +822,63 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
         }
         done += n;
     };
+    static_assert(!SKIP || !SHARED, "skipping runs with per-wave slabs");
+    // SKIP, opacity mode: whether the volume's bounds let a ray stop early, and the largest
+    // normalised sample (uniform)
+    float vmax_n = 0.0f;
+    bool can_stop = false;
+    if constexpr (SKIP && OPAC) {
+        can_stop = opacity_can_stop<T>(op, vol.vmin, vol.vmax, vmax_n);
+    }
     // per-wave slabs: the next box's rows are loaded into registers while this box is
-    // sampled (when the next box fits and needs at most kPrefetchRows rows per lane)
+    // sampled (when the next box fits and needs at most kPrefetchRows rows per lane); SKIP:
+    // with its brick bounds
     Box nb{};
-    bool pf = false;
+    bool pf = false, pf_keys = false;
     int pf_n = 0;
     U32x4 pv[kPrefetchRows];
+    BrickKey nk{};
     for (int i = g_begin; i < g_end; i += G) {
         const int ng = min(G, g_end - i);
+        if constexpr (SKIP && OPAC) {
+            if (can_stop && __ballot(active && !opacity_spent(op, col, vmax_n)) == 0) break;
+        }
         Box b;
         bool fits;
-        if (!SHARED && pf) {
-            staged_commit_rows(slab, lane, nb.nu, nb.rowb, nb.SY * nb.SZ, pf_n, pv);
-            visible();
-            b = nb;
-            fits = true;
+        if constexpr (SKIP && !OPAC) {
+            // the box's bricks: loaded with the prefetched rows, or now
+            BrickKey k = nk;
+            bool have = pf && pf_keys;
+            if (!pf) {
+                fits = box_of(std::false_type(), 0.0f, (float)(ng * CH - 1), b);
+                have = box_keys(b, k);
+            }
+            const int t = have ? skip_test(k) : 0;
+            if (t == 2) break;
+            if (t == 1) {
+                pf = false;  // prefetched rows (if any) are dropped
+                advance(ng * CH);
+                continue;
+            }
+            if (pf) {
+                staged_commit_rows(slab, lane, nb.nu, nb.rowb, nb.SY * nb.SZ, pf_n, pv);
+                visible();
+                b = nb;
+                fits = true;
+            } else if (fits) {
+                copy_box(b);
+            } else {
+                visible();
+            }
         } else {
-            fits = stage(ng * CH, b);
+            if (!SHARED && pf) {
+                staged_commit_rows(slab, lane, nb.nu, nb.rowb, nb.SY * nb.SZ, pf_n, pv);
+                visible();
+                b = nb;
+                fits = true;
+            } else {
+                fits = stage(ng * CH, b);
+            }
         }
         pf = false;
         if constexpr (!SHARED && !OPAC) {
@@ -728,6 +892,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
                         staged_prefetch_rows(vol.rsrc_plane(nb.LZ), lane, nb.LY, nb.lox * TS, nb.nu,
                                              nb.SY, PY, pitch, nb.SY * nb.SZ, pf_n, pv);
                         pf = true;
+                        if constexpr (SKIP) pf_keys = box_keys(nb, nk);
                     }
                 }
             }
@@ -830,7 +995,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
 // OPAC: the opacity mode (alpha_pow > 0, Opacity); boxes are not loaded ahead (a spent
 // lane's position does not advance with the group).
 template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE = false,
-          bool OPAC = false, bool HALVES = true>
+          bool OPAC = false, bool HALVES = true, bool SKIP = false>
 __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE>& vol, const RayParams& p,
                                                 uint32_t x, uint32_t y, bool inside, uint8_t* slab,
                                                 uint32_t* s_box, float* __restrict__ out,
@@ -852,7 +1017,7 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
     const bool live = act != 0;  // uniform over the wave
     if (live) shadow_first_active(act, active, px, py, pz, dx, dy, dz);
     float col = 0.0f, col2 = 0.0f;
-    mip_staged_march<T, POW2, SLAB_BYTES, CH, SHARED, WIDE, OPAC, HALVES>(
+    mip_staged_march<T, POW2, SLAB_BYTES, CH, SHARED, WIDE, OPAC, HALVES, SKIP>(
         vol, p, active, live, slab, s_box, px, py, pz, dx, dy, dz, 0,
         (p.nouter + 1) * (kLoopUnroll / CH), col, col2);
     if (active) {
@@ -864,7 +1029,7 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
 // one frame: one workgroup per 16x16 tile (tile map as k_max_project)
 // SLAB_BYTES: per wave (SHARED = false) or per workgroup (SHARED = true)
 template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE = false,
-          bool OPAC = false>
+          bool OPAC = false, bool SKIP = false>
 __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2, WIDE> vol, RayParams p,
                                                             float* __restrict__ out,
                                                             float* __restrict__ alpha) {
@@ -874,8 +1039,8 @@ __global__ __launch_bounds__(256) void k_max_project_staged(PaddedVolume<T, POW2
     uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (SHARED ? 0u : (threadIdx.x >> 6) * SLAB_BYTES);
     uint32_t x = 0, y = 0;
     const bool inside = tile_pixel(p, &x, &y);
-    mip_staged_rays<T, POW2, SLAB_BYTES, CH, SHARED, WIDE, OPAC>(vol, p, x, y, inside, slab, s_box,
-                                                                 out, alpha);
+    mip_staged_rays<T, POW2, SLAB_BYTES, CH, SHARED, WIDE, OPAC, true, SKIP>(vol, p, x, y, inside,
+                                                                             slab, s_box, out, alpha);
 }
 
 // Estimated cost of each wave of a frame of the per-wave staged kernels, for the
@@ -947,7 +1112,7 @@ __global__ __launch_bounds__(256) void k_tile_cost(PaddedVolume<T, POW2, WIDE> v
 // items: a count per cost value, then a stable scatter by ballots over 64 items at a time.
 constexpr uint32_t kCostValues = kCostProbes + 2;
 
-__global__ __launch_bounds__(512) void k_tile_order(const uint32_t* __restrict__ cost,
+static __global__ __launch_bounds__(512) void k_tile_order(const uint32_t* __restrict__ cost,
                                                     RayParams p, uint32_t per_wave,
                                                     uint32_t* __restrict__ order) {
     const uint32_t lane = threadIdx.x & 63u, x = threadIdx.x >> 6;
@@ -1022,14 +1187,15 @@ __device__ __forceinline__ bool wave_tile_pixel(const RayParams& p, uint32_t* x,
 // one frame with one wave per workgroup (64 threads, one slab of SLAB_BYTES): a CU takes a
 // new wave's work as soon as any of its waves ends, instead of when a workgroup's slowest
 // of four waves does.  Grid: 4 x tile_grid (interleaved or cost-ordered maps).
-template <typename T, bool POW2, int SLAB_BYTES, int CH, bool WIDE = false, bool OPAC = false>
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool WIDE = false, bool OPAC = false,
+          bool SKIP = false>
 __global__ __launch_bounds__(64) void k_max_project_wave(PaddedVolume<T, POW2, WIDE> vol,
                                                          RayParams p, float* __restrict__ out,
                                                          float* __restrict__ alpha) {
     __shared__ Unit16 lds[SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
     uint32_t x = 0, y = 0;
     const bool inside = wave_tile_pixel(p, &x, &y);
-    mip_staged_rays<T, POW2, SLAB_BYTES, CH, false, WIDE, OPAC>(
+    mip_staged_rays<T, POW2, SLAB_BYTES, CH, false, WIDE, OPAC, true, SKIP>(
         vol, p, x, y, inside, reinterpret_cast<uint8_t*>(lds), nullptr, out, alpha);
 }
 
@@ -1087,7 +1253,11 @@ __device__ __forceinline__ void iso_march_chunk(const V& vol, const uint8_t* sla
     }
 }
 
-template <typename T, bool POW2, int SLAB_BYTES, int CH, bool WIDE = false>
+// SKIP (a volume with brick bounds): a chunk after the first is skipped -- only the
+// positions advance, by the same additions -- when its box's bricks show that no marching
+// ray can change side there: bricks' maximum <= iso for rays that start below or at the
+// iso value, bricks' minimum > iso for rays that start above it.
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool WIDE = false, bool SKIP = false>
 __global__ __launch_bounds__(256) void k_iso_staged(PaddedVolume<T, POW2, WIDE> vol, RayParams p,
                                                     float* __restrict__ out,
                                                     float* __restrict__ alpha,
@@ -1150,17 +1320,38 @@ __global__ __launch_bounds__(256) void k_iso_staged(PaddedVolume<T, POW2, WIDE> 
         const uint32_t rowb = nu * 16u;
         const uint64_t bytes = nu <= 64u ? (uint64_t)rowb * SY * SZ : ~0ull;
         const bool fits = bytes <= (uint64_t)SLAB_BYTES;
-        if (fits)
-            staged_copy_rows<64>(vol.rsrc_plane(LZ), slab, lane, LY, 0u, lox * TS, nu, rowb, SY, SZ,
-                                 PY, pitch);
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (fits)
-            iso_march_chunk<PV, CH, true>(vol, slab, lox, LY, LZ, rowb, SY, sx, sy_, sz_, tx, ty, tz,
-                                          k0, nsteps, p.iso_val, march, gt, found, ihit);
-        else
-            iso_march_chunk<PV, CH, false>(vol, slab, lox, LY, LZ, rowb, SY, sx, sy_, sz_, tx, ty,
-                                           tz, k0, nsteps, p.iso_val, march, gt, found, ihit);
+        bool skip = false;
+        if constexpr (SKIP) {
+            BrickKey k;
+            if (k0 > 0 && brick_load(vol, LX, LY, LZ, HX, HY, HZ, lane, k)) {
+                uint32_t unused = 0;
+                wave_min3_u32(k.kmin, k.nkmax, unused);
+                const float bmin = key_value<T>(k.kmin), bmax = key_value<T>(~k.nkmax);
+                const float m = bound_margin<T>(bmin, bmax);
+                const bool same_side = gt ? bmin - m > p.iso_val : bmax + m <= p.iso_val;
+                skip = __ballot(march && !same_side) == 0;
+            }
+        }
+        if (skip) {
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                sx = sx + tx;
+                sy_ = sy_ + ty;
+                sz_ = sz_ + tz;
+            }
+        } else {
+            if (fits)
+                staged_copy_rows<64>(vol.rsrc_plane(LZ), slab, lane, LY, 0u, lox * TS, nu, rowb, SY,
+                                     SZ, PY, pitch);
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (fits)
+                iso_march_chunk<PV, CH, true>(vol, slab, lox, LY, LZ, rowb, SY, sx, sy_, sz_, tx, ty,
+                                              tz, k0, nsteps, p.iso_val, march, gt, found, ihit);
+            else
+                iso_march_chunk<PV, CH, false>(vol, slab, lox, LY, LZ, rowb, SY, sx, sy_, sz_, tx, ty,
+                                               tz, k0, nsteps, p.iso_val, march, gt, found, ihit);
+        }
         if (march) {  // a marching lane's own position moves on
             px = sx;
             py = sy_;
@@ -1189,8 +1380,10 @@ __host__ inline int staged_chunk(int nouter, int nmax) {
 
 // ---------------------------------------------------------------------------------
 // Batched frames (many volumes x poses in one launch).  A job is one frame: a resident
-// padded volume, the pose's inverse matrices and the two output planes; everything else
-// (frame size, min/max/gamma, max_steps, numParts) is shared by the batch.
+// padded volume, the pose's inverse matrices and the two output planes.  In a uniform
+// batch every other setting (frame size, box, min/max/gamma, max_steps, numParts) is the
+// batch's; in a mixed batch (BatchQueues::mixed) each job carries its own frame size, box,
+// transfer range, gamma and step count, and the voxel type, filter and alpha_pow are shared.
 
 struct BatchJob {
     const void* base;  // padded volume of the job (same voxel type across the batch)
@@ -1200,6 +1393,12 @@ struct BatchJob {
     float invM[16];
     float* out;
     float* alpha;
+    // mixed batches: the frame's own settings, and its first item in each queue
+    float box[6];
+    float min_val, max_val, gamma;
+    int nouter;
+    uint32_t W, H;
+    uint32_t first[8];
 };
 
 // Work queue of a persistent batch launch: the (job, tile) items whose tile row is
@@ -1208,7 +1407,9 @@ struct BatchJob {
 // in one L2), then shared by every workgroup once its own queue is empty.
 struct BatchQueues {
     uint32_t count[8];  // items per queue
-    uint32_t tiles[8];  // tiles per frame in queue q
+    uint32_t tiles[8];  // uniform batch: tiles per frame in queue q
+    uint32_t mixed;     // 1: jobs of their own sizes and settings (BatchJob::first)
+    uint32_t njobs;
 };
 
 template <typename T, bool POW2>
@@ -1231,11 +1432,14 @@ __device__ __forceinline__ PaddedVolume<T, POW2> job_volume(const BatchJob& j) {
     v.mx = (float)(j.nx - 1);
     v.my = (float)(j.ny - 1);
     v.mz = (float)(j.nz - 1);
+    v.bricks.keys = nullptr;  // batched frames sample every box
+    v.bricks.nbx = v.bricks.nby = 0;
+    v.vmin = v.vmax = 0.0f;
     return v;
 }
 
 // zero the queue counters (a kernel, so the batch can be captured in a HIP graph)
-__global__ void k_reset_counters(uint32_t* __restrict__ counters) {
+static __global__ void k_reset_counters(uint32_t* __restrict__ counters) {
     if (threadIdx.x < 8) counters[threadIdx.x] = 0u;
 }
 
@@ -1266,9 +1470,25 @@ __global__ __launch_bounds__(256) void k_max_project_batch(const BatchJob* __res
             ++drained;
             continue;
         }
-        const uint32_t per = qs.tiles[q];
-        const uint32_t jid = item / per, t = item - jid * per;
-        const uint32_t ty = q + 8u * (t / shared.ntx), tx = t % shared.ntx;
+        uint32_t jid, t;
+        if (qs.mixed) {
+            // the job whose items of queue q hold `item`: the last one whose first item is
+            // at most `item` (jobs without items in queue q share their successor's first)
+            uint32_t lo = 0, hi = qs.njobs;
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (jobs[mid].first[q] <= item)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            jid = lo;
+            t = item - jobs[jid].first[q];
+        } else {
+            const uint32_t per = qs.tiles[q];
+            jid = item / per;
+            t = item - jid * per;
+        }
         const BatchJob& j = jobs[jid];
         RayParams p = shared;
 #pragma unroll
@@ -1276,6 +1496,20 @@ __global__ __launch_bounds__(256) void k_max_project_batch(const BatchJob* __res
             p.invP[k] = j.invP[k];
             p.invM[k] = j.invM[k];
         }
+        if (qs.mixed) {
+            p.W = j.W;
+            p.H = j.H;
+            p.ntx = (j.W + 15u) / 16u;
+            p.nty = (j.H + 15u) / 16u;
+            p.ntiles = p.ntx * p.nty;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) p.box[k] = j.box[k];
+            p.min_val = j.min_val;
+            p.max_val = j.max_val;
+            p.gamma = j.gamma;
+            p.nouter = j.nouter;
+        }
+        const uint32_t ty = q + 8u * (t / p.ntx), tx = t % p.ntx;
         uint32_t x, y;
         tile_thread_pixel(tx, ty, &x, &y);
         const bool inside = x < p.W && y < p.H;

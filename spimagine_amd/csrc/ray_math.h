@@ -1,7 +1,7 @@
 // ray_math.h -- device-side float32 ray set-up shared by every render kernel.
 //
 // The reference leaves dot order, normalize, min/max on infinities and division
-// accuracy to the OpenCL implementation (spimagine/volumerender/kernels/utils.cl:70-99,
+// accuracy to the OpenCL implementation (spimagine/volumerender/kernels/utils.cl:41-70,
 // volume_kernel.cl:48-93).  This file fixes one float32 operation sequence; the library
 // is compiled with -ffp-contract=off and correctly rounded division/sqrt, and every fused
 // multiply-add below is an explicit fmaf, so the sample positions are reproducible bit for
@@ -38,7 +38,7 @@ __device__ __forceinline__ F4 normalize4(F4 v) {
     return mk4(v.x / l, v.y / l, v.z / l, v.w / l);
 }
 
-// utils.cl:92-99: component i = dot(v, row i of the row-major 4x4 M)
+// utils.cl:63-70: component i = dot(v, row i of the row-major 4x4 M)
 __device__ __forceinline__ F4 mult(const float* M, F4 v) {
     F4 r;
     r.x = dot4(v, mk4(M[0], M[1], M[2], M[3]));
@@ -52,7 +52,7 @@ __device__ __forceinline__ F4 mult(const float* M, F4 v) {
 __device__ __forceinline__ float cl_min(float x, float y) { return (y < x) ? y : x; }
 __device__ __forceinline__ float cl_max(float x, float y) { return (x < y) ? y : x; }
 
-// utils.cl:70-89 slab test (IEEE infinities for zero direction components)
+// utils.cl:41-60 slab test (IEEE infinities for zero direction components)
 __device__ __forceinline__ bool intersect_box(F4 ro, F4 rd, const float* box, float* tnear,
                                               float* tfar) {
     float ix = 1.0f / rd.x, iy = 1.0f / rd.y, iz = 1.0f / rd.z;
@@ -90,7 +90,7 @@ __device__ __forceinline__ void ray_setup(const float* invP, const float* invM, 
 // fused multiply-add lerp used by the trilinear sampler
 __device__ __forceinline__ float lerpf(float p, float q, float t) { return fmaf(t, q - p, p); }
 
-// utils.cl:39-52 uint32 LCG, 10 rounds, modulo 79197919
+// utils.cl:10-23 uint32 LCG, 10 rounds, modulo 79197919
 __device__ __forceinline__ float lcg_random(uint32_t x, uint32_t y) {
     uint32_t a = 4421u + (1u + x) * (1u + y) + x + y;
 #pragma unroll
@@ -98,7 +98,7 @@ __device__ __forceinline__ float lcg_random(uint32_t x, uint32_t y) {
     return ((float)a * 1.0f) / (float)79197919;
 }
 
-// utils.cl:54-67 (declared float, returns an int value)
+// utils.cl:25-38 (declared float, returns an int value)
 __device__ __forceinline__ float lcg_rand_int(uint32_t x, uint32_t y, int start, int end) {
     float rnd = lcg_random(x, y);
     return (float)(int)((float)start + rnd * (float)(end - start));

@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) void k_conv(const float* __restrict__ in, floa
 }
 
 // occlusion.cl:41-82
-__global__ __launch_bounds__(256) void k_occlusion(float* __restrict__ out, int W, int H,
+static __global__ __launch_bounds__(256) void k_occlusion(float* __restrict__ out, int W, int H,
                                                    int radius, int n_points,
                                                    const float* __restrict__ depth) {
     int x = blockIdx.x * 16 + (threadIdx.x & 15);
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void k_occlusion(float* __restrict__ out, int 
 }
 
 // iso_kernel.cl:505-588 -- the final iso output (overwrites `out`)
-__global__ __launch_bounds__(256) void k_shading(float* __restrict__ out, RayParams p,
+static __global__ __launch_bounds__(256) void k_shading(float* __restrict__ out, RayParams p,
                                                  float occ_strength,
                                                  const float* __restrict__ normals,
                                                  const float* __restrict__ depth,
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void k_shading(float* __restrict__ out, RayPar
 }
 
 // rays of a w x h frame whose box test hits (for the reference sample count)
-__global__ __launch_bounds__(256) void k_count_hits(RayParams p, unsigned long long* __restrict__ count) {
+static __global__ __launch_bounds__(256) void k_count_hits(RayParams p, unsigned long long* __restrict__ count) {
     uint32_t x, y;
     bool inside = tile_pixel(p, &x, &y);
     int hit = 0;

@@ -2,10 +2,13 @@
 // ray-casting kernels.  Host side of spimagine's VolumeRenderer
 // (spimagine/volumerender/volumerender.py): volume upload and layout, parameter state,
 // kernel dispatch in the reference's launch order, the device-to-host output copies, and
-// the batched (many volumes x poses, HIP graph) launch.
+// the batched (many volumes x poses, HIP graph) launch.  The ray-march kernels are
+// launched through TypedLaunch (launch_typed.hip, one compilation per voxel type and
+// power-of-two rule).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -16,29 +19,12 @@
 #include <type_traits>
 #include <vector>
 
-#include "mip_staged.h"
-#include "render_kernels.h"
-#include "spimrender.h"
+#include "spr_internal.h"
 
 using namespace spr;
 
-namespace {
-
-// one resident volume
-struct VolSlot {
-    bool used = false;
-    int layout = SPR_LAYOUT_AUTO;  // SPR_LAYOUT_LINEAR or SPR_LAYOUT_PADDED once filled
-    void* d_vol = nullptr;         // LINEAR: the volume as uploaded
-    size_t vol_bytes = 0;
-    void* d_pad = nullptr;         // PADDED: (nz+2, ny+2, pitch) apron copy
-    size_t pad_bytes = 0;
-    bool pad_wide = false;         // LINEAR volume above 4 GiB with a padded copy (d_pad)
-    int dtype = -1, nx = 0, ny = 0, nz = 0;
-};
-
-}  // namespace
-
 constexpr size_t kMaxSlots = 4096;
+
 
 struct spr_batch_graph {
     int n = 0, w = 0, h = 0;
@@ -48,17 +34,21 @@ struct spr_batch_graph {
     hipGraphExec_t exec = nullptr;
     hipStream_t last_stream = nullptr;
     int dtype = -1;
+    bool pow2 = false;  // the captured kernel's texel rule (every volume a power of two)
+    // (slot, generation) of every job: a launch after any of these slots was refilled or
+    // released is refused
+    std::vector<std::pair<int, uint32_t>> refs;
 };
 
 struct spr_ctx {
     int device = 0;
-    int num_cus = 0;
     hipStream_t stream = nullptr;
     std::mutex table_mu;  // serialises spr_volume_add / spr_volume_release
     int layout_req = SPR_LAYOUT_AUTO;  // layout for the next upload
     // slot 0: the renderer's volume.  Capacity is reserved once (kMaxSlots) so adding a
     // slot never moves the others while renders on other threads read them.
     std::vector<VolSlot> vols;
+    std::atomic<uint32_t> next_generation{1};
     spr_params params{};
     bool has_params = false;
     // w x h output buffers kept between spr_render calls (VolumeRenderer.buf*, reset_buffer)
@@ -70,13 +60,7 @@ struct spr_ctx {
     size_t scratch_px = 0;
     unsigned long long* d_count = nullptr;
     float* d_partial = nullptr;
-    // launch tuning (spr_set_tuning)
-    int tile_map = (int)kTileMapInterleaved;
-    int slab_kb = 19;  // per-wave LDS slab of the staged max projection, 0 = off
-    int staging = 0;   // 0 per-wave boxes, 1 one box per workgroup
-    int chunk = 0;     // samples per staged box, 0 = from the ray step
-    int row_pad = 0;   // pad staged slab rows to an odd number of 16-byte units
-    int wg_waves = 0;  // per-wave staged kernels: waves per workgroup (1, 4; 0 = by frame size)
+    LaunchTuning tune;  // spr_set_tuning state and the device's CU count
 };
 
 namespace {
@@ -147,6 +131,7 @@ void free_scratch(spr_ctx* c) {
 void free_slot(VolSlot& v) {
     if (v.d_vol) (void)hipFree(v.d_vol);
     if (v.d_pad) (void)hipFree(v.d_pad);
+    if (v.d_bricks) (void)hipFree(v.d_bricks);
     v = VolSlot();
 }
 
@@ -223,7 +208,7 @@ RayParams make_params(const spr_params& prm, int tile_map, int w, int h) {
     return p;
 }
 
-RayParams make_params(const spr_ctx* c, int w, int h) { return make_params(c->params, c->tile_map, w, h); }
+RayParams make_params(const spr_ctx* c, int w, int h) { return make_params(c->params, c->tune.tile_map, w, h); }
 
 int check_params(spr_ctx* c, const spr_params& p) {
     if (p.filter != SPR_FILTER_LINEAR && p.filter != SPR_FILTER_NEAREST)
@@ -252,63 +237,6 @@ ConvWeights conv_weights(int nh, float k) {
     return cw;
 }
 
-bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
-bool slot_pow2(const VolSlot& v) { return is_pow2(v.nx) && is_pow2(v.ny) && is_pow2(v.nz); }
-
-template <typename T>
-LinearVolume<T> linear_volume(const VolSlot& s) {
-    LinearVolume<T> v;
-    v.data = (const T*)s.d_vol;
-    v.nx = s.nx;
-    v.ny = s.ny;
-    v.nz = s.nz;
-    v.fnx = (float)s.nx;
-    v.fny = (float)s.ny;
-    v.fnz = (float)s.nz;
-    return v;
-}
-
-template <typename T, bool POW2, bool WIDE = false>
-PaddedVolume<T, POW2, WIDE> padded_volume(const VolSlot& s) {
-    PaddedVolume<T, POW2, WIDE> v;
-    v.base = s.d_pad;
-    v.nbytes = WIDE ? 0xFFFFFFFFu : (uint32_t)s.pad_bytes;
-    v.total_bytes = s.pad_bytes;
-    const uint32_t PY = (uint32_t)s.ny + 2;
-    v.px_bytes = padded_pitch<T>(s.nx) * (uint32_t)sizeof(T);
-    v.py = PY;
-    v.soff_y = v.px_bytes;
-    v.soff_z = v.px_bytes * PY;
-    v.soff_yz = v.soff_y + v.soff_z;
-    v.nx = s.nx;
-    v.ny = s.ny;
-    v.nz = s.nz;
-    v.fnx = (float)s.nx;
-    v.fny = (float)s.ny;
-    v.fnz = (float)s.nz;
-    v.mx = (float)(s.nx - 1);
-    v.my = (float)(s.ny - 1);
-    v.mz = (float)(s.nz - 1);
-    return v;
-}
-
-// Calls f(volume accessor) with the accessor matching the slot's voxel type, layout and
-// (padded layout) whether every dimension is a power of two.
-template <typename T, typename F>
-hipError_t visit_typed(const VolSlot& s, F&& f) {
-    if (s.layout == SPR_LAYOUT_LINEAR) return f(linear_volume<T>(s));
-    if (slot_pow2(s)) return f(padded_volume<T, true>(s));
-    return f(padded_volume<T, false>(s));
-}
-
-template <typename F>
-hipError_t visit_volume(const VolSlot& s, F&& f) {
-    switch (s.dtype) {
-        case SPR_DTYPE_F32: return visit_typed<float>(s, f);
-        case SPR_DTYPE_U16: return visit_typed<uint16_t>(s, f);
-        default: return visit_typed<uint8_t>(s, f);
-    }
-}
 
 int check_frame(spr_ctx* c, int w, int h) {
     if (!c->has_params) return fail(c, SPR_ERR_INVALID_ARG, "spr_set_params was not called");
@@ -332,232 +260,19 @@ int check_parts(spr_ctx* c, const spr_params& prm, int num_parts) {
     return SPR_OK;
 }
 
-template <typename T, bool P2, int CH, bool SH, typename... Args>
-void launch_staged_sh(int slab_kb, dim3 grid, hipStream_t s, Args... args) {
-    constexpr int F = SH ? 4 : 1;  // a workgroup slab holds the four waves' share
-    if (slab_kb == 8)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 8192 * F, CH, SH>), grid, dim3(256), 0, s, args...);
-    else if (slab_kb == 16 || (SH && slab_kb == 19))
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384 * F, CH, SH>), grid, dim3(256), 0, s, args...);
-    else if (slab_kb == 19)  // per-wave only: two workgroups of 4 x 19 KiB fill 152 of 160 KiB
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, CH, false>), grid, dim3(256), 0, s, args...);
-    else
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288 * F, CH, SH>), grid, dim3(256), 0, s, args...);
-}
-
-template <typename T, bool P2, int CH, typename... Args>
-void launch_staged_ch(int slab_kb, bool shared, dim3 grid, hipStream_t s, Args... args) {
-    if (shared)
-        launch_staged_sh<T, P2, CH, true>(slab_kb, grid, s, args...);
-    else
-        launch_staged_sh<T, P2, CH, false>(slab_kb, grid, s, args...);
-}
-
-// samples per unrolled group of the staged kernels: the tuning value (32 and 64 are boxes
-// of 2 and 4 groups of 16), else from the ray step
-int chunk_for(const spr_ctx* c, int nouter, int nmax) {
-    return c->chunk ? std::min(c->chunk, 16) : staged_chunk(nouter, nmax);
-}
-
-// the staged kernels' launch options carried in RayParams
-void staged_tuning(const spr_ctx* c, RayParams& p) {
-    p.row_pad = (uint32_t)c->row_pad;
-    p.box16 = c->chunk > 16 ? (uint32_t)(c->chunk / 16) : 1u;
-}
-
-// The cost-ordered tile map (kTileMapCostXcd) for the per-wave staged kernels: the waves'
-// estimated costs (k_tile_cost) and the workgroups' order (k_tile_order), computed on
-// stream s into *buf from the stream-ordered allocator (the caller frees it after the
-// frame's launch).  per_wave: one wave per workgroup (k_max_project_wave) instead of a
-// 16x16 tile.  Sets p.tile_map and p.order and the frame's grid; other tile maps, and
-// frames of more than kMaxOrderedTiles tiles, are left as they are.
-constexpr uint32_t kMaxOrderedTiles = 65536;
-
-template <typename T, bool P2, bool WIDE>
-hipError_t order_tiles(int tile_map, bool per_wave, const PaddedVolume<T, P2, WIDE>& v,
-                       uint32_t slab_bytes, RayParams& p, hipStream_t s, void** buf, dim3* grid) {
-    if (tile_map != (int)kTileMapCostXcd || p.ntiles > kMaxOrderedTiles) return hipSuccess;
-    const uint32_t slots = tile_grid((uint32_t)tile_map, p.ntx, p.nty) << (per_wave ? 2 : 0);
-    hipError_t e = hipMallocAsync(buf, (size_t)(4 * p.ntiles + slots) * sizeof(uint32_t), s);
-    if (e != hipSuccess) return e;
-    uint32_t* cost = (uint32_t*)*buf;
-    uint32_t* order = cost + 4 * p.ntiles;
-    hipLaunchKernelGGL((k_tile_cost<T, P2, WIDE>), dim3(p.ntiles), dim3(256), 0, s, v, p,
-                       slab_bytes, cost);
-    p.tile_map = (uint32_t)tile_map;
-    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(512), 0, s, (const uint32_t*)cost, p,
-                       per_wave ? 1u : 0u, order);
-    p.order = order;
-    *grid = dim3(slots);
-    return hipGetLastError();
-}
-
-// The slab of a one-wave workgroup for slab_kb 19: LDS is allocated in 512-byte granules,
-// so eight one-wave workgroups per CU fit 19,968 B each (the slab plus its 16 bytes of
-// slack), 2.5% more than 19 KiB (measured ~0.5-1% faster).
-constexpr uint32_t kWaveSlab19 = 19952;
-
-// one wave per workgroup: the plain max projection (slab 16 or 19 KiB) or the opacity mode
-template <typename T, bool P2, bool OPAC, int CH, typename... Args>
-void launch_wave_ch(int slab_kb, dim3 grid, hipStream_t s, Args... args) {
-    if (slab_kb == 16 && !OPAC)
-        hipLaunchKernelGGL((k_max_project_wave<T, P2, 16384, CH, false, OPAC>), grid, dim3(64), 0, s, args...);
-    else
-        hipLaunchKernelGGL((k_max_project_wave<T, P2, kWaveSlab19, CH, false, OPAC>), grid, dim3(64), 0, s, args...);
-}
-
-template <typename T, bool P2, bool OPAC, typename... Args>
-void launch_wave(int slab_kb, int ch, dim3 grid, hipStream_t s, Args... args) {
-    if (ch == 16)
-        launch_wave_ch<T, P2, OPAC, 16>(slab_kb, grid, s, args...);
-    else if (ch == 8)
-        launch_wave_ch<T, P2, OPAC, 8>(slab_kb, grid, s, args...);
-    else
-        launch_wave_ch<T, P2, OPAC, 4>(slab_kb, grid, s, args...);
-}
-
-// One wave per workgroup for the per-wave staged kernels?  wg_waves 0 (default) decides by
-// frame size: one wave while the frame has at most kOneWaveTilesPerCu 16x16 tiles per CU.
-// A frame of few tiles per CU ends on its slowest waves, and one-wave workgroups refill a
-// CU as soon as any wave ends (1024^2 frames, 16 tiles per CU: 7% faster); a frame of many
-// tiles has no such tail, and four waves of a tile side by side share the CU's L1
-// (2048^2 frames, 64 tiles per CU: four waves 10% faster).
-constexpr uint32_t kOneWaveTilesPerCu = 24;
-
-bool one_wave_for(const spr_ctx* c, const RayParams& p) {
-    if (c->wg_waves != 0) return c->wg_waves == 1;
-    return p.ntiles <= kOneWaveTilesPerCu * (uint32_t)std::max(c->num_cus, 1);
-}
-
-// a per-wave slab's bytes for the slab_kb tuning value
-uint32_t slab_bytes_of(int slab_kb) { return slab_kb == 19 ? 19456u : (uint32_t)slab_kb * 1024u; }
-
-// volumes above 4 GiB: per-wave 16 KiB slabs (19 KiB measured 2% slower on 2048^2
-// frames); OPAC: the opacity mode
-template <typename T, bool P2, bool OPAC>
-void launch_staged_wide_t(const VolSlot& vs, int ch, bool one_wave, dim3 grid, hipStream_t s,
-                          const RayParams& p, float* out, float* alpha) {
-    const PaddedVolume<T, P2, true> v = padded_volume<T, P2, true>(vs);
-    if (one_wave) {  // grid: one workgroup per 8x8 quarter of the interleaved tile grid
-        const dim3 g(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty));
-        if (ch == 16)
-            hipLaunchKernelGGL((k_max_project_wave<T, P2, 16384, 16, true, OPAC>), g, dim3(64), 0, s, v, p, out, alpha);
-        else if (ch == 8)
-            hipLaunchKernelGGL((k_max_project_wave<T, P2, 16384, 8, true, OPAC>), g, dim3(64), 0, s, v, p, out, alpha);
-        else
-            hipLaunchKernelGGL((k_max_project_wave<T, P2, 16384, 4, true, OPAC>), g, dim3(64), 0, s, v, p, out, alpha);
-        return;
-    }
-    if (ch == 16)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, 16, false, true, OPAC>), grid, dim3(256), 0, s, v, p, out, alpha);
-    else if (ch == 8)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, 8, false, true, OPAC>), grid, dim3(256), 0, s, v, p, out, alpha);
-    else
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384, 4, false, true, OPAC>), grid, dim3(256), 0, s, v, p, out, alpha);
-}
-
-template <bool OPAC>
-void launch_staged_wide(const VolSlot& vs, int ch, bool one_wave, dim3 grid, hipStream_t s,
-                        const RayParams& p, float* out, float* alpha) {
-    const bool p2 = slot_pow2(vs);
+// Calls f(TypedLaunch<T, P2>()) for the slot's voxel type; P2: the padded copy's texel rule
+// (every dimension a power of two; linear-layout slots without a padded copy use false).
+template <typename F>
+hipError_t visit_launch(const VolSlot& vs, F&& f) {
+    const bool p2 = (vs.layout == SPR_LAYOUT_PADDED || vs.pad_wide) && slot_pow2(vs);
     switch (vs.dtype) {
         case SPR_DTYPE_F32:
-            p2 ? launch_staged_wide_t<float, true, OPAC>(vs, ch, one_wave, grid, s, p, out, alpha)
-               : launch_staged_wide_t<float, false, OPAC>(vs, ch, one_wave, grid, s, p, out, alpha);
-            break;
+            return p2 ? f(TypedLaunch<float, true>()) : f(TypedLaunch<float, false>());
         case SPR_DTYPE_U16:
-            p2 ? launch_staged_wide_t<uint16_t, true, OPAC>(vs, ch, one_wave, grid, s, p, out, alpha)
-               : launch_staged_wide_t<uint16_t, false, OPAC>(vs, ch, one_wave, grid, s, p, out, alpha);
-            break;
+            return p2 ? f(TypedLaunch<uint16_t, true>()) : f(TypedLaunch<uint16_t, false>());
         default:
-            p2 ? launch_staged_wide_t<uint8_t, true, OPAC>(vs, ch, one_wave, grid, s, p, out, alpha)
-               : launch_staged_wide_t<uint8_t, false, OPAC>(vs, ch, one_wave, grid, s, p, out, alpha);
-            break;
+            return p2 ? f(TypedLaunch<uint8_t, true>()) : f(TypedLaunch<uint8_t, false>());
     }
-}
-
-// the iso-surface march of a volume above 4 GiB staged (16 KiB slabs)
-template <typename T, bool P2>
-void launch_iso_wide_t(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
-                       float* out, float* alpha, float* depth, float* normals) {
-    const PaddedVolume<T, P2, true> v = padded_volume<T, P2, true>(vs);
-    if (ch == 16)
-        hipLaunchKernelGGL((k_iso_staged<T, P2, 16384, 16, true>), grid, dim3(256), 0, s, v, p, out, alpha, depth, normals);
-    else if (ch == 8)
-        hipLaunchKernelGGL((k_iso_staged<T, P2, 16384, 8, true>), grid, dim3(256), 0, s, v, p, out, alpha, depth, normals);
-    else
-        hipLaunchKernelGGL((k_iso_staged<T, P2, 16384, 4, true>), grid, dim3(256), 0, s, v, p, out, alpha, depth, normals);
-}
-
-void launch_iso_wide(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
-                     float* out, float* alpha, float* depth, float* normals) {
-    const bool p2 = slot_pow2(vs);
-    switch (vs.dtype) {
-        case SPR_DTYPE_F32:
-            p2 ? launch_iso_wide_t<float, true>(vs, ch, grid, s, p, out, alpha, depth, normals)
-               : launch_iso_wide_t<float, false>(vs, ch, grid, s, p, out, alpha, depth, normals);
-            break;
-        case SPR_DTYPE_U16:
-            p2 ? launch_iso_wide_t<uint16_t, true>(vs, ch, grid, s, p, out, alpha, depth, normals)
-               : launch_iso_wide_t<uint16_t, false>(vs, ch, grid, s, p, out, alpha, depth, normals);
-            break;
-        default:
-            p2 ? launch_iso_wide_t<uint8_t, true>(vs, ch, grid, s, p, out, alpha, depth, normals)
-               : launch_iso_wide_t<uint8_t, false>(vs, ch, grid, s, p, out, alpha, depth, normals);
-            break;
-    }
-}
-
-template <typename T, bool P2, typename... Args>
-void launch_staged(int slab_kb, bool shared, int ch, dim3 grid, hipStream_t s, Args... args) {
-    if (ch == 16)
-        launch_staged_ch<T, P2, 16>(slab_kb, shared, grid, s, args...);
-    else if (ch == 8)
-        launch_staged_ch<T, P2, 8>(slab_kb, shared, grid, s, args...);
-    else
-        launch_staged_ch<T, P2, 4>(slab_kb, shared, grid, s, args...);
-}
-
-// the opacity mode (alpha_pow > 0) staged: per-wave 19 KiB slabs
-template <typename T, bool P2, typename... Args>
-void launch_staged_opac(int ch, dim3 grid, hipStream_t s, Args... args) {
-    if (ch == 16)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 16, false, false, true>), grid, dim3(256), 0, s, args...);
-    else if (ch == 8)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 8, false, false, true>), grid, dim3(256), 0, s, args...);
-    else
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 4, false, false, true>), grid, dim3(256), 0, s, args...);
-}
-
-// the iso-surface march staged: per-wave 19 KiB slabs (four waves per workgroup: one per
-// workgroup was 3.5% slower over five poses of the headline volume)
-template <typename T, bool P2, typename... Args>
-void launch_iso_staged(int ch, dim3 grid, hipStream_t s, Args... args) {
-    if (ch == 16)
-        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 16>), grid, dim3(256), 0, s, args...);
-    else if (ch == 8)
-        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 8>), grid, dim3(256), 0, s, args...);
-    else
-        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 4>), grid, dim3(256), 0, s, args...);
-}
-
-// both channels of a padded channel-pair slot (19 KiB per-wave slabs)
-template <bool P2, int CH>
-void launch_pair(const VolSlot& vs, dim3 grid, hipStream_t s, const RayParams& p, float* out,
-                 float* alpha) {
-    hipLaunchKernelGGL((k_max_project_staged<U16Pair, P2, 19456, CH, false>), grid, dim3(256), 0, s,
-                       padded_volume<U16Pair, P2>(vs), p, out, alpha);
-}
-
-template <bool P2>
-void launch_pair_ch(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
-                    float* out, float* alpha) {
-    if (ch == 16)
-        launch_pair<P2, 16>(vs, grid, s, p, out, alpha);
-    else if (ch == 8)
-        launch_pair<P2, 8>(vs, grid, s, p, out, alpha);
-    else
-        launch_pair<P2, 4>(vs, grid, s, p, out, alpha);
 }
 
 // render(method, numParts, currentPart) into device buffers; the reference's launch
@@ -566,9 +281,8 @@ void launch_pair_ch(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const R
 int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method, int num_parts,
                 int current_part, int w, int h, float* out, float* alpha, float* depth,
                 float* normals, float* occ, float* tmp, float* tmp_vec, hipStream_t s) {
-    RayParams p = make_params(prm, c->tile_map, w, h);
-    staged_tuning(c, p);
-    const dim3 grid(tile_grid(p.tile_map, p.ntx, p.nty)), block(256);
+    RayParams p = make_params(prm, c->tune.tile_map, w, h);
+    staged_tuning(c->tune, p);
     const bool nearest = prm.filter == SPR_FILTER_NEAREST;
     int rc;
     if (method == SPR_METHOD_MAX_PROJECT) {
@@ -577,66 +291,9 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
         if (!out || !alpha) return fail(c, SPR_ERR_INVALID_ARG, "out/alpha buffers are required");
         p.nouter = (prm.max_steps / num_parts) / kLoopUnroll;
         p.current_part = current_part;
-        const bool staged = !nearest && prm.alpha_pow == 0.0f && c->slab_kb > 0;
-        // the opacity mode: per-wave staging (19 KiB slabs; 16 KiB for wide volumes)
-        const bool staged_opac = !nearest && prm.alpha_pow != 0.0f && c->slab_kb > 0 &&
-                                 c->staging == 0;
-        if ((staged || staged_opac) && vs.pad_wide) {
-            const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
-            const bool one_wave = one_wave_for(c, p);
-            if (staged)
-                launch_staged_wide<false>(vs, chunk_for(c, p.nouter, nmax), one_wave, grid, s, p,
-                                          out, alpha);
-            else
-                launch_staged_wide<true>(vs, chunk_for(c, p.nouter, nmax), one_wave, grid, s, p,
-                                         out, alpha);
-            HIPCALL(c, hipGetLastError());
-            return SPR_OK;
-        }
-        hipError_t e = visit_volume(vs, [&](auto v) {
-            using V = decltype(v);
-            if constexpr (is_padded<V>::value) {
-                if (staged_opac || staged) {
-                    const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
-                    const int ch = chunk_for(c, p.nouter, nmax);
-                    // one wave per workgroup: per-wave slabs of 16 or 19 KiB
-                    const bool one_wave = one_wave_for(c, p) &&
-                                          (staged_opac || (c->staging == 0 && c->slab_kb >= 16));
-                    // (one wave: tile rows interleaved over the XCDs unless cost-ordered)
-                    dim3 g = one_wave ? dim3(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty)) : grid;
-                    void* obuf = nullptr;
-                    hipError_t e = hipSuccess;
-                    if (staged_opac || c->staging == 0)  // per-wave slabs
-                        e = order_tiles(c->tile_map, one_wave, v,
-                                        one_wave ? (staged_opac || c->slab_kb != 16 ? kWaveSlab19 : 16384u)
-                                                 : staged_opac ? 19456u : slab_bytes_of(c->slab_kb),
-                                        p, s, &obuf, &g);
-                    if (e == hipSuccess) {
-                        if (one_wave && staged_opac)
-                            launch_wave<typename V::value_type, V::kPow2, true>(19, ch, g, s, v, p,
-                                                                                out, alpha);
-                        else if (one_wave)
-                            launch_wave<typename V::value_type, V::kPow2, false>(c->slab_kb, ch, g, s,
-                                                                                 v, p, out, alpha);
-                        else if (staged_opac)
-                            launch_staged_opac<typename V::value_type, V::kPow2>(ch, g, s, v, p, out,
-                                                                                 alpha);
-                        else
-                            launch_staged<typename V::value_type, V::kPow2>(
-                                c->slab_kb, c->staging == 1, ch, g, s, v, p, out, alpha);
-                        e = hipGetLastError();
-                    }
-                    if (obuf) (void)hipFreeAsync(obuf, s);
-                    return e;
-                }
-            }
-            if (nearest)
-                hipLaunchKernelGGL((k_max_project<V, 1>), grid, block, 0, s, v, p, out, alpha);
-            else
-                hipLaunchKernelGGL((k_max_project<V, 0>), grid, block, 0, s, v, p, out, alpha);
-            return hipGetLastError();
-        });
-        HIPCALL(c, e);
+        HIPCALL(c, visit_launch(vs, [&](auto L) {
+            return L.max_project(c->tune, vs, p, nearest, out, alpha, s);
+        }));
         return SPR_OK;
     }
     if (method == SPR_METHOD_ISO_SURFACE) {
@@ -646,34 +303,9 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
             return fail(c, SPR_ERR_INVALID_ARG, "iso_surface needs all five output buffers");
         if (prm.occ_n_points < 1 || prm.occ_radius < 0)
             return fail(c, SPR_ERR_INVALID_ARG, "invalid occlusion parameters");
-        // the march staged in LDS: linear filter, per-wave staging, padded or wide
-        const bool staged_iso = !nearest && c->slab_kb > 0 && c->staging == 0;
-        hipError_t e = hipSuccess;
-        if (staged_iso && vs.pad_wide) {
-            const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
-            launch_iso_wide(vs, chunk_for(c, prm.max_steps / kLoopUnroll, nmax), grid, s, p, out,
-                            alpha, depth, normals);
-            e = hipGetLastError();
-        } else e = visit_volume(vs, [&](auto v) {
-            using V = decltype(v);
-            if constexpr (is_padded<V>::value) {
-                if (staged_iso) {
-                    const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
-                    launch_iso_staged<typename V::value_type, V::kPow2>(
-                        chunk_for(c, prm.max_steps / kLoopUnroll, nmax), grid, s, v, p, out, alpha,
-                        depth, normals);
-                    return hipGetLastError();
-                }
-            }
-            if (nearest)
-                hipLaunchKernelGGL((k_iso_surface<V, 1>), grid, block, 0, s, v, p, out, alpha,
-                                   depth, normals);
-            else
-                hipLaunchKernelGGL((k_iso_surface<V, 0>), grid, block, 0, s, v, p, out, alpha,
-                                   depth, normals);
-            return hipGetLastError();
-        });
-        HIPCALL(c, e);
+        HIPCALL(c, visit_launch(vs, [&](auto L) {
+            return L.iso_march(c->tune, vs, p, nearest, out, alpha, depth, normals, s);
+        }));
         dim3 g2((w + 15) / 16, (h + 15) / 16), b2(256);
         const ConvWeights wv = conv_weights(7, 5.0f), ws = conv_weights(5, 10.0f);
         // _convolve_vec(buf_normals, 7) (volumerender.py:405-416, 470)
@@ -692,6 +324,56 @@ int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method
     }
     // an unknown method is a silent no-op in the reference (volumerender.py:543-547)
     return SPR_OK;
+}
+
+
+// Bounds of the 8x8x8 bricks of a padded volume (brick_bounds.h): thread bx of block
+// (by, bz) scans brick (bx, by, bz) over the padded voxels [8 b, 8 b + 8) clipped to the
+// padded extent (PX = nx + 2, PY, PZ); total gets the minimum of every brick's keys (the
+// volume's bounds), one atomic per wave.  A NaN voxel gives its brick the widest keys.
+template <typename T>
+__global__ __launch_bounds__(64) void k_brick_bounds(const T* __restrict__ pad, uint32_t pitch,
+                                                     uint32_t PX, uint32_t PY, uint32_t PZ,
+                                                     uint32_t nbx, uint32_t nby,
+                                                     BrickKey* __restrict__ keys,
+                                                     BrickKey* __restrict__ total) {
+    const uint32_t bx = blockIdx.x * 64u + threadIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    uint32_t kmin = 0xFFFFFFFFu, nkmax = 0xFFFFFFFFu;
+    if (bx < nbx) {
+        const uint32_t x0 = bx * kBrick, x1 = min(x0 + kBrick, PX);
+        const uint32_t y1 = min(by * kBrick + kBrick, PY), z1 = min(bz * kBrick + kBrick, PZ);
+        for (uint32_t z = bz * kBrick; z < z1; ++z)
+            for (uint32_t y = by * kBrick; y < y1; ++y) {
+                const T* row = pad + ((size_t)z * PY + y) * pitch;
+                for (uint32_t x = x0; x < x1; ++x) {
+                    const T v = row[x];
+                    if (v != v) {  // NaN (float voxels only)
+                        kmin = nkmax = 0u;
+                    } else {
+                        const uint32_t k = order_key(v);
+                        kmin = min(kmin, k);
+                        nkmax = min(nkmax, ~k);
+                    }
+                }
+            }
+        keys[((size_t)bz * nby + by) * nbx + bx] = BrickKey{kmin, nkmax};
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        kmin = min(kmin, (uint32_t)__shfl_xor((int)kmin, off));
+        nkmax = min(nkmax, (uint32_t)__shfl_xor((int)nkmax, off));
+    }
+    if (threadIdx.x == 0) {
+        atomicMin(&total->kmin, kmin);
+        atomicMin(&total->nkmax, nkmax);
+    }
+}
+
+template <typename T>
+void launch_bricks(const VolSlot& vs, BrickKey* total, hipStream_t s) {
+    const uint32_t PX = (uint32_t)vs.nx + 2u, PY = (uint32_t)vs.ny + 2u, PZ = (uint32_t)vs.nz + 2u;
+    const dim3 grid((vs.nbx + 63u) / 64u, vs.nby, vs.nbz);
+    hipLaunchKernelGGL(k_brick_bounds<T>, grid, dim3(64), 0, s, (const T*)vs.d_pad,
+                       padded_pitch<T>(vs.nx), PX, PY, PZ, vs.nbx, vs.nby, vs.d_bricks, total);
 }
 
 template <typename T>
@@ -830,20 +512,27 @@ int fill_slot(spr_ctx* c, VolSlot& vs, const void* src, bool src_on_device, int 
     // does not synchronise with the null stream): wait for all prior work on the device
     if (src_on_device) HIPCALL(c, hipDeviceSynchronize());
 
+    // what a batch graph captured of this slot (device buffers, shape, type, layout): a
+    // refill that keeps all of it keeps the slot's generation
+    const VolSlot before = vs;
+    bool realloc = false;
     // linear copy: kept as the resident volume (LINEAR), or staged for padding (host source)
     const bool need_linear = layout == SPR_LAYOUT_LINEAR || !src_on_device;
     if (vs.d_pad && (!want_pad || vs.pad_bytes != pbytes)) {
+        realloc = true;
         HIPCALL(c, hipFree(vs.d_pad));
         vs.d_pad = nullptr;
         vs.pad_bytes = 0;
     }
     if (vs.d_vol && (!need_linear || vs.vol_bytes != bytes)) {
+        realloc = true;
         HIPCALL(c, hipFree(vs.d_vol));
         vs.d_vol = nullptr;
         vs.vol_bytes = 0;
     }
     vs.used = false;
     if (need_linear && !vs.d_vol) {
+        realloc = true;
         HIPCALL(c, hipMalloc(&vs.d_vol, bytes));
         vs.vol_bytes = bytes;
     }
@@ -853,6 +542,7 @@ int fill_slot(spr_ctx* c, VolSlot& vs, const void* src, bool src_on_device, int 
                                   c->stream));
     if (want_pad) {
         if (!vs.d_pad) {
+            realloc = true;
             HIPCALL(c, hipMalloc(&vs.d_pad, pbytes));
             vs.pad_bytes = pbytes;
         }
@@ -864,6 +554,52 @@ int fill_slot(spr_ctx* c, VolSlot& vs, const void* src, bool src_on_device, int 
             default: launch_pad<uint8_t>(from, vs.d_pad, nx, ny, nz, c->stream); break;
         }
         HIPCALL(c, hipGetLastError());
+    }
+    // per-brick bounds of the padded copy (one-channel volumes) and the volume's bounds
+    const uint32_t nbx = want_pad ? ((uint32_t)nx + 2u + kBrick - 1u) / kBrick : 0u;
+    const uint32_t nby = want_pad ? ((uint32_t)ny + 2u + kBrick - 1u) / kBrick : 0u;
+    const uint32_t nbz = want_pad ? ((uint32_t)nz + 2u + kBrick - 1u) / kBrick : 0u;
+    const bool want_bricks = want_pad && dtype != SPR_DTYPE_U16X2;
+    if (vs.d_bricks && (!want_bricks || vs.nbx != nbx || vs.nby != nby || vs.nbz != nbz)) {
+        realloc = true;
+        HIPCALL(c, hipFree(vs.d_bricks));
+        vs.d_bricks = nullptr;
+    }
+    vs.nbx = nbx;
+    vs.nby = nby;
+    vs.nbz = nbz;
+    if (want_bricks) {
+        const size_t nbricks = (size_t)nbx * nby * nbz;
+        if (!vs.d_bricks) {
+            realloc = true;
+            // one more entry: the volume's bounds
+            HIPCALL(c, hipMalloc(&vs.d_bricks, (nbricks + 1) * sizeof(BrickKey)));
+        }
+        BrickKey* total = vs.d_bricks + nbricks;
+        vs.dtype = dtype;
+        vs.nx = nx;
+        vs.ny = ny;
+        vs.nz = nz;
+        HIPCALL(c, hipMemsetAsync(total, 0xFF, sizeof(BrickKey), c->stream));
+        switch (dtype) {
+            case SPR_DTYPE_F32: launch_bricks<float>(vs, total, c->stream); break;
+            case SPR_DTYPE_U16: launch_bricks<uint16_t>(vs, total, c->stream); break;
+            default: launch_bricks<uint8_t>(vs, total, c->stream); break;
+        }
+        HIPCALL(c, hipGetLastError());
+        BrickKey t;
+        HIPCALL(c, hipMemcpyAsync(&t, total, sizeof(t), hipMemcpyDeviceToHost, c->stream));
+        HIPCALL(c, hipStreamSynchronize(c->stream));
+        switch (dtype) {
+            case SPR_DTYPE_F32:
+                vs.vmin = key_value<float>(t.kmin);
+                vs.vmax = key_value<float>(~t.nkmax);
+                break;
+            default:
+                vs.vmin = (float)t.kmin;
+                vs.vmax = (float)~t.nkmax;
+                break;
+        }
     }
     HIPCALL(c, hipStreamSynchronize(c->stream));
     if (layout == SPR_LAYOUT_PADDED && vs.d_vol) {  // staging copy no longer needed
@@ -878,6 +614,10 @@ int fill_slot(spr_ctx* c, VolSlot& vs, const void* src, bool src_on_device, int 
     vs.nx = nx;
     vs.ny = ny;
     vs.nz = nz;
+    if (realloc || !before.used || before.d_pad != vs.d_pad || before.d_vol != vs.d_vol ||
+        before.layout != layout || before.dtype != dtype || before.nx != nx || before.ny != ny ||
+        before.nz != nz)
+        vs.generation = c->next_generation++;
     return SPR_OK;
 }
 
@@ -890,12 +630,15 @@ void launch_minmax(const void* data, size_t n, float* partial, int blocks, hipSt
 
 // Converts the caller's jobs into device job records; checks that every slot is padded and
 // of one voxel type.  *pow2 = every volume's dimensions are powers of two.
-int build_jobs(spr_ctx* c, const spr_job* jobs, int n, std::vector<BatchJob>& out, int* dtype,
-               bool* pow2) {
+// Converts the caller's jobs into device job records in out[0 .. n-1]; checks that every
+// slot is padded and of one voxel type.  *pow2 = every volume's dimensions are powers of
+// two; *nmax = the largest volume dimension.
+int build_jobs_into(spr_ctx* c, const spr_job* jobs, int n, BatchJob* out, int* dtype, bool* pow2,
+                    int* nmax) {
     if (!jobs || n <= 0) return fail(c, SPR_ERR_INVALID_ARG, "empty batch");
-    out.resize(n);
     *dtype = -1;
     *pow2 = true;
+    *nmax = 1;
     for (int i = 0; i < n; ++i) {
         const VolSlot* vs = slot_of(c, jobs[i].slot);
         if (!vs) {
@@ -912,7 +655,9 @@ int build_jobs(spr_ctx* c, const spr_job* jobs, int n, std::vector<BatchJob>& ou
             return fail(c, SPR_ERR_INVALID_ARG, "job %d: NULL output", i);
         *dtype = vs->dtype;
         *pow2 = *pow2 && slot_pow2(*vs);
+        *nmax = std::max(*nmax, std::max(vs->nx, std::max(vs->ny, vs->nz)));
         BatchJob& b = out[i];
+        b = BatchJob{};
         b.base = vs->d_pad;
         b.nbytes = (uint32_t)vs->pad_bytes;
         b.px_bytes = (uint32_t)(((size_t)vs->nx + 2 + 16 / dtype_size(vs->dtype) - 1) /
@@ -929,98 +674,62 @@ int build_jobs(spr_ctx* c, const spr_job* jobs, int n, std::vector<BatchJob>& ou
     return SPR_OK;
 }
 
+int build_jobs(spr_ctx* c, const spr_job* jobs, int n, std::vector<BatchJob>& out, int* dtype,
+               bool* pow2, int* nmax) {
+    if (!jobs || n <= 0) return fail(c, SPR_ERR_INVALID_ARG, "empty batch");
+    out.resize(n);
+    return build_jobs_into(c, jobs, n, out.data(), dtype, pow2, nmax);
+}
+
+// tile rows of a frame of nty tile rows that fall in queue k (rows k, k + 8, ...)
+uint32_t queue_rows(uint32_t nty, uint32_t k) { return k < nty ? (nty - k + 7u) / 8u : 0u; }
+
 BatchQueues make_queues(const RayParams& p, int n) {
     BatchQueues q;
     for (uint32_t k = 0; k < 8; ++k) {
-        const uint32_t rows = k < p.nty ? (p.nty - k + 7u) / 8u : 0u;
-        q.tiles[k] = rows * p.ntx;
+        q.tiles[k] = queue_rows(p.nty, k) * p.ntx;
         q.count[k] = q.tiles[k] * (uint32_t)n;
     }
+    q.mixed = 0;
+    q.njobs = (uint32_t)n;
     return q;
 }
 
-// enqueue counter reset + persistent kernel on s
-template <typename T, bool P2, int SLAB, int MODE, int CH, bool SH>
-hipError_t launch_batch_kernel(const spr_ctx* c, const BatchJob* d_jobs, const RayParams& p,
-                               const BatchQueues& q, uint32_t* counters, hipStream_t s) {
-    int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, k_max_project_batch<T, P2, SLAB, MODE, CH, SH>, 256, 0);
-    if (e != hipSuccess) return e;
-    if (per_cu < 1) per_cu = 1;
-    const uint32_t grid = (uint32_t)(c->num_cus * per_cu + 7) / 8u * 8u;
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters);
-    hipLaunchKernelGGL((k_max_project_batch<T, P2, SLAB, MODE, CH, SH>), dim3(grid), dim3(256), 0, s,
-                       d_jobs, p, q, counters);
-    return hipGetLastError();
-}
-
-template <typename T, bool P2, int SLAB, bool SH>
-hipError_t launch_batch_staged(int ch, const spr_ctx* c, const BatchJob* d_jobs, const RayParams& p,
-                               const BatchQueues& q, uint32_t* counters, hipStream_t s) {
-    constexpr int B = SLAB * (SH ? 4 : 1);
-    if (ch == 16) return launch_batch_kernel<T, P2, B, 0, 16, SH>(c, d_jobs, p, q, counters, s);
-    if (ch == 8) return launch_batch_kernel<T, P2, B, 0, 8, SH>(c, d_jobs, p, q, counters, s);
-    return launch_batch_kernel<T, P2, B, 0, 4, SH>(c, d_jobs, p, q, counters, s);
-}
-
-template <typename T, bool P2, int SLAB>
-hipError_t launch_batch_slab(bool shared, int ch, const spr_ctx* c, const BatchJob* d_jobs,
-                             const RayParams& p, const BatchQueues& q, uint32_t* counters,
-                             hipStream_t s) {
-    return shared ? launch_batch_staged<T, P2, SLAB, true>(ch, c, d_jobs, p, q, counters, s)
-                  : launch_batch_staged<T, P2, SLAB, false>(ch, c, d_jobs, p, q, counters, s);
-}
-
-template <typename T, bool P2>
-hipError_t dispatch_batch_typed(const spr_ctx* c, int mode, int ch, const BatchJob* d_jobs,
-                                const RayParams& p, const BatchQueues& q, uint32_t* counters,
-                                hipStream_t s) {
-    if (mode == 1) return launch_batch_kernel<T, P2, 16, 1, 16, false>(c, d_jobs, p, q, counters, s);
-    if (mode == 2) return launch_batch_kernel<T, P2, 16, 2, 16, false>(c, d_jobs, p, q, counters, s);
-    if (mode == 3) {
-        if (ch == 16) return launch_batch_kernel<T, P2, 19456, 3, 16, false>(c, d_jobs, p, q, counters, s);
-        if (ch == 8) return launch_batch_kernel<T, P2, 19456, 3, 8, false>(c, d_jobs, p, q, counters, s);
-        return launch_batch_kernel<T, P2, 19456, 3, 4, false>(c, d_jobs, p, q, counters, s);
-    }
-    const bool sh = c->staging == 1;
-    if (c->slab_kb == 8) return launch_batch_slab<T, P2, 8192>(sh, ch, c, d_jobs, p, q, counters, s);
-    if (c->slab_kb == 16 || (sh && c->slab_kb == 19))
-        return launch_batch_slab<T, P2, 16384>(sh, ch, c, d_jobs, p, q, counters, s);
-    if (c->slab_kb == 19) return launch_batch_staged<T, P2, 19456, false>(ch, c, d_jobs, p, q, counters, s);
-    return launch_batch_slab<T, P2, 12288>(sh, ch, c, d_jobs, p, q, counters, s);
-}
-
-// nmax: the largest volume dimension over the batch's jobs (sets the staged box length)
-hipError_t dispatch_batch(const spr_ctx* c, const spr_params& prm, int dtype, bool pow2, int nmax,
+// ch: samples per staged box (chunk_for of the batch's longest ray step)
+hipError_t dispatch_batch(const spr_ctx* c, const spr_params& prm, int dtype, bool pow2, int ch,
                           const BatchJob* d_jobs, const RayParams& p, const BatchQueues& q,
                           uint32_t* counters, hipStream_t s) {
+    const LaunchTuning& t = c->tune;
     const bool nearest = prm.filter == SPR_FILTER_NEAREST;
     // 0: staged max projection, 3: staged opacity mode (per-wave 19 KiB slabs), 1 / 2: every
     // sample from global memory (linear / nearest)
     const int mode = nearest ? 2
-                     : c->slab_kb <= 0 ? 1
+                     : t.slab_kb <= 0 ? 1
                      : prm.alpha_pow == 0.0f ? 0
-                     : c->staging == 0 ? 3 : 1;
-    const int ch = chunk_for(c, p.nouter, nmax);
+                     : t.staging == 0 ? 3 : 1;
     switch (dtype) {
         case SPR_DTYPE_F32:
-            return pow2 ? dispatch_batch_typed<float, true>(c, mode, ch, d_jobs, p, q, counters, s)
-                        : dispatch_batch_typed<float, false>(c, mode, ch, d_jobs, p, q, counters, s);
+            return pow2 ? TypedLaunch<float, true>::batch(t, mode, ch, d_jobs, p, q, counters, s)
+                        : TypedLaunch<float, false>::batch(t, mode, ch, d_jobs, p, q, counters, s);
         case SPR_DTYPE_U16:
-            return pow2 ? dispatch_batch_typed<uint16_t, true>(c, mode, ch, d_jobs, p, q, counters, s)
-                        : dispatch_batch_typed<uint16_t, false>(c, mode, ch, d_jobs, p, q, counters, s);
+            return pow2 ? TypedLaunch<uint16_t, true>::batch(t, mode, ch, d_jobs, p, q, counters, s)
+                        : TypedLaunch<uint16_t, false>::batch(t, mode, ch, d_jobs, p, q, counters, s);
         default:
-            return pow2 ? dispatch_batch_typed<uint8_t, true>(c, mode, ch, d_jobs, p, q, counters, s)
-                        : dispatch_batch_typed<uint8_t, false>(c, mode, ch, d_jobs, p, q, counters, s);
+            return pow2 ? TypedLaunch<uint8_t, true>::batch(t, mode, ch, d_jobs, p, q, counters, s)
+                        : TypedLaunch<uint8_t, false>::batch(t, mode, ch, d_jobs, p, q, counters, s);
     }
 }
 
-int jobs_nmax(const std::vector<BatchJob>& jobs) {
-    int m = 1;
-    for (const BatchJob& j : jobs) m = std::max(m, std::max(j.nx, std::max(j.ny, j.nz)));
-    return m;
+// the (slot, generation) pairs a graph's jobs read (spr_batch_graph_launch checks them)
+void record_refs(const spr_ctx* c, const spr_job* jobs, int n,
+                 std::vector<std::pair<int, uint32_t>>& refs) {
+    refs.clear();
+    for (int i = 0; i < n; ++i) {
+        const std::pair<int, uint32_t> r(jobs[i].slot, c->vols[jobs[i].slot].generation);
+        if (std::find(refs.begin(), refs.end(), r) == refs.end()) refs.push_back(r);
+    }
 }
+
 
 }  // namespace
 
@@ -1062,7 +771,7 @@ int spr_create(int device, spr_ctx** out) {
     c->vols.reserve(kMaxSlots);
     c->vols.emplace_back();
     DeviceGuard g(device);
-    hipError_t e = hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+    hipError_t e = hipDeviceGetAttribute(&c->tune.num_cus, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_count, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&c->d_partial, 2 * 1024 * sizeof(float));
@@ -1114,30 +823,34 @@ int spr_set_tuning(spr_ctx* c, int key, int value) {
         case SPR_TUNE_TILE_MAP:
             if (value < 0 || value > (int)kTileMapCostXcd)
                 return fail(c, SPR_ERR_INVALID_ARG, "tile map %d (0 .. 3)", value);
-            c->tile_map = value;
+            c->tune.tile_map = value;
             return SPR_OK;
         case SPR_TUNE_SLAB_KB:
             if (value != 0 && value != 8 && value != 12 && value != 16 && value != 19)
                 return fail(c, SPR_ERR_INVALID_ARG, "slab size %d KiB (0, 8, 12, 16 or 19)", value);
-            c->slab_kb = value;
+            c->tune.slab_kb = value;
             return SPR_OK;
         case SPR_TUNE_STAGING:
             if (value != 0 && value != 1) return fail(c, SPR_ERR_INVALID_ARG, "staging %d (0 or 1)", value);
-            c->staging = value;
+            c->tune.staging = value;
             return SPR_OK;
         case SPR_TUNE_CHUNK:
             if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32 && value != 64)
                 return fail(c, SPR_ERR_INVALID_ARG, "chunk %d (0, 4, 8, 16, 32 or 64)", value);
-            c->chunk = value;
+            c->tune.chunk = value;
             return SPR_OK;
         case SPR_TUNE_ROW_PAD:
             if (value != 0 && value != 1) return fail(c, SPR_ERR_INVALID_ARG, "row pad %d (0 or 1)", value);
-            c->row_pad = value;
+            c->tune.row_pad = value;
             return SPR_OK;
         case SPR_TUNE_WG_WAVES:
             if (value != 0 && value != 1 && value != 4)
                 return fail(c, SPR_ERR_INVALID_ARG, "waves per workgroup %d (0, 1 or 4)", value);
-            c->wg_waves = value;
+            c->tune.wg_waves = value;
+            return SPR_OK;
+        case SPR_TUNE_SKIP:
+            if (value != 0 && value != 1) return fail(c, SPR_ERR_INVALID_ARG, "skip %d (0 or 1)", value);
+            c->tune.skip = value;
             return SPR_OK;
         default: return fail(c, SPR_ERR_INVALID_ARG, "unknown tuning key %d", key);
     }
@@ -1146,12 +859,13 @@ int spr_set_tuning(spr_ctx* c, int key, int value) {
 int spr_get_tuning(spr_ctx* c, int key, int* value) {
     if (!c || !value) return fail(c, SPR_ERR_INVALID_ARG, "NULL argument");
     switch (key) {
-        case SPR_TUNE_TILE_MAP: *value = c->tile_map; return SPR_OK;
-        case SPR_TUNE_SLAB_KB: *value = c->slab_kb; return SPR_OK;
-        case SPR_TUNE_STAGING: *value = c->staging; return SPR_OK;
-        case SPR_TUNE_CHUNK: *value = c->chunk; return SPR_OK;
-        case SPR_TUNE_ROW_PAD: *value = c->row_pad; return SPR_OK;
-        case SPR_TUNE_WG_WAVES: *value = c->wg_waves; return SPR_OK;
+        case SPR_TUNE_TILE_MAP: *value = c->tune.tile_map; return SPR_OK;
+        case SPR_TUNE_SLAB_KB: *value = c->tune.slab_kb; return SPR_OK;
+        case SPR_TUNE_STAGING: *value = c->tune.staging; return SPR_OK;
+        case SPR_TUNE_CHUNK: *value = c->tune.chunk; return SPR_OK;
+        case SPR_TUNE_ROW_PAD: *value = c->tune.row_pad; return SPR_OK;
+        case SPR_TUNE_WG_WAVES: *value = c->tune.wg_waves; return SPR_OK;
+        case SPR_TUNE_SKIP: *value = c->tune.skip; return SPR_OK;
         default: return fail(c, SPR_ERR_INVALID_ARG, "unknown tuning key %d", key);
     }
 }
@@ -1456,9 +1170,9 @@ int spr_render_batch(spr_ctx* c, const spr_params* params, const spr_job* jobs, 
     rc = check_parts(c, prm, num_parts);
     if (rc) return rc;
     std::vector<BatchJob> host;
-    int dtype;
+    int dtype, nmax;
     bool pow2;
-    rc = build_jobs(c, jobs, n, host, &dtype, &pow2);
+    rc = build_jobs(c, jobs, n, host, &dtype, &pow2, &nmax);
     if (rc) return rc;
     DeviceGuard g(c->device);
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
@@ -1470,13 +1184,85 @@ int spr_render_batch(spr_ctx* c, const spr_params* params, const spr_job* jobs, 
     BatchJob* d_jobs = (BatchJob*)buf;
     uint32_t* d_counters = (uint32_t*)((char*)buf + jbytes);
     hipError_t e = hipMemcpyAsync(d_jobs, host.data(), jbytes, hipMemcpyHostToDevice, s);
-    RayParams p = make_params(prm, c->tile_map, w, h);
-    staged_tuning(c, p);
+    RayParams p = make_params(prm, c->tune.tile_map, w, h);
+    staged_tuning(c->tune, p);
     p.nouter = (prm.max_steps / num_parts) / kLoopUnroll;
     p.current_part = current_part;
     const BatchQueues q = make_queues(p, n);
     if (e == hipSuccess)
-        e = dispatch_batch(c, prm, dtype, pow2, jobs_nmax(host), d_jobs, p, q, d_counters, s);
+        e = dispatch_batch(c, prm, dtype, pow2, chunk_for(c->tune, p.nouter, nmax), d_jobs, p, q, d_counters, s);
+    (void)hipFreeAsync(buf, s);
+    HIPCALL(c, e);
+    // blocking: the host job table must outlive the transfer
+    HIPCALL(c, hipStreamSynchronize(s));
+    return SPR_OK;
+}
+
+int spr_render_frames(spr_ctx* c, const spr_frame* frames, int n, float alpha_pow, int filter,
+                      void* stream) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (!frames || n <= 0) return fail(c, SPR_ERR_INVALID_ARG, "empty batch");
+    if (filter != SPR_FILTER_LINEAR && filter != SPR_FILTER_NEAREST)
+        return fail(c, SPR_ERR_INVALID_ARG, "unknown filter %d", filter);
+    std::vector<spr_job> jobs((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const spr_frame& f = frames[i];
+        if (f.w <= 0 || f.h <= 0 || f.w > 32768 || f.h > 32768)
+            return fail(c, SPR_ERR_INVALID_ARG, "frame %d: invalid output size %d x %d", i, f.w, f.h);
+        if (f.max_steps < kLoopUnroll)
+            return fail(c, SPR_ERR_INVALID_ARG, "frame %d: max_steps %d leaves no 16-sample block",
+                        i, f.max_steps);
+        jobs[i].slot = f.slot;
+        memcpy(jobs[i].inv_p, f.inv_p, sizeof(f.inv_p));
+        memcpy(jobs[i].inv_m, f.inv_m, sizeof(f.inv_m));
+        jobs[i].d_out = f.d_out;
+        jobs[i].d_alpha = f.d_alpha;
+    }
+    std::vector<BatchJob> host;
+    int dtype, nmax;
+    bool pow2;
+    int rc = build_jobs(c, jobs.data(), n, host, &dtype, &pow2, &nmax);
+    if (rc) return rc;
+    // each frame's own settings and its first item in each queue; the box length follows
+    // the frame with the longest ray step
+    BatchQueues q{};
+    int ch = kLoopUnroll;
+    for (int i = 0; i < n; ++i) {
+        const spr_frame& f = frames[i];
+        BatchJob& b = host[i];
+        memcpy(b.box, f.box, sizeof(b.box));
+        b.min_val = f.min_val;
+        b.max_val = f.max_val;
+        b.gamma = f.gamma;
+        b.nouter = f.max_steps / kLoopUnroll;
+        b.W = (uint32_t)f.w;
+        b.H = (uint32_t)f.h;
+        const uint32_t ntx = (b.W + 15u) / 16u, nty = (b.H + 15u) / 16u;
+        for (uint32_t k = 0; k < 8; ++k) {
+            b.first[k] = q.count[k];
+            q.count[k] += queue_rows(nty, k) * ntx;
+        }
+        ch = std::min(ch, chunk_for(c->tune, b.nouter, std::max(b.nx, std::max(b.ny, b.nz))));
+    }
+    q.mixed = 1;
+    q.njobs = (uint32_t)n;
+    spr_params prm{};
+    prm.alpha_pow = alpha_pow;
+    prm.filter = filter;
+    prm.max_steps = frames[0].max_steps;
+    DeviceGuard g(c->device);
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    RayParams p = make_params(prm, c->tune.tile_map, frames[0].w, frames[0].h);
+    staged_tuning(c->tune, p);
+    p.nouter = host[0].nouter;
+    p.current_part = 0;
+    const size_t jbytes = (size_t)n * sizeof(BatchJob);
+    void* buf = nullptr;
+    HIPCALL(c, hipMallocAsync(&buf, jbytes + 64, s));
+    BatchJob* d_jobs = (BatchJob*)buf;
+    uint32_t* d_counters = (uint32_t*)((char*)buf + jbytes);
+    hipError_t e = hipMemcpyAsync(d_jobs, host.data(), jbytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = dispatch_batch(c, prm, dtype, pow2, ch, d_jobs, p, q, d_counters, s);
     (void)hipFreeAsync(buf, s);
     HIPCALL(c, e);
     // blocking: the host job table must outlive the transfer
@@ -1503,39 +1289,27 @@ int spr_render_coarse_pair(spr_ctx* c, int slot_a, int slot_b, const spr_params*
     if (!d_out || !d_alpha) return fail(c, SPR_ERR_INVALID_ARG, "NULL output");
     DeviceGuard g(c->device);
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-    RayParams p = make_params(*params, c->tile_map, w, h);
-    staged_tuning(c, p);
+    RayParams p = make_params(*params, c->tune.tile_map, w, h);
+    staged_tuning(c->tune, p);
     p.nouter = params->max_steps / kLoopUnroll;
     p.current_part = 0;
-    const dim3 grid((unsigned)(((w + 7) / 8) * ((h + 7) / 8)), 2), block(256);
     const bool p2 = slot_pow2(*a);
+    hipError_t e;
     switch (a->dtype) {
         case SPR_DTYPE_F32:
-            if (p2)
-                hipLaunchKernelGGL((k_coarse_mip_pair<float, true, 12288, 4>), grid, block, 0, s,
-                                   padded_volume<float, true>(*a), padded_volume<float, true>(*b), p, d_out, d_alpha);
-            else
-                hipLaunchKernelGGL((k_coarse_mip_pair<float, false, 12288, 4>), grid, block, 0, s,
-                                   padded_volume<float, false>(*a), padded_volume<float, false>(*b), p, d_out, d_alpha);
+            e = p2 ? TypedLaunch<float, true>::coarse_pair(*a, *b, p, d_out, d_alpha, s)
+                   : TypedLaunch<float, false>::coarse_pair(*a, *b, p, d_out, d_alpha, s);
             break;
         case SPR_DTYPE_U16:
-            if (p2)
-                hipLaunchKernelGGL((k_coarse_mip_pair<uint16_t, true, 12288, 4>), grid, block, 0, s,
-                                   padded_volume<uint16_t, true>(*a), padded_volume<uint16_t, true>(*b), p, d_out, d_alpha);
-            else
-                hipLaunchKernelGGL((k_coarse_mip_pair<uint16_t, false, 12288, 4>), grid, block, 0, s,
-                                   padded_volume<uint16_t, false>(*a), padded_volume<uint16_t, false>(*b), p, d_out, d_alpha);
+            e = p2 ? TypedLaunch<uint16_t, true>::coarse_pair(*a, *b, p, d_out, d_alpha, s)
+                   : TypedLaunch<uint16_t, false>::coarse_pair(*a, *b, p, d_out, d_alpha, s);
             break;
         default:
-            if (p2)
-                hipLaunchKernelGGL((k_coarse_mip_pair<uint8_t, true, 12288, 4>), grid, block, 0, s,
-                                   padded_volume<uint8_t, true>(*a), padded_volume<uint8_t, true>(*b), p, d_out, d_alpha);
-            else
-                hipLaunchKernelGGL((k_coarse_mip_pair<uint8_t, false, 12288, 4>), grid, block, 0, s,
-                                   padded_volume<uint8_t, false>(*a), padded_volume<uint8_t, false>(*b), p, d_out, d_alpha);
+            e = p2 ? TypedLaunch<uint8_t, true>::coarse_pair(*a, *b, p, d_out, d_alpha, s)
+                   : TypedLaunch<uint8_t, false>::coarse_pair(*a, *b, p, d_out, d_alpha, s);
             break;
     }
-    HIPCALL(c, hipGetLastError());
+    HIPCALL(c, e);
     return SPR_OK;
 }
 
@@ -1560,18 +1334,12 @@ int spr_render_pair(spr_ctx* c, int slot, const spr_params* params, int num_part
     if (!d_out || !d_alpha) return fail(c, SPR_ERR_INVALID_ARG, "NULL output");
     DeviceGuard g(c->device);
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-    RayParams p = make_params(*params, c->tile_map, w, h);
-    staged_tuning(c, p);
+    RayParams p = make_params(*params, c->tune.tile_map, w, h);
+    staged_tuning(c->tune, p);
     p.nouter = (params->max_steps / num_parts) / kLoopUnroll;
     p.current_part = current_part;
-    const dim3 grid(tile_grid(p.tile_map, p.ntx, p.nty));
-    const int nmax = std::max(vs->nx, std::max(vs->ny, vs->nz));
-    const int ch = chunk_for(c, p.nouter, nmax);
-    if (slot_pow2(*vs))
-        launch_pair_ch<true>(*vs, ch, grid, s, p, d_out, d_alpha);
-    else
-        launch_pair_ch<false>(*vs, ch, grid, s, p, d_out, d_alpha);
-    HIPCALL(c, hipGetLastError());
+    HIPCALL(c, slot_pow2(*vs) ? launch_pair<true>(c->tune, *vs, p, d_out, d_alpha, s)
+                              : launch_pair<false>(c->tune, *vs, p, d_out, d_alpha, s));
     return SPR_OK;
 }
 
@@ -1583,9 +1351,9 @@ int spr_batch_graph_create(spr_ctx* c, const spr_job* jobs, int n, int w, int h,
     int rc = check_frame(c, w, h);
     if (rc) return rc;
     std::vector<BatchJob> host;
-    int dtype;
+    int dtype, nmax;
     bool pow2;
-    rc = build_jobs(c, jobs, n, host, &dtype, &pow2);
+    rc = build_jobs(c, jobs, n, host, &dtype, &pow2, &nmax);
     if (rc) return rc;
     DeviceGuard g(c->device);
     spr_batch_graph* bg = new (std::nothrow) spr_batch_graph();
@@ -1594,6 +1362,8 @@ int spr_batch_graph_create(spr_ctx* c, const spr_job* jobs, int n, int w, int h,
     bg->w = w;
     bg->h = h;
     bg->dtype = dtype;
+    bg->pow2 = pow2;
+    record_refs(c, jobs, n, bg->refs);
     auto cleanup = [&](int code) {
         spr_batch_graph_destroy(c, bg);
         return code;
@@ -1607,7 +1377,7 @@ int spr_batch_graph_create(spr_ctx* c, const spr_job* jobs, int n, int w, int h,
     rc = check_parts(c, c->params, 1);
     if (rc) return cleanup(rc);
     RayParams p = make_params(c, w, h);
-    staged_tuning(c, p);
+    staged_tuning(c->tune, p);
     p.nouter = c->params.max_steps / kLoopUnroll;
     p.current_part = 0;
     const BatchQueues q = make_queues(p, n);
@@ -1618,7 +1388,7 @@ int spr_batch_graph_create(spr_ctx* c, const spr_job* jobs, int n, int w, int h,
     hipError_t e2 = hipSuccess;
     if (e == hipSuccess) {
         // the box length follows the captured jobs' volumes (speed only, never results)
-        e2 = dispatch_batch(c, c->params, dtype, pow2, jobs_nmax(host), bg->d_jobs, p, q,
+        e2 = dispatch_batch(c, c->params, dtype, pow2, chunk_for(c->tune, p.nouter, nmax), bg->d_jobs, p, q,
                             bg->d_counters, cs);
         e = hipStreamEndCapture(cs, &bg->graph);
     }
@@ -1637,22 +1407,37 @@ int spr_batch_graph_update(spr_ctx* c, spr_batch_graph* bg, const spr_job* jobs,
     if (!bg) return fail(c, SPR_ERR_INVALID_ARG, "NULL graph");
     if (n != bg->n) return fail(c, SPR_ERR_INVALID_ARG, "graph holds %d jobs, got %d", bg->n, n);
     std::vector<BatchJob> host;
-    int dtype;
+    int dtype, nmax;
     bool pow2;
-    int rc = build_jobs(c, jobs, n, host, &dtype, &pow2);
+    int rc = build_jobs(c, jobs, n, host, &dtype, &pow2, &nmax);
     if (rc) return rc;
     if (dtype != bg->dtype)
         return fail(c, SPR_ERR_UNSUPPORTED_DTYPE, "voxel type differs from the captured batch");
+    // the captured kernel's texel rule (one fma for power-of-two dimensions, two roundings
+    // otherwise) must match every new volume, or its samples would differ from single renders
+    if (pow2 != bg->pow2)
+        return fail(c, SPR_ERR_INVALID_ARG,
+                    "the captured batch's volumes %s power-of-two dimensions and the new ones %s "
+                    "(capture a new graph)", bg->pow2 ? "all have" : "do not all have",
+                    pow2 ? "do" : "do not");
     DeviceGuard g(c->device);
     if (bg->last_stream) HIPCALL(c, hipStreamSynchronize(bg->last_stream));
     HIPCALL(c, hipMemcpy(bg->d_jobs, host.data(), (size_t)n * sizeof(BatchJob),
                          hipMemcpyHostToDevice));
+    record_refs(c, jobs, n, bg->refs);
     return SPR_OK;
 }
 
 int spr_batch_graph_launch(spr_ctx* c, spr_batch_graph* bg, void* stream) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
     if (!bg || !bg->exec) return fail(c, SPR_ERR_INVALID_ARG, "NULL graph");
+    for (const auto& r : bg->refs) {
+        const VolSlot* vs = slot_of(c, r.first);
+        if (!vs || vs->generation != r.second)
+            return fail(c, SPR_ERR_INVALID_ARG,
+                        "slot %d was released or refilled since the graph's jobs were set "
+                        "(spr_batch_graph_update or a new graph)", r.first);
+    }
     DeviceGuard g(c->device);
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     HIPCALL(c, hipGraphLaunch(bg->exec, s));
@@ -1695,7 +1480,7 @@ int spr_count_samples(spr_ctx* c, int num_parts, int w, int h, double* samples) 
     if (rc) return rc;
     DeviceGuard g(c->device);
     RayParams p = make_params(c, w, h);
-    staged_tuning(c, p);
+    staged_tuning(c->tune, p);
     HIPCALL(c, hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_count_hits, dim3(tile_grid(p.tile_map, p.ntx, p.nty)), dim3(256), 0,
                        c->stream, p, c->d_count);

@@ -1,6 +1,6 @@
 // volume_sampler.h -- software trilinear / nearest sampling of a resident volume.
 //
-// Semantics of the reference's image sampler (volume_kernel.cl:42-43, utils.cl:104):
+// Semantics of the reference's image sampler (volume_kernel.cl:42-43, utils.cl:75):
 // CLK_NORMALIZED_COORDS_TRUE | CLK_ADDRESS_CLAMP_TO_EDGE | CLK_FILTER_{LINEAR,NEAREST},
 // OpenCL 1.2 section 8.2.  LINEAR: u = s*N, u' = u - 0.5 (two float32 roundings),
 // i0 = floor(u'), a = u' - i0, corners clamped to [0, N-1], then lerp along x, y, z.
@@ -8,6 +8,7 @@
 // Hardware texture filtering is not used: its reduced-precision weights would not
 // reproduce these float32 weights.
 #pragma once
+#include "brick_bounds.h"
 #include "ray_math.h"
 
 namespace spr {
@@ -178,6 +179,8 @@ struct PaddedVolume {
     int nx, ny, nz;
     float fnx, fny, fnz;           // N
     float mx, my, mz;              // N - 1
+    BrickGrid bricks;              // per-brick bounds (bricks.keys == nullptr: none)
+    float vmin, vmax;              // the volume's minimum and maximum voxel
 
     // raw buffer resource over the padded volume (uniform: lives in SGPRs)
     __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc() const {

@@ -7,14 +7,17 @@ libspimrender context:
 
 - admission: a volume is uploaded into a slot on first use if it fits the HBM budget;
   otherwise least-recently-used volumes that no queued or running request needs are
-  released first; a request whose volume cannot be admitted waits in the queue;
+  released first; a request whose volume cannot be admitted waits in the queue.  The
+  scheduler reserves the budget under its lock and the worker uploads (and releases the
+  evicted volumes) outside it, so other workers keep taking batches meanwhile;
 - continuous batching: each worker thread (own HIP stream) repeatedly takes the oldest
-  waiting request plus every other waiting request with the same batch key (voxel type,
-  frame size, method and all parameters except the pose) up to ``max_batch``, and renders
-  max projections of padded-layout volumes in one ``spr_render_batch`` launch
-  (iso-surface requests and volumes too large for the padded layout are rendered one by
-  one with ``spr_render_slot``).  Requests that arrive while batches run are picked up by
-  the next free worker, so work from several batches overlaps on the device;
+  request whose volume is resident plus every other waiting request with the same batch
+  key -- voxel type, method, alpha_pow and filter; frame size, pose, box, transfer range,
+  gamma and step count may all differ -- whose volume is resident, up to ``max_batch``,
+  and renders max projections of padded-layout volumes in one ``spr_render_frames``
+  launch (iso-surface requests and volumes above 4 GiB padded are rendered one by one with
+  ``spr_render_slot``).  Requests that arrive while batches run are picked up by the next
+  free worker, so work from several batches overlaps on the device;
 - every request records its queueing and end-to-end latency.
 
 Every result is bit-identical to rendering the request alone (tested): batching changes
@@ -71,7 +74,9 @@ class _Volume(object):
         self.bytes = 0
         self.users = 0          # queued + running requests
         self.last_use = 0.
-        self.padded = None
+        self.loading = False    # an upload (admitted, budget reserved) is under way
+        # the padded layout alone (batchable); above 4 GiB a slot is linear + a padded copy
+        self.padded = padded_bytes(data.shape, data.dtype) <= 0xFFFFFFFF
 
 
 class RenderServer(object):
@@ -149,59 +154,106 @@ class RenderServer(object):
         self.ctx.close()
 
     # ------------------------------------------------------------------ residency
-    def _try_admit(self, vol):
-        """Make `vol` resident if the budget allows (evicting idle volumes). Caller holds cv."""
+    def _reserve(self, vol):
+        """Admission of `vol` (caller holds cv): "ready" when resident, "wait" when it is
+        being uploaded or cannot be admitted now, else the list of idle volumes to evict
+        for it -- its budget is then reserved and the caller must run _load(vol, victims)
+        outside the lock.  MemoryError when it can never fit."""
         if vol.slot is not None:
-            return True
+            return "ready"
+        if vol.loading:
+            return "wait"
         need = device_bytes(vol.data.shape, vol.data.dtype)
         if need > self.hbm_budget:
             raise MemoryError("volume %r larger than the HBM budget" % vol.name)
         idle = sorted((v for v in self.volumes.values() if v.slot is not None and v.users == 0
-                       and v is not vol), key=lambda v: v.last_use)
-        while self.resident_bytes + need > self.hbm_budget and idle:
-            self._evict(idle.pop(0))
-        if self.resident_bytes + need > self.hbm_budget:
+                       and not v.loading and v is not vol), key=lambda v: v.last_use)
+        victims = []
+        free = self.hbm_budget - self.resident_bytes
+        while free < need and idle:
+            v = idle.pop(0)
+            victims.append((v.slot, v.bytes))
+            free += v.bytes
+        if free < need:
+            return "wait"
+        for v in self.volumes.values():
+            if any(v.slot == slot for slot, _ in victims):
+                self.resident_bytes -= v.bytes
+                v.slot = None
+                v.bytes = 0
+        vol.loading = True
+        self.resident_bytes += need
+        return victims
+
+    def _load(self, vol, victims):
+        """Releases the evicted slots and uploads `vol` (no scheduler lock held)."""
+        need = device_bytes(vol.data.shape, vol.data.dtype)
+        try:
+            with self.table_lock:
+                for slot, _ in victims:
+                    self.ctx.volume_release(slot)
+                slot = self.ctx.volume_add(vol.data)
+                info = self.ctx.volume_info(slot)
+        except Exception as e:
+            with self.cv:
+                self.resident_bytes -= need
+                vol.loading = False
+                self._fail_volume(vol, e)
+                self.cv.notify_all()
             return False
-        with self.table_lock:
-            vol.slot = self.ctx.volume_add(vol.data)
-            info = self.ctx.volume_info(vol.slot)
-        vol.bytes = info["bytes"]
-        # the padded layout alone (batchable); above 4 GiB a slot is linear + a padded copy
-        vol.padded = vol.bytes == padded_bytes(vol.data.shape, vol.data.dtype)
-        self.resident_bytes += vol.bytes
+        with self.cv:
+            vol.slot = slot
+            vol.bytes = info["bytes"]
+            self.resident_bytes += vol.bytes - need
+            vol.loading = False
+            self.cv.notify_all()
         return True
 
-    def _evict(self, vol):
-        with self.table_lock:
-            self.ctx.volume_release(vol.slot)
-        self.resident_bytes -= vol.bytes
-        vol.slot = None
-        vol.bytes = 0
+    def _fail_volume(self, vol, err):
+        """Every waiting request of `vol` fails with err (caller holds cv)."""
+        for item in [it for it in self.waiting if self.volumes[it[0].volume] is vol]:
+            self.waiting.remove(item)
+            vol.users -= 1
+            item[1].set_exception(err)
 
     def _admit_blocking(self, vol):
-        with self.cv:
-            while not self._try_admit(vol):
-                self.cv.wait(.01)
+        while True:
+            with self.cv:
+                state = self._reserve(vol)
+                if state == "ready":
+                    return
+                if state == "wait":
+                    self.cv.wait(.01)
+                    continue
+            if not self._load(vol, state):
+                raise RuntimeError("upload of volume %r failed" % vol.name)
 
     # ------------------------------------------------------------------ scheduling
     def _key(self, req, vol):
+        """Requests with equal keys share a launch: voxel type, method, alpha_pow and filter
+        (frame size, pose, box, transfer range, gamma and steps are per frame)."""
         s = req.settings
         batchable = req.method == "max_project" and vol.padded
-        return (batchable, vol.data.dtype.str, req.size, req.method, s["min_val"], s["max_val"],
-                s["gamma"], s["alpha_pow"], s["max_steps"], s["interpolation"], s["box"])
+        return (batchable, vol.data.dtype.str, req.method, s["alpha_pow"], s["interpolation"])
 
-    def _take_batch(self):
-        """Oldest admissible request + same-key followers. Caller holds cv."""
+    def _next_task(self):
+        """The next piece of work (caller holds cv): ("batch", [(req, future), ...]) -- the
+        oldest request whose volume is resident plus same-key followers whose volumes are
+        resident -- or ("load", vol, victims) for the oldest request whose volume can be
+        admitted, or None."""
         for (req, fut) in list(self.waiting):
             vol = self.volumes[req.volume]
             try:
-                if not self._try_admit(vol):
-                    continue
-            except Exception as e:  # can never be admitted (or the upload failed)
+                state = self._reserve(vol)
+            except Exception as e:  # can never be admitted
                 self.waiting.remove((req, fut))
                 vol.users -= 1
                 fut.set_exception(e)
                 continue
+            if state == "wait":
+                continue
+            if state != "ready":
+                return ("load", vol, state)
             key = self._key(req, vol)
             batch = [(req, fut)]
             if key[0]:
@@ -211,31 +263,45 @@ class RenderServer(object):
                     if r2 is req:
                         continue
                     v2 = self.volumes[r2.volume]
-                    if v2.slot is None and not self._try_admit(v2):
-                        continue
-                    if self._key(r2, v2) == key:
+                    if v2.slot is not None and self._key(r2, v2) == key:
                         batch.append((r2, f2))
             for item in batch:
                 self.waiting.remove(item)
             now = time.perf_counter()
             for r, _ in batch:
                 self.volumes[r.volume].last_use = now
-            return batch
+            return ("batch", batch)
         return None
+
+    def _take_batch(self):
+        """The next batch, uploading volumes on the way (drives the scheduler without worker
+        threads); None when nothing can run now."""
+        while True:
+            with self.cv:
+                task = self._next_task()
+            if task is None:
+                return None
+            if task[0] == "batch":
+                return task[1]
+            self._load(task[1], task[2])
 
     def _worker(self):
         torch = self.torch
         stream = torch.cuda.Stream()
         while True:
             with self.cv:
-                batch = None
+                task = None
                 while not self.stop:
-                    batch = self._take_batch()
-                    if batch:
+                    task = self._next_task()
+                    if task:
                         break
                     self.cv.wait(.005)
-                if self.stop and not batch:
+                if self.stop and not task:
                     return
+            if task[0] == "load":
+                self._load(task[1], task[2])
+                continue
+            batch = task[1]
             try:
                 self._run(batch, stream)
             except Exception as e:  # deliver the error to every request of the batch
@@ -290,18 +356,21 @@ class RenderServer(object):
         if len(batch) == 1 or not self._key(req0, vol0)[0]:
             results = [self._render_single(r, self.volumes[r.volume], stream) for r, _ in batch]
         else:
-            w, h = req0.size
             with torch.cuda.stream(stream):
-                planes = torch.empty((len(batch), 2, h, w), device="cuda")
-                jobs = []
-                for k, (r, _) in enumerate(batch):
-                    p = self._params(r, self.volumes[r.volume])
-                    jobs.append((self.volumes[r.volume].slot, list(p.inv_p), list(p.inv_m),
-                                 planes[k, 0].data_ptr(), planes[k, 1].data_ptr()))
-                self.ctx.render_batch(jobs, w, h, stream=stream.cuda_stream,
-                                      params=self._params(req0, vol0))
-                host = planes.cpu().numpy()
-            results = [Result(host[k, 0], host[k, 1]) for k in range(len(batch))]
+                planes, frames = [], []
+                for r, _ in batch:
+                    w, h = r.size
+                    pl = torch.empty((2, h, w), device="cuda")
+                    planes.append(pl)
+                    vol = self.volumes[r.volume]
+                    frames.append(_lib.make_frame(vol.slot, self._params(r, vol), w, h,
+                                                  pl[0].data_ptr(), pl[1].data_ptr()))
+                filt = (_lib.FILTER_NEAREST if req0.settings["interpolation"] == "nearest"
+                        else _lib.FILTER_LINEAR)
+                self.ctx.render_frames(frames, req0.settings["alpha_pow"], filt,
+                                       stream=stream.cuda_stream)
+                host = [pl.cpu().numpy() for pl in planes]
+            results = [Result(hp[0], hp[1]) for hp in host]
         t1 = time.perf_counter()
         for (r, fut), res in zip(batch, results):
             res.queued_s = t0 - r.t_submit

@@ -32,6 +32,16 @@ CONFIGS = {
 }
 
 
+# the serving workload's resident volumes: name, numpy shape (z, y, x), voxel type, blobs
+SERVING_VOLUMES = [
+    ("f32_256", (256, 256, 256), np.float32, 500),
+    ("u8_256x512x512", (256, 512, 512), np.uint8, 1000),
+    ("u16_512", (512, 512, 512), np.uint16, 2000),
+    ("u16_512x1024x1024", (512, 1024, 1024), np.uint16, 8000),
+    ("u16_512x2048x2048", (512, 2048, 2048), np.uint16, 16000),
+]
+
+
 def steps_for_shape(shape):
     """max_steps = 16 * ceil(|(Nx,Ny,Nz)| / 16): every ray steps at most one voxel."""
     return 16 * int(math.ceil(math.sqrt(sum(n * n for n in shape)) / 16.))
@@ -162,6 +172,27 @@ def blob_series_device(shape, n_blobs, sigma, amp, background, poisson, read_noi
         v = torch.clamp(torch.round(vol), 0, 65535).to(torch.int32)
         del vol
         yield torch.where(v >= 32768, v - 65536, v).to(torch.int16)
+
+
+def serving_volume(shape, dtype, n_blobs, seed):
+    """Host (numpy) blob volume of the serving workload, built on the GPU: sigma U[3,10],
+    amplitude U[0.3,1] (float32), U[60,200] (uint8) or U[500,3000] (uint16), background
+    amplitude/10 (uint16: 100), no noise."""
+    import torch
+    dt = np.dtype(dtype)
+    amp = (.3, 1.) if dt == np.float32 else (60., 200.) if dt == np.uint8 else (500., 3000.)
+    bg = amp[0] / 10 if dt != np.uint16 else 100.
+    _, c, s, a = blob_params(shape, n_blobs, (3., 10.), amp, seed)
+    vol = torch.full(shape, bg, dtype=torch.float32, device="cuda")
+    _splat_blobs_device(vol, c, s, a)
+    if dt == np.float32:
+        return vol.cpu().numpy()
+    top = 255 if dt == np.uint8 else 65535
+    v = torch.clamp(torch.round(vol), 0, top).to(torch.int32)
+    del vol
+    if dt == np.uint8:
+        return v.to(torch.uint8).cpu().numpy()
+    return torch.where(v >= 32768, v - 65536, v).to(torch.int16).cpu().numpy().view(np.uint16)
 
 
 def device_volume_to_numpy(t, dtype):

@@ -22,6 +22,13 @@ class Quaternion(object):
         self.data[i] = val
 
     def conj(self):
+        """As quaternion.py:24 returns it: (w, -x, -y, -y) -- the last component repeats
+        -y, not -z.  Kept for drop-in parity (nothing in the reference calls it);
+        conjugate() is the mathematical conjugate."""
+        return Quaternion(self.data[0], -self.data[1], -self.data[2], -self.data[2])
+
+    def conjugate(self):
+        """(w, -x, -y, -z)."""
         return Quaternion(self.data[0], -self.data[1], -self.data[2], -self.data[3])
 
     def norm(self):

@@ -139,3 +139,61 @@ def test_batch_errors(torch):
     ctx.volume_release(s8)
     with pytest.raises(_lib.SprError):
         ctx.volume_info(s8)
+
+
+def test_graph_refuses_released_or_refilled_slots(torch):
+    """A captured batch records (slot, generation) per job: after its slot is released or
+    reused, spr_batch_graph_launch returns an error before launching anything; an update
+    to live slots makes it launchable again."""
+    W, H = 48, 40
+    ctx, vols, frames, filt = setup(torch, np.uint16, [(24, 32, 28), (20, 20, 20)], 4, W, H, seed=5)
+    jobs = [(f["slot"], f["iP"], f["iM"], f["out"].data_ptr(), f["alpha"].data_ptr()) for f in frames]
+    g = ctx.batch_graph(jobs, W, H)
+    g.launch()
+    ctx.synchronize()
+    gone = frames[1]["slot"]
+    ctx.volume_release(gone)
+    with pytest.raises(_lib.SprError, match="released or refilled"):
+        g.launch()
+    # the slot is reused by a new volume of another shape: still refused
+    again = ctx.volume_add(vols[0])
+    assert again == gone
+    with pytest.raises(_lib.SprError, match="released or refilled"):
+        g.launch()
+    # rewrite the jobs against live slots (the reused slot now holds vols[0])
+    for f in frames:
+        if f["slot"] == gone:
+            f["vol"] = 0
+            nz, ny, nx = vols[0].shape
+            P = mat4_perspective(50, W / float(H), .1, 10)
+            f["iP"], f["iM"] = render_matrices(poses(1, 7)[0], P, (nx, ny, nz))
+    g.update([(f["slot"], f["iP"], f["iM"], f["out"].data_ptr(), f["alpha"].data_ptr()) for f in frames])
+    s = torch.cuda.Stream()
+    g.launch(s.cuda_stream)
+    s.synchronize()
+    for f in frames:
+        o, a = single(torch, vols, f, W, H, 0., filt)
+        np.testing.assert_array_equal(f["out"].cpu().numpy(), o)
+        np.testing.assert_array_equal(f["alpha"].cpu().numpy(), a)
+    g.close()
+
+
+def test_graph_update_keeps_the_texel_rule(torch):
+    """A graph captured on power-of-two volumes runs the one-fma texel rule; an update to a
+    volume with other dimensions is refused (its samples would differ from single renders)."""
+    W, H = 40, 32
+    ctx, vols, frames, filt = setup(torch, np.uint8, [(32, 32, 32)], 2, W, H, seed=8)
+    jobs = [(f["slot"], f["iP"], f["iM"], f["out"].data_ptr(), f["alpha"].data_ptr()) for f in frames]
+    g = ctx.batch_graph(jobs, W, H)
+    odd = ctx.volume_add(synth.blob_volume((24, 40, 48), n_blobs=4, sigma=(2., 4.), amp=(60., 200.),
+                                           background=6., poisson=False, read_noise=0.,
+                                           dtype=np.uint8, seed=2))
+    with pytest.raises(_lib.SprError, match="power-of-two"):
+        g.update([(odd, f["iP"], f["iM"], f["out"].data_ptr(), f["alpha"].data_ptr())
+                  for f in frames])
+    g.launch()
+    ctx.synchronize()
+    for f in frames:
+        o, a = single(torch, vols, f, W, H, 0., filt)
+        np.testing.assert_array_equal(f["out"].cpu().numpy(), o)
+    g.close()

@@ -51,7 +51,9 @@ def test_rotation_is_orthonormal_and_right_handed():
 
 def test_quaternion_algebra_and_slerp():
     q = Quaternion(np.cos(.3), 0, np.sin(.3), 0)
-    assert np.allclose((q * q.conj()).data, [1, 0, 0, 0])
+    assert np.allclose((q * q.conjugate()).data, [1, 0, 0, 0])
+    # quaternion.py:24 as written: the last component repeats -y
+    assert np.array_equal(q.conj().data, [q.data[0], -q.data[1], -q.data[2], -q.data[2]])
     assert np.allclose((q * q).toRotation3(), mat4_rotation(1.2, 0, 1, 0)[:3, :3])
     a, b = Quaternion(1, 0, 0, 0), Quaternion(np.cos(.5), np.sin(.5), 0, 0)
     mid = quaternion_slerp(a, b, .5)
@@ -105,3 +107,24 @@ def test_blob_volume_recipe():
     assert np.array_equal(vol, again)
     f = synth.config_volume("parity")
     assert f.dtype == np.float32 and f.shape == (128, 128, 128)
+
+
+def test_downsampling_rule():
+    """volumerender.py:199-212 (Q16): a volume above the memory budget is rendered from
+    every Nstep-th voxel per axis, Nstep = ceil((nbytes / memMax)^(1/3)) in float64."""
+    from spimagine_amd.volumerender.volumerender import VolumeRenderer
+
+    class Budget(object):
+        _get_downsampled_data_slices = VolumeRenderer._get_downsampled_data_slices
+
+        def __init__(self, mem_max):
+            self.memMax = mem_max
+
+    data = np.zeros((30, 40, 50), np.uint16)      # 120000 bytes
+    assert Budget(data.nbytes)._get_downsampled_data_slices(data) is None
+    assert Budget(data.nbytes * 10)._get_downsampled_data_slices(data) is None
+    for ratio, step in ((1.001, 2), (8., 2), (8.01, 3), (26.5, 3), (27., 3), (27.01, 4), (64., 4), (125., 5)):
+        sl = Budget(data.nbytes / ratio)._get_downsampled_data_slices(data)
+        assert sl is not None and all(s.step == step for s in sl), (ratio, step, sl)
+        assert [s.start for s in sl] == [0, 0, 0] and [s.stop for s in sl] == list(data.shape)
+        assert data[sl].shape == tuple(-(-n // step) for n in data.shape)

@@ -11,6 +11,7 @@ import pytest
 from scipy.ndimage import correlate1d, map_coordinates
 
 import oracle
+from spimagine_amd import synth
 from spimagine_amd.utils import (mat4_identity, mat4_ortho, mat4_perspective, mat4_rotation,
                                  mat4_translate)
 from spimagine_amd.volumerender.volumerender import render_matrices
@@ -461,3 +462,86 @@ def test_iso_pipeline_order(oracle_lib):
     np.testing.assert_array_equal(occ, oc)
     np.testing.assert_array_equal(out, oracle.shading(n2, d1, oc, invP, invM, occ_strength=.3))
     assert np.isfinite(depth).sum() > 100
+
+
+@pytest.mark.parametrize("rows", [(0, 9), (41, 58), (110, 120), (5, 6)])
+def test_iso_rows_pipeline_is_the_full_pipeline(oracle_lib, rows):
+    """render_iso_rows (the band form the full-size GPU tests compare against) equals
+    render_iso on its rows: it marches only the rows within max(3, occ_radius + 3) of the
+    band, so a margin too small (or a wrong row offset) would change the blurred normals,
+    the occlusion or the shading of the band's edge rows."""
+    vol = synth.blob_volume((40, 48, 44), n_blobs=12, sigma=(3., 6.), amp=(500., 3000.),
+                            background=100., poisson=True, read_noise=5., dtype="uint16", seed=3)
+    W, H = 90, 120
+    M = np.dot(mat4_translate(0, 0, -3.2), mat4_rotation(.7, .3, 1, -.2))
+    invP, invM = render_matrices(M, mat4_perspective(45, W / H, .1, 10), vol.shape[::-1])
+    lo, hi = float(vol.min()), float(vol.max())
+    kw = dict(iso_val=np.float32(lo + .3 * (hi - lo)), max_steps=120, occ_strength=.3,
+              occ_radius=11, occ_n_points=20, width=W, height=H)
+    full = oracle.render_iso(vol, invP, invM, **kw)
+    band = oracle.render_iso_rows(vol, invP, invM, rows, **kw)
+    r0, r1 = rows
+    for a, b in zip(full, band):
+        np.testing.assert_array_equal(a[r0:r1], b[r0:r1])
+    assert np.isfinite(full[2]).sum() > 1000
+
+
+# ------------------------------------------------------------------ gamma != 1
+
+@pytest.mark.parametrize("gamma", [.5, 1.7, 2.])
+def test_mip_gamma_closed_form(oracle_lib, gamma):
+    """volume_kernel.cl:162: out = clamp(pow((v - min)/(max - min), gamma), 0, 1).  On the
+    rend_const volume (123, maxVal 200) every pixel is (123/200)^gamma (float64 closed
+    form); gamma 1 would give 0.615 and an exponent of gamma^2 or 1/gamma other values."""
+    data = (123 * np.ones((40, 40, 40))).astype(np.uint16)
+    invP, invM = render_matrices(mat4_identity(), mat4_perspective(), (40, 40, 40))
+    out, _ = oracle.max_project(data, invP, invM, max_val=200., gamma=gamma, width=6, height=5)
+    want = (123. / 200.) ** gamma
+    np.testing.assert_allclose(out, want, rtol=3e-7)
+    for wrong in ((123. / 200.) ** (gamma * gamma), (123. / 200.) ** (1. / gamma), .615):
+        assert abs(want - wrong) > 1e-3
+
+
+def test_iso_gradient_step_is_dt_gamma_squared(oracle_lib):
+    """iso_kernel.cl:178-196: the normal is a 4-tap difference at h = dt * gamma^2 (in
+    normalised texture units) with weights (2, 1) in x, y and (1, 1) in z.  Field
+    f = q(x) + b y + c z with q(x) = x^3/64: trilinear filtering reproduces the y and z
+    terms exactly and q piecewise linearly (np.interp of the voxel values), so along x the
+    4-tap difference depends on h -- the float64 closed form with h = dt gamma^2 matches
+    the oracle's normals, h = dt gamma or h = dt does not.  Orthographic view along -z: x
+    and y are constant along each ray."""
+    N, b, c = 32, 4., 30.
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    vol = (x ** 3 / 64. + b * y + c * z).astype(np.float32)
+    gamma, S = 1.6, 40
+    invP, invM = ortho_mats(vol.shape)
+    W = H = 32
+    iso = 600.
+    out, alpha, depth, normals = oracle.iso_surface(vol, invP, invM, iso_val=iso, gamma=gamma,
+                                                    max_steps=S, width=W, height=H)
+    dt = 2. / (S - 1)
+    q = np.arange(N, dtype=np.float64) ** 3 / 64.
+
+    def F(s):  # q interpolated at normalised coordinate s (texel s N - 0.5, clamped)
+        return np.interp(s * N - .5, np.arange(N), q)
+
+    xs = np.arange(W)
+    sx = .5 * (1 + (xs / W * 2 - 1))       # normalised x of pixel column xs
+    inner = (sx * N - .5 > 10) & (sx * N - .5 < 21)
+    rows = slice(12, 21)  # y taps (up to 2 h = 8.4 voxels) stay inside the volume
+    hit = np.isfinite(depth[rows][:, inner])
+    assert hit.all()
+
+    def normal(h, flip):
+        dx = 2 * (F(sx + h) - F(sx - h)) + F(sx + 2 * h) - F(sx - 2 * h)
+        dy = np.full_like(dx, 8 * h * N * b)
+        dz = np.full_like(dx, 6 * h * N * c)
+        n = np.stack([dx, dy, dz], -1)
+        return flip * n / np.linalg.norm(n, axis=-1, keepdims=True)
+
+    # the first sample (front face, z = N - 1) lies above the iso value: the normal flips
+    want = normal(dt * gamma * gamma, -1.)[inner]
+    got = normals[rows][:, inner]
+    np.testing.assert_allclose(got, np.broadcast_to(want, got.shape), atol=3e-5)
+    for h in (dt * gamma, dt):
+        assert np.abs(normal(h, -1.)[inner] - want).max() > 1e-3

@@ -89,20 +89,47 @@ def test_busy_volumes_are_never_evicted_and_requests_wait():
 
 
 def test_batches_group_same_key_requests_only():
+    """A batch mixes volumes, frame sizes and transfer settings (spr_render_frames takes
+    them per frame) but not alpha_pow, method or filter."""
     vol = np.zeros((30, 30, 30), np.uint16)
     srv = server(1 << 30)
     srv.register("a", vol)
     srv.register("b", vol)
-    for n, size, ap in [("a", (64, 64), 0.), ("b", (64, 64), 0.), ("a", (32, 32), 0.),
-                        ("b", (64, 64), .5), ("a", (64, 64), 0.), ("b", (64, 64), 0.),
-                        ("a", (64, 64), 0.)]:
-        srv.submit(req(n, size=size, alpha_pow=ap))
+    for n, size, ap, mx in [("a", (64, 64), 0., 1.), ("b", (64, 64), 0., 2.), ("a", (32, 32), 0., 1.),
+                            ("b", (64, 64), .5, 1.), ("a", (64, 48), 0., 3.), ("b", (64, 64), 0., 1.),
+                            ("a", (64, 64), 0., 1.)]:
+        srv.submit(req(n, size=size, alpha_pow=ap, max_val=mx))
+    srv.submit(req("a", method="iso_surface"))
+    # only volumes already resident join a batch (b is admitted when its own request is
+    # the oldest): a's four plain projections, of three sizes and two transfer ranges
     batch = srv._take_batch()
-    # same frame size and parameters (any volume, any pose), at most max_batch = 4
-    assert len(batch) == 4
-    assert all(r.size == (64, 64) and r.settings["alpha_pow"] == 0. for r, _ in batch)
-    rest = [(r.volume, r.size, r.settings["alpha_pow"]) for r, _ in srv.waiting]
-    assert rest == [("a", (32, 32), 0.), ("b", (64, 64), .5), ("a", (64, 64), 0.)]
+    assert len(batch) == 4  # max_batch
+    assert [(r.volume, r.size) for r, _ in batch] == [("a", (64, 64)), ("a", (32, 32)),
+                                                      ("a", (64, 48)), ("a", (64, 64))]
+    assert all(r.settings["alpha_pow"] == 0. for r, _ in batch)
+    rest = [(r.volume, r.settings["alpha_pow"], r.method) for r, _ in srv.waiting]
+    assert rest == [("b", 0., "max_project"), ("b", .5, "max_project"), ("b", 0., "max_project"),
+                    ("a", 0., "iso_surface")]
+    assert [len(srv._take_batch()) for _ in range(3)] == [2, 1, 1]
+    assert srv._take_batch() is None
+
+
+def test_oversized_request_behind_an_admissible_one():
+    """An oversized request queued behind an admissible one fails on its own; the other
+    is served and the scheduler keeps going."""
+    srv = server(1 << 20)
+    srv.register("ok", np.zeros((16, 16, 16), np.uint16))
+    srv.register("big", np.zeros((64, 64, 256), np.uint16))  # 2 MiB > 1 MiB budget
+    f_ok = srv.submit(req("ok"))
+    f_big = srv.submit(req("big"))
+    f_ok2 = srv.submit(req("ok", size=(32, 32)))
+    batch = srv._take_batch()
+    assert [r.volume for r, _ in batch] == ["ok", "ok"]
+    assert srv._take_batch() is None
+    with pytest.raises(MemoryError):
+        f_big.result(timeout=1)
+    assert not f_ok.done() and not f_ok2.done()  # the batch is the caller's to render
+    assert srv.volumes["big"].users == 0
 
 
 def test_oversized_and_unknown_requests_fail():
