@@ -65,10 +65,19 @@ def traffic_from_profile(name):
 
 
 def roofline(b_alg, kernel_ms, traffic):
+    """achieved = algorithmic bytes / kernel time (the gather bandwidth the reference's
+    sample count implies); with a measured traffic figure (bytes beyond L2 per launch,
+    rocprofv3 counters) also the HBM bandwidth the launch actually drew."""
     achieved = b_alg / (kernel_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel_ms": round(kernel_ms, 4), "algorithmic_bytes": int(b_alg)}
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+           "kernel_ms": round(kernel_ms, 4), "algorithmic_bytes": int(b_alg),
+           "achieved_is": "effective gather bandwidth (reference samples x 8 corners)"}
+    if traffic:
+        measured = traffic / (kernel_ms * 1e-3) / 1e9
+        out["measured_hbm_GBps"] = round(measured, 1)
+        out["measured_hbm_frac"] = round(measured / HBM_PEAK_GBS, 4)
+    return out
 
 
 def cpu_baseline(vol, invP, invM, lo, hi, S, W, H, seconds, gpu_out, gpu_alpha, what):
@@ -97,7 +106,8 @@ def cpu_baseline(vol, invP, invM, lo, hi, S, W, H, seconds, gpu_out, gpu_alpha, 
     cpu = {"value": round(cpu_samples / tcpu / 1e9, 4), "unit": "Gsamples/s", "cores": cores,
            "kind": "port", "cpu": cpu_model(),
            "sample": "rows %d-%d of %s (%d hit rays, %d samples), %.1f s"
-                     % (r0, r1 - 1, what, hits, cpu_samples, tcpu)}
+                     % (r0, r1 - 1, what, hits, cpu_samples, tcpu),
+           "rows": [r0, r1 - 1], "seconds": round(tcpu, 3)}
     parity = float(np.abs(gpu_out[r0:r1] - o[r0:r1]).max())
     if not np.array_equal(gpu_alpha[r0:r1], a[r0:r1]):
         parity = float("inf")
@@ -160,6 +170,27 @@ def bench_headline(args):
         ctx.render_device(_lib.METHOD_MAX_PROJECT, 1, 0, W, H, out.data_ptr(), alpha.data_ptr(),
                           stream=sptr)
 
+    # launch tuning: the autotuner on the headline pose and three of the seeded poses
+    # (the mean frame time), unless --tune fixes it
+    poses = synth.random_poses(8, seed=0)
+    mats = [render_matrices(Mi, P, (nx, ny, nz)) for Mi in poses]
+    if args.tune:
+        tuned = {"source": "--tune"}
+    else:
+        from spimagine_amd import autotune
+        sample = [(invP, invM)] + mats[:3]
+
+        def render_k(k):
+            set_pose(*sample[k])
+            frame()
+
+        log("bench: autotuning (%d candidates)" % len(autotune.FRAME_CANDIDATES))
+        best, table = autotune.tune_frame(ctx, "u16:%dx%dx%d:%dx%d:%d" % (nx, ny, nz, W, H, S),
+                                          render_k, len(sample), stream)
+        tuned = {"source": "autotune", "best": best,
+                 "candidates_ms": {k: round(v, 4) for k, v in table.items()}}
+        set_pose(invP, invM)
+
     log("bench: warmup %d" % args.warmup)
     for _ in range(args.warmup):
         frame()
@@ -182,8 +213,6 @@ def bench_headline(args):
     gpu_alpha = alpha.cpu().numpy()
 
     # per-frame latency over 8 seeded random poses
-    poses = synth.random_poses(8, seed=0)
-    mats = [render_matrices(Mi, P, (nx, ny, nz)) for Mi in poses]
     lat_ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(args.latency_frames)]
     samples_pose = []
@@ -229,7 +258,8 @@ def bench_headline(args):
                      "translate(0,0,-4).rotY(30 deg), linear sampler",
          "voxel": "u16", "frame": [W, H], "max_steps": S, "samples_per_frame": int(n_samp),
          "samples_per_hit_ray": per_ray,
-         "tuning": {k: ctx.get_tuning(k) for k in _lib.TUNING_KEYS}},
+         "tuning": {k: ctx.get_tuning(k) for k in _lib.TUNING_KEYS},
+         "tuning_source": tuned},
         frames_per_s=round(1e3 / ms_per_step, 2),
         latency_ms={"p50": round(pct(lat, 50), 4), "p99": round(pct(lat, 99), 4),
                     "frames": len(lat_ev), "poses": 8, "gsamples_per_s": round(pose_gsps, 3)},
@@ -827,17 +857,36 @@ def bench_learned(args):
     coarse_ms = timed(lambda p: lm.coarse(slot, p, W, H, planes, 0, stream.cuda_stream), reps)
     net_ms = timed(lambda p: lm.net.forward(planes.data_ptr(), 1, H // 2, W // 2, out.data_ptr(),
                                             stream.cuda_stream), reps)
-    # per-frame judged rendering of the 8 poses as one batch (one host synchronisation per
-    # batch): wall clock per frame, the exact re-renders of rejected frames included
+    # per-frame judged rendering of the 8 poses as one batch per call, the calls queued
+    # back to back (the judge decides on the device; one host synchronisation at the end):
+    # wall clock per frame, the exact re-renders of rejected frames included
     outs = [torch.empty((H, W), device="cuda") for _ in params]
     alphas = [torch.empty_like(o) for o in outs]
     lm.render_judged_many(slot, params, outs, alphas, stream)  # warm
     stream.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
-        kept, est = lm.render_judged_many(slot, params, outs, alphas, stream)
+        res = lm.render_judged_many(slot, params, outs, alphas, stream, wait=False)
     stream.synchronize()
     judged_ms = 1e3 * (time.perf_counter() - t0) / (reps * len(params))
+    kept, est = res.kept(), res.estimates()
+    # the judged frames (learned or fallen back) against the exact ones
+    judged_err = []
+    for p, o in zip(params, outs):
+        ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, W, H, out.data_ptr(), alpha.data_ptr(),
+                        stream=stream.cuda_stream)
+        stream.synchronize()
+        e, g = out.cpu().numpy(), o.cpu().numpy()
+        judged_err.append((learned.psnr(g, e), float(np.abs(g - e).max())))
+    cpu = None
+    if not args.no_cpu:
+        vol_host = ctx.volume_read(slot)
+        rec, _ = cpu_baseline(vol_host, list(params[0].inv_p), list(params[0].inv_m), lo, hi, S,
+                              W, H, args.cpu_seconds, np.zeros((H, W), np.float32),
+                              np.zeros((H, W), np.float32), "the exact frame of pose 0")
+        rows = rec["rows"]
+        cpu = dict(rec, value=round((rows[1] - rows[0] + 1) / rec["seconds"] / H, 4),
+                   unit="frames/s (exact frames)")
     # matrix-core work of one network pass: 32x32x9 MACs per pixel in 4 layers at h*w,
     # 3 at h*w/4 and 2 at h*w/16, here h = H/2
     px = (H // 2) * (W // 2)
@@ -867,13 +916,15 @@ def bench_learned(args):
         "judged": {"estimated_psnr_db": [round(e, 2) for e in est], "learned_kept": kept,
                    "false_accepts": int(sum(1 for k, r in zip(kept, rep)
                                             if k and r["psnr"] < lm.threshold)),
+                   "psnr_db": [round(v[0], 2) for v in judged_err],
+                   "max_abs_err": [round(v[1], 4) for v in judged_err],
                    "ms_per_frame_wall": round(judged_ms, 4),
                    "frames_per_s": round(1e3 / judged_ms, 2),
                    "speedup_vs_exact": round(exact_ms / judged_ms, 3)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": 2500.0,
                      "unit": "TFLOP/s", "frac": round(achieved / 2500., 5), "traffic": None,
                      "kernel": "network forward (all launches)"},
-        "cpu_baseline": None}), flush=True)
+        "cpu_baseline": cpu}), flush=True)
 
 
 def main():

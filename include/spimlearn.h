@@ -86,6 +86,13 @@ int spl_judge_sse(spl_net *net, const float *d_frame, int w, int h, const float 
                   int patch, int gx, int gy, int ox, int oy, float *d_sse, int accumulate,
                   void *stream);
 
+/* The judge's decision on the device: d_flags[k] = 1 when d_sse[k] / npix > mse_max (the
+   frame's estimated PSNR below the acceptance level; not-a-number included), else 0, for
+   k < n, on `stream`.  spr_render_slot_if reads a flag to render the exact fallback without
+   a host round trip. */
+int spl_judge_flags(spl_net *net, const float *d_sse, int n, int npix, float mse_max,
+                    unsigned int *d_flags, void *stream);
+
 /* d_dst (2h x 2w) = nearest-neighbour x2 upsampling of d_src (h x w), on `stream` (the
    learned frame's alpha from the coarse pass). */
 int spl_upsample2(spl_net *net, const float *d_src, int w, int h, float *d_dst, void *stream);

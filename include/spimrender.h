@@ -242,8 +242,10 @@ typedef struct spr_job {
    `params` (NULL: the spr_set_params state; its matrices are ignored, each job carries
    its own).  Frames share w, h and every parameter except the matrices; all their slots
    must use the padded layout and one voxel type.  Results equal n separate
-   spr_render_device calls.  Blocks until the batch completes; calls from several host
-   threads on different streams run concurrently on the device. */
+   spr_render_device calls.  Returns once the batch is enqueued on `stream` (its job table
+   is copied through pinned host memory the context keeps); synchronise the stream before
+   reading the frames.  Calls from several host threads on different streams run
+   concurrently on the device. */
 int spr_render_batch(spr_ctx *ctx, const spr_params *params, const spr_job *jobs, int n, int w,
                      int h, int num_parts, int current_part, void *stream);
 
@@ -265,8 +267,9 @@ typedef struct spr_frame {
    any sizes and settings in one persistent launch: the continuous batching of a render
    server, which takes whatever requests are waiting.  The frames share the voxel type,
    alpha_pow and the filter; every slot uses the padded layout.  Each frame equals
-   spr_render_slot with its own parameters (num_parts 1).  Blocks until the batch
-   completes; calls from several host threads on different streams run concurrently. */
+   spr_render_slot with its own parameters (num_parts 1).  Returns once the batch is
+   enqueued on `stream`, as spr_render_batch; calls from several host threads on different
+   streams run concurrently. */
 int spr_render_frames(spr_ctx *ctx, const spr_frame *frames, int n, float alpha_pow, int filter,
                       void *stream);
 
@@ -276,6 +279,13 @@ int spr_render_frames(spr_ctx *ctx, const spr_frame *frames, int n, float alpha_
 int spr_render_slot(spr_ctx *ctx, int slot, const spr_params *params, int method, int num_parts,
                     int current_part, int w, int h, float *d_out, float *d_alpha, float *d_depth,
                     float *d_normals, float *d_occlusion, void *stream);
+
+/* spr_render_slot's max projection (num_parts 1) performed only if *d_flag (a device
+   word) is nonzero when its kernels run, else leaving d_out and d_alpha untouched: a
+   decision taken on the device by earlier work on `stream`, without a host round trip
+   (the learned reconstruction's exact fallback, include/spimlearn.h). */
+int spr_render_slot_if(spr_ctx *ctx, int slot, const spr_params *params, int w, int h,
+                       float *d_out, float *d_alpha, const unsigned int *d_flag, void *stream);
 
 /* Max projections (linear filter, alpha_pow 0) of both channels of a channel-pair slot
    with one set of rays: d_out holds two w x h planes (channel 0's, then channel 1's),

@@ -40,7 +40,8 @@ EXPORTS = (
     "spr_count_samples", "spr_volume_add", "spr_volume_add_dilated", "spr_volume_add_pooled",
     "spr_volume_add_pooled_pair", "spr_volume_read",
     "spr_volume_release", "spr_volume_info",
-    "spr_render_batch", "spr_render_frames", "spr_render_slot", "spr_render_pair", "spr_render_coarse_pair",
+    "spr_render_batch", "spr_render_frames", "spr_render_slot",
+    "spr_render_slot_if", "spr_render_pair", "spr_render_coarse_pair",
     "spr_batch_graph_create",
     "spr_batch_graph_update",
     "spr_batch_graph_launch", "spr_batch_graph_destroy",
@@ -225,6 +226,8 @@ def load():
     L.spr_render_slot.argtypes = [c_void_p, c_int, ctypes.POINTER(SprParams), c_int, c_int, c_int,
                                   c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p]
+    L.spr_render_slot_if.argtypes = [c_void_p, c_int, ctypes.POINTER(SprParams), c_int, c_int,
+                                     c_void_p, c_void_p, c_void_p, c_void_p]
     L.spr_render_pair.argtypes = [c_void_p, c_int, ctypes.POINTER(SprParams), c_int, c_int, c_int,
                                   c_int, c_void_p, c_void_p, c_void_p]
     L.spr_render_coarse_pair.argtypes = [c_void_p, c_int, c_int, ctypes.POINTER(SprParams), c_int,
@@ -442,6 +445,12 @@ class Context:
         self._check(self.lib.spr_render_slot(self.handle, int(slot), ctypes.byref(params), method,
                                              num_parts, current_part, w, h, d_out, d_alpha, d_depth,
                                              d_normals, d_occlusion, stream))
+
+    def render_slot_if(self, slot, params, w, h, d_out, d_alpha, d_flag, stream=None):
+        """The max projection of render_slot, performed only if the device word at d_flag
+        is nonzero when the kernels run (spr_render_slot_if)."""
+        self._check(self.lib.spr_render_slot_if(self.handle, int(slot), ctypes.byref(params), w, h,
+                                                d_out, d_alpha, d_flag, stream))
 
     def render_pair(self, slot, params, w, h, d_out, d_alpha, num_parts=1, current_part=0,
                     stream=None):

@@ -1002,6 +1002,7 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
                                                 float* __restrict__ alpha) {
     constexpr bool kShort = !std::is_same<T, float>::value;
     constexpr bool kPair = std::is_same<T, U16Pair>::value;  // out: two planes of W x H
+    if (gate_closed(p)) return;  // uniform over the launch
     float px = 0.5f, py = 0.5f, pz = 0.5f, dx = 0.0f, dy = 0.0f, dz = 0.0f, tnear = 0.0f;
     const bool active = staged_ray_begin<kShort>(p, x, y, inside, out, alpha, px, py, pz, dx, dy,
                                                  dz, tnear);

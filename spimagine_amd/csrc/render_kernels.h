@@ -28,7 +28,14 @@ struct RayParams {
     uint32_t row_pad;           // staged kernels: pad LDS slab rows to an odd multiple of 16 B
     uint32_t box16;             // staged kernels: 16-sample groups per staged box (1, 2, 4)
     const uint32_t* order;      // kTileMapCostXcd: workgroup -> work item (k_tile_order)
+    const uint32_t* gate;       // max projections: render only if *gate != 0 (nullptr: always)
 };
+
+// a max projection whose device-side gate is closed leaves its outputs untouched (uniform
+// over the launch: every thread reads the same word)
+__device__ __forceinline__ bool gate_closed(const RayParams& p) {
+    return p.gate != nullptr && *p.gate == 0u;
+}
 
 // Workgroup -> 16x16 tile assignment.  The dispatcher hands consecutive workgroups to
 // the 8 XCDs round-robin (workgroup b runs on XCD b % 8), each XCD with its own L2.
@@ -98,7 +105,7 @@ __device__ __forceinline__ void mip_rays(const Vol& vol, const RayParams& p, uin
                                          bool inside, float* __restrict__ out,
                                          float* __restrict__ alpha) {
     constexpr bool kShort = !std::is_same<typename Vol::value_type, float>::value;
-    if (!inside) return;
+    if (!inside || gate_closed(p)) return;
     const uint32_t o = x + p.W * y;
     F4 orig, direc;
     ray_setup(p.invP, p.invM, x, y, p.W, p.H, &orig, &direc);

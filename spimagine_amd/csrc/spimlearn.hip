@@ -515,6 +515,14 @@ __global__ __launch_bounds__(256) void k_judge_patches(const float* __restrict__
     if (threadIdx.x == 0) sse[0] = (accumulate ? sse[0] : 0.0f) + red[0];
 }
 
+// flags[k] = 1 when frame k's judged mean squared error sse[k] / npix exceeds mse_max (or
+// is not a number): the frame falls back to the exact render
+__global__ void k_judge_flags(const float* __restrict__ sse, int n, float inv_npix, float mse_max,
+                              uint32_t* __restrict__ flags) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) flags[k] = sse[k] * inv_npix <= mse_max ? 0u : 1u;
+}
+
 __global__ void k_fill(float* __restrict__ p, int n, float v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -901,6 +909,19 @@ int spl_judge_sse(spl_net* t, const float* d_frame, int w, int h, const float* d
     hipStream_t s = stream ? (hipStream_t)stream : t->stream;
     hipLaunchKernelGGL(k_judge_patches, dim3(1), dim3(256), 0, s, d_frame, w, h, d_patches, patch,
                        gx, gy, ox, oy, accumulate, d_sse);
+    HIPCALL(t, hipGetLastError());
+    return SPL_OK;
+}
+
+int spl_judge_flags(spl_net* t, const float* d_sse, int n, int npix, float mse_max,
+                    unsigned int* d_flags, void* stream) {
+    if (!t) return fail(nullptr, SPL_ERR_INVALID_ARG, "NULL network");
+    if (!d_sse || !d_flags || n <= 0 || npix <= 0)
+        return fail(t, SPL_ERR_INVALID_ARG, "invalid judge-flag arguments");
+    DeviceGuard g(t->device);
+    hipStream_t s = stream ? (hipStream_t)stream : t->stream;
+    hipLaunchKernelGGL(k_judge_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_sse, n,
+                       1.0f / (float)npix, mse_max, (uint32_t*)d_flags);
     HIPCALL(t, hipGetLastError());
     return SPL_OK;
 }

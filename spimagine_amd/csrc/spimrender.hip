@@ -61,6 +61,15 @@ struct spr_ctx {
     unsigned long long* d_count = nullptr;
     float* d_partial = nullptr;
     LaunchTuning tune;  // spr_set_tuning state and the device's CU count
+    // pinned staging of batch job tables: a ring of host buffers, each reused once the
+    // copy that last read it has run (stage_jobs)
+    std::mutex stage_mu;
+    struct Stage {
+        void* host = nullptr;
+        size_t bytes = 0;
+        hipEvent_t done = nullptr;
+    } stage[8];
+    int stage_next = 0;
 };
 
 namespace {
@@ -205,6 +214,7 @@ RayParams make_params(const spr_params& prm, int tile_map, int w, int h) {
     p.row_pad = 0;
     p.box16 = 1;
     p.order = nullptr;
+    p.gate = nullptr;
     return p;
 }
 
@@ -278,11 +288,14 @@ hipError_t visit_launch(const VolSlot& vs, F&& f) {
 // render(method, numParts, currentPart) into device buffers; the reference's launch
 // order (volumerender.py:366-386 for max_project, :451-497 for iso_surface)
 // tmp (w*h floats) and tmp_vec (3*w*h floats) are the iso post-pass scratch
+// gate: max projection only, rendered only if *gate != 0 when the kernels run
 int render_impl(spr_ctx* c, const VolSlot& vs, const spr_params& prm, int method, int num_parts,
                 int current_part, int w, int h, float* out, float* alpha, float* depth,
-                float* normals, float* occ, float* tmp, float* tmp_vec, hipStream_t s) {
+                float* normals, float* occ, float* tmp, float* tmp_vec, hipStream_t s,
+                const uint32_t* gate = nullptr) {
     RayParams p = make_params(prm, c->tune.tile_map, w, h);
     staged_tuning(c->tune, p);
+    p.gate = gate;
     const bool nearest = prm.filter == SPR_FILTER_NEAREST;
     int rc;
     if (method == SPR_METHOD_MAX_PROJECT) {
@@ -720,6 +733,31 @@ hipError_t dispatch_batch(const spr_ctx* c, const spr_params& prm, int dtype, bo
     }
 }
 
+// Enqueues the copy of a batch's job table to d_jobs on stream s from pinned host memory
+// (the ring of the context): the call returns without waiting for the copy; a ring buffer
+// is refilled only after the copy that last read it has completed.
+hipError_t stage_jobs(spr_ctx* c, const BatchJob* jobs, size_t bytes, BatchJob* d_jobs,
+                      hipStream_t s) {
+    std::lock_guard<std::mutex> lock(c->stage_mu);
+    spr_ctx::Stage& st = c->stage[c->stage_next];
+    c->stage_next = (c->stage_next + 1) % (int)(sizeof(c->stage) / sizeof(c->stage[0]));
+    hipError_t e = hipSuccess;
+    if (st.done) e = hipEventSynchronize(st.done);
+    if (e == hipSuccess && st.bytes < bytes) {
+        if (st.host) (void)hipHostFree(st.host);
+        st.host = nullptr;
+        st.bytes = 0;
+        e = hipHostMalloc(&st.host, bytes, hipHostMallocDefault);
+        if (e == hipSuccess) st.bytes = bytes;
+    }
+    if (e == hipSuccess && !st.done) e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+    memcpy(st.host, jobs, bytes);
+    e = hipMemcpyAsync(d_jobs, st.host, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipEventRecord(st.done, s);
+    return e;
+}
+
 // the (slot, generation) pairs a graph's jobs read (spr_batch_graph_launch checks them)
 void record_refs(const spr_ctx* c, const spr_job* jobs, int n,
                  std::vector<std::pair<int, uint32_t>>& refs) {
@@ -794,6 +832,11 @@ int spr_destroy(spr_ctx* c) {
         for (VolSlot& v : c->vols) free_slot(v);
         if (c->d_count) (void)hipFree(c->d_count);
         if (c->d_partial) (void)hipFree(c->d_partial);
+        for (spr_ctx::Stage& st : c->stage) {
+            if (st.done) (void)hipEventSynchronize(st.done);
+            if (st.done) (void)hipEventDestroy(st.done);
+            if (st.host) (void)hipHostFree(st.host);
+        }
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -1158,6 +1201,22 @@ int spr_render_slot(spr_ctx* c, int slot, const spr_params* params, int method, 
     return rc;
 }
 
+int spr_render_slot_if(spr_ctx* c, int slot, const spr_params* params, int w, int h,
+                       float* d_out, float* d_alpha, const unsigned int* d_flag, void* stream) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    const VolSlot* vs = slot_of(c, slot);
+    if (!vs) return no_volume(c, slot);
+    if (!params || !d_flag) return fail(c, SPR_ERR_INVALID_ARG, "NULL params or flag");
+    int rc = check_params(c, *params);
+    if (rc) return rc;
+    if (w <= 0 || h <= 0 || w > 32768 || h > 32768)
+        return fail(c, SPR_ERR_INVALID_ARG, "invalid output size %d x %d", w, h);
+    DeviceGuard g(c->device);
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    return render_impl(c, *vs, *params, SPR_METHOD_MAX_PROJECT, 1, 0, w, h, d_out, d_alpha, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, s, (const uint32_t*)d_flag);
+}
+
 int spr_render_batch(spr_ctx* c, const spr_params* params, const spr_job* jobs, int n, int w, int h,
                      int num_parts, int current_part, void* stream) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
@@ -1183,18 +1242,17 @@ int spr_render_batch(spr_ctx* c, const spr_params* params, const spr_job* jobs, 
     HIPCALL(c, hipMallocAsync(&buf, jbytes + 64, s));
     BatchJob* d_jobs = (BatchJob*)buf;
     uint32_t* d_counters = (uint32_t*)((char*)buf + jbytes);
-    hipError_t e = hipMemcpyAsync(d_jobs, host.data(), jbytes, hipMemcpyHostToDevice, s);
+    hipError_t e = stage_jobs(c, host.data(), jbytes, d_jobs, s);
     RayParams p = make_params(prm, c->tune.tile_map, w, h);
     staged_tuning(c->tune, p);
     p.nouter = (prm.max_steps / num_parts) / kLoopUnroll;
     p.current_part = current_part;
     const BatchQueues q = make_queues(p, n);
     if (e == hipSuccess)
-        e = dispatch_batch(c, prm, dtype, pow2, chunk_for(c->tune, p.nouter, nmax), d_jobs, p, q, d_counters, s);
+        e = dispatch_batch(c, prm, dtype, pow2, chunk_for(c->tune, p.nouter, nmax), d_jobs, p, q,
+                           d_counters, s);
     (void)hipFreeAsync(buf, s);
     HIPCALL(c, e);
-    // blocking: the host job table must outlive the transfer
-    HIPCALL(c, hipStreamSynchronize(s));
     return SPR_OK;
 }
 
@@ -1261,12 +1319,10 @@ int spr_render_frames(spr_ctx* c, const spr_frame* frames, int n, float alpha_po
     HIPCALL(c, hipMallocAsync(&buf, jbytes + 64, s));
     BatchJob* d_jobs = (BatchJob*)buf;
     uint32_t* d_counters = (uint32_t*)((char*)buf + jbytes);
-    hipError_t e = hipMemcpyAsync(d_jobs, host.data(), jbytes, hipMemcpyHostToDevice, s);
+    hipError_t e = stage_jobs(c, host.data(), jbytes, d_jobs, s);
     if (e == hipSuccess) e = dispatch_batch(c, prm, dtype, pow2, ch, d_jobs, p, q, d_counters, s);
     (void)hipFreeAsync(buf, s);
     HIPCALL(c, e);
-    // blocking: the host job table must outlive the transfer
-    HIPCALL(c, hipStreamSynchronize(s));
     return SPR_OK;
 }
 

@@ -25,7 +25,7 @@ LIB_PATH = os.path.join(HERE, "libspimlearn.so")
 WEIGHTS_PATH = os.path.join(HERE, "weights", "mip_x2_q4.npz")
 EXPORTS = ("spl_abi_version", "spl_param_count", "spl_create", "spl_destroy", "spl_last_error",
            "spl_set_params", "spl_get_params", "spl_forward", "spl_gradients", "spl_train_step",
-           "spl_judge_sse", "spl_upsample2")
+           "spl_judge_sse", "spl_judge_flags", "spl_upsample2")
 C = 32
 LAYER_NAMES = ("e0", "e0b", "e1", "e1b", "e2a", "e2", "u1", "u0", "u0b")
 PSNR_THRESHOLD = 40.0
@@ -60,6 +60,7 @@ def load():
                                  ctypes.POINTER(ctypes.c_double), c_void_p]
     L.spl_judge_sse.argtypes = [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int,
                                 c_int, c_int, c_void_p, c_int, c_void_p]
+    L.spl_judge_flags.argtypes = [c_void_p, c_void_p, c_int, c_int, c_float, c_void_p, c_void_p]
     L.spl_upsample2.argtypes = [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]
     _lib_learn = L
     return L
@@ -160,6 +161,11 @@ class Net(object):
         self._check(self.lib.spl_judge_sse(self.handle, d_frame, w, h, d_patches, patch, gx, gy,
                                            ox, oy, d_sse, int(bool(accumulate)), stream))
 
+    def judge_flags(self, d_sse, n, npix, mse_max, d_flags, stream=None):
+        """flags[k] = sse[k] / npix > mse_max, on the device (spl_judge_flags)."""
+        self._check(self.lib.spl_judge_flags(self.handle, d_sse, int(n), int(npix), float(mse_max),
+                                             d_flags, stream))
+
     def upsample2(self, d_src, w, h, d_dst, stream=None):
         self._check(self.lib.spl_upsample2(self.handle, d_src, w, h, d_dst, stream))
 
@@ -219,6 +225,23 @@ def coarse_params(p):
     return q
 
 
+class JudgedBatch(object):
+    """Device results of one render_judged_many call: the judged error sums and the
+    fallback flags (read them once the stream has passed the call)."""
+
+    def __init__(self, sse, flags, npix):
+        self.sse, self.flags, self.npix = sse, flags, npix
+
+    def kept(self):
+        """Per frame: True when the learned frame was kept (no exact fallback)."""
+        return [int(f) == 0 for f in self.flags.cpu().numpy()]
+
+    def estimates(self):
+        """Per frame: the PSNR estimated on the judge's pixels (dB)."""
+        m = self.sse.cpu().numpy() / float(self.npix)
+        return [float("inf") if v == 0 else 10. * math.log10(1. / float(v)) for v in m]
+
+
 class LearnedMIP(object):
     """Learned max projection over the volumes of one renderer context."""
 
@@ -241,6 +264,7 @@ class LearnedMIP(object):
         self.dilated = {}
         self.enabled = {}   # slot -> validated (min PSNR >= threshold)
         self._bufs = {}
+        self._side = None   # the judge's second stream (sparse exact frames)
 
     def attach(self, slot):
         """The slots whose coarse projections feed the network (created once): for pooled
@@ -289,19 +313,25 @@ class LearnedMIP(object):
         kept, est = self.render_judged_many(slot, [p], [out], [alpha], stream, margin)
         return kept[0], est[0]
 
-    def render_judged_many(self, slot, plist, outs, alphas, stream, margin=0.5):
-        """Frames with a per-frame judge: the learned frames, then the exact renderer on one
-        pixel in 8 x 8 of every frame (a (w/8) x (h/8) render, sparse_params; the sparse
-        frames of all frames that share their non-pose parameters in one batched launch)
-        and each learned frame's PSNR on those pixels, computed on the device; the exact
-        full frame for every frame whose estimate is below threshold + margin.  A scattered
-        grid: the error of a learned frame sits in few pixels, and 1/64 of the pixels
-        estimates the PSNR well only when they are spread over the whole frame (pixel
-        blocks of the same count miss or over-weight the few bright structures).
+    def render_judged_many(self, slot, plist, outs, alphas, stream, margin=0.5, wait=True):
+        """Frames with a per-frame judge: the learned frames, the exact renderer on one pixel
+        in 8 x 8 of every frame (a (w/8) x (h/8) render, sparse_params; the sparse frames of
+        all frames that share their non-pose parameters in one batched launch), each learned
+        frame's PSNR on those pixels, and the exact full frame for every frame whose
+        estimate is below threshold + margin.  A scattered grid: the error of a learned
+        frame sits in few pixels, and 1/64 of the pixels estimates the PSNR well only when
+        they are spread over the whole frame (pixel blocks of the same count miss or
+        over-weight the few bright structures).
+        Nothing waits on the host: the sparse exact frames render on a second stream while
+        the learned frames render on `stream`; then, on `stream`, the judge's error sums
+        (spl_judge_sse), its decisions on the device (spl_judge_flags) and the exact
+        fallbacks, each rendered only if its frame's flag is set (spr_render_slot_if).
         outs / alphas: (h, w) float32 torch tensors; stream: a torch.cuda.Stream.  Learned
         frames take alpha from the coarse pass (exact at even pixels, nearest-upsampled).
-        Returns (kept flags, estimated PSNRs).  The slot must be batchable (padded layout,
-        below 4 GiB)."""
+        wait=True: synchronises `stream` and returns (kept flags, estimated PSNRs);
+        wait=False: returns a JudgedBatch whose results are read later.  The slot must be
+        batchable (padded layout, below 4 GiB)."""
+        torch = self.torch
         h, w = outs[0].shape
         S = JUDGE_STRIDE
         if w % 64 or h % 64:
@@ -309,14 +339,16 @@ class LearnedMIP(object):
         sh = stream.cuda_stream
         n = len(plist)
         sw, sh_ = w // S, h // S
-        sparse = self._buffer(("judge", n, w, h), (n, sh_, sw))
-        salpha = self._buffer(("judge_alpha", n, w, h), (n, sh_, sw))
-        sse = self._buffer(("judge_sse", n), (n,))
-        coarse_alpha = self._buffer(("alpha", w, h), (h // 2, w // 2))
-        for p, out, alpha in zip(plist, outs, alphas):
-            self.render(slot, p, w, h, out.data_ptr(), sh)
-            self.net.upsample2(coarse_alpha.data_ptr(), w // 2, h // 2, alpha.data_ptr(), sh)
-        # the sparse exact frames (spr_render_batch returns once its launch has completed)
+        if self._side is None:
+            self._side = torch.cuda.Stream()
+        side = self._side
+        with torch.cuda.stream(stream):
+            sparse = torch.empty((n, sh_, sw), device="cuda")
+            salpha = torch.empty_like(sparse)
+            sse = torch.empty((n,), device="cuda")
+            flags = torch.empty((n,), dtype=torch.int32, device="cuda")
+        # the sparse exact frames on the side stream, concurrently with the learned frames
+        side.wait_stream(stream)
         groups = {}
         for k, p in enumerate(plist):
             groups.setdefault(bytes(p)[_lib.SprParams.box.offset:], []).append(k)
@@ -324,19 +356,29 @@ class LearnedMIP(object):
             qs = [sparse_params(plist[k], w, h) for k in ks]
             jobs = [(slot, list(q.inv_p), list(q.inv_m), sparse[k].data_ptr(),
                      salpha[k].data_ptr()) for k, q in zip(ks, qs)]
-            self.ctx.render_batch(jobs, sw, sh_, params=qs[0], stream=sh)
+            self.ctx.render_batch(jobs, sw, sh_, params=qs[0], stream=side.cuda_stream)
+        sparse.record_stream(side)
+        salpha.record_stream(side)
+        done = torch.cuda.Event()
+        done.record(side)
+        coarse_alpha = self._buffer(("alpha", w, h), (h // 2, w // 2))
+        for p, out, alpha in zip(plist, outs, alphas):
+            self.render(slot, p, w, h, out.data_ptr(), sh)
+            self.net.upsample2(coarse_alpha.data_ptr(), w // 2, h // 2, alpha.data_ptr(), sh)
+        stream.wait_event(done)
         for k, out in enumerate(outs):
             self.net.judge_sse(out.data_ptr(), w, h, sparse[k].data_ptr(), 1, sw, sh_,
                                JUDGE_OFFSET, JUDGE_OFFSET, sse.data_ptr() + 4 * k, stream=sh)
+        mse_max = 10. ** (-(self.threshold + margin) / 10.)
+        self.net.judge_flags(sse.data_ptr(), n, sw * sh_, mse_max, flags.data_ptr(), stream=sh)
+        for k, (p, out, alpha) in enumerate(zip(plist, outs, alphas)):
+            self.ctx.render_slot_if(slot, p, w, h, out.data_ptr(), alpha.data_ptr(),
+                                    flags.data_ptr() + 4 * k, stream=sh)
+        res = JudgedBatch(sse, flags, sw * sh_)
+        if not wait:
+            return res
         stream.synchronize()
-        m = sse.cpu().numpy() / float(sw * sh_)
-        est = [float("inf") if v == 0 else 10. * math.log10(1. / float(v)) for v in m]
-        kept = [e >= self.threshold + margin for e in est]
-        for p, out, alpha, ok in zip(plist, outs, alphas, kept):
-            if not ok:
-                self.ctx.render_slot(slot, p, _lib.METHOD_MAX_PROJECT, w, h, out.data_ptr(),
-                                     alpha.data_ptr(), stream=sh)
-        return kept, est
+        return res.kept(), res.estimates()
 
     def render_or_exact(self, slot, p, w, h, d_out, d_alpha, stream):
         """Learned frame when the slot validated, else the exact render."""
