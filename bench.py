@@ -862,13 +862,30 @@ def bench_learned(args):
     # wall clock per frame, the exact re-renders of rejected frames included
     outs = [torch.empty((H, W), device="cuda") for _ in params]
     alphas = [torch.empty_like(o) for o in outs]
-    lm.render_judged_many(slot, params, outs, alphas, stream)  # warm
-    stream.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        res = lm.render_judged_many(slot, params, outs, alphas, stream, wait=False)
-    stream.synchronize()
-    judged_ms = 1e3 * (time.perf_counter() - t0) / (reps * len(params))
+
+    def judged_run(stride, slab_kb):
+        lm.judge_stride, lm.judge_slab_kb = stride, slab_kb
+        lm.render_judged_many(slot, params, outs, alphas, stream)  # warm
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            res = lm.render_judged_many(slot, params, outs, alphas, stream, wait=False)
+        stream.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / (reps * len(params)), res
+
+    # the judge's sampling (one pixel in stride x stride) and the sparse frames' staging
+    # (19 KiB slabs or global gathers: their rays run stride pixels apart)
+    sweep = []
+    for stride, slab in ((8, None), (8, 0), (16, None), (16, 0)):
+        ms, res = judged_run(stride, slab)
+        sweep.append({"stride": stride, "sparse_slab_kb": ctx.get_tuning("slab_kb")
+                      if slab is None else slab, "ms_per_frame_wall": round(ms, 4),
+                      "learned_kept": res.kept(),
+                      "estimated_psnr_db": [round(e, 2) for e in res.estimates()]})
+        log("judge stride %d slab %s: %.4f ms/frame, kept %s" % (stride, slab, ms, res.kept()))
+    # the reported run: the library's defaults
+    lm.judge_stride, lm.judge_slab_kb = learned.JUDGE_STRIDE, None
+    judged_ms, res = judged_run(learned.JUDGE_STRIDE, None)
     kept, est = res.kept(), res.estimates()
     # the judged frames (learned or fallen back) against the exact ones
     judged_err = []
@@ -920,7 +937,8 @@ def bench_learned(args):
                    "max_abs_err": [round(v[1], 4) for v in judged_err],
                    "ms_per_frame_wall": round(judged_ms, 4),
                    "frames_per_s": round(1e3 / judged_ms, 2),
-                   "speedup_vs_exact": round(exact_ms / judged_ms, 3)},
+                   "speedup_vs_exact": round(exact_ms / judged_ms, 3),
+                   "stride": learned.JUDGE_STRIDE, "sweep": sweep},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": 2500.0,
                      "unit": "TFLOP/s", "frac": round(achieved / 2500., 5), "traffic": None,
                      "kernel": "network forward (all launches)"},

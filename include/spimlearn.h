@@ -86,6 +86,16 @@ int spl_judge_sse(spl_net *net, const float *d_frame, int w, int h, const float 
                   int patch, int gx, int gy, int ox, int oy, float *d_sse, int accumulate,
                   void *stream);
 
+/* The judge's error of n frames at once against their sparse exact frames (the patch-1
+   form of spl_judge_sse): frames = a host array of n device pointers to h x w frames,
+   d_sparse = n consecutive gx x gy sparse frames (sparse pixel (tx, ty) of frame k is
+   frame pixel (tx w/gx + ox, ty h/gy + oy)); d_sse[k] = frame k's summed squared
+   difference, summed in a fixed order over many workgroups.  Enqueued on `stream`; one
+   stream at a time per network (the partial sums live in the network). */
+int spl_judge_sse_frames(spl_net *net, const float *const *frames, int n, int w, int h,
+                         const float *d_sparse, int gx, int gy, int ox, int oy, float *d_sse,
+                         void *stream);
+
 /* The judge's decision on the device: d_flags[k] = 1 when d_sse[k] / npix > mse_max (the
    frame's estimated PSNR below the acceptance level; not-a-number included), else 0, for
    k < n, on `stream`.  spr_render_slot_if reads a flag to render the exact fallback without

@@ -515,6 +515,51 @@ __global__ __launch_bounds__(256) void k_judge_patches(const float* __restrict__
     if (threadIdx.x == 0) sse[0] = (accumulate ? sse[0] : 0.0f) + red[0];
 }
 
+// The same sum for up to kJudgeFrames frames at once against their sparse exact frames
+// (patch 1: the gx x gy grid of pixels (tx w/gx + ox, ty h/gy + oy), sparse frame k at
+// sparse + k gx gy, row-major): block b of frame k (grid kJudgeBlocks x n) sums its
+// contiguous share of the grid in a fixed order into part[k kJudgeBlocks + b];
+// k_judge_reduce adds a frame's partial sums in order (deterministic).
+constexpr int kJudgeFrames = 16;
+constexpr int kJudgeBlocks = 32;
+struct JudgeFrames {
+    const float* f[kJudgeFrames];
+};
+
+__global__ __launch_bounds__(256) void k_judge_grid(JudgeFrames frames, int w, int h,
+                                                    const float* __restrict__ sparse, int gx,
+                                                    int gy, int ox, int oy,
+                                                    float* __restrict__ part) {
+    __shared__ float red[256];
+    const int k = blockIdx.y, b = blockIdx.x;
+    const int n = gx * gy, per = (n + kJudgeBlocks - 1) / kJudgeBlocks;
+    const int i0 = b * per, i1 = min(n, i0 + per);
+    const int cw = w / gx, ch = h / gy;
+    const float* frame = frames.f[k];
+    const float* sp = sparse + (size_t)k * n;
+    float v = 0.0f;
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
+        const int ty = i / gx, tx = i - ty * gx;
+        const float d = frame[(size_t)(ty * ch + oy) * w + tx * cw + ox] - sp[i];
+        v = fmaf(d, d, v);
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int r = 128; r > 0; r >>= 1) {
+        if (threadIdx.x < r) red[threadIdx.x] += red[threadIdx.x + r];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[k * kJudgeBlocks + b] = red[0];
+}
+
+__global__ void k_judge_reduce(const float* __restrict__ part, int n, float* __restrict__ sse) {
+    const int k = threadIdx.x;
+    if (k >= n) return;
+    float v = 0.0f;
+    for (int b = 0; b < kJudgeBlocks; ++b) v += part[k * kJudgeBlocks + b];
+    sse[k] = v;
+}
+
 // flags[k] = 1 when frame k's judged mean squared error sse[k] / npix exceeds mse_max (or
 // is not a number): the frame falls back to the exact render
 __global__ void k_judge_flags(const float* __restrict__ sse, int n, float inv_npix, float mse_max,
@@ -555,6 +600,7 @@ struct spl_net {
           *gQ1 = nullptr, *dr = nullptr, *y = nullptr;
     float *wpart = nullptr, *lpart = nullptr;
     size_t lpart_n = 0;
+    float* judge_part = nullptr;  // spl_judge_sse_frames partial sums (kJudgeFrames blocks)
 };
 
 namespace {
@@ -803,7 +849,8 @@ int spl_destroy(spl_net* t) {
         DeviceGuard g(t->device);
         if (t->stream) (void)hipStreamSynchronize(t->stream);
         free_work(t);
-        void* bufs[] = {t->params, t->grad, t->adam_m, t->adam_v, t->frag_fwd, t->frag_dgrad, t->wpart};
+        void* bufs[] = {t->params,   t->grad,       t->adam_m, t->adam_v,
+                        t->frag_fwd, t->frag_dgrad, t->wpart,  t->judge_part};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (t->stream) (void)hipStreamDestroy(t->stream);
@@ -909,6 +956,33 @@ int spl_judge_sse(spl_net* t, const float* d_frame, int w, int h, const float* d
     hipStream_t s = stream ? (hipStream_t)stream : t->stream;
     hipLaunchKernelGGL(k_judge_patches, dim3(1), dim3(256), 0, s, d_frame, w, h, d_patches, patch,
                        gx, gy, ox, oy, accumulate, d_sse);
+    HIPCALL(t, hipGetLastError());
+    return SPL_OK;
+}
+
+int spl_judge_sse_frames(spl_net* t, const float* const* frames, int n, int w, int h,
+                         const float* d_sparse, int gx, int gy, int ox, int oy, float* d_sse,
+                         void* stream) {
+    if (!t) return fail(nullptr, SPL_ERR_INVALID_ARG, "NULL network");
+    if (!frames || !d_sparse || !d_sse || n <= 0 || w <= 0 || h <= 0 || gx < 1 || gy < 1 ||
+        ox < 0 || oy < 0 || ox >= w / gx || oy >= h / gy)
+        return fail(t, SPL_ERR_INVALID_ARG, "invalid judge arguments");
+    for (int k = 0; k < n; ++k)
+        if (!frames[k]) return fail(t, SPL_ERR_INVALID_ARG, "NULL frame %d", k);
+    DeviceGuard g(t->device);
+    hipStream_t s = stream ? (hipStream_t)stream : t->stream;
+    if (!t->judge_part)
+        HIPCALL(t, hipMalloc(&t->judge_part, sizeof(float) * kJudgeFrames * kJudgeBlocks));
+    const size_t per = (size_t)gx * gy;
+    for (int k0 = 0; k0 < n; k0 += kJudgeFrames) {
+        const int m = std::min(kJudgeFrames, n - k0);
+        JudgeFrames jf{};
+        for (int k = 0; k < m; ++k) jf.f[k] = frames[k0 + k];
+        hipLaunchKernelGGL(k_judge_grid, dim3(kJudgeBlocks, m), dim3(256), 0, s, jf, w, h,
+                           d_sparse + k0 * per, gx, gy, ox, oy, t->judge_part);
+        hipLaunchKernelGGL(k_judge_reduce, dim3(1), dim3(64), 0, s, (const float*)t->judge_part, m,
+                           d_sse + k0);
+    }
     HIPCALL(t, hipGetLastError());
     return SPL_OK;
 }

@@ -25,7 +25,7 @@ LIB_PATH = os.path.join(HERE, "libspimlearn.so")
 WEIGHTS_PATH = os.path.join(HERE, "weights", "mip_x2_q4.npz")
 EXPORTS = ("spl_abi_version", "spl_param_count", "spl_create", "spl_destroy", "spl_last_error",
            "spl_set_params", "spl_get_params", "spl_forward", "spl_gradients", "spl_train_step",
-           "spl_judge_sse", "spl_judge_flags", "spl_upsample2")
+           "spl_judge_sse", "spl_judge_sse_frames", "spl_judge_flags", "spl_upsample2")
 C = 32
 LAYER_NAMES = ("e0", "e0b", "e1", "e1b", "e2a", "e2", "u1", "u0", "u0b")
 PSNR_THRESHOLD = 40.0
@@ -60,6 +60,8 @@ def load():
                                  ctypes.POINTER(ctypes.c_double), c_void_p]
     L.spl_judge_sse.argtypes = [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int,
                                 c_int, c_int, c_void_p, c_int, c_void_p]
+    L.spl_judge_sse_frames.argtypes = [c_void_p, ctypes.POINTER(c_void_p), c_int, c_int, c_int,
+                                       c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p]
     L.spl_judge_flags.argtypes = [c_void_p, c_void_p, c_int, c_int, c_float, c_void_p, c_void_p]
     L.spl_upsample2.argtypes = [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]
     _lib_learn = L
@@ -160,6 +162,13 @@ class Net(object):
         """Squared error of a frame against gx x gy exact patches (spl_judge_sse)."""
         self._check(self.lib.spl_judge_sse(self.handle, d_frame, w, h, d_patches, patch, gx, gy,
                                            ox, oy, d_sse, int(bool(accumulate)), stream))
+
+    def judge_sse_frames(self, d_frames, w, h, d_sparse, gx, gy, ox, oy, d_sse, stream=None):
+        """Squared errors of n frames (device pointers d_frames) against their sparse exact
+        frames in one pass (spl_judge_sse_frames)."""
+        arr = (ctypes.c_void_p * len(d_frames))(*d_frames)
+        self._check(self.lib.spl_judge_sse_frames(self.handle, arr, len(d_frames), w, h, d_sparse,
+                                                  gx, gy, ox, oy, d_sse, stream))
 
     def judge_flags(self, d_sse, n, npix, mse_max, d_flags, stream=None):
         """flags[k] = sse[k] / npix > mse_max, on the device (spl_judge_flags)."""
@@ -265,6 +274,12 @@ class LearnedMIP(object):
         self.enabled = {}   # slot -> validated (min PSNR >= threshold)
         self._bufs = {}
         self._side = None   # the judge's second stream (sparse exact frames)
+        # the judge: one pixel in judge_stride x judge_stride, frames kept when their
+        # estimated PSNR reaches threshold + margin; judge_slab_kb: the sparse frames'
+        # staging (their rays run judge_stride pixels apart; None: the context's)
+        self.judge_stride = JUDGE_STRIDE
+        self.margin = .5
+        self.judge_slab_kb = None
 
     def attach(self, slot):
         """The slots whose coarse projections feed the network (created once): for pooled
@@ -308,12 +323,12 @@ class LearnedMIP(object):
         self.coarse(slot, p, w, h, planes, 0, stream)
         self.net.forward(planes.data_ptr(), 1, h // 2, w // 2, d_out, stream)
 
-    def render_judged(self, slot, p, out, alpha, stream, margin=0.5):
+    def render_judged(self, slot, p, out, alpha, stream, margin=None):
         """One frame with a per-frame judge (render_judged_many for one frame)."""
         kept, est = self.render_judged_many(slot, [p], [out], [alpha], stream, margin)
         return kept[0], est[0]
 
-    def render_judged_many(self, slot, plist, outs, alphas, stream, margin=0.5, wait=True):
+    def render_judged_many(self, slot, plist, outs, alphas, stream, margin=None, wait=True):
         """Frames with a per-frame judge: the learned frames, the exact renderer on one pixel
         in 8 x 8 of every frame (a (w/8) x (h/8) render, sparse_params; the sparse frames of
         all frames that share their non-pose parameters in one batched launch), each learned
@@ -333,9 +348,10 @@ class LearnedMIP(object):
         batchable (padded layout, below 4 GiB)."""
         torch = self.torch
         h, w = outs[0].shape
-        S = JUDGE_STRIDE
-        if w % 64 or h % 64:
-            raise ValueError("judged frames need w and h multiples of 64")
+        S = self.judge_stride
+        margin = self.margin if margin is None else margin
+        if w % (8 * S) or h % (8 * S):
+            raise ValueError("judged frames need w and h multiples of %d" % (8 * S))
         sh = stream.cuda_stream
         n = len(plist)
         sw, sh_ = w // S, h // S
@@ -352,11 +368,19 @@ class LearnedMIP(object):
         groups = {}
         for k, p in enumerate(plist):
             groups.setdefault(bytes(p)[_lib.SprParams.box.offset:], []).append(k)
-        for ks in groups.values():
-            qs = [sparse_params(plist[k], w, h) for k in ks]
-            jobs = [(slot, list(q.inv_p), list(q.inv_m), sparse[k].data_ptr(),
-                     salpha[k].data_ptr()) for k, q in zip(ks, qs)]
-            self.ctx.render_batch(jobs, sw, sh_, params=qs[0], stream=side.cuda_stream)
+        keep = None
+        if self.judge_slab_kb is not None:  # the sparse frames' own launch tuning
+            keep = self.ctx.get_tuning("slab_kb")
+            self.ctx.set_tuning("slab_kb", self.judge_slab_kb)
+        try:
+            for ks in groups.values():
+                qs = [sparse_params(plist[k], w, h, stride=S) for k in ks]
+                jobs = [(slot, list(q.inv_p), list(q.inv_m), sparse[k].data_ptr(),
+                         salpha[k].data_ptr()) for k, q in zip(ks, qs)]
+                self.ctx.render_batch(jobs, sw, sh_, params=qs[0], stream=side.cuda_stream)
+        finally:
+            if keep is not None:
+                self.ctx.set_tuning("slab_kb", keep)
         sparse.record_stream(side)
         salpha.record_stream(side)
         done = torch.cuda.Event()
@@ -366,9 +390,8 @@ class LearnedMIP(object):
             self.render(slot, p, w, h, out.data_ptr(), sh)
             self.net.upsample2(coarse_alpha.data_ptr(), w // 2, h // 2, alpha.data_ptr(), sh)
         stream.wait_event(done)
-        for k, out in enumerate(outs):
-            self.net.judge_sse(out.data_ptr(), w, h, sparse[k].data_ptr(), 1, sw, sh_,
-                               JUDGE_OFFSET, JUDGE_OFFSET, sse.data_ptr() + 4 * k, stream=sh)
+        self.net.judge_sse_frames([o.data_ptr() for o in outs], w, h, sparse.data_ptr(), sw, sh_,
+                                  JUDGE_OFFSET, JUDGE_OFFSET, sse.data_ptr(), stream=sh)
         mse_max = 10. ** (-(self.threshold + margin) / 10.)
         self.net.judge_flags(sse.data_ptr(), n, sw * sh_, mse_max, flags.data_ptr(), stream=sh)
         for k, (p, out, alpha) in enumerate(zip(plist, outs, alphas)):

@@ -277,6 +277,30 @@ def test_judge_sse_patch_layout(torch):
         net.judge_sse(frame.data_ptr(), w, h, pat.data_ptr(), c, gx, gy, 6, oy, sse.data_ptr())
 
 
+def test_judge_sse_frames_layout(torch):
+    """spl_judge_sse_frames on 21 frames (two launches of at most 16) against their sparse
+    grids of 9 x 7 pixels at offset (3, 2), against the same sums on the host."""
+    net = learned.Net(0, learned.init_params(0))
+    n, w, h, gx, gy, ox, oy = 21, 90, 56, 9, 7, 3, 2
+    g = torch.Generator("cuda").manual_seed(4)
+    frames = [torch.rand((h, w), device="cuda", generator=g) for _ in range(n)]
+    sparse = torch.rand((n, gy, gx), device="cuda", generator=g)
+    sse = torch.full((n,), -1., device="cuda")
+    torch.cuda.synchronize()
+    net.judge_sse_frames([f.data_ptr() for f in frames], w, h, sparse.data_ptr(), gx, gy, ox, oy,
+                         sse.data_ptr())
+    torch.cuda.synchronize()
+    S = sparse.cpu().double().numpy()
+    got = sse.cpu().double().numpy()
+    for k, f in enumerate(frames):
+        F = f.cpu().double().numpy()
+        want = ((F[oy::h // gy, ox::w // gx][:gy, :gx] - S[k]) ** 2).sum()
+        assert abs(got[k] - want) < 1e-4 * want, k
+    with pytest.raises(learned.SplError):
+        net.judge_sse_frames([f.data_ptr() for f in frames], w, h, sparse.data_ptr(), gx, gy,
+                             w // gx, oy, sse.data_ptr())
+
+
 @pytest.mark.parametrize("dtype,shape", [(np.uint16, (64, 80, 72)), (np.float32, (48, 64, 64)),
                                          (np.uint8, (40, 52, 60))])
 def test_coarse_pair_matches_exact_renders(torch, dtype, shape):
