@@ -305,6 +305,20 @@ int spr_render_pair(spr_ctx *ctx, int slot, const spr_params *params, int num_pa
 int spr_render_coarse_pair(spr_ctx *ctx, int slot_a, int slot_b, const spr_params *params, int w,
                            int h, float *d_out, float *d_alpha, void *stream);
 
+/* spr_render_batch without any device memory allocation (no counterpart in the reference;
+   for callers that cannot allocate: code between hipMalloc-free checkpoints, tight serving
+   loops, harnesses that forbid hipMalloc).  Same arguments, checks and results as
+   spr_render_batch, which takes its job table and queue counters from the stream-ordered
+   allocator; here they live in a fixed device scratch array (one per device, held until
+   the process ends) of 256 + SPR_NOALLOC_MAX_JOBS x 256 bytes (~1 MiB), so n is limited
+   to SPR_NOALLOC_MAX_JOBS (SPR_ERR_INVALID_ARG beyond).  The scratch keeps the last call's
+   job table.  Calls share one host staging table and each device's scratch: the library
+   runs them one at a time (concurrent callers wait), and each call blocks until its batch
+   completes. */
+#define SPR_NOALLOC_MAX_JOBS 4096
+int spr_render_batch_noalloc(spr_ctx *ctx, const spr_params *params, const spr_job *jobs, int n,
+                             int w, int h, int num_parts, int current_part, void *stream);
+
 /* The same batch (num_parts = 1, the spr_set_params state) captured once in a HIP graph: replay it any number of
    times on any stream, rewrite its jobs (same count; waits for the last replay first) and
    destroy it.  Launches are asynchronous.  A graph records the slots its jobs read:

@@ -25,6 +25,7 @@ DTYPE_CODES = {np.dtype(np.float32): DTYPE_F32, np.dtype(np.uint16): DTYPE_U16,
 FILTER_LINEAR, FILTER_NEAREST = 0, 1
 METHOD_MAX_PROJECT, METHOD_ISO_SURFACE = 0, 1
 LAYOUT_AUTO, LAYOUT_LINEAR, LAYOUT_PADDED = 0, 1, 2
+NOALLOC_MAX_JOBS = 4096
 TUNE_TILE_MAP, TUNE_SLAB_KB = 0, 1
 TUNE_STAGING, TUNE_CHUNK, TUNE_ROW_PAD, TUNE_WG_WAVES, TUNE_SKIP = 2, 3, 4, 5, 6
 TUNING_KEYS = {"tile_map": TUNE_TILE_MAP, "slab_kb": TUNE_SLAB_KB, "staging": TUNE_STAGING,
@@ -40,7 +41,7 @@ EXPORTS = (
     "spr_count_samples", "spr_volume_add", "spr_volume_add_dilated", "spr_volume_add_pooled",
     "spr_volume_add_pooled_pair", "spr_volume_read",
     "spr_volume_release", "spr_volume_info",
-    "spr_render_batch", "spr_render_frames", "spr_render_slot",
+    "spr_render_batch", "spr_render_batch_noalloc", "spr_render_frames", "spr_render_slot",
     "spr_render_slot_if", "spr_render_pair", "spr_render_coarse_pair",
     "spr_batch_graph_create",
     "spr_batch_graph_update",
@@ -221,6 +222,8 @@ def load():
     jp = ctypes.POINTER(SprJob)
     L.spr_render_batch.argtypes = [c_void_p, ctypes.POINTER(SprParams), jp, c_int, c_int, c_int,
                                    c_int, c_int, c_void_p]
+    L.spr_render_batch_noalloc.argtypes = [c_void_p, ctypes.POINTER(SprParams), jp, c_int, c_int,
+                                           c_int, c_int, c_int, c_void_p]
     L.spr_render_frames.argtypes = [c_void_p, ctypes.POINTER(SprFrame), c_int, c_float, c_int,
                                     c_void_p]
     L.spr_render_slot.argtypes = [c_void_p, c_int, ctypes.POINTER(SprParams), c_int, c_int, c_int,
@@ -433,6 +436,15 @@ class Context:
         pp = ctypes.byref(params) if params is not None else None
         self._check(self.lib.spr_render_batch(self.handle, pp, arr, len(arr), w, h, num_parts,
                                               current_part, stream))
+
+    def render_batch_noalloc(self, jobs, w, h, num_parts=1, current_part=0, stream=None,
+                             params=None):
+        """render_batch through spr_render_batch_noalloc (job table and queue counters in the
+        library's fixed device scratch; at most NOALLOC_MAX_JOBS jobs)."""
+        arr = jobs if isinstance(jobs, ctypes.Array) else self.make_jobs(jobs)
+        pp = ctypes.byref(params) if params is not None else None
+        self._check(self.lib.spr_render_batch_noalloc(self.handle, pp, arr, len(arr), w, h,
+                                                      num_parts, current_part, stream))
 
     def render_frames(self, frames, alpha_pow=0., filter=FILTER_LINEAR, stream=None):
         """frames: a list of SprFrame (make_frame); one persistent launch (spr_render_frames)."""

@@ -25,6 +25,24 @@ using namespace spr;
 
 constexpr size_t kMaxSlots = 4096;
 
+// Device scratch of spr_render_batch_noalloc: the persistent launch's eight queue counters
+// (the first 256 bytes) and a job table of up to SPR_NOALLOC_MAX_JOBS jobs.  Each device
+// holds its own copy until the process ends.  Host code never passes this name to a kernel
+// or a copy: it uses the device address hipGetSymbolAddress returns for each device
+// (noalloc_scratch_address, called at spr_create with that device current).
+constexpr size_t kScratchCounterBytes = 256;
+constexpr size_t kScratchBytes =
+    kScratchCounterBytes + (size_t)SPR_NOALLOC_MAX_JOBS * sizeof(BatchJob);
+static_assert(sizeof(BatchJob) == 256, "the header states the scratch size per job");
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+// scratch's device address per device, and the host staging table of the no-allocation
+// batch (one call at a time, across every context and device)
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_noalloc_mu;
+static BatchJob g_noalloc_jobs[SPR_NOALLOC_MAX_JOBS];
 
 struct spr_batch_graph {
     int n = 0, w = 0, h = 0;
@@ -70,6 +88,7 @@ struct spr_ctx {
         hipEvent_t done = nullptr;
     } stage[8];
     int stage_next = 0;
+    void* d_scratch = nullptr;  // the device's scratch (spr_render_batch_noalloc)
 };
 
 namespace {
@@ -769,6 +788,20 @@ void record_refs(const spr_ctx* c, const spr_job* jobs, int n,
 }
 
 
+// scratch's address on `device` (the current device), looked up once per device
+hipError_t noalloc_scratch_address(int device, void** ptr) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -803,6 +836,8 @@ int spr_create(int device, spr_ctx** out) {
         return fail(nullptr, SPR_ERR_NO_DEVICE, "no HIP device visible");
     if (device < 0 || device >= n)
         return fail(nullptr, SPR_ERR_INVALID_ARG, "device %d out of range [0, %d)", device, n);
+    if (device >= kMaxDevices)
+        return fail(nullptr, SPR_ERR_INVALID_ARG, "device %d beyond the library's %d", device, kMaxDevices);
     spr_ctx* c = new (std::nothrow) spr_ctx();
     if (!c) return fail(nullptr, SPR_ERR_OUT_OF_MEMORY, "out of host memory");
     c->device = device;
@@ -813,6 +848,7 @@ int spr_create(int device, spr_ctx** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_count, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&c->d_partial, 2 * 1024 * sizeof(float));
+    if (e == hipSuccess) e = noalloc_scratch_address(device, &c->d_scratch);
     if (e != hipSuccess) {
         int rc = fail(nullptr, SPR_ERR_HIP, "context creation failed: %s", hipGetErrorString(e));
         spr_destroy(c);
@@ -1323,6 +1359,45 @@ int spr_render_frames(spr_ctx* c, const spr_frame* frames, int n, float alpha_po
     if (e == hipSuccess) e = dispatch_batch(c, prm, dtype, pow2, ch, d_jobs, p, q, d_counters, s);
     (void)hipFreeAsync(buf, s);
     HIPCALL(c, e);
+    return SPR_OK;
+}
+
+int spr_render_batch_noalloc(spr_ctx* c, const spr_params* params, const spr_job* jobs, int n,
+                             int w, int h, int num_parts, int current_part, void* stream) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (n > SPR_NOALLOC_MAX_JOBS)
+        return fail(c, SPR_ERR_INVALID_ARG,
+                    "%d jobs exceed the no-allocation batch's maximum of %d (SPR_NOALLOC_MAX_JOBS)",
+                    n, SPR_NOALLOC_MAX_JOBS);
+    const spr_params& prm = params ? *params : c->params;
+    if (!params && !c->has_params) return fail(c, SPR_ERR_INVALID_ARG, "no parameters");
+    int rc = check_params(c, prm);
+    if (rc) return rc;
+    if (w <= 0 || h <= 0 || w > 32768 || h > 32768)
+        return fail(c, SPR_ERR_INVALID_ARG, "invalid output size %d x %d", w, h);
+    rc = check_parts(c, prm, num_parts);
+    if (rc) return rc;
+    if (!c->d_scratch) return fail(c, SPR_ERR_HIP, "no device scratch for this context");
+    // one call at a time: the host staging table and each device's scratch are shared
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    int dtype, nmax;
+    bool pow2;
+    rc = build_jobs_into(c, jobs, n, g_noalloc_jobs, &dtype, &pow2, &nmax);
+    if (rc) return rc;
+    DeviceGuard g(c->device);
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    uint32_t* d_counters = (uint32_t*)c->d_scratch;  // cleared by the launch's reset kernel
+    BatchJob* d_jobs = (BatchJob*)((uint8_t*)c->d_scratch + kScratchCounterBytes);
+    HIPCALL(c, hipMemcpyAsync(d_jobs, g_noalloc_jobs, (size_t)n * sizeof(BatchJob),
+                              hipMemcpyHostToDevice, s));
+    RayParams p = make_params(prm, c->tune.tile_map, w, h);
+    staged_tuning(c->tune, p);
+    p.nouter = (prm.max_steps / num_parts) / kLoopUnroll;
+    p.current_part = current_part;
+    const BatchQueues q = make_queues(p, n);
+    HIPCALL(c, dispatch_batch(c, prm, dtype, pow2, chunk_for(c->tune, p.nouter, nmax), d_jobs, p, q, d_counters, s));
+    // blocking: the next call may overwrite the staging table and the scratch
+    HIPCALL(c, hipStreamSynchronize(s));
     return SPR_OK;
 }
 
