@@ -80,11 +80,17 @@ def roofline(b_alg, kernel_ms, traffic):
     return out
 
 
+def cpu_threads():
+    """Host threads of the CPU baselines: the CPUs this process may run on, at most 16 (one
+    GPU's share of the host on an 8-GPU node; the sibling GPUs' host work uses the rest)."""
+    return min(len(os.sched_getaffinity(0)), 16)
+
+
 def cpu_baseline(vol, invP, invM, lo, hi, S, W, H, seconds, gpu_out, gpu_alpha, what):
     """The oracle on a bounded block of rows around the middle of the frame; returns the
     cpu_baseline record and the max |GPU - oracle| over those rows (inf if alpha differs)."""
     import oracle
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = cpu_threads()
     mid = H // 2
     per_ray = (S // 16 + 1) * 16
 
@@ -101,13 +107,18 @@ def cpu_baseline(vol, invP, invM, lo, hi, S, W, H, seconds, gpu_out, gpu_alpha, 
     r0 = max(0, mid - nrows // 2)
     r1 = min(H, r0 + nrows)
     tcpu, o, a = run_rows(r0, r1)
+    reps = 1
+    # a whole frame shorter than the budget: render it again until the budget is used
+    while r1 - r0 == H and tcpu * (reps + 1) / reps <= seconds and reps < 64:
+        tcpu += run_rows(r0, r1)[0]
+        reps += 1
     hits = int((a[r0:r1] > 0).sum())
-    cpu_samples = hits * per_ray
+    cpu_samples = hits * per_ray * reps
     cpu = {"value": round(cpu_samples / tcpu / 1e9, 4), "unit": "Gsamples/s", "cores": cores,
-           "kind": "port", "cpu": cpu_model(),
-           "sample": "rows %d-%d of %s (%d hit rays, %d samples), %.1f s"
-                     % (r0, r1 - 1, what, hits, cpu_samples, tcpu),
-           "rows": [r0, r1 - 1], "seconds": round(tcpu, 3)}
+           "kind": "port", "cpu": cpu_model(), "host_threads_visible": os.cpu_count(),
+           "sample": "rows %d-%d of %s, %d time(s) (%d hit rays, %d samples), %.1f s"
+                     % (r0, r1 - 1, what, reps, hits, cpu_samples, tcpu),
+           "rows": [r0, r1 - 1], "reps": reps, "seconds": round(tcpu, 3)}
     parity = float(np.abs(gpu_out[r0:r1] - o[r0:r1]).max())
     if not np.array_equal(gpu_alpha[r0:r1], a[r0:r1]):
         parity = float("inf")
@@ -272,8 +283,10 @@ def skip_table(ctx, stream, frame, set_pose, samples, reps):
     """Per-pose frame times (HIP events on the frame stream) with SPR_TUNE_SKIP 0 and 1:
     pose 0 is the headline pose, the rest the seeded random poses.  Restores the context's
     skip setting.  Samples are the reference count either way: skipping raises the
-    effective rate, not the work."""
+    effective rate, not the work.  reps 0: no table (None)."""
     import torch
+    if reps <= 0:
+        return None
     keep = ctx.get_tuning("skip")
     out = {}
     for skip in (0, 1):
@@ -472,7 +485,7 @@ def cpu_baseline_iso(vol, mat, iso_val, S, W, H, seconds):
     """The oracle's iso march on a block of rows around the middle of the frame, in
     reference samples per second (counted as iso_reference_samples does)."""
     import oracle
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = cpu_threads()
     invP, invM = mat
     mid = H // 2
     t = time.perf_counter()
@@ -486,6 +499,14 @@ def cpu_baseline_iso(vol, mat, iso_val, S, W, H, seconds):
     _, a, d, _ = oracle.iso_surface(vol, invP, invM, iso_val=np.float32(iso_val), max_steps=S,
                                     width=W, height=H, rows=(r0, r1), nthreads=cores)
     tcpu = time.perf_counter() - t
+    reps = 1
+    # a whole frame shorter than the budget: render it again until the budget is used
+    while r1 - r0 == H and tcpu * (reps + 1) / reps <= seconds and reps < 64:
+        t = time.perf_counter()
+        oracle.iso_surface(vol, invP, invM, iso_val=np.float32(iso_val), max_steps=S, width=W,
+                           height=H, rows=(r0, r1), nthreads=cores)
+        tcpu += time.perf_counter() - t
+        reps += 1
     # rays of those rows that enter the box: iso misses and box misses both read alpha 0, so
     # count the box hits with the max projection's sample counter geometry on the host
     iP = np.asarray(invP, np.float64).reshape(4, 4)
@@ -509,10 +530,12 @@ def cpu_baseline_iso(vol, mat, iso_val, S, W, H, seconds):
     tmax = np.nanmin(np.maximum(t0, t1), axis=-1)
     rays = int((tmax > tmin).sum())
     samples = iso_reference_samples(d, a, invP, invM, W, H, S, (r0, r1), rays)
-    return {"value": round(samples / tcpu / 1e9, 4), "unit": "Gsamples/s", "cores": cores,
-            "kind": "port", "cpu": cpu_model(),
-            "sample": "iso march (oracle) of rows %d-%d of pose 0 (%d rays in the box, ~%d "
-                      "reference samples), %.1f s" % (r0, r1 - 1, rays, samples, tcpu)}
+    return {"value": round(samples * reps / tcpu / 1e9, 4), "unit": "Gsamples/s",
+            "cores": cores, "kind": "port", "cpu": cpu_model(),
+            "host_threads_visible": os.cpu_count(),
+            "sample": "iso march (oracle) of rows %d-%d of pose 0, %d time(s) (%d rays in the "
+                      "box, ~%d reference samples each), %.1f s"
+                      % (r0, r1 - 1, reps, rays, samples, tcpu)}
 
 
 def cpu_model():
@@ -767,7 +790,7 @@ def cpu_baseline_requests(volumes, mip, seconds):
     """The oracle on a bounded block of rows of each of a few max-projection requests of the
     run; Gsamples/s over all of them."""
     import oracle
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = cpu_threads()
     per = seconds / max(1, len(mip))
     total_s = total_t = 0.
     parts = []
@@ -863,29 +886,41 @@ def bench_learned(args):
     outs = [torch.empty((H, W), device="cuda") for _ in params]
     alphas = [torch.empty_like(o) for o in outs]
 
-    def judged_run(stride, slab_kb):
-        lm.judge_stride, lm.judge_slab_kb = stride, slab_kb
+    def judged_run(stride, seg, slab_kb=None, concurrent=False, z=learned.JUDGE_Z):
+        lm.judge_stride, lm.judge_seg, lm.judge_slab_kb = stride, seg, slab_kb
+        lm.judge_concurrent, lm.judge_z = concurrent, z
         lm.render_judged_many(slot, params, outs, alphas, stream)  # warm
         stream.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
             res = lm.render_judged_many(slot, params, outs, alphas, stream, wait=False)
+        t1 = time.perf_counter()
         stream.synchronize()
-        return 1e3 * (time.perf_counter() - t0) / (reps * len(params)), res
+        n = reps * len(params)
+        log("judged: host enqueue %.4f ms/frame" % (1e3 * (t1 - t0) / n))
+        return 1e3 * (time.perf_counter() - t0) / n, res
 
-    # the judge's sampling (one pixel in stride x stride) and the sparse frames' staging
-    # (19 KiB slabs or global gathers: their rays run stride pixels apart)
+    # the judge's sampling (one pixel in stride x stride) and how its sparse frames render:
+    # split rays (segments of seg samples) or one persistent batch launch (19 KiB slabs
+    # or global gathers: the rays run stride pixels apart)
     sweep = []
-    for stride, slab in ((8, None), (8, 0), (16, None), (16, 0)):
-        ms, res = judged_run(stride, slab)
-        sweep.append({"stride": stride, "sparse_slab_kb": ctx.get_tuning("slab_kb")
-                      if slab is None else slab, "ms_per_frame_wall": round(ms, 4),
-                      "learned_kept": res.kept(),
-                      "estimated_psnr_db": [round(e, 2) for e in res.estimates()]})
-        log("judge stride %d slab %s: %.4f ms/frame, kept %s" % (stride, slab, ms, res.kept()))
+    for stride, seg, slab, conc, z in ((8, 64, None, False, 3.), (8, 32, None, False, 3.),
+                                       (8, 64, None, True, 3.), (8, 64, 0, False, 3.),
+                                       (16, 64, None, False, 3.), (16, 64, None, False, 2.),
+                                       (16, 64, None, True, 3.), (8, None, None, True, 3.),
+                                       (16, None, None, True, 3.)):
+        ms, res = judged_run(stride, seg, slab, conc, z)
+        slab_kb = ctx.get_tuning("slab_kb") if slab is None else slab
+        sweep.append({"stride": stride, "z": z, "concurrent": conc,
+                      "sparse": "split rays, %d-sample segments, slab_kb %d"
+                      % (seg, slab_kb) if seg else "batch launch, slab_kb %d" % slab_kb,
+                      "ms_per_frame_wall": round(ms, 4), "learned_kept": res.kept(),
+                      "estimated_psnr_db": [round(e, 2) for e in res.estimates()],
+                      "bound_psnr_db": [round(e, 2) for e in res.bounds()]})
+        log("judge stride %d seg %s slab %s concurrent %s z %g: %.4f ms/frame, kept %s"
+            % (stride, seg, slab, conc, z, ms, res.kept()))
     # the reported run: the library's defaults
-    lm.judge_stride, lm.judge_slab_kb = learned.JUDGE_STRIDE, None
-    judged_ms, res = judged_run(learned.JUDGE_STRIDE, None)
+    judged_ms, res = judged_run(learned.JUDGE_STRIDE, learned.JUDGE_SEG)
     kept, est = res.kept(), res.estimates()
     # the judged frames (learned or fallen back) against the exact ones
     judged_err = []
@@ -902,8 +937,8 @@ def bench_learned(args):
                               W, H, args.cpu_seconds, np.zeros((H, W), np.float32),
                               np.zeros((H, W), np.float32), "the exact frame of pose 0")
         rows = rec["rows"]
-        cpu = dict(rec, value=round((rows[1] - rows[0] + 1) / rec["seconds"] / H, 4),
-                   unit="frames/s (exact frames)")
+        cpu = dict(rec, value=round(rec["reps"] * (rows[1] - rows[0] + 1) / rec["seconds"] / H,
+                                    4), unit="frames/s (exact frames)")
     # matrix-core work of one network pass: 32x32x9 MACs per pixel in 4 layers at h*w,
     # 3 at h*w/4 and 2 at h*w/16, here h = H/2
     px = (H // 2) * (W // 2)
@@ -930,7 +965,10 @@ def bench_learned(args):
         "psnr_db": [round(r["psnr"], 2) for r in rep],
         "max_abs_err": [round(r["max_abs"], 4) for r in rep],
         "psnr_threshold_db": lm.threshold, "all_validation_frames_pass": ok,
-        "judged": {"estimated_psnr_db": [round(e, 2) for e in est], "learned_kept": kept,
+        "judged": {"estimated_psnr_db": [round(e, 2) for e in est],
+                   "bound_psnr_db": [round(e, 2) for e in res.bounds()],
+                   "acceptance_db": lm.threshold + lm.margin, "z": lm.judge_z,
+                   "learned_kept": kept,
                    "false_accepts": int(sum(1 for k, r in zip(kept, rep)
                                             if k and r["psnr"] < lm.threshold)),
                    "psnr_db": [round(v[0], 2) for v in judged_err],
@@ -938,7 +976,8 @@ def bench_learned(args):
                    "ms_per_frame_wall": round(judged_ms, 4),
                    "frames_per_s": round(1e3 / judged_ms, 2),
                    "speedup_vs_exact": round(exact_ms / judged_ms, 3),
-                   "stride": learned.JUDGE_STRIDE, "sweep": sweep},
+                   "stride": learned.JUDGE_STRIDE, "segment": learned.JUDGE_SEG,
+                   "sweep": sweep},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": 2500.0,
                      "unit": "TFLOP/s", "frac": round(achieved / 2500., 5), "traffic": None,
                      "kernel": "network forward (all launches)"},

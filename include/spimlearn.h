@@ -90,11 +90,21 @@ int spl_judge_sse(spl_net *net, const float *d_frame, int w, int h, const float 
    form of spl_judge_sse): frames = a host array of n device pointers to h x w frames,
    d_sparse = n consecutive gx x gy sparse frames (sparse pixel (tx, ty) of frame k is
    frame pixel (tx w/gx + ox, ty h/gy + oy)); d_sse[k] = frame k's summed squared
-   difference, summed in a fixed order over many workgroups.  Enqueued on `stream`; one
-   stream at a time per network (the partial sums live in the network). */
+   difference and, when d_sse4 is not NULL, d_sse4[k] = its summed fourth power, each
+   summed in a fixed order over many workgroups.  Enqueued on `stream`; one stream at a
+   time per network (the partial sums live in the network). */
 int spl_judge_sse_frames(spl_net *net, const float *const *frames, int n, int w, int h,
                          const float *d_sparse, int gx, int gy, int ox, int oy, float *d_sse,
-                         void *stream);
+                         float *d_sse4, void *stream);
+
+/* The judge's decision with a confidence bound, on the device: with m2 = d_sse[k] / npix
+   and m4 = d_sse4[k] / npix, d_flags[k] = 0 (keep the learned frame) only when
+   m2 + z sqrt((m4 - m2^2) / npix) <= mse_max -- the mean squared error of the judge's
+   npix pixels plus z standard errors of that mean -- else 1 (not-a-number included).
+   Frames whose error sits in few pixels get a wider bound than frames with an even error
+   of the same mean. */
+int spl_judge_flags_bound(spl_net *net, const float *d_sse, const float *d_sse4, int n, int npix,
+                          float mse_max, float z, unsigned int *d_flags, void *stream);
 
 /* The judge's decision on the device: d_flags[k] = 1 when d_sse[k] / npix > mse_max (the
    frame's estimated PSNR below the acceptance level; not-a-number included), else 0, for

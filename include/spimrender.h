@@ -273,6 +273,20 @@ typedef struct spr_frame {
 int spr_render_frames(spr_ctx *ctx, const spr_frame *frames, int n, float alpha_pow, int filter,
                       void *stream);
 
+/* Plain max projections (alpha_pow 0, linear filter) of n frames whose rays lie far apart,
+   such as a sparse grid of a frame's pixels (no counterpart in the reference, whose frames
+   are always full pixel grids; here for the learned renderer's exact judge): each ray's
+   samples are split into segments of `seg` samples (32, 64 or 128; 0 = 64) marched by
+   different waves, so few rays still fill the device; a segment is staged through LDS
+   slabs as in the single-frame kernel (SPR_TUNE_SLAB_KB > 0, 16-sample boxes) or gathered
+   from global memory (SPR_TUNE_SLAB_KB 0).  Sample positions follow the
+   reference's running sum exactly (volume_kernel.cl:121-131, 293-298), so each frame equals
+   spr_render_frames / spr_render_slot of the same frame up to the sign of zero outputs.
+   Slots as spr_render_frames (padded layout, one voxel type).  The frames' d_out planes
+   are cleared and used as the accumulators of their rays.  Returns once enqueued on
+   `stream`. */
+int spr_render_frames_split(spr_ctx *ctx, const spr_frame *frames, int n, int seg, void *stream);
+
 /* One frame of any method from any slot into device buffers on `stream`, with explicit
    `params` (matrices included): the stateless form of spr_render_device for callers that
    keep many volumes resident and render with per-request settings from several threads. */

@@ -41,7 +41,8 @@ EXPORTS = (
     "spr_count_samples", "spr_volume_add", "spr_volume_add_dilated", "spr_volume_add_pooled",
     "spr_volume_add_pooled_pair", "spr_volume_read",
     "spr_volume_release", "spr_volume_info",
-    "spr_render_batch", "spr_render_batch_noalloc", "spr_render_frames", "spr_render_slot",
+    "spr_render_batch", "spr_render_batch_noalloc", "spr_render_frames",
+    "spr_render_frames_split", "spr_render_slot",
     "spr_render_slot_if", "spr_render_pair", "spr_render_coarse_pair",
     "spr_batch_graph_create",
     "spr_batch_graph_update",
@@ -224,6 +225,8 @@ def load():
                                    c_int, c_int, c_void_p]
     L.spr_render_batch_noalloc.argtypes = [c_void_p, ctypes.POINTER(SprParams), jp, c_int, c_int,
                                            c_int, c_int, c_int, c_void_p]
+    L.spr_render_frames_split.argtypes = [c_void_p, ctypes.POINTER(SprFrame), c_int, c_int,
+                                          c_void_p]
     L.spr_render_frames.argtypes = [c_void_p, ctypes.POINTER(SprFrame), c_int, c_float, c_int,
                                     c_void_p]
     L.spr_render_slot.argtypes = [c_void_p, c_int, ctypes.POINTER(SprParams), c_int, c_int, c_int,
@@ -451,6 +454,13 @@ class Context:
         arr = (SprFrame * len(frames))(*frames)
         self._check(self.lib.spr_render_frames(self.handle, arr, len(frames), float(alpha_pow),
                                                int(filter), stream))
+
+    def render_frames_split(self, frames, seg=0, stream=None):
+        """Max projections of frames of scattered rays (spr_render_frames_split): each ray's
+        samples split into segments of seg (32, 64, 128; 0 = 64) samples."""
+        arr = (SprFrame * len(frames))(*frames)
+        self._check(self.lib.spr_render_frames_split(self.handle, arr, len(frames), int(seg),
+                                                     stream))
 
     def render_slot(self, slot, params, method, w, h, d_out, d_alpha, d_depth=None,
                     d_normals=None, d_occlusion=None, num_parts=1, current_part=0, stream=None):

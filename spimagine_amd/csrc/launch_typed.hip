@@ -92,6 +92,28 @@ void launch_wave(int slab_kb, bool skip, int ch, dim3 grid, hipStream_t s, Args.
         launch_wave_ch<T, P2, OPAC, 4>(slab_kb, skip, grid, s, args...);
 }
 
+// the nearest filter staged: one wave per workgroup, 19 KiB slabs; OPAC: the opacity mode;
+// SKIP: boxes skipped (max projection) or waves stopped (opacity mode) by brick bounds
+template <typename T, bool P2, bool OPAC, bool SKIP, typename... Args>
+void launch_wave_nearest_sk(int ch, dim3 grid, hipStream_t s, Args... args) {
+    if (ch == 16)
+        hipLaunchKernelGGL((k_max_project_wave<T, P2, kWaveSlab19, 16, false, OPAC, SKIP, 1>), grid, dim3(64), 0, s, args...);
+    else if (ch == 8)
+        hipLaunchKernelGGL((k_max_project_wave<T, P2, kWaveSlab19, 8, false, OPAC, SKIP, 1>), grid, dim3(64), 0, s, args...);
+    else
+        hipLaunchKernelGGL((k_max_project_wave<T, P2, kWaveSlab19, 4, false, OPAC, SKIP, 1>), grid, dim3(64), 0, s, args...);
+}
+
+template <typename T, bool P2, typename... Args>
+void launch_wave_nearest(bool opac, bool skip, int ch, dim3 grid, hipStream_t s, Args... args) {
+    if (opac)
+        skip ? launch_wave_nearest_sk<T, P2, true, true>(ch, grid, s, args...)
+             : launch_wave_nearest_sk<T, P2, true, false>(ch, grid, s, args...);
+    else
+        skip ? launch_wave_nearest_sk<T, P2, false, true>(ch, grid, s, args...)
+             : launch_wave_nearest_sk<T, P2, false, false>(ch, grid, s, args...);
+}
+
 // One wave per workgroup for the per-wave staged kernels?  wg_waves 0 (default) decides by
 // frame size: one wave while the frame has at most kOneWaveTilesPerCu 16x16 tiles per CU.
 // A frame of few tiles per CU ends on its slowest waves, and one-wave workgroups refill a
@@ -183,18 +205,18 @@ void launch_staged_opac(int ch, dim3 grid, hipStream_t s, Args... args) {
 
 // the iso-surface march staged: per-wave 19 KiB slabs (four waves per workgroup: one per
 // workgroup was 3.5% slower over five poses of the headline volume)
-template <typename T, bool P2, bool SKIP, typename... Args>
+template <typename T, bool P2, bool SKIP, int FILT = 0, typename... Args>
 void launch_iso_staged(int ch, dim3 grid, hipStream_t s, Args... args) {
     if (ch == 16)
-        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 16, false, SKIP>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 16, false, SKIP, FILT>), grid, dim3(256), 0, s, args...);
     else if (ch == 8)
-        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 8, false, SKIP>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 8, false, SKIP, FILT>), grid, dim3(256), 0, s, args...);
     else
-        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 4, false, SKIP>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 4, false, SKIP, FILT>), grid, dim3(256), 0, s, args...);
 }
 
-// the plain (unstaged) kernels on the slot's layout: nearest filter, linear layout (volumes
-// above 4 GiB included), or staging switched off
+// the plain (unstaged) kernels on the slot's layout: linear layout (a volume above 4 GiB
+// with the nearest filter included), or staging switched off
 template <typename T, bool P2, typename F>
 hipError_t visit_layout(const VolSlot& vs, F&& f) {
     if (vs.layout == SPR_LAYOUT_LINEAR) return f(linear_volume<T>(vs));
@@ -255,6 +277,22 @@ hipError_t TypedLaunch<T, P2>::max_project(const LaunchTuning& t, const VolSlot&
     // skipping by brick bounds: per-wave slabs of the default size, a volume with bounds
     const bool skip = t.skip != 0 && vs.d_bricks != nullptr && t.staging == 0 &&
                       (t.slab_kb == 19 || vs.pad_wide);
+    // the nearest filter (plain or opacity mode) staged: padded slots below 4 GiB, per-wave
+    // 19 KiB slabs, one wave per workgroup
+    if (nearest && t.slab_kb > 0 && t.staging == 0 && vs.layout == SPR_LAYOUT_PADDED) {
+        const PaddedVolume<T, P2> v = padded_volume<T, P2>(vs);
+        const int ch = chunk_for(t, p.nouter, nmax);
+        dim3 g(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty));
+        void* obuf = nullptr;
+        hipError_t e = order_tiles(t.tile_map, true, v, kWaveSlab19, p, s, &obuf, &g);
+        if (e == hipSuccess) {
+            launch_wave_nearest<T, P2>(opac, t.skip != 0 && vs.d_bricks != nullptr, ch, g, s, v,
+                                       p, out, alpha);
+            e = hipGetLastError();
+        }
+        if (obuf) (void)hipFreeAsync(obuf, s);
+        return e;
+    }
     if ((staged || staged_opac) && vs.pad_wide) {
         const bool one_wave = one_wave_for(t, p);
         const int ch = chunk_for(t, p.nouter, nmax);
@@ -326,6 +364,13 @@ hipError_t TypedLaunch<T, P2>::iso_march(const LaunchTuning& t, const VolSlot& v
              : launch_iso_staged<T, P2, false>(ch, grid, s, v, p, out, alpha, depth, normals);
         return hipGetLastError();
     }
+    // the nearest filter's march staged the same way (padded slots below 4 GiB)
+    if (nearest && t.slab_kb > 0 && t.staging == 0 && vs.layout == SPR_LAYOUT_PADDED) {
+        const PaddedVolume<T, P2> v = padded_volume<T, P2>(vs);
+        skip ? launch_iso_staged<T, P2, true, 1>(ch, grid, s, v, p, out, alpha, depth, normals)
+             : launch_iso_staged<T, P2, false, 1>(ch, grid, s, v, p, out, alpha, depth, normals);
+        return hipGetLastError();
+    }
     return visit_layout<T, P2>(vs, [&](auto v) {
         using V = decltype(v);
         if (nearest)
@@ -357,6 +402,34 @@ hipError_t TypedLaunch<T, P2>::batch(const LaunchTuning& t, int mode, int ch, co
     if (t.slab_kb == 19)
         return launch_batch_staged<T, P2, 19456, false>(ch, cus, d_jobs, p, q, counters, s);
     return launch_batch_slab<T, P2, 12288>(sh, ch, cus, d_jobs, p, q, counters, s);
+}
+
+template <typename T, bool P2>
+hipError_t TypedLaunch<T, P2>::split(int seg, bool staged, const BatchJob* d_jobs, uint32_t njobs,
+                                     uint32_t nitems, uint32_t npix, const RayParams& base,
+                                     hipStream_t s) {
+    constexpr bool kShort = !std::is_same<T, float>::value;
+    const dim3 pg((npix + 255u) / 256u), block(256);
+    hipLaunchKernelGGL(k_split_clear, pg, block, 0, s, d_jobs, njobs, npix);
+    if (staged) {  // one wave (work item) per workgroup
+        const dim3 ig(nitems), wave(64);
+        if (seg == 32)
+            hipLaunchKernelGGL((k_split_staged<T, P2, 32>), ig, wave, 0, s, d_jobs, njobs, nitems, base);
+        else if (seg == 128)
+            hipLaunchKernelGGL((k_split_staged<T, P2, 128>), ig, wave, 0, s, d_jobs, njobs, nitems, base);
+        else
+            hipLaunchKernelGGL((k_split_staged<T, P2, 64>), ig, wave, 0, s, d_jobs, njobs, nitems, base);
+    } else {  // four work items per workgroup
+        const dim3 ig((nitems + 3u) / 4u);
+        if (seg == 32)
+            hipLaunchKernelGGL((k_split_march<T, P2, 32>), ig, block, 0, s, d_jobs, njobs, nitems, base);
+        else if (seg == 128)
+            hipLaunchKernelGGL((k_split_march<T, P2, 128>), ig, block, 0, s, d_jobs, njobs, nitems, base);
+        else
+            hipLaunchKernelGGL((k_split_march<T, P2, 64>), ig, block, 0, s, d_jobs, njobs, nitems, base);
+    }
+    hipLaunchKernelGGL(k_split_finish<kShort>, pg, block, 0, s, d_jobs, njobs, npix, base);
+    return hipGetLastError();
 }
 
 template <typename T, bool P2>

@@ -461,8 +461,9 @@ __device__ __forceinline__ void opacity_step(Opacity& o, float v, float& col, fl
 
 // CH samples from the slab holding the box (lox, LY, LZ, row bytes rowb, SY rows a plane);
 // slab offset of corner (c+1) relative to the box computed exactly in float (every term is
-// an integer below 2^24)
-template <typename V, int CH, bool OPAC = false>
+// an integer below 2^24).  FILT 1: the nearest filter, one voxel per sample (nearest's
+// voxel is one of the linear sample's 8 corners, so it lies in the box staged for them).
+template <typename V, int CH, bool OPAC = false, int FILT = 0>
 __device__ __forceinline__ void staged_sample_slab(const V& vol,
                                                    const uint8_t* slab, uint32_t lox, uint32_t LY,
                                                    uint32_t LZ, uint32_t rowb, uint32_t SY,
@@ -474,6 +475,26 @@ __device__ __forceinline__ void staged_sample_slab(const V& vol,
     const float fsy = (float)sy, fsz = (float)sz;
     const float oy = 1.0f - (float)LY, oz = 1.0f - (float)LZ;
     const float ox = (1.0f - (float)lox) * (float)TS;
+    if constexpr (FILT == 1) {
+        const float ny = -(float)LY, nz = -(float)LZ, nx = -(float)(lox * TS);
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            float ix, iy, iz;
+            vol.nearest_f(px, py, pz, &ix, &iy, &iz);
+            const uint32_t q =
+                (uint32_t)fmaf(iz + nz, fsz, fmaf(iy + ny, fsy, fmaf(ix, (float)TS, nx)));
+            const float v = (float)*reinterpret_cast<const T*>(slab + q);
+            if constexpr (OPAC) {
+                opacity_step<!std::is_same<T, float>::value>(op, v, col, px, py, pz, dx, dy, dz);
+            } else {
+                col = fmaxf(col, v);
+                px = px + dx;
+                py = py + dy;
+                pz = pz + dz;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
         float cx, cy, cz, ax, ay, az;
@@ -499,13 +520,13 @@ __device__ __forceinline__ void staged_sample_slab(const V& vol,
     }
 }
 
-template <typename V, int CH, bool OPAC = false>
+template <typename V, int CH, bool OPAC = false, int FILT = 0>
 __device__ __forceinline__ void staged_sample_global(const V& vol, float& px,
                                                      float& py, float& pz, float dx, float dy,
                                                      float dz, float& col, Opacity& op) {
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
-        const float v = vol.linear(px, py, pz);
+        const float v = FILT == 1 ? vol.nearest(px, py, pz) : vol.linear(px, py, pz);
         if constexpr (OPAC) {
             opacity_step<!std::is_same<typename V::value_type, float>::value>(op, v, col, px, py,
                                                                              pz, dx, dy, dz);
@@ -581,7 +602,7 @@ __device__ __forceinline__ void staged_sample_global_pair(const V& vol, float& p
 }
 
 // the sampler matching the voxel type (col2: the second channel of a U16Pair volume)
-template <typename V, int N, bool OPAC>
+template <typename V, int N, bool OPAC, int FILT = 0>
 __device__ __forceinline__ void sample_slab_any(const V& vol, const uint8_t* slab, uint32_t lox,
                                                 uint32_t LY, uint32_t LZ, uint32_t rowb,
                                                 uint32_t SY, float& px, float& py, float& pz,
@@ -591,18 +612,18 @@ __device__ __forceinline__ void sample_slab_any(const V& vol, const uint8_t* sla
         staged_sample_slab_pair<V, N>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy, dz, col,
                                       col2);
     else
-        staged_sample_slab<V, N, OPAC>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy, dz, col,
-                                       op);
+        staged_sample_slab<V, N, OPAC, FILT>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy,
+                                             dz, col, op);
 }
 
-template <typename V, int N, bool OPAC>
+template <typename V, int N, bool OPAC, int FILT = 0>
 __device__ __forceinline__ void sample_global_any(const V& vol, float& px, float& py, float& pz,
                                                   float dx, float dy, float dz, float& col,
                                                   float& col2, Opacity& op) {
     if constexpr (std::is_same<typename V::value_type, U16Pair>::value)
         staged_sample_global_pair<V, N>(vol, px, py, pz, dx, dy, dz, col, col2);
     else
-        staged_sample_global<V, N, OPAC>(vol, px, py, pz, dx, dy, dz, col, op);
+        staged_sample_global<V, N, OPAC, FILT>(vol, px, py, pz, dx, dy, dz, col, op);
 }
 
 // the transfer and the outputs (volume_kernel.cl:159-183); OPAC: col is already normalised
@@ -658,7 +679,7 @@ __device__ __forceinline__ void shadow_first_active(uint64_t act, bool active, f
 // additions) and stops once every ray holds the volume's maximum; the opacity mode stops
 // once no later sample can raise any ray's colour (opacity_spent).
 template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE, bool OPAC,
-          bool HALVES = true, bool SKIP = false>
+          bool HALVES = true, bool SKIP = false, int FILT = 0>
 __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WIDE>& vol,
                                                  const RayParams& p, bool active, bool live,
                                                  uint8_t* slab, uint32_t* s_box, float& px,
@@ -806,6 +827,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
 
     static_assert(kLoopUnroll % CH == 0, "a chunk is a whole number of boxes");
     static_assert(!(OPAC && kPair), "pair renders are plain max projections");
+    static_assert(FILT == 0 || (!kPair && !WIDE), "nearest: one-channel volumes below 4 GiB");
     const int G = CH == 16 ? (int)p.box16 : 1;  // groups per box, uniform
     Opacity op;
     op.cumsum = 1.0f;
@@ -902,7 +924,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
 #pragma unroll 1
                 for (int k = 0; k < ng; ++k) {
                     group(CH);
-                    sample_slab_any<PV, CH, OPAC>(vol, slab, b.lox, b.LY, b.LZ, b.rowb, b.SY, px, py,
+                    sample_slab_any<PV, CH, OPAC, FILT>(vol, slab, b.lox, b.LY, b.LZ, b.rowb, b.SY, px, py,
                                                   pz, dx, dy, dz, col, col2, op);
                 }
             }
@@ -919,7 +941,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
                 if (live) {
                     const bool second = active && lane >= 32u;
                     group(CH);
-                    sample_slab_any<PV, CH, OPAC>(
+                    sample_slab_any<PV, CH, OPAC, FILT>(
                         vol, second ? slab + ha.bytes : slab,
                         second ? hb.lox : ha.lox, second ? hb.LY : ha.LY, second ? hb.LZ : ha.LZ,
                         second ? hb.rowb : ha.rowb, second ? hb.SY : ha.SY, px, py, pz, dx, dy, dz,
@@ -940,7 +962,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
                     Box hb;
                     if (stage(8, hb)) {
                         group(8);
-                        sample_slab_any<PV, 8, OPAC>(vol, slab, hb.lox, hb.LY, hb.LZ, hb.rowb, hb.SY,
+                        sample_slab_any<PV, 8, OPAC, FILT>(vol, slab, hb.lox, hb.LY, hb.LZ, hb.rowb, hb.SY,
                                                      px, py, pz, dx, dy, dz, col, col2, op);
                         release();
                         continue;
@@ -951,10 +973,10 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
                         const bool qfits = stage(4, qb);
                         group(4);
                         if (qfits)
-                            sample_slab_any<PV, 4, OPAC>(vol, slab, qb.lox, qb.LY, qb.LZ, qb.rowb,
+                            sample_slab_any<PV, 4, OPAC, FILT>(vol, slab, qb.lox, qb.LY, qb.LZ, qb.rowb,
                                                          qb.SY, px, py, pz, dx, dy, dz, col, col2, op);
                         else
-                            sample_global_any<PV, 4, OPAC>(vol, px, py, pz, dx, dy, dz, col, col2, op);
+                            sample_global_any<PV, 4, OPAC, FILT>(vol, px, py, pz, dx, dy, dz, col, col2, op);
                         release();
                     }
                 }
@@ -962,7 +984,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
 #pragma unroll 1
                 for (int k = 0; k < ng; ++k) {
                     group(CH);
-                    sample_global_any<PV, CH, OPAC>(vol, px, py, pz, dx, dy, dz, col, col2, op);
+                    sample_global_any<PV, CH, OPAC, FILT>(vol, px, py, pz, dx, dy, dz, col, col2, op);
                 }
                 release();
             }
@@ -971,7 +993,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
 #pragma unroll 1
                 for (int k = 0; k < ng; ++k) {
                     group(CH);
-                    sample_global_any<PV, CH, OPAC>(vol, px, py, pz, dx, dy, dz, col, col2, op);
+                    sample_global_any<PV, CH, OPAC, FILT>(vol, px, py, pz, dx, dy, dz, col, col2, op);
                 }
             }
             release();
@@ -995,7 +1017,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
 // OPAC: the opacity mode (alpha_pow > 0, Opacity); boxes are not loaded ahead (a spent
 // lane's position does not advance with the group).
 template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE = false,
-          bool OPAC = false, bool HALVES = true, bool SKIP = false>
+          bool OPAC = false, bool HALVES = true, bool SKIP = false, int FILT = 0>
 __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE>& vol, const RayParams& p,
                                                 uint32_t x, uint32_t y, bool inside, uint8_t* slab,
                                                 uint32_t* s_box, float* __restrict__ out,
@@ -1018,7 +1040,7 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
     const bool live = act != 0;  // uniform over the wave
     if (live) shadow_first_active(act, active, px, py, pz, dx, dy, dz);
     float col = 0.0f, col2 = 0.0f;
-    mip_staged_march<T, POW2, SLAB_BYTES, CH, SHARED, WIDE, OPAC, HALVES, SKIP>(
+    mip_staged_march<T, POW2, SLAB_BYTES, CH, SHARED, WIDE, OPAC, HALVES, SKIP, FILT>(
         vol, p, active, live, slab, s_box, px, py, pz, dx, dy, dz, 0,
         (p.nouter + 1) * (kLoopUnroll / CH), col, col2);
     if (active) {
@@ -1189,14 +1211,14 @@ __device__ __forceinline__ bool wave_tile_pixel(const RayParams& p, uint32_t* x,
 // new wave's work as soon as any of its waves ends, instead of when a workgroup's slowest
 // of four waves does.  Grid: 4 x tile_grid (interleaved or cost-ordered maps).
 template <typename T, bool POW2, int SLAB_BYTES, int CH, bool WIDE = false, bool OPAC = false,
-          bool SKIP = false>
+          bool SKIP = false, int FILT = 0>
 __global__ __launch_bounds__(64) void k_max_project_wave(PaddedVolume<T, POW2, WIDE> vol,
                                                          RayParams p, float* __restrict__ out,
                                                          float* __restrict__ alpha) {
     __shared__ Unit16 lds[SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
     uint32_t x = 0, y = 0;
     const bool inside = wave_tile_pixel(p, &x, &y);
-    mip_staged_rays<T, POW2, SLAB_BYTES, CH, false, WIDE, OPAC, true, SKIP>(
+    mip_staged_rays<T, POW2, SLAB_BYTES, CH, false, WIDE, OPAC, true, SKIP, FILT>(
         vol, p, x, y, inside, reinterpret_cast<uint8_t*>(lds), nullptr, out, alpha);
 }
 
@@ -1208,7 +1230,9 @@ __global__ __launch_bounds__(64) void k_max_project_wave(PaddedVolume<T, POW2, W
 // positions, so every LDS read stays in the box, and their samples are ignored.  The
 // wave stops marching when no lane marches; the crossing's refinement, gradient and
 // shading then run per lane from global memory (iso_finish), as in k_iso_surface.
-template <typename V, int CH, bool SLAB>
+// FILT 1: the nearest filter (one voxel per sample, inside the box staged for the linear
+// sample's corners).
+template <typename V, int CH, bool SLAB, int FILT = 0>
 __device__ __forceinline__ void iso_march_chunk(const V& vol, const uint8_t* slab, uint32_t lox,
                                                 uint32_t LY, uint32_t LZ, uint32_t rowb,
                                                 uint32_t SY, float& px, float& py, float& pz,
@@ -1224,7 +1248,15 @@ __device__ __forceinline__ void iso_march_chunk(const V& vol, const uint8_t* sla
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
         float v;
-        if constexpr (SLAB) {
+        if constexpr (SLAB && FILT == 1) {
+            float ix, iy, iz;
+            vol.nearest_f(px, py, pz, &ix, &iy, &iz);
+            const uint32_t q = (uint32_t)fmaf(iz - (float)LZ, fsz,
+                                              fmaf(iy - (float)LY, fsy, (ix - (float)lox) * (float)TS));
+            v = (float)*reinterpret_cast<const T*>(slab + q);
+        } else if constexpr (FILT == 1) {
+            v = vol.nearest(px, py, pz);
+        } else if constexpr (SLAB) {
             float cx, cy, cz, ax, ay, az;
             vol.locate_f(px, py, pz, &cx, &cy, &cz, &ax, &ay, &az);
             const float bf = fmaf(cz + oz, fsz, fmaf(cy + oy, fsy, fmaf(cx, (float)TS, ox)));
@@ -1258,13 +1290,15 @@ __device__ __forceinline__ void iso_march_chunk(const V& vol, const uint8_t* sla
 // positions advance, by the same additions -- when its box's bricks show that no marching
 // ray can change side there: bricks' maximum <= iso for rays that start below or at the
 // iso value, bricks' minimum > iso for rays that start above it.
-template <typename T, bool POW2, int SLAB_BYTES, int CH, bool WIDE = false, bool SKIP = false>
+template <typename T, bool POW2, int SLAB_BYTES, int CH, bool WIDE = false, bool SKIP = false,
+          int FILT = 0>
 __global__ __launch_bounds__(256) void k_iso_staged(PaddedVolume<T, POW2, WIDE> vol, RayParams p,
                                                     float* __restrict__ out,
                                                     float* __restrict__ alpha,
                                                     float* __restrict__ depth,
                                                     float* __restrict__ normals) {
     using PV = PaddedVolume<T, POW2, WIDE>;
+    static_assert(FILT == 0 || !WIDE, "nearest: volumes below 4 GiB");
     constexpr uint32_t E = 16u / sizeof(T);
     constexpr uint32_t TS = sizeof(T);
     __shared__ Unit16 lds[4 * SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
@@ -1347,11 +1381,13 @@ __global__ __launch_bounds__(256) void k_iso_staged(PaddedVolume<T, POW2, WIDE> 
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             if (fits)
-                iso_march_chunk<PV, CH, true>(vol, slab, lox, LY, LZ, rowb, SY, sx, sy_, sz_, tx, ty,
-                                              tz, k0, nsteps, p.iso_val, march, gt, found, ihit);
+                iso_march_chunk<PV, CH, true, FILT>(vol, slab, lox, LY, LZ, rowb, SY, sx, sy_, sz_,
+                                                    tx, ty, tz, k0, nsteps, p.iso_val, march, gt,
+                                                    found, ihit);
             else
-                iso_march_chunk<PV, CH, false>(vol, slab, lox, LY, LZ, rowb, SY, sx, sy_, sz_, tx, ty,
-                                               tz, k0, nsteps, p.iso_val, march, gt, found, ihit);
+                iso_march_chunk<PV, CH, false, FILT>(vol, slab, lox, LY, LZ, rowb, SY, sx, sy_, sz_,
+                                                     tx, ty, tz, k0, nsteps, p.iso_val, march, gt,
+                                                     found, ihit);
         }
         if (march) {  // a marching lane's own position moves on
             px = sx;
@@ -1366,7 +1402,8 @@ __global__ __launch_bounds__(256) void k_iso_staged(PaddedVolume<T, POW2, WIDE> 
         iso_miss(o, out, alpha, depth, normals);
         return;
     }
-    iso_finish<0>(vol, p, o, tnear, dt, delta, pos0, ihit, gt, direc, out, alpha, depth, normals);
+    iso_finish<FILT>(vol, p, o, tnear, dt, delta, pos0, ihit, gt, direc, out, alpha, depth,
+                     normals);
 }
 
 // Samples per staged box for a frame: the ray advances at most N_a / (16 nouter) voxels

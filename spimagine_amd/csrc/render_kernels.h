@@ -326,23 +326,55 @@ __global__ __launch_bounds__(256) void k_conv(const float* __restrict__ in, floa
     for (int k = 0; k < C; ++k) out[C * o + k] = r[k] / s;
 }
 
-// occlusion.cl:41-82
+// occlusion.cl:41-82.  Point i of every pixel draws its radius and angle from
+// lcg_random(x + a_i, y + b_i) and lcg_random(x + c_i, y + d_i), whose offsets a_i .. d_i
+// (rand_int of i and i*i) do not depend on the pixel: the workgroup computes them once
+// into an LDS table for the first kOccTable points (the same expressions, so the same
+// values), and the pixels read them from there.
+constexpr int kOccTable = 256;
+
+__device__ __forceinline__ void occ_offsets(uint32_t i, float& ra, float& rb, float& pa,
+                                            float& pb) {
+    const uint32_t ii = i * i;
+    ra = lcg_rand_int(i, ii, 0, 1000);
+    rb = lcg_rand_int(ii, i, 294, 97701);
+    pa = lcg_rand_int(ii, i, 0, 1997);
+    pb = lcg_rand_int(i, ii, 569, 17633);
+}
+
 static __global__ __launch_bounds__(256) void k_occlusion(float* __restrict__ out, int W, int H,
-                                                   int radius, int n_points,
-                                                   const float* __restrict__ depth) {
+                                                          int radius, int n_points,
+                                                          const float* __restrict__ depth) {
+    __shared__ float4 tab[kOccTable];
+    const int nt = min(n_points, kOccTable);
+    for (int i = (int)threadIdx.x; i < nt; i += 256) {
+        float4 t;
+        occ_offsets((uint32_t)i, t.x, t.y, t.z, t.w);
+        tab[i] = t;
+    }
+    __syncthreads();
     int x = blockIdx.x * 16 + (threadIdx.x & 15);
     int y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= W || y >= H) return;
     const float MPI_2 = 6.2831853071795f;
     const float depth0 = depth[x + y * W];
     float occ = 0.0f;
-    for (uint32_t i = 0; i < (uint32_t)n_points; ++i) {
-        uint32_t ii = i * i;
-        float ra = (float)x + lcg_rand_int(i, ii, 0, 1000);
-        float rb = (float)y + lcg_rand_int(ii, i, 294, 97701);
+    for (int i = 0; i < n_points; ++i) {
+        float oa, ob, oc, od;
+        if (i < kOccTable) {
+            const float4 t = tab[i];
+            oa = t.x;
+            ob = t.y;
+            oc = t.z;
+            od = t.w;
+        } else {
+            occ_offsets((uint32_t)i, oa, ob, oc, od);
+        }
+        float ra = (float)x + oa;
+        float rb = (float)y + ob;
         float r = (float)(uint32_t)radius * lcg_random((uint32_t)ra, (uint32_t)rb);
-        float pa = (float)x + lcg_rand_int(ii, i, 0, 1997);
-        float pb = (float)y + lcg_rand_int(i, ii, 569, 17633);
+        float pa = (float)x + oc;
+        float pb = (float)y + od;
         float phi = MPI_2 * lcg_random((uint32_t)pa, (uint32_t)pb);
         float sn, cs;
         spr_sincosf(phi, &sn, &cs);

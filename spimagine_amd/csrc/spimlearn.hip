@@ -517,8 +517,10 @@ __global__ __launch_bounds__(256) void k_judge_patches(const float* __restrict__
 
 // The same sum for up to kJudgeFrames frames at once against their sparse exact frames
 // (patch 1: the gx x gy grid of pixels (tx w/gx + ox, ty h/gy + oy), sparse frame k at
-// sparse + k gx gy, row-major): block b of frame k (grid kJudgeBlocks x n) sums its
-// contiguous share of the grid in a fixed order into part[k kJudgeBlocks + b];
+// sparse + k gx gy, row-major), with the sum of fourth powers beside it (the spread of the
+// squared errors, for the judge's confidence bound): block b of frame k (grid
+// kJudgeBlocks x n) sums its contiguous share of the grid in a fixed order into
+// part[k kJudgeBlocks + b] and part[kJudgeFrames kJudgeBlocks + k kJudgeBlocks + b];
 // k_judge_reduce adds a frame's partial sums in order (deterministic).
 constexpr int kJudgeFrames = 16;
 constexpr int kJudgeBlocks = 32;
@@ -530,34 +532,62 @@ __global__ __launch_bounds__(256) void k_judge_grid(JudgeFrames frames, int w, i
                                                     const float* __restrict__ sparse, int gx,
                                                     int gy, int ox, int oy,
                                                     float* __restrict__ part) {
-    __shared__ float red[256];
+    __shared__ float red[2][256];
     const int k = blockIdx.y, b = blockIdx.x;
     const int n = gx * gy, per = (n + kJudgeBlocks - 1) / kJudgeBlocks;
     const int i0 = b * per, i1 = min(n, i0 + per);
     const int cw = w / gx, ch = h / gy;
     const float* frame = frames.f[k];
     const float* sp = sparse + (size_t)k * n;
-    float v = 0.0f;
+    float v2 = 0.0f, v4 = 0.0f;
     for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
         const int ty = i / gx, tx = i - ty * gx;
         const float d = frame[(size_t)(ty * ch + oy) * w + tx * cw + ox] - sp[i];
-        v = fmaf(d, d, v);
+        const float d2 = d * d;
+        v2 += d2;
+        v4 = fmaf(d2, d2, v4);
     }
-    red[threadIdx.x] = v;
+    red[0][threadIdx.x] = v2;
+    red[1][threadIdx.x] = v4;
     __syncthreads();
     for (int r = 128; r > 0; r >>= 1) {
-        if (threadIdx.x < r) red[threadIdx.x] += red[threadIdx.x + r];
+        if (threadIdx.x < r) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + r];
+            red[1][threadIdx.x] += red[1][threadIdx.x + r];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[k * kJudgeBlocks + b] = red[0];
+    if (threadIdx.x == 0) {
+        part[k * kJudgeBlocks + b] = red[0][0];
+        part[kJudgeFrames * kJudgeBlocks + k * kJudgeBlocks + b] = red[1][0];
+    }
 }
 
-__global__ void k_judge_reduce(const float* __restrict__ part, int n, float* __restrict__ sse) {
+__global__ void k_judge_reduce(const float* __restrict__ part, int n, float* __restrict__ sse,
+                               float* __restrict__ sse4) {
     const int k = threadIdx.x;
     if (k >= n) return;
-    float v = 0.0f;
-    for (int b = 0; b < kJudgeBlocks; ++b) v += part[k * kJudgeBlocks + b];
+    float v = 0.0f, v4 = 0.0f;
+    for (int b = 0; b < kJudgeBlocks; ++b) {
+        v += part[k * kJudgeBlocks + b];
+        v4 += part[kJudgeFrames * kJudgeBlocks + k * kJudgeBlocks + b];
+    }
     sse[k] = v;
+    if (sse4) sse4[k] = v4;
+}
+
+// The judge's decision with a confidence bound: with m2 = sse / npix and m4 = sse4 / npix
+// the mean squared and fourth-power errors of the judge's pixels, the squared error's
+// variance is m4 - m2^2 and the standard error of m2 is sqrt((m4 - m2^2) / npix);
+// flags[k] = 1 (fall back to the exact frame) unless m2 + z standard errors <= mse_max.
+__global__ void k_judge_flags_bound(const float* __restrict__ sse, const float* __restrict__ sse4,
+                                    int n, float inv_npix, float mse_max, float z,
+                                    uint32_t* __restrict__ flags) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const float m2 = sse[k] * inv_npix, m4 = sse4[k] * inv_npix;
+    const float ucb = m2 + z * sqrtf(fmaxf(m4 - m2 * m2, 0.0f) * inv_npix);
+    flags[k] = ucb <= mse_max ? 0u : 1u;
 }
 
 // flags[k] = 1 when frame k's judged mean squared error sse[k] / npix exceeds mse_max (or
@@ -600,7 +630,7 @@ struct spl_net {
           *gQ1 = nullptr, *dr = nullptr, *y = nullptr;
     float *wpart = nullptr, *lpart = nullptr;
     size_t lpart_n = 0;
-    float* judge_part = nullptr;  // spl_judge_sse_frames partial sums (kJudgeFrames blocks)
+    float* judge_part = nullptr;  // spl_judge_sse_frames partial sums (2 x kJudgeFrames blocks)
 };
 
 namespace {
@@ -962,7 +992,7 @@ int spl_judge_sse(spl_net* t, const float* d_frame, int w, int h, const float* d
 
 int spl_judge_sse_frames(spl_net* t, const float* const* frames, int n, int w, int h,
                          const float* d_sparse, int gx, int gy, int ox, int oy, float* d_sse,
-                         void* stream) {
+                         float* d_sse4, void* stream) {
     if (!t) return fail(nullptr, SPL_ERR_INVALID_ARG, "NULL network");
     if (!frames || !d_sparse || !d_sse || n <= 0 || w <= 0 || h <= 0 || gx < 1 || gy < 1 ||
         ox < 0 || oy < 0 || ox >= w / gx || oy >= h / gy)
@@ -972,7 +1002,7 @@ int spl_judge_sse_frames(spl_net* t, const float* const* frames, int n, int w, i
     DeviceGuard g(t->device);
     hipStream_t s = stream ? (hipStream_t)stream : t->stream;
     if (!t->judge_part)
-        HIPCALL(t, hipMalloc(&t->judge_part, sizeof(float) * kJudgeFrames * kJudgeBlocks));
+        HIPCALL(t, hipMalloc(&t->judge_part, sizeof(float) * 2 * kJudgeFrames * kJudgeBlocks));
     const size_t per = (size_t)gx * gy;
     for (int k0 = 0; k0 < n; k0 += kJudgeFrames) {
         const int m = std::min(kJudgeFrames, n - k0);
@@ -981,8 +1011,21 @@ int spl_judge_sse_frames(spl_net* t, const float* const* frames, int n, int w, i
         hipLaunchKernelGGL(k_judge_grid, dim3(kJudgeBlocks, m), dim3(256), 0, s, jf, w, h,
                            d_sparse + k0 * per, gx, gy, ox, oy, t->judge_part);
         hipLaunchKernelGGL(k_judge_reduce, dim3(1), dim3(64), 0, s, (const float*)t->judge_part, m,
-                           d_sse + k0);
+                           d_sse + k0, d_sse4 ? d_sse4 + k0 : nullptr);
     }
+    HIPCALL(t, hipGetLastError());
+    return SPL_OK;
+}
+
+int spl_judge_flags_bound(spl_net* t, const float* d_sse, const float* d_sse4, int n, int npix,
+                          float mse_max, float z, unsigned int* d_flags, void* stream) {
+    if (!t) return fail(nullptr, SPL_ERR_INVALID_ARG, "NULL network");
+    if (!d_sse || !d_sse4 || !d_flags || n <= 0 || npix <= 0 || !(z >= 0.0f))
+        return fail(t, SPL_ERR_INVALID_ARG, "invalid judge arguments");
+    DeviceGuard g(t->device);
+    hipStream_t s = stream ? (hipStream_t)stream : t->stream;
+    hipLaunchKernelGGL(k_judge_flags_bound, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       d_sse, d_sse4, n, 1.0f / (float)npix, mse_max, z, (uint32_t*)d_flags);
     HIPCALL(t, hipGetLastError());
     return SPL_OK;
 }

@@ -1292,12 +1292,14 @@ int spr_render_batch(spr_ctx* c, const spr_params* params, const spr_job* jobs, 
     return SPR_OK;
 }
 
-int spr_render_frames(spr_ctx* c, const spr_frame* frames, int n, float alpha_pow, int filter,
-                      void* stream) {
-    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+}  // extern "C"
+
+namespace {
+
+// The job table of a batch of frames of their own sizes and settings (first[] left 0).
+int frames_to_jobs(spr_ctx* c, const spr_frame* frames, int n, std::vector<BatchJob>& host,
+                   int* dtype, bool* pow2, int* nmax) {
     if (!frames || n <= 0) return fail(c, SPR_ERR_INVALID_ARG, "empty batch");
-    if (filter != SPR_FILTER_LINEAR && filter != SPR_FILTER_NEAREST)
-        return fail(c, SPR_ERR_INVALID_ARG, "unknown filter %d", filter);
     std::vector<spr_job> jobs((size_t)n);
     for (int i = 0; i < n; ++i) {
         const spr_frame& f = frames[i];
@@ -1312,15 +1314,8 @@ int spr_render_frames(spr_ctx* c, const spr_frame* frames, int n, float alpha_po
         jobs[i].d_out = f.d_out;
         jobs[i].d_alpha = f.d_alpha;
     }
-    std::vector<BatchJob> host;
-    int dtype, nmax;
-    bool pow2;
-    int rc = build_jobs(c, jobs.data(), n, host, &dtype, &pow2, &nmax);
+    const int rc = build_jobs(c, jobs.data(), n, host, dtype, pow2, nmax);
     if (rc) return rc;
-    // each frame's own settings and its first item in each queue; the box length follows
-    // the frame with the longest ray step
-    BatchQueues q{};
-    int ch = kLoopUnroll;
     for (int i = 0; i < n; ++i) {
         const spr_frame& f = frames[i];
         BatchJob& b = host[i];
@@ -1331,6 +1326,30 @@ int spr_render_frames(spr_ctx* c, const spr_frame* frames, int n, float alpha_po
         b.nouter = f.max_steps / kLoopUnroll;
         b.W = (uint32_t)f.w;
         b.H = (uint32_t)f.h;
+    }
+    return SPR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spr_render_frames(spr_ctx* c, const spr_frame* frames, int n, float alpha_pow, int filter,
+                      void* stream) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (filter != SPR_FILTER_LINEAR && filter != SPR_FILTER_NEAREST)
+        return fail(c, SPR_ERR_INVALID_ARG, "unknown filter %d", filter);
+    std::vector<BatchJob> host;
+    int dtype, nmax;
+    bool pow2;
+    int rc = frames_to_jobs(c, frames, n, host, &dtype, &pow2, &nmax);
+    if (rc) return rc;
+    // each frame's first item in each queue; the box length follows the frame with the
+    // longest ray step
+    BatchQueues q{};
+    int ch = kLoopUnroll;
+    for (int i = 0; i < n; ++i) {
+        BatchJob& b = host[i];
         const uint32_t ntx = (b.W + 15u) / 16u, nty = (b.H + 15u) / 16u;
         for (uint32_t k = 0; k < 8; ++k) {
             b.first[k] = q.count[k];
@@ -1358,6 +1377,64 @@ int spr_render_frames(spr_ctx* c, const spr_frame* frames, int n, float alpha_po
     hipError_t e = stage_jobs(c, host.data(), jbytes, d_jobs, s);
     if (e == hipSuccess) e = dispatch_batch(c, prm, dtype, pow2, ch, d_jobs, p, q, d_counters, s);
     (void)hipFreeAsync(buf, s);
+    HIPCALL(c, e);
+    return SPR_OK;
+}
+
+int spr_render_frames_split(spr_ctx* c, const spr_frame* frames, int n, int seg, void* stream) {
+    if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
+    if (seg == 0) seg = kSplitSegDefault;
+    if (seg != 32 && seg != 64 && seg != 128)
+        return fail(c, SPR_ERR_INVALID_ARG, "segment length %d (32, 64 or 128)", seg);
+    std::vector<BatchJob> host;
+    int dtype, nmax;
+    bool pow2;
+    int rc = frames_to_jobs(c, frames, n, host, &dtype, &pow2, &nmax);
+    if (rc) return rc;
+    // job k's first work item (8 x 8 rays x one segment) and first pixel; staged in LDS
+    // slabs (tuning slab_kb > 0) when every frame's ray step allows 16-sample boxes
+    uint64_t items = 0, pix = 0;
+    int ch = kLoopUnroll;
+    for (int i = 0; i < n; ++i) {
+        BatchJob& b = host[i];
+        ch = std::min(ch, chunk_for(c->tune, b.nouter, std::max(b.nx, std::max(b.ny, b.nz))));
+        const uint64_t total = ((uint64_t)b.nouter + 1u) * kLoopUnroll;
+        const uint64_t groups = (uint64_t)((b.W + 7u) / 8u) * ((b.H + 7u) / 8u);
+        b.first[0] = (uint32_t)items;
+        b.first[1] = (uint32_t)pix;
+        items += groups * ((total + (uint64_t)seg - 1u) / (uint64_t)seg);
+        pix += (uint64_t)b.W * b.H;
+        if (items >= (1ull << 31) || pix >= (1ull << 31))
+            return fail(c, SPR_ERR_INVALID_ARG, "batch too large for one split launch");
+    }
+    const bool staged = c->tune.slab_kb > 0 && ch == kLoopUnroll;
+    DeviceGuard g(c->device);
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    spr_params prm{};
+    prm.max_steps = frames[0].max_steps;
+    RayParams base = make_params(prm, c->tune.tile_map, frames[0].w, frames[0].h);
+    staged_tuning(c->tune, base);
+    const size_t jbytes = (size_t)n * sizeof(BatchJob);
+    BatchJob* d_jobs = nullptr;
+    HIPCALL(c, hipMallocAsync((void**)&d_jobs, jbytes, s));
+    hipError_t e = stage_jobs(c, host.data(), jbytes, d_jobs, s);
+    if (e == hipSuccess) {
+        const uint32_t nj = (uint32_t)n, ni = (uint32_t)items, np = (uint32_t)pix;
+        switch (dtype) {
+            case SPR_DTYPE_F32:
+                e = pow2 ? TypedLaunch<float, true>::split(seg, staged, d_jobs, nj, ni, np, base, s)
+                         : TypedLaunch<float, false>::split(seg, staged, d_jobs, nj, ni, np, base, s);
+                break;
+            case SPR_DTYPE_U16:
+                e = pow2 ? TypedLaunch<uint16_t, true>::split(seg, staged, d_jobs, nj, ni, np, base, s)
+                         : TypedLaunch<uint16_t, false>::split(seg, staged, d_jobs, nj, ni, np, base, s);
+                break;
+            default:
+                e = pow2 ? TypedLaunch<uint8_t, true>::split(seg, staged, d_jobs, nj, ni, np, base, s)
+                         : TypedLaunch<uint8_t, false>::split(seg, staged, d_jobs, nj, ni, np, base, s);
+        }
+    }
+    (void)hipFreeAsync(d_jobs, s);
     HIPCALL(c, e);
     return SPR_OK;
 }

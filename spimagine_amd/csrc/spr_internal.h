@@ -14,6 +14,7 @@
 
 #include "mip_staged.h"
 #include "render_kernels.h"
+#include "split_rays.h"
 #include "spimrender.h"
 
 namespace spr {
@@ -120,6 +121,13 @@ struct TypedLaunch {
     // the persistent batch kernel (mode: see dispatch in spimrender.hip)
     static hipError_t batch(const LaunchTuning& t, int mode, int ch, const BatchJob* d_jobs,
                             const RayParams& p, const BatchQueues& q, uint32_t* counters,
+                            hipStream_t s);
+    // max projections of frames of scattered rays, each ray's samples split into segments
+    // of seg (32, 64 or 128) samples (split_rays.h), staged in LDS slabs or gathered from
+    // global memory; job k's first item and first pixel in first[0] and first[1]; base: the
+    // launch's staging settings
+    static hipError_t split(int seg, bool staged, const BatchJob* d_jobs, uint32_t njobs,
+                            uint32_t nitems, uint32_t npix, const RayParams& base,
                             hipStream_t s);
     // coarse max projections of two padded slots of this type and shape
     static hipError_t coarse_pair(const VolSlot& a, const VolSlot& b, const RayParams& p,

@@ -277,6 +277,15 @@ struct PaddedVolume {
         return trilerp(v000, v100, v010, v110, v001, v101, v011, v111, ax, ay, az);
     }
 
+    // padded indices of nearest's voxel as floats (exact integers): the staged kernels'
+    // slab offsets of a nearest sample
+    __device__ __forceinline__ void nearest_f(float px, float py_, float pz, float* ix1,
+                                              float* iy1, float* iz1) const {
+        *ix1 = fminf(fmaxf(floorf(px * fnx), 0.0f), mx) + 1.0f;
+        *iy1 = fminf(fmaxf(floorf(py_ * fny), 0.0f), my) + 1.0f;
+        *iz1 = fminf(fmaxf(floorf(pz * fnz), 0.0f), mz) + 1.0f;
+    }
+
     __device__ __forceinline__ float nearest(float px, float py_, float pz) const {
         float ux = px * fnx, uy = py_ * fny, uz = pz * fnz;
         uint32_t ix1 = (uint32_t)((int)fminf(fmaxf(floorf(ux), 0.0f), mx) + 1);

@@ -25,7 +25,8 @@ LIB_PATH = os.path.join(HERE, "libspimlearn.so")
 WEIGHTS_PATH = os.path.join(HERE, "weights", "mip_x2_q4.npz")
 EXPORTS = ("spl_abi_version", "spl_param_count", "spl_create", "spl_destroy", "spl_last_error",
            "spl_set_params", "spl_get_params", "spl_forward", "spl_gradients", "spl_train_step",
-           "spl_judge_sse", "spl_judge_sse_frames", "spl_judge_flags", "spl_upsample2")
+           "spl_judge_sse", "spl_judge_sse_frames", "spl_judge_flags", "spl_judge_flags_bound",
+           "spl_upsample2")
 C = 32
 LAYER_NAMES = ("e0", "e0b", "e1", "e1b", "e2a", "e2", "u1", "u0", "u0b")
 PSNR_THRESHOLD = 40.0
@@ -61,7 +62,10 @@ def load():
     L.spl_judge_sse.argtypes = [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int,
                                 c_int, c_int, c_void_p, c_int, c_void_p]
     L.spl_judge_sse_frames.argtypes = [c_void_p, ctypes.POINTER(c_void_p), c_int, c_int, c_int,
-                                       c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p]
+                                       c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                       c_void_p]
+    L.spl_judge_flags_bound.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_float,
+                                        c_float, c_void_p, c_void_p]
     L.spl_judge_flags.argtypes = [c_void_p, c_void_p, c_int, c_int, c_float, c_void_p, c_void_p]
     L.spl_upsample2.argtypes = [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]
     _lib_learn = L
@@ -163,12 +167,20 @@ class Net(object):
         self._check(self.lib.spl_judge_sse(self.handle, d_frame, w, h, d_patches, patch, gx, gy,
                                            ox, oy, d_sse, int(bool(accumulate)), stream))
 
-    def judge_sse_frames(self, d_frames, w, h, d_sparse, gx, gy, ox, oy, d_sse, stream=None):
-        """Squared errors of n frames (device pointers d_frames) against their sparse exact
-        frames in one pass (spl_judge_sse_frames)."""
+    def judge_sse_frames(self, d_frames, w, h, d_sparse, gx, gy, ox, oy, d_sse, d_sse4=None,
+                         stream=None):
+        """Summed squared (and, with d_sse4, fourth-power) errors of n frames (device
+        pointers d_frames) against their sparse exact frames in one pass
+        (spl_judge_sse_frames)."""
         arr = (ctypes.c_void_p * len(d_frames))(*d_frames)
         self._check(self.lib.spl_judge_sse_frames(self.handle, arr, len(d_frames), w, h, d_sparse,
-                                                  gx, gy, ox, oy, d_sse, stream))
+                                                  gx, gy, ox, oy, d_sse, d_sse4, stream))
+
+    def judge_flags_bound(self, d_sse, d_sse4, n, npix, mse_max, z, d_flags, stream=None):
+        """flags[k] = 1 unless the judged MSE + z standard errors <= mse_max, on the device
+        (spl_judge_flags_bound)."""
+        self._check(self.lib.spl_judge_flags_bound(self.handle, d_sse, d_sse4, int(n), int(npix),
+                                                   float(mse_max), float(z), d_flags, stream))
 
     def judge_flags(self, d_sse, n, npix, mse_max, d_flags, stream=None):
         """flags[k] = sse[k] / npix > mse_max, on the device (spl_judge_flags)."""
@@ -205,8 +217,10 @@ def psnr(a, b):
     return float("inf") if mse == 0 else 10. * math.log10(1. / mse)
 
 
-JUDGE_STRIDE = 8
-JUDGE_OFFSET = 1   # judge pixels (8i+1, 8j+1): between the coarse rays, where errors are largest
+JUDGE_STRIDE = 16  # the judge traces one pixel in 16 x 16 of every frame
+JUDGE_OFFSET = 1   # judge pixels (S i + 1, S j + 1): off the coarse rays, where errors are largest
+JUDGE_SEG = 64     # samples per segment of the judge's split rays (spr_render_frames_split)
+JUDGE_Z = 3.       # standard errors in the judge's upper confidence bound on a frame's MSE
 
 
 def sparse_params(p, w, h, stride=JUDGE_STRIDE, offset=JUDGE_OFFSET):
@@ -234,12 +248,17 @@ def coarse_params(p):
     return q
 
 
-class JudgedBatch(object):
-    """Device results of one render_judged_many call: the judged error sums and the
-    fallback flags (read them once the stream has passed the call)."""
+def _db(mse):
+    return float("inf") if mse <= 0 else 10. * math.log10(1. / float(mse))
 
-    def __init__(self, sse, flags, npix):
-        self.sse, self.flags, self.npix = sse, flags, npix
+
+class JudgedBatch(object):
+    """Device results of one render_judged_many call: the judged error sums (squared and
+    fourth powers) and the fallback flags (read them once the stream has passed the
+    call); z: the confidence bound's standard errors."""
+
+    def __init__(self, sse, sse4, flags, npix, z):
+        self.sse, self.sse4, self.flags, self.npix, self.z = sse, sse4, flags, npix, z
 
     def kept(self):
         """Per frame: True when the learned frame was kept (no exact fallback)."""
@@ -247,8 +266,15 @@ class JudgedBatch(object):
 
     def estimates(self):
         """Per frame: the PSNR estimated on the judge's pixels (dB)."""
-        m = self.sse.cpu().numpy() / float(self.npix)
-        return [float("inf") if v == 0 else 10. * math.log10(1. / float(v)) for v in m]
+        return [_db(v) for v in self.sse.cpu().numpy().astype(np.float64) / self.npix]
+
+    def bounds(self):
+        """Per frame: the PSNR of the judge's upper confidence bound on the MSE (dB), the
+        value compared with the acceptance level."""
+        m2 = self.sse.cpu().numpy().astype(np.float64) / self.npix
+        m4 = self.sse4.cpu().numpy().astype(np.float64) / self.npix
+        ucb = m2 + self.z * np.sqrt(np.maximum(m4 - m2 * m2, 0.) / self.npix)
+        return [_db(v) for v in ucb]
 
 
 class LearnedMIP(object):
@@ -274,12 +300,21 @@ class LearnedMIP(object):
         self.enabled = {}   # slot -> validated (min PSNR >= threshold)
         self._bufs = {}
         self._side = None   # the judge's second stream (sparse exact frames)
-        # the judge: one pixel in judge_stride x judge_stride, frames kept when their
-        # estimated PSNR reaches threshold + margin; judge_slab_kb: the sparse frames'
-        # staging (their rays run judge_stride pixels apart; None: the context's)
+        # the judge: one pixel in judge_stride x judge_stride; a frame is kept when the
+        # PSNR of its judged MSE plus judge_z standard errors reaches threshold + margin;
+        # judge_slab_kb: the sparse frames' staging (their rays run judge_stride pixels
+        # apart; None: the context's); judge_seg: samples per segment of the sparse
+        # frames' split rays (spr_render_frames_split; None: one persistent batch launch
+        # per set of non-pose parameters, spr_render_batch)
         self.judge_stride = JUDGE_STRIDE
-        self.margin = .5
+        self.judge_z = JUDGE_Z
+        self.margin = 0.
         self.judge_slab_kb = None
+        self.judge_seg = JUDGE_SEG
+        # the sparse frames on a second stream, concurrently with the learned frames (False:
+        # on the caller's stream before them; measured faster: concurrent, the two thrash
+        # each other's caches and LDS)
+        self.judge_concurrent = False
 
     def attach(self, slot):
         """The slots whose coarse projections feed the network (created once): for pooled
@@ -330,17 +365,22 @@ class LearnedMIP(object):
 
     def render_judged_many(self, slot, plist, outs, alphas, stream, margin=None, wait=True):
         """Frames with a per-frame judge: the learned frames, the exact renderer on one pixel
-        in 8 x 8 of every frame (a (w/8) x (h/8) render, sparse_params; the sparse frames of
-        all frames that share their non-pose parameters in one batched launch), each learned
-        frame's PSNR on those pixels, and the exact full frame for every frame whose
-        estimate is below threshold + margin.  A scattered grid: the error of a learned
-        frame sits in few pixels, and 1/64 of the pixels estimates the PSNR well only when
-        they are spread over the whole frame (pixel blocks of the same count miss or
-        over-weight the few bright structures).
-        Nothing waits on the host: the sparse exact frames render on a second stream while
-        the learned frames render on `stream`; then, on `stream`, the judge's error sums
-        (spl_judge_sse), its decisions on the device (spl_judge_flags) and the exact
-        fallbacks, each rendered only if its frame's flag is set (spr_render_slot_if).
+        in S x S of every frame (S = judge_stride; a (w/S) x (h/S) render, sparse_params;
+        the sparse frames of all frames in one launch of split rays,
+        spr_render_frames_split, or with judge_seg None one batched launch per set of
+        non-pose parameters), each learned frame's mean squared error on those pixels and
+        its standard error (from the fourth-power sum), and the exact full frame for every
+        frame whose MSE + judge_z standard errors is above the MSE of threshold + margin
+        dB.  A scattered grid: the error of a learned frame sits in few pixels, and a
+        sample of the pixels estimates it well only when spread over the whole frame
+        (pixel blocks of the same count miss or over-weight the few bright structures);
+        the bound widens where the sampled errors are concentrated.
+        Nothing waits on the host: the sparse exact frames render on `stream` before the
+        learned frames (or, with judge_concurrent, on a second stream while the learned
+        frames render on `stream`); then, on `stream`, the judge's error sums
+        (spl_judge_sse_frames), its decisions on the device (spl_judge_flags_bound) and
+        the exact fallbacks, each rendered only if its frame's flag is set
+        (spr_render_slot_if).
         outs / alphas: (h, w) float32 torch tensors; stream: a torch.cuda.Stream.  Learned
         frames take alpha from the coarse pass (exact at even pixels, nearest-upsampled).
         wait=True: synchronises `stream` and returns (kept flags, estimated PSNRs);
@@ -350,54 +390,71 @@ class LearnedMIP(object):
         h, w = outs[0].shape
         S = self.judge_stride
         margin = self.margin if margin is None else margin
-        if w % (8 * S) or h % (8 * S):
-            raise ValueError("judged frames need w and h multiples of %d" % (8 * S))
+        if w % S or h % S:
+            raise ValueError("judged frames need w and h multiples of the judge stride %d" % S)
         sh = stream.cuda_stream
         n = len(plist)
         sw, sh_ = w // S, h // S
-        if self._side is None:
-            self._side = torch.cuda.Stream()
-        side = self._side
+        if self.judge_concurrent:
+            if self._side is None:
+                self._side = torch.cuda.Stream()
+            side = self._side
+        else:
+            side = stream
         with torch.cuda.stream(stream):
             sparse = torch.empty((n, sh_, sw), device="cuda")
             salpha = torch.empty_like(sparse)
-            sse = torch.empty((n,), device="cuda")
+            sse = torch.empty((2, n), device="cuda")  # squared, fourth-power error sums
             flags = torch.empty((n,), dtype=torch.int32, device="cuda")
-        # the sparse exact frames on the side stream, concurrently with the learned frames
-        side.wait_stream(stream)
+        # the sparse exact frames: on `stream` before the learned frames, or on a second
+        # stream concurrently with them (judge_concurrent)
+        if side is not stream:
+            side.wait_stream(stream)
         groups = {}
         for k, p in enumerate(plist):
             groups.setdefault(bytes(p)[_lib.SprParams.box.offset:], []).append(k)
+        split = self.judge_seg is not None and all(
+            p.alpha_pow == 0 and p.filter == _lib.FILTER_LINEAR for p in plist)
         keep = None
-        if self.judge_slab_kb is not None:  # the sparse frames' own launch tuning
+        if self.judge_slab_kb is not None:  # the sparse frames' launch tuning
             keep = self.ctx.get_tuning("slab_kb")
             self.ctx.set_tuning("slab_kb", self.judge_slab_kb)
         try:
-            for ks in groups.values():
-                qs = [sparse_params(plist[k], w, h, stride=S) for k in ks]
-                jobs = [(slot, list(q.inv_p), list(q.inv_m), sparse[k].data_ptr(),
-                         salpha[k].data_ptr()) for k, q in zip(ks, qs)]
-                self.ctx.render_batch(jobs, sw, sh_, params=qs[0], stream=side.cuda_stream)
+            if split:  # every ray's samples split over several waves (few, scattered rays)
+                frames = [_lib.make_frame(slot, sparse_params(p, w, h, stride=S), sw, sh_,
+                                          sparse[k].data_ptr(), salpha[k].data_ptr())
+                          for k, p in enumerate(plist)]
+                self.ctx.render_frames_split(frames, seg=self.judge_seg, stream=side.cuda_stream)
+            else:
+                for ks in groups.values():
+                    qs = [sparse_params(plist[k], w, h, stride=S) for k in ks]
+                    jobs = [(slot, list(q.inv_p), list(q.inv_m), sparse[k].data_ptr(),
+                             salpha[k].data_ptr()) for k, q in zip(ks, qs)]
+                    self.ctx.render_batch(jobs, sw, sh_, params=qs[0], stream=side.cuda_stream)
         finally:
             if keep is not None:
                 self.ctx.set_tuning("slab_kb", keep)
-        sparse.record_stream(side)
-        salpha.record_stream(side)
-        done = torch.cuda.Event()
-        done.record(side)
+        if side is not stream:
+            sparse.record_stream(side)
+            salpha.record_stream(side)
+            done = torch.cuda.Event()
+            done.record(side)
         coarse_alpha = self._buffer(("alpha", w, h), (h // 2, w // 2))
         for p, out, alpha in zip(plist, outs, alphas):
             self.render(slot, p, w, h, out.data_ptr(), sh)
             self.net.upsample2(coarse_alpha.data_ptr(), w // 2, h // 2, alpha.data_ptr(), sh)
-        stream.wait_event(done)
+        if side is not stream:
+            stream.wait_event(done)
         self.net.judge_sse_frames([o.data_ptr() for o in outs], w, h, sparse.data_ptr(), sw, sh_,
-                                  JUDGE_OFFSET, JUDGE_OFFSET, sse.data_ptr(), stream=sh)
+                                  JUDGE_OFFSET, JUDGE_OFFSET, sse[0].data_ptr(), sse[1].data_ptr(),
+                                  stream=sh)
         mse_max = 10. ** (-(self.threshold + margin) / 10.)
-        self.net.judge_flags(sse.data_ptr(), n, sw * sh_, mse_max, flags.data_ptr(), stream=sh)
+        self.net.judge_flags_bound(sse[0].data_ptr(), sse[1].data_ptr(), n, sw * sh_, mse_max,
+                                   self.judge_z, flags.data_ptr(), stream=sh)
         for k, (p, out, alpha) in enumerate(zip(plist, outs, alphas)):
             self.ctx.render_slot_if(slot, p, w, h, out.data_ptr(), alpha.data_ptr(),
                                     flags.data_ptr() + 4 * k, stream=sh)
-        res = JudgedBatch(sse, flags, sw * sh_)
+        res = JudgedBatch(sse[0], sse[1], flags, sw * sh_, self.judge_z)
         if not wait:
             return res
         stream.synchronize()

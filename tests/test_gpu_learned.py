@@ -239,15 +239,17 @@ def test_sparse_judge_rays_are_the_full_frame_pixels(torch, W, H):
     s.synchronize()
     assert kept is True and abs(est2 - est) < 1e-6
     assert torch.equal(out, ref)
-    # the judge's pixels: (8i+1, 8j+1), traced by the shifted sparse render
-    shifted = torch.empty_like(sparse)
-    ctx.render_slot(slot, learned.sparse_params(p, W, H), _lib.METHOD_MAX_PROJECT, W // 8, H // 8,
-                    shifted.data_ptr(), sa.data_ptr(), stream=s.cuda_stream)
+    # the judge's pixels: (S i + 1, S j + 1), traced by the shifted sparse render
+    S = learned.JUDGE_STRIDE
+    shifted = torch.empty((H // S, W // S), device="cuda")
+    sa2 = torch.empty_like(shifted)
+    ctx.render_slot(slot, learned.sparse_params(p, W, H), _lib.METHOD_MAX_PROJECT, W // S, H // S,
+                    shifted.data_ptr(), sa2.data_ptr(), stream=s.cuda_stream)
     s.synchronize()
-    mse = float(((ref[1::8, 1::8].double() - shifted.double()) ** 2).mean())
+    mse = float(((ref[1::S, 1::S].double() - shifted.double()) ** 2).mean())
     assert abs(est2 - 10 * np.log10(1. / mse)) < 1e-3
     # and they are the full frame's rays at those pixels up to float rounding
-    assert float((shifted - full[1::8, 1::8]).abs().max()) < 1e-3
+    assert float((shifted - full[1::S, 1::S]).abs().max()) < 1e-3
     # learned frames carry the coarse pass's alpha: exact at even pixels
     assert torch.equal(alpha[::2, ::2], fa[::2, ::2])
 
@@ -287,15 +289,29 @@ def test_judge_sse_frames_layout(torch):
     sparse = torch.rand((n, gy, gx), device="cuda", generator=g)
     sse = torch.full((n,), -1., device="cuda")
     torch.cuda.synchronize()
+    sse4 = torch.full((n,), -1., device="cuda")
     net.judge_sse_frames([f.data_ptr() for f in frames], w, h, sparse.data_ptr(), gx, gy, ox, oy,
-                         sse.data_ptr())
+                         sse.data_ptr(), sse4.data_ptr())
     torch.cuda.synchronize()
     S = sparse.cpu().double().numpy()
-    got = sse.cpu().double().numpy()
+    got, got4 = sse.cpu().double().numpy(), sse4.cpu().double().numpy()
     for k, f in enumerate(frames):
         F = f.cpu().double().numpy()
-        want = ((F[oy::h // gy, ox::w // gx][:gy, :gx] - S[k]) ** 2).sum()
-        assert abs(got[k] - want) < 1e-4 * want, k
+        d2 = (F[oy::h // gy, ox::w // gx][:gy, :gx] - S[k]) ** 2
+        assert abs(got[k] - d2.sum()) < 1e-4 * d2.sum(), k
+        assert abs(got4[k] - (d2 ** 2).sum()) < 1e-4 * (d2 ** 2).sum(), k
+    # the bounded decision: keep iff m2 + z sqrt((m4 - m2^2) / npix) <= mse_max
+    npix = gx * gy
+    m2, m4 = got / npix, got4 / npix
+    ucb = m2 + 2.5 * np.sqrt(np.maximum(m4 - m2 * m2, 0) / npix)
+    flags = torch.full((n,), 7, dtype=torch.int32, device="cuda")
+    mse_max = float(np.median(ucb))
+    net.judge_flags_bound(sse.data_ptr(), sse4.data_ptr(), n, npix, mse_max, 2.5, flags.data_ptr())
+    torch.cuda.synchronize()
+    want = (ucb > mse_max).astype(np.int32)
+    close = np.abs(ucb - mse_max) < 1e-6 * mse_max  # float32 rounding at the boundary
+    np.testing.assert_array_equal(flags.cpu().numpy()[~close], want[~close])
+    assert 0 < want.sum() < n
     with pytest.raises(learned.SplError):
         net.judge_sse_frames([f.data_ptr() for f in frames], w, h, sparse.data_ptr(), gx, gy,
                              w // gx, oy, sse.data_ptr())
