@@ -161,8 +161,8 @@ def bench_headline(args):
     M, P = synth.headline_camera()
     invP, invM = render_matrices(M, P, (nx, ny, nz))
 
-    def set_pose(iP, iM):
-        ctx.set_params(iP, iM, BOX, lo, hi, 1., 0., .1, 21, 30, S, _lib.FILTER_LINEAR)
+    def set_pose(iP, iM, filt=_lib.FILTER_LINEAR):
+        ctx.set_params(iP, iM, BOX, lo, hi, 1., 0., .1, 21, 30, S, filt)
 
     set_pose(invP, invM)
     out = torch.empty((H, W), dtype=torch.float32, device="cuda")
@@ -244,6 +244,11 @@ def bench_headline(args):
     # empty-space skipping off and on (SPR_TUNE_SKIP), the headline pose and the 8 poses
     skipping = skip_table(ctx, stream, frame, lambda k: set_pose(*([(invP, invM)] + mats)[k]),
                           [n_samp] + samples_pose, args.skip_frames)
+    # the nearest filter (interpolation="nearest"): staged as the linear filter, or every
+    # sample gathered from global memory (slab_kb 0)
+    nearest = filter_table(ctx, stream, frame,
+                           lambda k: set_pose(*([(invP, invM)] + mats)[k], _lib.FILTER_NEAREST),
+                           [n_samp] + samples_pose, args.skip_frames)
     set_pose(invP, invM)
 
     if args.calibrate_copy_mib:
@@ -274,7 +279,7 @@ def bench_headline(args):
         frames_per_s=round(1e3 / ms_per_step, 2),
         latency_ms={"p50": round(pct(lat, 50), 4), "p99": round(pct(lat, 99), 4),
                     "frames": len(lat_ev), "poses": 8, "gsamples_per_s": round(pose_gsps, 3)},
-        skipping=skipping,
+        skipping=skipping, nearest_filter=nearest,
         roofline=roofline(b_alg, kernel_ms, traffic_from_profile("headline")),
         cpu_baseline=cpu, parity_max_abs_diff_vs_oracle=parity, upload=upload)
 
@@ -309,6 +314,35 @@ def skip_table(ctx, stream, frame, set_pose, samples, reps):
             "poses_gsamples_per_s": round(float(np.mean([n / (t * 1e6) for n, t in
                                                          zip(samples[1:], ms[1:])])), 2)}
     ctx.set_tuning("skip", keep)
+    return out
+
+
+def filter_table(ctx, stream, frame, set_pose, samples, reps):
+    """Per-pose frame times of the nearest filter (set_pose(k) selects it) staged in LDS
+    slabs (the context's slab_kb) and with every sample gathered from global memory
+    (slab_kb 0).  Restores slab_kb.  reps 0: no table (None)."""
+    import torch
+    if reps <= 0:
+        return None
+    keep = ctx.get_tuning("slab_kb")
+    out = {}
+    for name, slab in (("staged", keep), ("global", 0)):
+        ctx.set_tuning("slab_kb", slab)
+        ms = []
+        for k in range(len(samples)):
+            set_pose(k)
+            frame()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                frame()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1) / reps)
+        out[name] = {"headline_ms": round(ms[0], 4),
+                     "headline_gsamples_per_s": round(samples[0] / (ms[0] * 1e6), 2),
+                     "poses_mean_ms": round(float(np.mean(ms[1:])), 4)}
+    ctx.set_tuning("slab_kb", keep)
     return out
 
 

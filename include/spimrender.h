@@ -283,9 +283,13 @@ int spr_render_frames(spr_ctx *ctx, const spr_frame *frames, int n, float alpha_
    reference's running sum exactly (volume_kernel.cl:121-131, 293-298), so each frame equals
    spr_render_frames / spr_render_slot of the same frame up to the sign of zero outputs.
    Slots as spr_render_frames (padded layout, one voxel type).  The frames' d_out planes
-   are cleared and used as the accumulators of their rays.  Returns once enqueued on
-   `stream`. */
-int spr_render_frames_split(spr_ctx *ctx, const spr_frame *frames, int n, int seg, void *stream);
+   are cleared and used as the accumulators of their rays.  d_gates: NULL, or n device
+   flags: frame k is rendered only when d_gates[k] != 0 (read on the device when the
+   launch runs; a closed frame's planes are left untouched), so a caller can render
+   frames that an earlier kernel on the stream selects without a host round trip.
+   Returns once enqueued on `stream`. */
+int spr_render_frames_split(spr_ctx *ctx, const spr_frame *frames, int n, int seg,
+                            const unsigned int *d_gates, void *stream);
 
 /* One frame of any method from any slot into device buffers on `stream`, with explicit
    `params` (matrices included): the stateless form of spr_render_device for callers that

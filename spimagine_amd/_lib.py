@@ -158,6 +158,27 @@ def jobs_from_arrays(slots, inv_p, inv_m, d_out, d_alpha):
     return arr
 
 
+# numpy views of struct spr_params and struct spr_frame (vectorised frame tables)
+PARAMS_DTYPE = np.dtype({"names": [f[0] for f in SprParams._fields_],
+                         "formats": [np.int32 if f[1] is ctypes.c_int else
+                                     np.float32 if f[1] is ctypes.c_float else
+                                     (np.float32, f[1]._length_) for f in SprParams._fields_],
+                         "offsets": [getattr(SprParams, f[0]).offset for f in SprParams._fields_],
+                         "itemsize": ctypes.sizeof(SprParams)})
+FRAME_DTYPE = np.dtype({"names": [f[0] for f in SprFrame._fields_],
+                        "formats": [np.int32 if f[1] is ctypes.c_int else
+                                    np.float32 if f[1] is ctypes.c_float else
+                                    np.uint64 if f[1] is ctypes.c_void_p else
+                                    (np.float32, f[1]._length_) for f in SprFrame._fields_],
+                        "offsets": [getattr(SprFrame, f[0]).offset for f in SprFrame._fields_],
+                        "itemsize": ctypes.sizeof(SprFrame)})
+
+
+def params_array(plist):
+    """The SprParams of plist as one PARAMS_DTYPE array."""
+    return np.frombuffer(b"".join(bytes(p) for p in plist), PARAMS_DTYPE)
+
+
 class SprError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("spimrender error %d: %s" % (code, msg))
@@ -226,7 +247,7 @@ def load():
     L.spr_render_batch_noalloc.argtypes = [c_void_p, ctypes.POINTER(SprParams), jp, c_int, c_int,
                                            c_int, c_int, c_int, c_void_p]
     L.spr_render_frames_split.argtypes = [c_void_p, ctypes.POINTER(SprFrame), c_int, c_int,
-                                          c_void_p]
+                                          c_void_p, c_void_p]
     L.spr_render_frames.argtypes = [c_void_p, ctypes.POINTER(SprFrame), c_int, c_float, c_int,
                                     c_void_p]
     L.spr_render_slot.argtypes = [c_void_p, c_int, ctypes.POINTER(SprParams), c_int, c_int, c_int,
@@ -455,12 +476,18 @@ class Context:
         self._check(self.lib.spr_render_frames(self.handle, arr, len(frames), float(alpha_pow),
                                                int(filter), stream))
 
-    def render_frames_split(self, frames, seg=0, stream=None):
+    def render_frames_split(self, frames, seg=0, stream=None, d_gates=None):
         """Max projections of frames of scattered rays (spr_render_frames_split): each ray's
-        samples split into segments of seg (32, 64, 128; 0 = 64) samples."""
-        arr = (SprFrame * len(frames))(*frames)
+        samples split into segments of seg (32, 64, 128; 0 = 64) samples; frames: a list of
+        SprFrame or a FRAME_DTYPE array; d_gates: None or a device pointer to one uint32
+        flag per frame (0: leave the frame untouched)."""
+        if isinstance(frames, np.ndarray):
+            assert frames.dtype == FRAME_DTYPE and frames.flags.c_contiguous
+            arr = frames.ctypes.data_as(ctypes.POINTER(SprFrame))
+        else:
+            arr = (SprFrame * len(frames))(*frames)
         self._check(self.lib.spr_render_frames_split(self.handle, arr, len(frames), int(seg),
-                                                     stream))
+                                                     d_gates, stream))
 
     def render_slot(self, slot, params, method, w, h, d_out, d_alpha, d_depth=None,
                     d_normals=None, d_occlusion=None, num_parts=1, current_part=0, stream=None):

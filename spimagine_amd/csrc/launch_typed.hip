@@ -256,11 +256,17 @@ hipError_t launch_batch_slab(bool shared, int ch, int num_cus, const BatchJob* d
                   : launch_batch_staged<T, P2, SLAB, false>(ch, num_cus, d_jobs, p, q, counters, s);
 }
 
+// both channels of a pair slot: one wave per workgroup (19,952-byte slabs) or four
 template <bool P2, int CH>
-void launch_pair_ch(const VolSlot& vs, dim3 grid, hipStream_t s, const RayParams& p, float* out,
-                    float* alpha) {
-    hipLaunchKernelGGL((k_max_project_staged<U16Pair, P2, 19456, CH, false>), grid, dim3(256), 0, s,
-                       padded_volume<U16Pair, P2>(vs), p, out, alpha);
+void launch_pair_ch(const VolSlot& vs, bool one_wave, dim3 grid, hipStream_t s, const RayParams& p,
+                    float* out, float* alpha) {
+    if (one_wave)
+        hipLaunchKernelGGL((k_max_project_wave<U16Pair, P2, kWaveSlab19, CH>),
+                           dim3(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty)), dim3(64), 0, s,
+                           padded_volume<U16Pair, P2>(vs), p, out, alpha);
+    else
+        hipLaunchKernelGGL((k_max_project_staged<U16Pair, P2, 19456, CH, false>), grid, dim3(256), 0,
+                           s, padded_volume<U16Pair, P2>(vs), p, out, alpha);
 }
 
 }  // namespace
@@ -407,28 +413,28 @@ hipError_t TypedLaunch<T, P2>::batch(const LaunchTuning& t, int mode, int ch, co
 template <typename T, bool P2>
 hipError_t TypedLaunch<T, P2>::split(int seg, bool staged, const BatchJob* d_jobs, uint32_t njobs,
                                      uint32_t nitems, uint32_t npix, const RayParams& base,
-                                     hipStream_t s) {
+                                     const uint32_t* gates, hipStream_t s) {
     constexpr bool kShort = !std::is_same<T, float>::value;
     const dim3 pg((npix + 255u) / 256u), block(256);
-    hipLaunchKernelGGL(k_split_clear, pg, block, 0, s, d_jobs, njobs, npix);
+    hipLaunchKernelGGL(k_split_clear, pg, block, 0, s, d_jobs, njobs, npix, gates);
     if (staged) {  // one wave (work item) per workgroup
         const dim3 ig(nitems), wave(64);
         if (seg == 32)
-            hipLaunchKernelGGL((k_split_staged<T, P2, 32>), ig, wave, 0, s, d_jobs, njobs, nitems, base);
+            hipLaunchKernelGGL((k_split_staged<T, P2, 32>), ig, wave, 0, s, d_jobs, njobs, nitems, base, gates);
         else if (seg == 128)
-            hipLaunchKernelGGL((k_split_staged<T, P2, 128>), ig, wave, 0, s, d_jobs, njobs, nitems, base);
+            hipLaunchKernelGGL((k_split_staged<T, P2, 128>), ig, wave, 0, s, d_jobs, njobs, nitems, base, gates);
         else
-            hipLaunchKernelGGL((k_split_staged<T, P2, 64>), ig, wave, 0, s, d_jobs, njobs, nitems, base);
+            hipLaunchKernelGGL((k_split_staged<T, P2, 64>), ig, wave, 0, s, d_jobs, njobs, nitems, base, gates);
     } else {  // four work items per workgroup
         const dim3 ig((nitems + 3u) / 4u);
         if (seg == 32)
-            hipLaunchKernelGGL((k_split_march<T, P2, 32>), ig, block, 0, s, d_jobs, njobs, nitems, base);
+            hipLaunchKernelGGL((k_split_march<T, P2, 32>), ig, block, 0, s, d_jobs, njobs, nitems, base, gates);
         else if (seg == 128)
-            hipLaunchKernelGGL((k_split_march<T, P2, 128>), ig, block, 0, s, d_jobs, njobs, nitems, base);
+            hipLaunchKernelGGL((k_split_march<T, P2, 128>), ig, block, 0, s, d_jobs, njobs, nitems, base, gates);
         else
-            hipLaunchKernelGGL((k_split_march<T, P2, 64>), ig, block, 0, s, d_jobs, njobs, nitems, base);
+            hipLaunchKernelGGL((k_split_march<T, P2, 64>), ig, block, 0, s, d_jobs, njobs, nitems, base, gates);
     }
-    hipLaunchKernelGGL(k_split_finish<kShort>, pg, block, 0, s, d_jobs, njobs, npix, base);
+    hipLaunchKernelGGL(k_split_finish<kShort>, pg, block, 0, s, d_jobs, njobs, npix, base, gates);
     return hipGetLastError();
 }
 
@@ -452,12 +458,14 @@ hipError_t launch_pair(const LaunchTuning& t, const VolSlot& vs, RayParams p, fl
     const dim3 grid(tile_grid(p.tile_map, p.ntx, p.nty));
     const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
     const int ch = chunk_for(t, p.nouter, nmax);
+    // (the one-wave kernel deals tiles interleaved; a cost-ordered map falls back to four)
+    const bool one_wave = one_wave_for(t, p) && p.tile_map != kTileMapCostXcd;
     if (ch == 16)
-        launch_pair_ch<P2, 16>(vs, grid, s, p, out, alpha);
+        launch_pair_ch<P2, 16>(vs, one_wave, grid, s, p, out, alpha);
     else if (ch == 8)
-        launch_pair_ch<P2, 8>(vs, grid, s, p, out, alpha);
+        launch_pair_ch<P2, 8>(vs, one_wave, grid, s, p, out, alpha);
     else
-        launch_pair_ch<P2, 4>(vs, grid, s, p, out, alpha);
+        launch_pair_ch<P2, 4>(vs, one_wave, grid, s, p, out, alpha);
     return hipGetLastError();
 }
 template hipError_t launch_pair<(SPR_POW2 != 0)>(const LaunchTuning&, const VolSlot&, RayParams,

@@ -931,6 +931,20 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
             release();
             continue;
         }
+        if constexpr (FILT == 1) {
+            // the nearest filter reads one voxel per sample: a box that does not fit is
+            // cheaper gathered from global memory than split and staged piecewise
+            if (live) {
+#pragma unroll 1
+                for (int k = 0; k < ng; ++k) {
+                    group(CH);
+                    sample_global_any<PV, CH, OPAC, FILT>(vol, px, py, pz, dx, dy, dz, col, col2,
+                                                          op);
+                }
+            }
+            release();
+            continue;
+        }
         if constexpr (HALVES && CH == 16 && !SHARED && !kPair) {
             // a box too large for the slab because the wave's two halves (rows 0-3 and 4-7
             // of its 8x8 rays) run apart, e.g. along the volume's silhouette where the rays'

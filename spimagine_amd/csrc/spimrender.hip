@@ -1381,7 +1381,8 @@ int spr_render_frames(spr_ctx* c, const spr_frame* frames, int n, float alpha_po
     return SPR_OK;
 }
 
-int spr_render_frames_split(spr_ctx* c, const spr_frame* frames, int n, int seg, void* stream) {
+int spr_render_frames_split(spr_ctx* c, const spr_frame* frames, int n, int seg,
+                            const unsigned int* d_gates, void* stream) {
     if (!c) return fail(nullptr, SPR_ERR_INVALID_ARG, "NULL context");
     if (seg == 0) seg = kSplitSegDefault;
     if (seg != 32 && seg != 64 && seg != 128)
@@ -1420,18 +1421,19 @@ int spr_render_frames_split(spr_ctx* c, const spr_frame* frames, int n, int seg,
     hipError_t e = stage_jobs(c, host.data(), jbytes, d_jobs, s);
     if (e == hipSuccess) {
         const uint32_t nj = (uint32_t)n, ni = (uint32_t)items, np = (uint32_t)pix;
+        const uint32_t* gates = reinterpret_cast<const uint32_t*>(d_gates);
         switch (dtype) {
             case SPR_DTYPE_F32:
-                e = pow2 ? TypedLaunch<float, true>::split(seg, staged, d_jobs, nj, ni, np, base, s)
-                         : TypedLaunch<float, false>::split(seg, staged, d_jobs, nj, ni, np, base, s);
+                e = pow2 ? TypedLaunch<float, true>::split(seg, staged, d_jobs, nj, ni, np, base, gates, s)
+                         : TypedLaunch<float, false>::split(seg, staged, d_jobs, nj, ni, np, base, gates, s);
                 break;
             case SPR_DTYPE_U16:
-                e = pow2 ? TypedLaunch<uint16_t, true>::split(seg, staged, d_jobs, nj, ni, np, base, s)
-                         : TypedLaunch<uint16_t, false>::split(seg, staged, d_jobs, nj, ni, np, base, s);
+                e = pow2 ? TypedLaunch<uint16_t, true>::split(seg, staged, d_jobs, nj, ni, np, base, gates, s)
+                         : TypedLaunch<uint16_t, false>::split(seg, staged, d_jobs, nj, ni, np, base, gates, s);
                 break;
             default:
-                e = pow2 ? TypedLaunch<uint8_t, true>::split(seg, staged, d_jobs, nj, ni, np, base, s)
-                         : TypedLaunch<uint8_t, false>::split(seg, staged, d_jobs, nj, ni, np, base, s);
+                e = pow2 ? TypedLaunch<uint8_t, true>::split(seg, staged, d_jobs, nj, ni, np, base, gates, s)
+                         : TypedLaunch<uint8_t, false>::split(seg, staged, d_jobs, nj, ni, np, base, gates, s);
         }
     }
     (void)hipFreeAsync(d_jobs, s);

@@ -40,6 +40,11 @@ __device__ __forceinline__ uint32_t split_job_of(const BatchJob* __restrict__ jo
     return lo;
 }
 
+// frame jid is left untouched when its gate is closed (gates == nullptr: every frame runs)
+__device__ __forceinline__ bool split_gated(const uint32_t* __restrict__ gates, uint32_t jid) {
+    return gates != nullptr && gates[jid] == 0u;
+}
+
 // a job's ray parameters: the launch's (staging settings) with the job's matrices, frame
 // size, box, transfer and step count; one pass
 __device__ __forceinline__ RayParams split_params(const RayParams& base, const BatchJob& j) {
@@ -65,10 +70,13 @@ __device__ __forceinline__ RayParams split_params(const RayParams& base, const B
 // Output words of every job's pixels to +0 (the identity of the segments' maximum).
 // Pixels: job k's pixels start at jobs[k].first[1].
 static __global__ __launch_bounds__(256) void k_split_clear(const BatchJob* __restrict__ jobs,
-                                                            uint32_t njobs, uint32_t npix) {
+                                                            uint32_t njobs, uint32_t npix,
+                                                            const uint32_t* __restrict__ gates) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= npix) return;
-    const BatchJob& j = jobs[split_job_of(jobs, njobs, 1, i)];
+    const uint32_t jid = split_job_of(jobs, njobs, 1, i);
+    if (split_gated(gates, jid)) return;
+    const BatchJob& j = jobs[jid];
     j.out[i - j.first[1]] = 0.0f;
 }
 
@@ -77,10 +85,13 @@ static __global__ __launch_bounds__(256) void k_split_clear(const BatchJob* __re
 template <typename T, bool POW2, int SEG>
 __global__ __launch_bounds__(256) void k_split_march(const BatchJob* __restrict__ jobs,
                                                      uint32_t njobs, uint32_t nitems,
-                                                     RayParams base) {
+                                                     RayParams base,
+                                                     const uint32_t* __restrict__ gates) {
     const uint32_t item = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (item >= nitems) return;  // uniform per wave
-    const BatchJob& j = jobs[split_job_of(jobs, njobs, 0, item)];
+    const uint32_t jid = split_job_of(jobs, njobs, 0, item);
+    if (split_gated(gates, jid)) return;
+    const BatchJob& j = jobs[jid];
     const uint32_t total = (j.nouter + 1u) * (uint32_t)kLoopUnroll;
     const uint32_t nseg = (total + SEG - 1u) / SEG;
     const uint32_t local = item - j.first[0];
@@ -122,12 +133,15 @@ constexpr int kSplitSlab = 19952;
 template <typename T, bool POW2, int SEG>
 __global__ __launch_bounds__(64) void k_split_staged(const BatchJob* __restrict__ jobs,
                                                      uint32_t njobs, uint32_t nitems,
-                                                     RayParams base) {
+                                                     RayParams base,
+                                                     const uint32_t* __restrict__ gates) {
     static_assert(SEG % kLoopUnroll == 0, "segments of whole 16-sample groups");
     __shared__ Unit16 lds[kSplitSlab / 16 + 1];  // +16 B: lds_pair reads past a pair
     const uint32_t item = blockIdx.x;
     if (item >= nitems) return;
-    const BatchJob& j = jobs[split_job_of(jobs, njobs, 0, item)];
+    const uint32_t jid = split_job_of(jobs, njobs, 0, item);
+    if (split_gated(gates, jid)) return;  // uniform
+    const BatchJob& j = jobs[jid];
     const uint32_t total = (j.nouter + 1u) * (uint32_t)kLoopUnroll;
     const uint32_t nseg = (total + SEG - 1u) / SEG;
     const uint32_t local = item - j.first[0];
@@ -164,10 +178,13 @@ __global__ __launch_bounds__(64) void k_split_staged(const BatchJob* __restrict_
 template <bool kShort>
 __global__ __launch_bounds__(256) void k_split_finish(const BatchJob* __restrict__ jobs,
                                                       uint32_t njobs, uint32_t npix,
-                                                      RayParams base) {
+                                                      RayParams base,
+                                                      const uint32_t* __restrict__ gates) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= npix) return;
-    const BatchJob& j = jobs[split_job_of(jobs, njobs, 1, i)];
+    const uint32_t jid = split_job_of(jobs, njobs, 1, i);
+    if (split_gated(gates, jid)) return;
+    const BatchJob& j = jobs[jid];
     const uint32_t o = i - j.first[1];
     const uint32_t y = o / j.W, x = o - y * j.W;
     const RayParams p = split_params(base, j);

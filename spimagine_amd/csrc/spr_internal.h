@@ -125,10 +125,10 @@ struct TypedLaunch {
     // max projections of frames of scattered rays, each ray's samples split into segments
     // of seg (32, 64 or 128) samples (split_rays.h), staged in LDS slabs or gathered from
     // global memory; job k's first item and first pixel in first[0] and first[1]; base: the
-    // launch's staging settings
+    // launch's staging settings; gates: nullptr or one device flag per job (0: skip the job)
     static hipError_t split(int seg, bool staged, const BatchJob* d_jobs, uint32_t njobs,
                             uint32_t nitems, uint32_t npix, const RayParams& base,
-                            hipStream_t s);
+                            const uint32_t* gates, hipStream_t s);
     // coarse max projections of two padded slots of this type and shape
     static hipError_t coarse_pair(const VolSlot& a, const VolSlot& b, const RayParams& p,
                                   float* out, float* alpha, hipStream_t s);

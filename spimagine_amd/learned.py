@@ -223,20 +223,50 @@ JUDGE_SEG = 64     # samples per segment of the judge's split rays (spr_render_f
 JUDGE_Z = 3.       # standard errors in the judge's upper confidence bound on a frame's MSE
 
 
-def sparse_params(p, w, h, stride=JUDGE_STRIDE, offset=JUDGE_OFFSET):
+def _shifted_inv_p(inv_p, w, h, ox, oy):
+    """inv_p (n, 16) float32 times the clip-space translation by (2 ox / w, 2 oy / h),
+    computed in float64 and rounded to float32 (n, 16)."""
+    T = np.eye(4)
+    T[0, 3] = 2. * ox / w
+    T[1, 3] = 2. * oy / h
+    iP = np.asarray(inv_p, np.float64).reshape(-1, 4, 4)
+    return np.matmul(iP, T).astype(np.float32).reshape(-1, 16)
+
+
+def sparse_frames(slot, plist, w, h, stride, offsets, d_out, d_alpha):
+    """A FRAME_DTYPE table of the sparse frames of every frame of plist for each (ox, oy) in
+    offsets (offset-major: entry j n + k is frame k at offset j; see sparse_params), each
+    (w/stride) x (h/stride); the planes of entry i at d_out / d_alpha + 4 i (w/stride)
+    (h/stride)."""
+    P = _lib.params_array(plist)
+    n, m = len(plist), len(offsets)
+    sw, sh = w // stride, h // stride
+    F = np.zeros(n * m, _lib.FRAME_DTYPE)
+    F["slot"] = slot
+    F["w"], F["h"] = sw, sh
+    for name in ("max_steps", "inv_m", "box", "min_val", "max_val", "gamma"):
+        F[name] = np.tile(P[name], (m,) + (1,) * (P[name].ndim - 1))
+    for j, (ox, oy) in enumerate(offsets):
+        F["inv_p"][j * n:(j + 1) * n] = (_shifted_inv_p(P["inv_p"], w, h, ox, oy)
+                                         if ox or oy else P["inv_p"])
+    step = 4 * sw * sh
+    F["d_out"] = d_out + step * np.arange(n * m, dtype=np.uint64)
+    F["d_alpha"] = d_alpha + step * np.arange(n * m, dtype=np.uint64)
+    return F
+
+
+def sparse_params(p, w, h, stride=JUDGE_STRIDE, offset=JUDGE_OFFSET, offset_y=None):
     """Parameters of a (w/stride) x (h/stride) exact render whose pixel (i, j) traces the ray
-    of pixel (stride i + offset, stride j + offset) of the w x h frame: the pixel mapping
-    u = x / W * 2 - 1 of the small frame is shifted by 2 offset / w in clip space, i.e.
-    inv_p is multiplied by a clip-space translation (offset 0: the rays of pixels
-    (stride i, stride j) exactly, bit for bit)."""
+    of pixel (stride i + offset, stride j + offset_y) of the w x h frame (offset_y None: the
+    same as offset): the pixel mapping u = x / W * 2 - 1 of the small frame is shifted by
+    2 offset / w in clip space, i.e. inv_p is multiplied by a clip-space translation
+    (offset 0: the rays of pixels (stride i, stride j) exactly, bit for bit)."""
     q = _lib.SprParams()
     ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(q))
-    if offset:
-        T = np.eye(4)
-        T[0, 3] = 2. * offset / w
-        T[1, 3] = 2. * offset / h
-        iP = np.asarray(list(p.inv_p), np.float64).reshape(4, 4).dot(T)
-        q.inv_p[:] = [float(v) for v in iP.astype(np.float32).ravel()]
+    oy = offset if offset_y is None else offset_y
+    if offset or oy:
+        iP = _shifted_inv_p(np.asarray(list(p.inv_p), np.float32), w, h, offset, oy)
+        q.inv_p[:] = [float(v) for v in iP.ravel()]
     return q
 
 
@@ -253,27 +283,46 @@ def _db(mse):
 
 
 class JudgedBatch(object):
-    """Device results of one render_judged_many call: the judged error sums (squared and
-    fourth powers) and the fallback flags (read them once the stream has passed the
-    call); z: the confidence bound's standard errors."""
+    """Device results of one render_judged_many call (read them once the stream has passed
+    the call): the first stage's error sums sums1 = (squared, fourth-power) over npix
+    pixels per frame and its flags; with a second stage, the sums over all 4 npix pixels
+    (sums2, meaningful for the frames the first stage did not accept) and the final flags;
+    z: the confidence bound's standard errors."""
 
-    def __init__(self, sse, sse4, flags, npix, z):
-        self.sse, self.sse4, self.flags, self.npix, self.z = sse, sse4, flags, npix, z
+    def __init__(self, sums1, flags1, npix, z, sums2=None, flags=None):
+        self.sums1, self.flags1, self.npix, self.z = sums1, flags1, npix, z
+        self.sums2 = sums2
+        self.flags = flags1 if flags is None else flags
 
     def kept(self):
         """Per frame: True when the learned frame was kept (no exact fallback)."""
         return [int(f) == 0 for f in self.flags.cpu().numpy()]
 
+    def refined(self):
+        """Per frame: True when the second stage judged the frame."""
+        if self.sums2 is None:
+            return [False] * len(self.flags1)
+        return [int(f) != 0 for f in self.flags1.cpu().numpy()]
+
+    def _moments(self):
+        s1 = self.sums1.cpu().numpy().astype(np.float64) / self.npix
+        if self.sums2 is None:
+            return s1
+        s2 = self.sums2.cpu().numpy().astype(np.float64) / (4 * self.npix)
+        return np.where(np.asarray(self.refined())[None, :], s2, s1)
+
+    def _n(self):
+        return np.where(np.asarray(self.refined()), 4 * self.npix, self.npix)
+
     def estimates(self):
-        """Per frame: the PSNR estimated on the judge's pixels (dB)."""
-        return [_db(v) for v in self.sse.cpu().numpy().astype(np.float64) / self.npix]
+        """Per frame: the PSNR estimated on the judge's pixels (dB; the last stage's)."""
+        return [_db(v) for v in self._moments()[0]]
 
     def bounds(self):
         """Per frame: the PSNR of the judge's upper confidence bound on the MSE (dB), the
-        value compared with the acceptance level."""
-        m2 = self.sse.cpu().numpy().astype(np.float64) / self.npix
-        m4 = self.sse4.cpu().numpy().astype(np.float64) / self.npix
-        ucb = m2 + self.z * np.sqrt(np.maximum(m4 - m2 * m2, 0.) / self.npix)
+        value the last stage compared with the acceptance level."""
+        m2, m4 = self._moments()
+        ucb = m2 + self.z * np.sqrt(np.maximum(m4 - m2 * m2, 0.) / self._n())
         return [_db(v) for v in ucb]
 
 
@@ -315,6 +364,9 @@ class LearnedMIP(object):
         # on the caller's stream before them; measured faster: concurrent, the two thrash
         # each other's caches and LDS)
         self.judge_concurrent = False
+        # a second judge stage on the stride-(S/2) grid for the frames the first stage does
+        # not accept (split rays only)
+        self.judge_refine = True
 
     def attach(self, slot):
         """The slots whose coarse projections feed the network (created once): for pooled
@@ -410,9 +462,6 @@ class LearnedMIP(object):
         # stream concurrently with them (judge_concurrent)
         if side is not stream:
             side.wait_stream(stream)
-        groups = {}
-        for k, p in enumerate(plist):
-            groups.setdefault(bytes(p)[_lib.SprParams.box.offset:], []).append(k)
         split = self.judge_seg is not None and all(
             p.alpha_pow == 0 and p.filter == _lib.FILTER_LINEAR for p in plist)
         keep = None
@@ -421,11 +470,13 @@ class LearnedMIP(object):
             self.ctx.set_tuning("slab_kb", self.judge_slab_kb)
         try:
             if split:  # every ray's samples split over several waves (few, scattered rays)
-                frames = [_lib.make_frame(slot, sparse_params(p, w, h, stride=S), sw, sh_,
-                                          sparse[k].data_ptr(), salpha[k].data_ptr())
-                          for k, p in enumerate(plist)]
+                frames = sparse_frames(slot, plist, w, h, S, [(JUDGE_OFFSET, JUDGE_OFFSET)],
+                                       sparse.data_ptr(), salpha.data_ptr())
                 self.ctx.render_frames_split(frames, seg=self.judge_seg, stream=side.cuda_stream)
             else:
+                groups = {}
+                for k, p in enumerate(plist):
+                    groups.setdefault(bytes(p)[_lib.SprParams.box.offset:], []).append(k)
                 for ks in groups.values():
                     qs = [sparse_params(plist[k], w, h, stride=S) for k in ks]
                     jobs = [(slot, list(q.inv_p), list(q.inv_m), sparse[k].data_ptr(),
@@ -445,16 +496,48 @@ class LearnedMIP(object):
             self.net.upsample2(coarse_alpha.data_ptr(), w // 2, h // 2, alpha.data_ptr(), sh)
         if side is not stream:
             stream.wait_event(done)
-        self.net.judge_sse_frames([o.data_ptr() for o in outs], w, h, sparse.data_ptr(), sw, sh_,
-                                  JUDGE_OFFSET, JUDGE_OFFSET, sse[0].data_ptr(), sse[1].data_ptr(),
-                                  stream=sh)
+        fptrs = [o.data_ptr() for o in outs]
+        self.net.judge_sse_frames(fptrs, w, h, sparse.data_ptr(), sw, sh_, JUDGE_OFFSET,
+                                  JUDGE_OFFSET, sse[0].data_ptr(), sse[1].data_ptr(), stream=sh)
         mse_max = 10. ** (-(self.threshold + margin) / 10.)
         self.net.judge_flags_bound(sse[0].data_ptr(), sse[1].data_ptr(), n, sw * sh_, mse_max,
                                    self.judge_z, flags.data_ptr(), stream=sh)
+        sums2 = flags1 = None
+        if split and self.judge_refine and S % 2 == 0:
+            # second stage, for the frames the first did not accept: the other three pixels
+            # of every (S/2) x (S/2) cell, offsets (S/2, 0), (0, S/2), (S/2, S/2) from the
+            # first grid, rendered only for those frames (device gates), and the bound over
+            # all four grids' pixels
+            flags1 = flags
+            offs = [(JUDGE_OFFSET + S // 2, JUDGE_OFFSET), (JUDGE_OFFSET, JUDGE_OFFSET + S // 2),
+                    (JUDGE_OFFSET + S // 2, JUDGE_OFFSET + S // 2)]
+            with torch.cuda.stream(stream):
+                extra = torch.empty((3, n, sh_, sw), device="cuda")
+                xalpha = torch.empty_like(extra)
+                xsums = torch.empty((3, 2, n), device="cuda")
+                gates = flags1.repeat(3)
+            frames = sparse_frames(slot, plist, w, h, S, offs, extra.data_ptr(),
+                                   xalpha.data_ptr())
+            self.ctx.render_frames_split(frames, seg=self.judge_seg, stream=sh,
+                                         d_gates=gates.data_ptr())
+            for j, (ox, oy) in enumerate(offs):
+                self.net.judge_sse_frames(fptrs, w, h, extra[j].data_ptr(), sw, sh_, ox, oy,
+                                          xsums[j, 0].data_ptr(), xsums[j, 1].data_ptr(),
+                                          stream=sh)
+            with torch.cuda.stream(stream):
+                sums2 = sse + xsums.sum(0)
+                flags2 = torch.empty_like(flags1)
+            self.net.judge_flags_bound(sums2[0].data_ptr(), sums2[1].data_ptr(), n, 4 * sw * sh_,
+                                       mse_max, self.judge_z, flags2.data_ptr(), stream=sh)
+            with torch.cuda.stream(stream):
+                flags = flags1 & flags2
         for k, (p, out, alpha) in enumerate(zip(plist, outs, alphas)):
             self.ctx.render_slot_if(slot, p, w, h, out.data_ptr(), alpha.data_ptr(),
                                     flags.data_ptr() + 4 * k, stream=sh)
-        res = JudgedBatch(sse[0], sse[1], flags, sw * sh_, self.judge_z)
+        if flags1 is None:
+            res = JudgedBatch(sse, flags, sw * sh_, self.judge_z)
+        else:
+            res = JudgedBatch(sse, flags1, sw * sh_, self.judge_z, sums2, flags)
         if not wait:
             return res
         stream.synchronize()

@@ -101,6 +101,20 @@ def test_split_equals_single_and_mixed(torch, dtype, seg, slab_kb):
     s.synchronize()
     for p, fw, fh, pl in frames:
         np.testing.assert_array_equal(pl[4].cpu().numpy(), pl[0].cpu().numpy())
+    # device gates: frames 1 and 3 closed (their planes untouched), 0 and 2 rendered
+    gates = torch.tensor([1, 0, 7, 0], dtype=torch.int32, device="cuda")
+    for p, fw, fh, pl in frames:
+        pl[4].fill_(-3.)
+        pl[5].fill_(-3.)
+    torch.cuda.synchronize()
+    ctx.render_frames_split(split, seg=seg, stream=s.cuda_stream, d_gates=gates.data_ptr())
+    s.synchronize()
+    for k, (p, fw, fh, pl) in enumerate(frames):
+        if k in (0, 2):
+            np.testing.assert_array_equal(pl[4].cpu().numpy(), pl[0].cpu().numpy())
+            np.testing.assert_array_equal(pl[5].cpu().numpy(), pl[1].cpu().numpy())
+        else:
+            assert (pl[4].cpu().numpy() == -3.).all() and (pl[5].cpu().numpy() == -3.).all()
 
 
 def test_split_matches_oracle(torch):

@@ -128,3 +128,25 @@ def test_downsampling_rule():
         assert sl is not None and all(s.step == step for s in sl), (ratio, step, sl)
         assert [s.start for s in sl] == [0, 0, 0] and [s.stop for s in sl] == list(data.shape)
         assert data[sl].shape == tuple(-(-n // step) for n in data.shape)
+
+
+def test_sparse_frame_table_matches_single_frames():
+    """learned.sparse_frames (the judge's vectorised frame table) holds, entry for entry and
+    byte for byte, the spr_frame that sparse_params + make_frame build for one frame."""
+    import ctypes
+    from spimagine_amd import _lib, learned
+    rng = np.random.default_rng(0)
+    plist = [_lib.make_params(rng.normal(size=(4, 4)), rng.normal(size=(4, 4)),
+                              (-1, 1, -.5, 1, -1, .7), 3., 9., 1.3, 0., .1, 21, 30, 200 + k, 0)
+             for k in range(5)]
+    w, h, S = 1024, 768, 16
+    offs = [(1, 1), (9, 1), (1, 9), (9, 9)]
+    F = learned.sparse_frames(3, plist, w, h, S, offs, 1000, 5000)
+    assert F.dtype.itemsize == ctypes.sizeof(_lib.SprFrame)
+    n = (w // S) * (h // S)
+    for j, (ox, oy) in enumerate(offs):
+        for k, p in enumerate(plist):
+            q = learned.sparse_params(p, w, h, S, ox, oy)
+            i = j * len(plist) + k
+            f = _lib.make_frame(3, q, w // S, h // S, 1000 + 4 * i * n, 5000 + 4 * i * n)
+            assert bytes(f) == F[i].tobytes(), (j, k)
