@@ -69,7 +69,30 @@ __global__ __launch_bounds__(256) void k_conv32(const void* __restrict__ in,
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTH, img = blockIdx.z;
     const size_t plane = (size_t)h * w;
-    for (int c = tid; c < kLH * kLW * 4; c += 256) {
+    constexpr int kChunks = kLH * kLW * 4;  // 16-byte chunks of the halo tile
+    if constexpr (!DGRAD) {
+        // every chunk's load issued before the first store: one round trip to memory
+        // instead of one per chunk
+        constexpr int kIter = (kChunks + 255) / 256;
+        uint4 v[kIter];
+#pragma unroll
+        for (int k = 0; k < kIter; ++k) {
+            const int c = tid + 256 * k;
+            const int pix = c >> 2, q = c & 3;
+            const int ty = pix / kLW, tx = pix - ty * kLW;
+            const int gy = y0 - 1 + ty, gx = x0 - 1 + tx;
+            v[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (c < kChunks && gy >= 0 && gy < h && gx >= 0 && gx < w)
+                v[k] = *(const uint4*)((const uint16_t*)in +
+                                       ((size_t)img * plane + (size_t)gy * w + gx) * C + q * 8);
+        }
+#pragma unroll
+        for (int k = 0; k < kIter; ++k) {
+            const int c = tid + 256 * k;
+            if (c < kChunks) tile[(c >> 2) * kPixU4 + (c & 3)] = v[k];
+        }
+    }
+    for (int c = DGRAD ? tid : kChunks; c < kChunks; c += 256) {
         const int pix = c >> 2, q = c & 3;
         const int ty = pix / kLW, tx = pix - ty * kLW;
         const int gy = y0 - 1 + ty, gx = x0 - 1 + tx;
