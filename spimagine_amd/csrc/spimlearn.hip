@@ -57,6 +57,32 @@ __device__ __forceinline__ bool bf_pos(uint32_t b) { return (b & 0x8000u) == 0u 
 // ------------------------------------------------------------------ 32->32 conv (MFMA)
 constexpr int kTW = 32, kTH = 16, kLW = kTW + 2, kLH = kTH + 2, kPixU4 = 5;
 
+// The (kTH + 2) x (kTW + 2) halo tile of bf16 activations (C channels, 80-byte pixel
+// stride) at (x0 - 1, y0 - 1) of image img into LDS, zero outside the image: every chunk's
+// load issued before the first store (one round trip to memory instead of one per chunk).
+__device__ __forceinline__ void load_halo_tile(const uint16_t* __restrict__ in, uint4* tile,
+                                               int x0, int y0, int img, int h, int w) {
+    constexpr int kChunks = kLH * kLW * 4, kIter = (kChunks + 255) / 256;
+    const int tid = threadIdx.x;
+    const size_t plane = (size_t)h * w;
+    uint4 v[kIter];
+#pragma unroll
+    for (int k = 0; k < kIter; ++k) {
+        const int c = tid + 256 * k;
+        const int pix = c >> 2, q = c & 3;
+        const int ty = pix / kLW, tx = pix - ty * kLW;
+        const int gy = y0 - 1 + ty, gx = x0 - 1 + tx;
+        v[k] = make_uint4(0u, 0u, 0u, 0u);
+        if (c < kChunks && gy >= 0 && gy < h && gx >= 0 && gx < w)
+            v[k] = *(const uint4*)(in + ((size_t)img * plane + (size_t)gy * w + gx) * C + q * 8);
+    }
+#pragma unroll
+    for (int k = 0; k < kIter; ++k) {
+        const int c = tid + 256 * k;
+        if (c < kChunks) tile[(c >> 2) * kPixU4 + (c & 3)] = v[k];
+    }
+}
+
 // DGRAD = false: out(bf16) = relu(conv(in bf16) + bias).
 // DGRAD = true : out(f32)  = conv(in f32 * [mask > 0]) with the input-gradient fragments.
 template <bool DGRAD>
@@ -70,28 +96,7 @@ __global__ __launch_bounds__(256) void k_conv32(const void* __restrict__ in,
     const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTH, img = blockIdx.z;
     const size_t plane = (size_t)h * w;
     constexpr int kChunks = kLH * kLW * 4;  // 16-byte chunks of the halo tile
-    if constexpr (!DGRAD) {
-        // every chunk's load issued before the first store: one round trip to memory
-        // instead of one per chunk
-        constexpr int kIter = (kChunks + 255) / 256;
-        uint4 v[kIter];
-#pragma unroll
-        for (int k = 0; k < kIter; ++k) {
-            const int c = tid + 256 * k;
-            const int pix = c >> 2, q = c & 3;
-            const int ty = pix / kLW, tx = pix - ty * kLW;
-            const int gy = y0 - 1 + ty, gx = x0 - 1 + tx;
-            v[k] = make_uint4(0u, 0u, 0u, 0u);
-            if (c < kChunks && gy >= 0 && gy < h && gx >= 0 && gx < w)
-                v[k] = *(const uint4*)((const uint16_t*)in +
-                                       ((size_t)img * plane + (size_t)gy * w + gx) * C + q * 8);
-        }
-#pragma unroll
-        for (int k = 0; k < kIter; ++k) {
-            const int c = tid + 256 * k;
-            if (c < kChunks) tile[(c >> 2) * kPixU4 + (c & 3)] = v[k];
-        }
-    }
+    if constexpr (!DGRAD) load_halo_tile((const uint16_t*)in, tile, x0, y0, img, h, w);
     for (int c = DGRAD ? tid : kChunks; c < kChunks; c += 256) {
         const int pix = c >> 2, q = c & 3;
         const int ty = pix / kLW, tx = pix - ty * kLW;
@@ -162,10 +167,35 @@ __global__ __launch_bounds__(256) void k_conv32(const void* __restrict__ in,
 // fragments of the 6 layers: forward A[t][hh](lane, j) = W[t][16 hh + 8 (lane/32) + j][lane%32];
 // input gradient (taps mirrored, channels swapped) A'[t][hh](lane, j) =
 // W[8 - t][lane%32][16 hh + 8 (lane/32) + j]
+// Head fragments (fwd + kLayers kFragPerLayer): the 32->4 head as a 32->32 tile whose
+// output rows 0-3 hold the bf16-rounded fp32 weights W[t][ci][co] and rows 8-11 the bf16
+// remainders W - hi (hi + lo carries 16 bits of each weight's mantissa); other rows 0.
 __global__ void k_pack(const float* __restrict__ params, uint4* __restrict__ fwd,
                        uint4* __restrict__ dgrad) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kLayers * kFragPerLayer) return;
+    if (i >= kLayers * kFragPerLayer) {
+        const int hi_ = i - kLayers * kFragPerLayer;
+        if (hi_ >= kFragPerLayer) return;
+        const int k = hi_ / 64, lane = hi_ % 64, t = k >> 1, hh = k & 1, row = lane & 31;
+        const float* W = params + kOffHead;
+        uint32_t f[4];
+        for (int j = 0; j < 8; j += 2) {
+            uint32_t b[2];
+            for (int e = 0; e < 2; ++e) {
+                const int ci = 16 * hh + 8 * (lane >> 5) + j + e;
+                b[e] = 0u;
+                if (row < 4) {
+                    b[e] = f2bf(W[(t * C + ci) * 4 + row]);
+                } else if (row >= 8 && row < 12) {
+                    const float wv = W[(t * C + ci) * 4 + row - 8];
+                    b[e] = f2bf(wv - bf2f(f2bf(wv)));
+                }
+            }
+            f[j / 2] = b[0] | (b[1] << 16);
+        }
+        fwd[i] = make_uint4(f[0], f[1], f[2], f[3]);
+        return;
+    }
     const int L = i / kFragPerLayer, k = (i / 64) % 18, lane = i % 64;
     const int t = k >> 1, hh = k & 1;
     const float* W = params + kOffL0 + L * kLParams;
@@ -214,20 +244,6 @@ __global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x,
                           pack2(fmaxf(acc[8 * q + 6], 0.f), fmaxf(acc[8 * q + 7], 0.f)));
 }
 
-__device__ __forceinline__ void load_bf16x32(const uint16_t* p, float* v) {
-    const uint4* q = (const uint4*)p;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint4 u = q[k];
-        const uint32_t ww[4] = {u.x, u.y, u.z, u.w};
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            v[8 * k + 2 * j] = bf_lo(ww[j]);
-            v[8 * k + 2 * j + 1] = bf_hi(ww[j]);
-        }
-    }
-}
-
 // bilinear x2 upsampling of plane 0 (half-pixel centres, edge-clamped: the
 // align_corners=False convention), sampled at output pixel (Y, X)
 __device__ __forceinline__ float bilinear_up2(const float* p, int h, int w, int Y, int X) {
@@ -240,37 +256,52 @@ __device__ __forceinline__ float bilinear_up2(const float* p, int h, int w, int 
     return (1.0f - ly) * ((1.0f - lx) * a + lx * b) + ly * ((1.0f - lx) * c + lx * d);
 }
 
-// y = pixel_shuffle2(head(u0)) + bilinear_up2(x plane 0); one thread per low-res pixel
+// y = pixel_shuffle2(head(u0)) + bilinear_up2(x plane 0) on the matrix cores: the 32->4
+// head as k_conv32's tiles (32 x 16 low-res pixels a workgroup, 18 MFMAs a row) with the
+// head fragments of k_pack; accumulator registers 0-3 and 4-7 of lanes 0-31 hold the
+// high and low weight parts' sums of channels 0-3 for pixel lane%32.
 __global__ __launch_bounds__(256) void k_head(const uint16_t* __restrict__ u0,
+                                              const uint4* __restrict__ frag,
                                               const float* __restrict__ x,
                                               const float* __restrict__ prm,
                                               float* __restrict__ y, int h, int w) {
-    __shared__ float sw[kHeadParams];
-    for (int i = threadIdx.x; i < kHeadParams; i += 256) sw[i] = prm[kOffHead + i];
-    __syncthreads();
-    const int xx = blockIdx.x * 16 + (threadIdx.x & 15), yy = blockIdx.y * 16 + (threadIdx.x >> 4);
-    const int img = blockIdx.z;
-    if (xx >= w || yy >= h) return;
-    float acc[4] = {sw[kHeadW], sw[kHeadW + 1], sw[kHeadW + 2], sw[kHeadW + 3]};
-    for (int t = 0; t < 9; ++t) {
-        const int sy = yy + t / 3 - 1, sx = xx + t % 3 - 1;
-        if (sy < 0 || sy >= h || sx < 0 || sx >= w) continue;
-        float v[C];
-        load_bf16x32(u0 + (((size_t)img * h + sy) * w + sx) * C, v);
+    __shared__ uint4 tile[kLH * kLW * kPixU4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTH, img = blockIdx.z;
+    load_halo_tile(u0, tile, x0, y0, img, h, w);
+    uint4 wf[18];
 #pragma unroll
-        for (int ci = 0; ci < C; ++ci)
-#pragma unroll
-            for (int co = 0; co < 4; ++co) acc[co] = fmaf(v[ci], sw[(t * C + ci) * 4 + co], acc[co]);
-    }
+    for (int k = 0; k < 18; ++k) wf[k] = frag[k * 64 + lane];
+    const float* bias = prm + kOffHead + kHeadW;
     const float* x0p = x + (size_t)img * 2 * h * w;
     float* yo = y + (size_t)img * 4 * h * w;
+    __syncthreads();
+    const int px = lane & 31, kh = lane >> 5;
+    const int ox = x0 + px;
+    for (int rr = 0; rr < kTH / 4; ++rr) {
+        const int r = wv * (kTH / 4) + rr, oy = y0 + r;
+        if (oy >= h) break;  // uniform over the wave: the MFMAs below run with all lanes
+        f32x16 acc = {};
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+        for (int t = 0; t < 9; ++t) {
+            const int base = ((r + t / 3) * kLW + px + t % 3) * kPixU4 + kh;
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int Y = 2 * yy + a, X = 2 * xx + b;
-            yo[(size_t)Y * 2 * w + X] = acc[2 * a + b] + bilinear_up2(x0p, h, w, Y, X);
+            for (int hh = 0; hh < 2; ++hh) {
+                const uint4 b = tile[base + 2 * hh];
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    __builtin_bit_cast(bf16x8, wf[2 * t + hh]), __builtin_bit_cast(bf16x8, b),
+                    acc, 0, 0, 0);
+            }
         }
+        if (kh == 0 && ox < w) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int Y = 2 * oy + (c >> 1), X = 2 * ox + (c & 1);
+                yo[(size_t)Y * 2 * w + X] =
+                    (acc[c] + acc[4 + c] + bias[c]) + bilinear_up2(x0p, h, w, Y, X);
+            }
+        }
+    }
 }
 
 // du0(p, ci) = sum_t sum_co dr(p - off(t), co) W[t][ci][co]
@@ -783,7 +814,9 @@ void forward(spl_net* t, const float* x, int n, int h, int w, float* y, hipStrea
     hipLaunchKernelGGL(k_up2_add, dim3(blocks_for((size_t)n * h * w * 4)), dim3(256), 0, s, t->u1, t->e0b, t->s0, n, h, w);
     conv32_fwd(t, kU0, t->s0, t->u0, n, h, w, s);
     conv32_fwd(t, kU0b, t->u0, t->u0b, n, h, w, s);
-    hipLaunchKernelGGL(k_head, grid16(n, h, w), dim3(256), 0, s, t->u0b, x, t->params, y, h, w);
+    hipLaunchKernelGGL(k_head, dim3((w + kTW - 1) / kTW, (h + kTH - 1) / kTH, n), dim3(256), 0, s,
+                       t->u0b, t->frag_fwd + (size_t)kLayers * kFragPerLayer, x, t->params, y, h,
+                       w);
 }
 
 float* layer_grad(spl_net* t, int L) { return t->grad + kOffL0 + L * kLParams; }
@@ -843,8 +876,8 @@ int read_sse(spl_net* t, double* sse, hipStream_t s) {
 }
 
 int repack(spl_net* t, hipStream_t s) {
-    hipLaunchKernelGGL(k_pack, dim3((kLayers * kFragPerLayer + 255) / 256), dim3(256), 0, s, t->params,
-                       t->frag_fwd, t->frag_dgrad);
+    hipLaunchKernelGGL(k_pack, dim3(((kLayers + 1) * kFragPerLayer + 255) / 256), dim3(256), 0, s,
+                       t->params, t->frag_fwd, t->frag_dgrad);
     HIPCALL(t, hipGetLastError());
     return SPL_OK;
 }
@@ -876,7 +909,7 @@ int spl_create(int device, spl_net** out) {
     if (e == hipSuccess) e = hipMalloc(&t->grad, pb);
     if (e == hipSuccess) e = hipMalloc(&t->adam_m, pb);
     if (e == hipSuccess) e = hipMalloc(&t->adam_v, pb);
-    if (e == hipSuccess) e = hipMalloc(&t->frag_fwd, kLayers * kFragPerLayer * sizeof(uint4));
+    if (e == hipSuccess) e = hipMalloc(&t->frag_fwd, (kLayers + 1) * kFragPerLayer * sizeof(uint4));
     if (e == hipSuccess) e = hipMalloc(&t->frag_dgrad, kLayers * kFragPerLayer * sizeof(uint4));
     if (e == hipSuccess) e = hipMalloc(&t->wpart, (size_t)kMaxWgradBlocks * kLParams * sizeof(float));
     if (e == hipSuccess) e = hipMemsetAsync(t->params, 0, pb, t->stream);
