@@ -312,7 +312,8 @@ def skip_table(ctx, stream, frame, set_pose, samples, reps):
             "headline_gsamples_per_s": round(samples[0] / (ms[0] * 1e6), 2),
             "poses_mean_ms": round(float(np.mean(ms[1:])), 4),
             "poses_gsamples_per_s": round(float(np.mean([n / (t * 1e6) for n, t in
-                                                         zip(samples[1:], ms[1:])])), 2)}
+                                                         zip(samples[1:], ms[1:])])), 2),
+            "poses_ms": [round(t, 4) for t in ms[1:]]}
     ctx.set_tuning("skip", keep)
     return out
 
@@ -511,7 +512,12 @@ def bench_iso(args):
         "skipping_ms": table,
         "roofline": dict(roofline(b_alg, kernel_ms, None),
                          kernel="iso pipeline per frame (march k_iso_staged + 6 post-pass "
-                                "kernels; the march dominates, see profiles/)"),
+                                "kernels; the march dominates, see profiles/)",
+                         frac_skipping_off=round(
+                             float(np.mean([b / (t * 1e-3) for b, t in zip(
+                                 [v * 16 + W * H * 24 for v in iso_samples],
+                                 table["iso"]["off"]["poses_ms"])])) / 1e9 / HBM_PEAK_GBS, 4)
+                         if table else None),
         "cpu_baseline": cpu}), flush=True)
 
 
@@ -938,7 +944,8 @@ def bench_learned(args):
     # split rays (segments of seg samples) or one persistent batch launch (19 KiB slabs
     # or global gathers: the rays run stride pixels apart)
     sweep = []
-    for stride, seg, slab, conc, z in ((8, 64, None, False, 3.), (8, 32, None, False, 3.),
+    for stride, seg, slab, conc, z in () if args.no_sweep else (
+                                       (8, 64, None, False, 3.), (8, 32, None, False, 3.),
                                        (8, 64, None, True, 3.), (8, 64, 0, False, 3.),
                                        (16, 64, None, False, 3.), (16, 64, None, False, 2.),
                                        (16, 64, None, True, 3.), (8, None, None, True, 3.),
@@ -1030,6 +1037,8 @@ def main():
     ap.add_argument("--skip-frames", type=int, default=5,
                     help="frames per pose timed with skipping off and on")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="learned: time the default judge only (no judge-settings sweep)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="launch tuning passed to spr_set_tuning, e.g. tile_map=1 "
                          "(batch: replaces the autotuner)")
