@@ -915,6 +915,19 @@ def bench_learned(args):
                                                out.data_ptr(), alpha.data_ptr(),
                                                stream=stream.cuda_stream), reps)
     learned_ms = timed(lambda p: lm.render(slot, p, W, H, out.data_ptr(), stream.cuda_stream), reps)
+    frames_buf = torch.empty((len(params), H, W), device="cuda")
+    outs = list(frames_buf)  # consecutive views: the network writes them in place
+    alphas = [torch.empty_like(o) for o in outs]
+    # the 8 learned frames with one network pass (render_many)
+    lm.render_many(slot, params, W, H, outs, stream)  # warm
+    stream.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        lm.render_many(slot, params, W, H, outs, stream)
+    e1.record(stream)
+    stream.synchronize()
+    learned_batch_ms = e0.elapsed_time(e1) / (reps * len(params))
     planes = torch.empty((1, 2, H // 2, W // 2), device="cuda")
     lm.coarse(slot, params[0], W, H, planes, 0, stream.cuda_stream)
     coarse_ms = timed(lambda p: lm.coarse(slot, p, W, H, planes, 0, stream.cuda_stream), reps)
@@ -923,10 +936,8 @@ def bench_learned(args):
     # per-frame judged rendering of the 8 poses as one batch per call, the calls queued
     # back to back (the judge decides on the device; one host synchronisation at the end):
     # wall clock per frame, the exact re-renders of rejected frames included
-    outs = [torch.empty((H, W), device="cuda") for _ in params]
-    alphas = [torch.empty_like(o) for o in outs]
 
-    def judged_run(stride, seg, slab_kb=None, concurrent=False, z=learned.JUDGE_Z):
+    def judged_run(stride, seg, slab_kb=None, concurrent=True, z=learned.JUDGE_Z):
         lm.judge_stride, lm.judge_seg, lm.judge_slab_kb = stride, seg, slab_kb
         lm.judge_concurrent, lm.judge_z = concurrent, z
         lm.render_judged_many(slot, params, outs, alphas, stream)  # warm
@@ -1002,6 +1013,8 @@ def bench_learned(args):
                    "training_pairs": int(w["pairs"]) if "pairs" in w else None},
         "exact_ms": round(exact_ms, 4), "exact_frames_per_s": round(1e3 / exact_ms, 2),
         "learned_ms": round(learned_ms, 4), "speedup": round(exact_ms / learned_ms, 3),
+        "learned_batch_ms": round(learned_batch_ms, 4),
+        "speedup_batch": round(exact_ms / learned_batch_ms, 3),
         "coarse_ms": round(coarse_ms, 4), "net_ms": round(net_ms, 4),
         "psnr_db": [round(r["psnr"], 2) for r in rep],
         "max_abs_err": [round(r["max_abs"], 4) for r in rep],

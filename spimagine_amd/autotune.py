@@ -2,7 +2,8 @@
 
 The kernels' launch knobs (spr_set_tuning) change speed, never results:
   slab_kb    per-wave LDS slab of the staged sampler (0 = global gathers only, 12, 16, 19)
-  chunk      samples per staged box (0 = from the ray step, 32 = two 16-sample groups)
+  chunk      samples per staged box (0 = from the ray step: two 16-sample groups where the
+             step allows 16; 16 = one group)
   staging    0 = each wave stages its own box, 1 = a workgroup's four waves stage one box
   tile_map   single frames: how 16x16 tiles are dealt to the XCDs (0 interleaved rows,
              3 = the same in descending estimated cost)
@@ -26,12 +27,12 @@ from . import _lib
 BATCH_CANDIDATES = tuple(
     [dict(slab_kb=0, chunk=0, staging=0), dict(slab_kb=12, chunk=0, staging=0)] +
     [dict(slab_kb=s, chunk=c, staging=st) for s, c, st in
-     itertools.product((16, 19), (0, 32), (0, 1))])
+     itertools.product((16, 19), (0, 16), (0, 1))])
 # (skipping runs with the default 19 KiB slabs only)
 FRAME_CANDIDATES = tuple(
     dict(tile_map=tm, wg_waves=ww, slab_kb=s, chunk=c, skip=sk)
     for tm, ww, (s, sk), c in itertools.product((0, 3), (0, 1, 4), ((16, 0), (19, 0), (19, 1)),
-                                                 (0, 32)))
+                                                 (0, 16)))
 CANDIDATES = BATCH_CANDIDATES  # the batch tuner's default
 
 _cache = {}

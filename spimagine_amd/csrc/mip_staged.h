@@ -945,73 +945,101 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
             release();
             continue;
         }
-        if constexpr (HALVES && CH == 16 && !SHARED && !kPair) {
-            // a box too large for the slab because the wave's two halves (rows 0-3 and 4-7
-            // of its 8x8 rays) run apart, e.g. along the volume's silhouette where the rays'
-            // lengths change from row to row: one box per half, side by side in the slab.
-            // Lanes without a ray follow the wave's first active lane, in the first half.
-            Box ha, hb;
-            if (ng == 1 && stage_halves(ha, hb)) {
+        // a box that does not fit the slab: the ngf groups of box bf staged per half-wave,
+        // split into 8- and 4-sample boxes, or gathered from global memory
+        auto fallback = [&](const Box& bf, int ngf) {
+            if constexpr (HALVES && CH == 16 && !SHARED && !kPair) {
+                // a box too large for the slab because the wave's two halves (rows 0-3 and 4-7
+                // of its 8x8 rays) run apart, e.g. along the volume's silhouette where the rays'
+                // lengths change from row to row: one box per half, side by side in the slab.
+                // Lanes without a ray follow the wave's first active lane, in the first half.
+                Box ha, hb;
+                if (ngf == 1 && stage_halves(ha, hb)) {
+                    if (live) {
+                        const bool second = active && lane >= 32u;
+                        group(CH);
+                        sample_slab_any<PV, CH, OPAC, FILT>(
+                            vol, second ? slab + ha.bytes : slab,
+                            second ? hb.lox : ha.lox, second ? hb.LY : ha.LY, second ? hb.LZ : ha.LZ,
+                            second ? hb.rowb : ha.rowb, second ? hb.SY : ha.SY, px, py, pz, dx, dy, dz,
+                            col, col2, op);
+                    }
+                    release();
+                    return;
+                }
+            }
+            if constexpr (CH == 16 && !SHARED) {
+                if (bf.bytes <= (uint64_t)(4 * ngf * SLAB_BYTES)) {
+                    // a box up to 4x the slab: halves of 8 samples with a box each, then quarters
+                    // of 4; a quarter that still does not fit samples global memory.  (Rays far
+                    // apart, e.g. a sparse frame, give boxes that no split brings under the
+                    // slab: those go to global memory directly.)
+#pragma unroll 1
+                    for (int h = 0; h < 2 * ngf; ++h) {
+                        Box hb;
+                        if (stage(8, hb)) {
+                            group(8);
+                            sample_slab_any<PV, 8, OPAC, FILT>(vol, slab, hb.lox, hb.LY, hb.LZ, hb.rowb, hb.SY,
+                                                         px, py, pz, dx, dy, dz, col, col2, op);
+                            release();
+                            continue;
+                        }
+#pragma unroll 1
+                        for (int q = 0; q < 2; ++q) {
+                            Box qb;
+                            const bool qfits = stage(4, qb);
+                            group(4);
+                            if (qfits)
+                                sample_slab_any<PV, 4, OPAC, FILT>(vol, slab, qb.lox, qb.LY, qb.LZ, qb.rowb,
+                                                             qb.SY, px, py, pz, dx, dy, dz, col, col2, op);
+                            else
+                                sample_global_any<PV, 4, OPAC, FILT>(vol, px, py, pz, dx, dy, dz, col, col2, op);
+                            release();
+                        }
+                    }
+                } else {
+#pragma unroll 1
+                    for (int k = 0; k < ngf; ++k) {
+                        group(CH);
+                        sample_global_any<PV, CH, OPAC, FILT>(vol, px, py, pz, dx, dy, dz, col, col2, op);
+                    }
+                    release();
+                }
+            } else {
                 if (live) {
-                    const bool second = active && lane >= 32u;
-                    group(CH);
-                    sample_slab_any<PV, CH, OPAC, FILT>(
-                        vol, second ? slab + ha.bytes : slab,
-                        second ? hb.lox : ha.lox, second ? hb.LY : ha.LY, second ? hb.LZ : ha.LZ,
-                        second ? hb.rowb : ha.rowb, second ? hb.SY : ha.SY, px, py, pz, dx, dy, dz,
-                        col, col2, op);
+#pragma unroll 1
+                    for (int k = 0; k < ngf; ++k) {
+                        group(CH);
+                        sample_global_any<PV, CH, OPAC, FILT>(vol, px, py, pz, dx, dy, dz, col, col2, op);
+                    }
                 }
                 release();
+            }
+        };
+        if constexpr (CH == 16 && !SHARED && !kPair) {
+            if (ng > 1) {
+                // a box of several 16-sample groups that does not fit: each group on its own,
+                // as a box of one group (oblique rays: 32-sample boxes often do not fit where
+                // 16-sample boxes do)
+#pragma unroll 1
+                for (int k = 0; k < ng; ++k) {
+                    Box bk;
+                    if (stage(CH, bk)) {
+                        if (live) {
+                            group(CH);
+                            sample_slab_any<PV, CH, OPAC, FILT>(vol, slab, bk.lox, bk.LY, bk.LZ,
+                                                                bk.rowb, bk.SY, px, py, pz, dx, dy,
+                                                                dz, col, col2, op);
+                        }
+                        release();
+                    } else {
+                        fallback(bk, 1);
+                    }
+                }
                 continue;
             }
         }
-        if constexpr (CH == 16 && !SHARED) {
-            if (b.bytes <= (uint64_t)(4 * ng * SLAB_BYTES)) {
-                // a box up to 4x the slab: halves of 8 samples with a box each, then quarters
-                // of 4; a quarter that still does not fit samples global memory.  (Rays far
-                // apart, e.g. a sparse frame, give boxes that no split brings under the
-                // slab: those go to global memory directly.)
-#pragma unroll 1
-                for (int h = 0; h < 2 * ng; ++h) {
-                    Box hb;
-                    if (stage(8, hb)) {
-                        group(8);
-                        sample_slab_any<PV, 8, OPAC, FILT>(vol, slab, hb.lox, hb.LY, hb.LZ, hb.rowb, hb.SY,
-                                                     px, py, pz, dx, dy, dz, col, col2, op);
-                        release();
-                        continue;
-                    }
-#pragma unroll 1
-                    for (int q = 0; q < 2; ++q) {
-                        Box qb;
-                        const bool qfits = stage(4, qb);
-                        group(4);
-                        if (qfits)
-                            sample_slab_any<PV, 4, OPAC, FILT>(vol, slab, qb.lox, qb.LY, qb.LZ, qb.rowb,
-                                                         qb.SY, px, py, pz, dx, dy, dz, col, col2, op);
-                        else
-                            sample_global_any<PV, 4, OPAC, FILT>(vol, px, py, pz, dx, dy, dz, col, col2, op);
-                        release();
-                    }
-                }
-            } else {
-#pragma unroll 1
-                for (int k = 0; k < ng; ++k) {
-                    group(CH);
-                    sample_global_any<PV, CH, OPAC, FILT>(vol, px, py, pz, dx, dy, dz, col, col2, op);
-                }
-                release();
-            }
-        } else {
-            if (live) {
-#pragma unroll 1
-                for (int k = 0; k < ng; ++k) {
-                    group(CH);
-                    sample_global_any<PV, CH, OPAC, FILT>(vol, px, py, pz, dx, dy, dz, col, col2, op);
-                }
-            }
-            release();
-        }
+        fallback(b, ng);
     }
 }
 

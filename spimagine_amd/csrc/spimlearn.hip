@@ -744,9 +744,14 @@ int check_shape(const spl_net* t, int n, int h, int w) {
     return SPL_OK;
 }
 
+// Work buffers for n images of h x w (grown, never shrunk: calls of different batch sizes
+// reuse the largest allocation).
 int ensure_work(spl_net* t, int n, int h, int w) {
     const size_t px = (size_t)n * h * w;
-    if (t->cap == px && t->a0) return SPL_OK;
+    if (t->cap >= px && t->a0) {
+        t->lpart_n = (px + 255) / 256;  // the loss's blocks for this call's pixels
+        return SPL_OK;
+    }
     free_work(t);
     const size_t b16 = px * C * 2, f32 = px * C * 4;
     struct {
@@ -976,7 +981,7 @@ int spl_forward(spl_net* t, const float* d_in, int n, int h, int w, float* d_out
     if (!d_in || !d_out) return fail(t, SPL_ERR_INVALID_ARG, "NULL buffer");
     DeviceGuard g(t->device);
     hipStream_t s = stream ? (hipStream_t)stream : t->stream;
-    if (t->cap != (size_t)n * h * w) HIPCALL(t, hipStreamSynchronize(s));  // buffers in use
+    if (t->cap < (size_t)n * h * w) HIPCALL(t, hipStreamSynchronize(s));  // buffers regrown
     rc = ensure_work(t, n, h, w);
     if (rc) return rc;
     forward(t, d_in, n, h, w, d_out, s);
@@ -1015,7 +1020,7 @@ int spl_train_step(spl_net* t, const float* d_in, const float* d_target, int n, 
     if (!(lr > 0.0f)) return fail(t, SPL_ERR_INVALID_ARG, "learning rate must be positive");
     DeviceGuard g(t->device);
     hipStream_t s = stream ? (hipStream_t)stream : t->stream;
-    if (t->cap != (size_t)n * h * w) HIPCALL(t, hipStreamSynchronize(s));
+    if (t->cap < (size_t)n * h * w) HIPCALL(t, hipStreamSynchronize(s));  // buffers regrown
     rc = ensure_work(t, n, h, w);
     if (rc) return rc;
     forward(t, d_in, n, h, w, t->y, s);

@@ -42,7 +42,7 @@ struct LaunchTuning {
     int tile_map = (int)kTileMapInterleaved;
     int slab_kb = 19;  // per-wave LDS slab of the staged max projection, 0 = off
     int staging = 0;   // 0 per-wave boxes, 1 one box per workgroup
-    int chunk = 0;     // samples per staged box, 0 = from the ray step
+    int chunk = 0;     // samples per staged box, 0 = from the ray step (32 when it allows 16)
     int row_pad = 0;   // pad staged slab rows to an odd number of 16-byte units
     int wg_waves = 0;  // per-wave staged kernels: waves per workgroup (1, 4; 0 = by frame size)
     int skip = 1;      // skip boxes / stop rays by the volume's brick bounds (SPR_TUNE_SKIP)
@@ -103,7 +103,9 @@ inline int chunk_for(const LaunchTuning& t, int nouter, int nmax) {
 // the staged kernels' launch options carried in RayParams
 inline void staged_tuning(const LaunchTuning& t, RayParams& p) {
     p.row_pad = (uint32_t)t.row_pad;
-    p.box16 = t.chunk > 16 ? (uint32_t)(t.chunk / 16) : 1u;
+    // chunk 0 (from the ray step): where the step allows 16-sample groups, boxes of two
+    // (a box that does not fit the slab falls back to one group at a time)
+    p.box16 = t.chunk > 16 ? (uint32_t)(t.chunk / 16) : t.chunk == 0 ? 2u : 1u;
 }
 
 // Launchers of one voxel type T (float, uint16_t, uint8_t) and one power-of-two rule P2
