@@ -756,6 +756,18 @@ def bench_serve(args):
     wall = time.perf_counter() - t0
     lat = [r.latency_s * 1e3 for r in res]
     batch = [r.batch_size for r in res]
+    # the tail by request kind: (volume, frame side, method) -> latency p50 / p99 / max and
+    # the mean time from dispatch to result (latency minus queueing)
+    kinds = {}
+    for q, r in zip(reqs, res):
+        m = "alpha" if q.settings["alpha_pow"] else q.method
+        kinds.setdefault("%s %d %s" % (q.volume, q.size[0], m), []).append(
+            (r.latency_s * 1e3, (r.latency_s - r.queued_s) * 1e3))
+    by_kind = {k: {"n": len(v), "p50": round(pct([a for a, _ in v], 50), 2),
+                   "p99": round(pct([a for a, _ in v], 99), 2),
+                   "max": round(max(a for a, _ in v), 2),
+                   "service_mean": round(float(np.mean([b for _, b in v])), 2)}
+               for k, v in sorted(kinds.items())}
     px = sum(r.size[0] * r.size[1] for r in reqs)
     server.close()
     # algorithmic bytes of the max-projection requests (reference samples of the rays that
@@ -791,6 +803,7 @@ def bench_serve(args):
                    "hbm_budget_bytes": server.hbm_budget},
         "latency_ms": {"p50": round(pct(lat, 50), 3), "p99": round(pct(lat, 99), 3),
                        "mean": round(float(np.mean(lat)), 3)},
+        "latency_by_kind_ms": by_kind,
         "mean_batch": round(float(np.mean(batch)), 2), "gpixels_per_s": round(px / wall / 1e9, 4),
         "batches": server.batches,
         "roofline": dict(roofline(b_alg, 1e3 * wall, None),
