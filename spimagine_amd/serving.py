@@ -345,8 +345,22 @@ class RenderServer(object):
             else:
                 self.ctx.render_slot(vol.slot, p, _lib.METHOD_MAX_PROJECT, w, h, out.data_ptr(),
                                      alpha.data_ptr(), stream=stream.cuda_stream)
-            return Result(out.cpu().numpy(), alpha.cpu().numpy(),
-                          **{k: v.cpu().numpy() for k, v in extra.items()})
+            host = self._download([out, alpha] + list(extra.values()), stream)
+        return Result(host[0], host[1], **dict(zip(extra, host[2:])))
+
+    def _download(self, tensors, stream):
+        """Device frames -> host arrays: pinned host tensors (torch's caching host
+        allocator), copied asynchronously on the render stream, one wait for all of them.
+        Pinned memory is what lets the copy run at DMA rate: a copy into pageable memory
+        is staged through a bounce buffer and, for a 2048^2 iso-surface result (112 MB),
+        took longer than the render."""
+        torch = self.torch
+        with torch.cuda.stream(stream):
+            host = [torch.empty(t.shape, dtype=t.dtype, pin_memory=True) for t in tensors]
+            for h, t in zip(host, tensors):
+                h.copy_(t, non_blocking=True)
+        stream.synchronize()
+        return [h.numpy() for h in host]
 
     def _run(self, batch, stream):
         torch = self.torch
@@ -369,7 +383,7 @@ class RenderServer(object):
                         else _lib.FILTER_LINEAR)
                 self.ctx.render_frames(frames, req0.settings["alpha_pow"], filt,
                                        stream=stream.cuda_stream)
-                host = [pl.cpu().numpy() for pl in planes]
+            host = self._download(planes, stream)
             results = [Result(hp[0], hp[1]) for hp in host]
         t1 = time.perf_counter()
         for (r, fut), res in zip(batch, results):
