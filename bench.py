@@ -416,6 +416,9 @@ def bench_iso(args):
     log("bench: generating 512^3 uint16 blob volume on cuda:0")
     vt = synth.config_volume("iso", device="cuda")
     ctx = _lib.Context(0)
+    for kv in args.tune:  # launch settings fixed by hand (default: the library's)
+        k, v = kv.split("=")
+        ctx.set_tuning(k, int(v))
     ctx.set_volume_device(vt.data_ptr(), np.uint16, cfg["shape"])
     lo, hi = ctx.volume_minmax()
     iso_max = 2 * (lo + .3 * (hi - lo))
@@ -465,14 +468,15 @@ def bench_iso(args):
         iso_samples.append(iso_reference_samples(d, a, mats[k][0], mats[k][1], W, H, S, (0, H),
                                                  rays))
     table = {}
+    keep = ctx.get_tuning("skip")
     for skip in (0, 1):
         ctx.set_tuning("skip", skip)
         for mode in ("iso", "alpha"):
             ms = [pose_ms(mode, k, args.skip_frames) for k in range(8)]
             table.setdefault(mode, {})["on" if skip else "off"] = {
                 "poses_ms": [round(t, 4) for t in ms], "mean_ms": round(float(np.mean(ms)), 4)}
-    ctx.set_tuning("skip", 1)
-    log("bench: timing %d iso frames (8 poses, skipping on)" % args.steps)
+    ctx.set_tuning("skip", keep)
+    log("bench: timing %d iso frames (8 poses, skipping %s)" % (args.steps, "on" if keep else "off"))
     for k in range(8):
         set_pose(k, "iso")
         frame("iso")

@@ -203,11 +203,16 @@ void launch_staged_opac(int ch, dim3 grid, hipStream_t s, Args... args) {
         hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 4, false, false, true, SKIP>), grid, dim3(256), 0, s, args...);
 }
 
-// the iso-surface march staged: per-wave 19 KiB slabs (four waves per workgroup: one per
-// workgroup was 3.5% slower over five poses of the headline volume)
+// the iso-surface march staged: per-wave 19 KiB slabs; one_wave (16-sample boxes): one
+// wave per workgroup, grid 4 x the interleaved tile grid
 template <typename T, bool P2, bool SKIP, int FILT = 0, typename... Args>
-void launch_iso_staged(int ch, dim3 grid, hipStream_t s, Args... args) {
-    if (ch == 16)
+void launch_iso_staged(int ch, bool one_wave, dim3 grid, hipStream_t s, const RayParams& p,
+                       Args... args) {
+    if (one_wave && ch == 16)
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 16, false, SKIP, FILT, true>),
+                           dim3(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty)), dim3(64), 0, s,
+                           args...);
+    else if (ch == 16)
         hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 16, false, SKIP, FILT>), grid, dim3(256), 0, s, args...);
     else if (ch == 8)
         hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 8, false, SKIP, FILT>), grid, dim3(256), 0, s, args...);
@@ -364,17 +369,19 @@ hipError_t TypedLaunch<T, P2>::iso_march(const LaunchTuning& t, const VolSlot& v
              : launch_iso_wide<T, P2, false>(vs, ch, grid, s, p, out, alpha, depth, normals);
         return hipGetLastError();
     }
+    // one wave per workgroup as for the max projection (interleaved tile maps)
+    const bool one_wave = one_wave_for(t, p) && p.tile_map == kTileMapInterleaved;
     if (staged_iso && vs.layout == SPR_LAYOUT_PADDED) {
         const PaddedVolume<T, P2> v = padded_volume<T, P2>(vs);
-        skip ? launch_iso_staged<T, P2, true>(ch, grid, s, v, p, out, alpha, depth, normals)
-             : launch_iso_staged<T, P2, false>(ch, grid, s, v, p, out, alpha, depth, normals);
+        skip ? launch_iso_staged<T, P2, true>(ch, one_wave, grid, s, p, v, p, out, alpha, depth, normals)
+             : launch_iso_staged<T, P2, false>(ch, one_wave, grid, s, p, v, p, out, alpha, depth, normals);
         return hipGetLastError();
     }
     // the nearest filter's march staged the same way (padded slots below 4 GiB)
     if (nearest && t.slab_kb > 0 && t.staging == 0 && vs.layout == SPR_LAYOUT_PADDED) {
         const PaddedVolume<T, P2> v = padded_volume<T, P2>(vs);
-        skip ? launch_iso_staged<T, P2, true, 1>(ch, grid, s, v, p, out, alpha, depth, normals)
-             : launch_iso_staged<T, P2, false, 1>(ch, grid, s, v, p, out, alpha, depth, normals);
+        skip ? launch_iso_staged<T, P2, true, 1>(ch, one_wave, grid, s, p, v, p, out, alpha, depth, normals)
+             : launch_iso_staged<T, P2, false, 1>(ch, one_wave, grid, s, p, v, p, out, alpha, depth, normals);
         return hipGetLastError();
     }
     return visit_layout<T, P2>(vs, [&](auto v) {

@@ -1332,22 +1332,26 @@ __device__ __forceinline__ void iso_march_chunk(const V& vol, const uint8_t* sla
 // positions advance, by the same additions -- when its box's bricks show that no marching
 // ray can change side there: bricks' maximum <= iso for rays that start below or at the
 // iso value, bricks' minimum > iso for rays that start above it.
+// WAVE: one wave per workgroup (grid 4 x the interleaved tile grid, as k_max_project_wave),
+// else four waves, one 16x16 tile, per workgroup.
 template <typename T, bool POW2, int SLAB_BYTES, int CH, bool WIDE = false, bool SKIP = false,
-          int FILT = 0>
-__global__ __launch_bounds__(256) void k_iso_staged(PaddedVolume<T, POW2, WIDE> vol, RayParams p,
-                                                    float* __restrict__ out,
-                                                    float* __restrict__ alpha,
-                                                    float* __restrict__ depth,
-                                                    float* __restrict__ normals) {
+          int FILT = 0, bool WAVE = false>
+__global__ __launch_bounds__(WAVE ? 64 : 256) void k_iso_staged(PaddedVolume<T, POW2, WIDE> vol,
+                                                                RayParams p,
+                                                                float* __restrict__ out,
+                                                                float* __restrict__ alpha,
+                                                                float* __restrict__ depth,
+                                                                float* __restrict__ normals) {
     using PV = PaddedVolume<T, POW2, WIDE>;
     static_assert(FILT == 0 || !WIDE, "nearest: volumes below 4 GiB");
     constexpr uint32_t E = 16u / sizeof(T);
     constexpr uint32_t TS = sizeof(T);
-    __shared__ Unit16 lds[4 * SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
+    constexpr uint32_t NW = WAVE ? 1u : 4u;
+    __shared__ Unit16 lds[NW * SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
     const uint32_t lane = threadIdx.x & 63u;
-    uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * SLAB_BYTES;
+    uint8_t* slab = reinterpret_cast<uint8_t*>(lds) + (WAVE ? 0u : (threadIdx.x >> 6) * SLAB_BYTES);
     uint32_t x = 0, y = 0;
-    const bool inside = tile_pixel(p, &x, &y);
+    const bool inside = WAVE ? wave_tile_pixel(p, &x, &y) : tile_pixel(p, &x, &y);
     const uint32_t o = x + p.W * y;
     F4 orig = mk4(0, 0, 0, 0), direc = mk4(0, 0, 0, 0), pos0 = mk4(0.5f, 0.5f, 0.5f, 0.0f),
        delta = mk4(0, 0, 0, 0);

@@ -203,11 +203,22 @@ __device__ __forceinline__ void iso_finish(const Vol& vol, const RayParams& p, u
     F4 delta2 = mk4(delta.x / fb, delta.y / fb, delta.z / fb, delta.w / fb);
     float dt2 = dt / fb;
     F4 pos = add4(pos0, scl4((float)(i - 1), delta));
+    // the ten substep samples are loaded together (their positions do not depend on the
+    // values), then scanned for the first one across: the same samples and the same
+    // running sums as a loop that loads one substep at a time and breaks
+    float vs[10];
+    {
+        F4 q = pos;
+#pragma unroll
+        for (int j = 0; j < 10; j++) {
+            vs[j] = vol.template sample<FILTER>(q.x, q.y, q.z);
+            q = add4(q, delta2);
+        }
+    }
     for (int j = 1; j <= 10; j++) {
-        float v = vol.template sample<FILTER>(pos.x, pos.y, pos.z);
         pos = add4(pos, delta2);
         t_hit += dt2;
-        if ((int)(v > iso) != isGreater) break;
+        if ((int)(vs[j - 1] > iso) != isGreater) break;
     }
     // Phong at the refined point, 4-tap gradient: weights (2,1) in x,y and (1,1) in z
     F4 light = normalize4(mult(p.invM, mk4(2.0f, -1.0f, -2.0f, 0.0f)));
