@@ -20,9 +20,16 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspimrender.so")
 LEARN_LIB = os.path.join(HERE, "libspimlearn.so")
 
-# (object name, source, extra defines)
+# The ray-marching kernels (launch_typed.hip) run at two waves per SIMD (their LDS slabs
+# bound occupancy), so the scheduler may spend registers on latency: LLVM's max-ILP
+# strategy issues a sample's LDS reads further ahead of their use (1.5% on the 1024^2 max
+# projection, 3% on the iso march; the instruction sequence per sample is unchanged).
+SCHED_ILP = ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
+
+# (object name, source, extra flags)
 RENDER_OBJECTS = [("spimrender", "spimrender.hip", [])] + [
-    ("launch_v%d_p%d" % (v, p2), "launch_typed.hip", ["-DSPR_VOXEL=%d" % v, "-DSPR_POW2=%d" % p2])
+    ("launch_v%d_p%d" % (v, p2), "launch_typed.hip",
+     ["-DSPR_VOXEL=%d" % v, "-DSPR_POW2=%d" % p2] + SCHED_ILP)
     for v in (0, 1, 2) for p2 in (0, 1)]
 LEARN_OBJECTS = [("spimlearn", "spimlearn.hip", [])]
 LIBRARIES = {LIB: RENDER_OBJECTS, LEARN_LIB: LEARN_OBJECTS}
@@ -57,10 +64,10 @@ def _run(cmd, what, verbose):
         raise RuntimeError("hipcc failed building %s" % what)
 
 
-def compile_object(name, source, defines, verbose=False):
+def compile_object(name, source, flags, verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     obj = os.path.join(OBJ, name + ".o")
-    cmd = [hipcc()] + HIPCC_FLAGS + defines + ["-I", os.path.join(ROOT, "include"), "-I", CSRC,
+    cmd = [hipcc()] + HIPCC_FLAGS + flags + ["-I", os.path.join(ROOT, "include"), "-I", CSRC,
                                                 "-c", "-o", obj, os.path.join(CSRC, source)]
     _run(cmd, obj, verbose)
     return obj
