@@ -264,6 +264,7 @@ def bench_headline(args):
     cpu = parity = None
     vol = synth.device_volume_to_numpy(vt, np.uint16)
     upload = measure_upload(ctx, vol)
+    host_frame = measure_host_frames(ctx, W, H, 10)
     if not args.no_cpu:
         cpu, parity = cpu_baseline(vol, invP, invM, lo, hi, S, W, H, args.cpu_seconds,
                                    gpu_out, gpu_alpha, "the same 1024^2 frame")
@@ -281,7 +282,8 @@ def bench_headline(args):
                     "frames": len(lat_ev), "poses": 8, "gsamples_per_s": round(pose_gsps, 3)},
         skipping=skipping, nearest_filter=nearest,
         roofline=roofline(b_alg, kernel_ms, traffic_from_profile("headline")),
-        cpu_baseline=cpu, parity_max_abs_diff_vs_oracle=parity, upload=upload)
+        cpu_baseline=cpu, parity_max_abs_diff_vs_oracle=parity, upload=upload,
+        host_frame_ms=host_frame)
 
 
 def skip_table(ctx, stream, frame, set_pose, samples, reps):
@@ -345,6 +347,28 @@ def filter_table(ctx, stream, frame, set_pose, samples, reps):
                      "poses_mean_ms": round(float(np.mean(ms[1:])), 4)}
     ctx.set_tuning("slab_kb", keep)
     return out
+
+
+def measure_host_frames(ctx, W, H, n):
+    """The drop-in host path (spr_render, as VolumeRenderer.render calls it): one frame of
+    the context's current parameters rendered and its three planes (output, alpha, depth)
+    copied into fresh host arrays, wall time per call (median of n).  Pageable numpy arrays
+    (what VolumeRenderer allocates) and, beside them, arrays in pinned host memory."""
+    import torch
+    from spimagine_amd import _lib
+    res = {}
+    for kind in ("pageable", "pinned"):
+        ts = []
+        for i in range(n + 2):
+            if kind == "pageable":
+                planes = [np.empty((H, W), np.float32) for _ in range(3)]
+            else:
+                planes = [torch.empty((H, W), pin_memory=True).numpy() for _ in range(3)]
+            t0 = time.perf_counter()
+            ctx.render(_lib.METHOD_MAX_PROJECT, 1, 0, W, H, *planes)
+            ts.append(time.perf_counter() - t0)
+        res[kind + "_ms"] = round(1e3 * float(np.median(ts[2:])), 3)
+    return res
 
 
 def measure_upload(ctx, vol):

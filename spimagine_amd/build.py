@@ -23,13 +23,17 @@ LEARN_LIB = os.path.join(HERE, "libspimlearn.so")
 # The ray-marching kernels (launch_typed.hip) run at two waves per SIMD (their LDS slabs
 # bound occupancy), so the scheduler may spend registers on latency: LLVM's max-ILP
 # strategy issues a sample's LDS reads further ahead of their use (1.5% on the 1024^2 max
-# projection, 3% on the iso march; the instruction sequence per sample is unchanged).
+# projection, 3% on the iso march; the instruction sequence per sample is unchanged).  The
+# persistent batch launch (SPR_PART 1) was 0.7% slower with it and keeps the default.
 SCHED_ILP = ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
 
 # (object name, source, extra flags)
 RENDER_OBJECTS = [("spimrender", "spimrender.hip", [])] + [
     ("launch_v%d_p%d" % (v, p2), "launch_typed.hip",
-     ["-DSPR_VOXEL=%d" % v, "-DSPR_POW2=%d" % p2] + SCHED_ILP)
+     ["-DSPR_VOXEL=%d" % v, "-DSPR_POW2=%d" % p2, "-DSPR_PART=0"] + SCHED_ILP)
+    for v in (0, 1, 2) for p2 in (0, 1)] + [
+    ("batch_v%d_p%d" % (v, p2), "launch_typed.hip",
+     ["-DSPR_VOXEL=%d" % v, "-DSPR_POW2=%d" % p2, "-DSPR_PART=1"])
     for v in (0, 1, 2) for p2 in (0, 1)]
 LEARN_OBJECTS = [("spimlearn", "spimlearn.hip", [])]
 LIBRARIES = {LIB: RENDER_OBJECTS, LEARN_LIB: LEARN_OBJECTS}

@@ -14,6 +14,12 @@
 #error "compile with -DSPR_VOXEL=0|1|2 -DSPR_POW2=0|1"
 #endif
 
+// SPR_PART 0: every launcher but the persistent batch launch; 1: the batch launch alone
+// (two objects, so each can take its own scheduling flags); undefined: both.
+#ifndef SPR_PART
+#define SPR_PART 2
+#endif
+
 namespace spr {
 namespace {
 
@@ -276,6 +282,7 @@ void launch_pair_ch(const VolSlot& vs, bool one_wave, dim3 grid, hipStream_t s, 
 
 }  // namespace
 
+#if SPR_PART != 1
 template <typename T, bool P2>
 hipError_t TypedLaunch<T, P2>::max_project(const LaunchTuning& t, const VolSlot& vs, RayParams p,
                                            bool nearest, float* out, float* alpha, hipStream_t s) {
@@ -396,6 +403,9 @@ hipError_t TypedLaunch<T, P2>::iso_march(const LaunchTuning& t, const VolSlot& v
     });
 }
 
+#endif  // SPR_PART != 1
+
+#if SPR_PART != 0
 template <typename T, bool P2>
 hipError_t TypedLaunch<T, P2>::batch(const LaunchTuning& t, int mode, int ch, const BatchJob* d_jobs,
                                      const RayParams& p, const BatchQueues& q, uint32_t* counters,
@@ -417,6 +427,9 @@ hipError_t TypedLaunch<T, P2>::batch(const LaunchTuning& t, int mode, int ch, co
     return launch_batch_slab<T, P2, 12288>(sh, ch, cus, d_jobs, p, q, counters, s);
 }
 
+#endif  // SPR_PART != 0
+
+#if SPR_PART != 1
 template <typename T, bool P2>
 hipError_t TypedLaunch<T, P2>::split(int seg, bool staged, const BatchJob* d_jobs, uint32_t njobs,
                                      uint32_t nitems, uint32_t npix, const RayParams& base,
@@ -454,11 +467,14 @@ hipError_t TypedLaunch<T, P2>::coarse_pair(const VolSlot& a, const VolSlot& b, c
     return hipGetLastError();
 }
 
+#endif  // SPR_PART != 1
+
 #if SPR_VOXEL == 0
 template struct TypedLaunch<float, (SPR_POW2 != 0)>;
 #elif SPR_VOXEL == 1
 template struct TypedLaunch<uint16_t, (SPR_POW2 != 0)>;
 
+#if SPR_PART != 1
 template <bool P2>
 hipError_t launch_pair(const LaunchTuning& t, const VolSlot& vs, RayParams p, float* out,
                        float* alpha, hipStream_t s) {
@@ -477,6 +493,7 @@ hipError_t launch_pair(const LaunchTuning& t, const VolSlot& vs, RayParams p, fl
 }
 template hipError_t launch_pair<(SPR_POW2 != 0)>(const LaunchTuning&, const VolSlot&, RayParams,
                                                  float*, float*, hipStream_t);
+#endif  // SPR_PART != 1
 #elif SPR_VOXEL == 2
 template struct TypedLaunch<uint8_t, (SPR_POW2 != 0)>;
 #else
