@@ -419,6 +419,50 @@ def test_render_device_matches_host_render(VR):
     np.testing.assert_array_equal(a.cpu().numpy(), rend.output_alpha)
 
 
+def test_each_render_returns_fresh_arrays(VR):
+    """render() leaves fresh arrays in output / output_alpha / output_depth (the reference
+    assigns buffer.get() copies, volumerender.py:388-390, 502-506): a later render of
+    another pose changes none of the arrays an earlier one returned, and the arrays are
+    writable and independent (they live in pinned host memory where torch can pin)."""
+    vol = synth.blob_volume((36, 40, 44), n_blobs=8, sigma=(2., 4.), amp=(500., 3000.),
+                            background=100., poisson=False, read_noise=0., dtype="uint16",
+                            seed=12)
+    rend = VR((64, 48))
+    rend.set_data(vol)
+    lo, hi = float(vol.min()), float(vol.max())
+    poses = [np.dot(mat4_translate(0, 0, -3.2), mat4_rotation(a, .2, 1, .1)) for a in (.3, 1.4)]
+    got = []
+    for M in poses:
+        rend.set_modelView(M)
+        rend.render(minVal=lo, maxVal=hi)
+        got.append((rend.output, rend.output_alpha, rend.output_depth))
+    first = [a.copy() for a in got[0]]
+    assert not np.array_equal(got[0][0], got[1][0])
+    for a, b in zip(got[0], first):
+        np.testing.assert_array_equal(a, b)
+    for a in got[0] + got[1]:
+        assert a.dtype == np.float32 and a.flags.writeable and a.flags.c_contiguous
+    assert not np.shares_memory(got[0][0], got[1][0])
+    assert not np.shares_memory(got[1][0], got[1][1])
+    got[1][0][:] = -1.  # the caller owns the array
+    rend.set_modelView(poses[0])
+    rend.render(minVal=lo, maxVal=hi)
+    np.testing.assert_array_equal(rend.output, first[0])
+    np.testing.assert_array_equal(rend.output_alpha, first[1])
+    want = oracle_mip(rend, vol)
+    assert np.abs(rend.output - want[0]).max() <= TOL
+    # iso: all five planes fresh as well
+    rend.render(method="iso_surface", maxVal=2 * (lo + .3 * (hi - lo)))
+    iso1 = [rend.output, rend.output_alpha, rend.output_depth, rend.output_normals,
+            rend.output_occlusion]
+    keep = [a.copy() for a in iso1]
+    rend.set_modelView(poses[1])
+    rend.render(method="iso_surface", maxVal=2 * (lo + .3 * (hi - lo)))
+    for a, b in zip(iso1, keep):
+        np.testing.assert_array_equal(a, b)
+    assert rend.output_normals.shape == (48, 64, 3)
+
+
 def test_headline_size_rows_parity():
     """1024x1024x512 uint16 (8000 blobs) -> 1024^2, max_steps 1536: GPU frame against the
     oracle on 12 full rows spread over the frame (12 x 1024 rays x 1552 samples)."""
