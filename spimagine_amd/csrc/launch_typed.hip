@@ -72,9 +72,10 @@ hipError_t order_tiles(int tile_map, bool per_wave, const PaddedVolume<T, P2, WI
 }
 
 // The slab of a one-wave workgroup for slab_kb 19: LDS is allocated in 512-byte granules,
-// so eight one-wave workgroups per CU fit 19,968 B each (the slab plus its 16 bytes of
-// slack), 2.5% more than 19 KiB (measured ~0.5-1% faster).
-constexpr uint32_t kWaveSlab19 = 19952;
+// so eight one-wave workgroups per CU fit 20,480 B each (the slab plus its 16 bytes of
+// slack: all 160 KiB), 5% more than 19 KiB (19,952 B was ~0.5-1% faster than 19 KiB, and
+// 20,464 B 1.3% faster again on the 1024^2 max projection).
+constexpr uint32_t kWaveSlab19 = 20464;
 
 // one wave per workgroup: the plain max projection (slab 16 or 19 KiB) or the opacity mode;
 // skip: the variant that skips boxes by their brick bounds (19 KiB slabs)
@@ -267,7 +268,7 @@ hipError_t launch_batch_slab(bool shared, int ch, int num_cus, const BatchJob* d
                   : launch_batch_staged<T, P2, SLAB, false>(ch, num_cus, d_jobs, p, q, counters, s);
 }
 
-// both channels of a pair slot: one wave per workgroup (19,952-byte slabs) or four
+// both channels of a pair slot: one wave per workgroup (20,464-byte slabs) or four
 template <bool P2, int CH>
 void launch_pair_ch(const VolSlot& vs, bool one_wave, dim3 grid, hipStream_t s, const RayParams& p,
                     float* out, float* alpha) {

@@ -123,12 +123,12 @@ __global__ __launch_bounds__(256) void k_split_march(const BatchJob* __restrict_
 }
 
 // The same work items marched through the per-wave LDS slab of the staged kernels (one
-// wave per workgroup, 19,952-byte slab: eight workgroups per CU): the segment's 16-sample
+// wave per workgroup, 20,464-byte slab: eight workgroups per CU): the segment's 16-sample
 // boxes are staged whole, per half-wave, or split into 8- and 4-sample boxes, as in the
 // single-frame kernel (mip_staged_march), and sampled from global memory only when none
 // fits.  Rays a few voxels apart (one pixel in 8 x 8 of a 1024^2 frame of a 512^3
 // volume) fit as half-wave boxes.  Lanes without a ray follow the first active lane.
-constexpr int kSplitSlab = 19952;
+constexpr int kSplitSlab = 20464;
 
 template <typename T, bool POW2, int SEG>
 __global__ __launch_bounds__(64) void k_split_staged(const BatchJob* __restrict__ jobs,
