@@ -81,7 +81,7 @@ extern "C" {
                       opacity mode; two small kernels order the tiles before the frame,
                       on its stream)
    SPR_TUNE_SLAB_KB   per-wave LDS slab (KiB) of the staged max projection: 8, 12, 16 or
-                      19 (default: 19,456 B, or 20,464 B with one wave per workgroup;
+                      19 (default: 20,448 B, or 20,464 B with one wave per workgroup;
                       per-wave staging only, 16 for workgroup staging); 0 = sample every
                       chunk from global memory
    SPR_TUNE_STAGING   0 = each wave stages its own box (default), 1 = the four waves of a

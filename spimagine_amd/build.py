@@ -24,7 +24,8 @@ LEARN_LIB = os.path.join(HERE, "libspimlearn.so")
 # bound occupancy), so the scheduler may spend registers on latency: LLVM's max-ILP
 # strategy issues a sample's LDS reads further ahead of their use (1.5% on the 1024^2 max
 # projection, 3% on the iso march; the instruction sequence per sample is unchanged).  The
-# persistent batch launch (SPR_PART 1) was 0.7% slower with it and keeps the default.
+# persistent batch launch (0.7% slower with it) and the plain gather kernels (the nearest
+# one 23% slower) are compiled apart (SPR_PART 1) with the default scheduler.
 SCHED_ILP = ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
 
 # (object name, source, extra flags)

@@ -14,13 +14,18 @@
 #error "compile with -DSPR_VOXEL=0|1|2 -DSPR_POW2=0|1"
 #endif
 
-// SPR_PART 0: every launcher but the persistent batch launch; 1: the batch launch alone
-// (two objects, so each can take its own scheduling flags); undefined: both.
+// SPR_PART 0: the staged launchers (built with the max-ILP scheduler); 1: the persistent
+// batch launch and the plain gather kernels (default scheduler); undefined: both.
 #ifndef SPR_PART
 #define SPR_PART 2
 #endif
 
 namespace spr {
+// The per-wave slab of the four-wave staged kernels for slab_kb 19: two workgroups per CU
+// of four slabs, their 16 bytes of slack and up to 52 bytes of box and queue words, within
+// 81,920 B each (LDS is allocated in 512-byte granules): 5% more than 19 KiB, 1.4% faster
+// on the batch launch.
+constexpr uint32_t kQuadSlab19 = 20448;
 namespace {
 
 template <typename T, bool P2, int CH, bool SH, typename... Args>
@@ -30,8 +35,8 @@ void launch_staged_sh(int slab_kb, dim3 grid, hipStream_t s, Args... args) {
         hipLaunchKernelGGL((k_max_project_staged<T, P2, 8192 * F, CH, SH>), grid, dim3(256), 0, s, args...);
     else if (slab_kb == 16 || (SH && slab_kb == 19))
         hipLaunchKernelGGL((k_max_project_staged<T, P2, 16384 * F, CH, SH>), grid, dim3(256), 0, s, args...);
-    else if (slab_kb == 19)  // per-wave only: two workgroups of 4 x 19 KiB fill 152 of 160 KiB
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, CH, false>), grid, dim3(256), 0, s, args...);
+    else if (slab_kb == 19)  // per-wave only: two workgroups of 4 slabs fill the 160 KiB
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, kQuadSlab19, CH, false>), grid, dim3(256), 0, s, args...);
     else
         hipLaunchKernelGGL((k_max_project_staged<T, P2, 12288 * F, CH, SH>), grid, dim3(256), 0, s, args...);
 }
@@ -135,7 +140,7 @@ bool one_wave_for(const LaunchTuning& t, const RayParams& p) {
 }
 
 // a per-wave slab's bytes for the slab_kb tuning value
-uint32_t slab_bytes_of(int slab_kb) { return slab_kb == 19 ? 19456u : (uint32_t)slab_kb * 1024u; }
+uint32_t slab_bytes_of(int slab_kb) { return slab_kb == 19 ? kQuadSlab19 : (uint32_t)slab_kb * 1024u; }
 
 // volumes above 4 GiB: per-wave 16 KiB slabs (19 KiB measured 2% slower on 2048^2
 // frames); OPAC: the opacity mode; SKIP: boxes skipped by their brick bounds
@@ -183,11 +188,11 @@ void launch_staged(int slab_kb, bool shared, bool skip, int ch, dim3 grid, hipSt
                    Args... args) {
     if (skip) {
         if (ch == 16)
-            hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 16, false, false, false, true>), grid, dim3(256), 0, s, args...);
+            hipLaunchKernelGGL((k_max_project_staged<T, P2, kQuadSlab19, 16, false, false, false, true>), grid, dim3(256), 0, s, args...);
         else if (ch == 8)
-            hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 8, false, false, false, true>), grid, dim3(256), 0, s, args...);
+            hipLaunchKernelGGL((k_max_project_staged<T, P2, kQuadSlab19, 8, false, false, false, true>), grid, dim3(256), 0, s, args...);
         else
-            hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 4, false, false, false, true>), grid, dim3(256), 0, s, args...);
+            hipLaunchKernelGGL((k_max_project_staged<T, P2, kQuadSlab19, 4, false, false, false, true>), grid, dim3(256), 0, s, args...);
         return;
     }
     if (ch == 16)
@@ -203,11 +208,11 @@ void launch_staged(int slab_kb, bool shared, bool skip, int ch, dim3 grid, hipSt
 template <typename T, bool P2, bool SKIP, typename... Args>
 void launch_staged_opac(int ch, dim3 grid, hipStream_t s, Args... args) {
     if (ch == 16)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 16, false, false, true, SKIP>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, kQuadSlab19, 16, false, false, true, SKIP>), grid, dim3(256), 0, s, args...);
     else if (ch == 8)
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 8, false, false, true, SKIP>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, kQuadSlab19, 8, false, false, true, SKIP>), grid, dim3(256), 0, s, args...);
     else
-        hipLaunchKernelGGL((k_max_project_staged<T, P2, 19456, 4, false, false, true, SKIP>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_max_project_staged<T, P2, kQuadSlab19, 4, false, false, true, SKIP>), grid, dim3(256), 0, s, args...);
 }
 
 // the iso-surface march staged: per-wave 19 KiB slabs; one_wave (16-sample boxes): one
@@ -220,11 +225,11 @@ void launch_iso_staged(int ch, bool one_wave, dim3 grid, hipStream_t s, const Ra
                            dim3(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty)), dim3(64), 0, s,
                            args...);
     else if (ch == 16)
-        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 16, false, SKIP, FILT>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kQuadSlab19, 16, false, SKIP, FILT>), grid, dim3(256), 0, s, args...);
     else if (ch == 8)
-        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 8, false, SKIP, FILT>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kQuadSlab19, 8, false, SKIP, FILT>), grid, dim3(256), 0, s, args...);
     else
-        hipLaunchKernelGGL((k_iso_staged<T, P2, 19456, 4, false, SKIP, FILT>), grid, dim3(256), 0, s, args...);
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kQuadSlab19, 4, false, SKIP, FILT>), grid, dim3(256), 0, s, args...);
 }
 
 // the plain (unstaged) kernels on the slot's layout: linear layout (a volume above 4 GiB
@@ -277,7 +282,7 @@ void launch_pair_ch(const VolSlot& vs, bool one_wave, dim3 grid, hipStream_t s, 
                            dim3(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty)), dim3(64), 0, s,
                            padded_volume<U16Pair, P2>(vs), p, out, alpha);
     else
-        hipLaunchKernelGGL((k_max_project_staged<U16Pair, P2, 19456, CH, false>), grid, dim3(256), 0,
+        hipLaunchKernelGGL((k_max_project_staged<U16Pair, P2, kQuadSlab19, CH, false>), grid, dim3(256), 0,
                            s, padded_volume<U16Pair, P2>(vs), p, out, alpha);
 }
 
@@ -335,7 +340,7 @@ hipError_t TypedLaunch<T, P2>::max_project(const LaunchTuning& t, const VolSlot&
         if (staged_opac || t.staging == 0)  // per-wave slabs
             e = order_tiles(t.tile_map, one_wave, v,
                             one_wave ? (staged_opac || t.slab_kb != 16 ? kWaveSlab19 : 16384u)
-                                     : staged_opac ? 19456u : slab_bytes_of(t.slab_kb),
+                                     : staged_opac ? kQuadSlab19 : slab_bytes_of(t.slab_kb),
                             p, s, &obuf, &g);
         if (e == hipSuccess) {
             if (one_wave && staged_opac)
@@ -352,14 +357,7 @@ hipError_t TypedLaunch<T, P2>::max_project(const LaunchTuning& t, const VolSlot&
         if (obuf) (void)hipFreeAsync(obuf, s);
         return e;
     }
-    return visit_layout<T, P2>(vs, [&](auto v) {
-        using V = decltype(v);
-        if (nearest)
-            hipLaunchKernelGGL((k_max_project<V, 1>), grid, block, 0, s, v, p, out, alpha);
-        else
-            hipLaunchKernelGGL((k_max_project<V, 0>), grid, block, 0, s, v, p, out, alpha);
-        return hipGetLastError();
-    });
+    return plain_max_project(vs, p, nearest, out, alpha, s);
 }
 
 template <typename T, bool P2>
@@ -392,6 +390,34 @@ hipError_t TypedLaunch<T, P2>::iso_march(const LaunchTuning& t, const VolSlot& v
              : launch_iso_staged<T, P2, false, 1>(ch, one_wave, grid, s, p, v, p, out, alpha, depth, normals);
         return hipGetLastError();
     }
+    return plain_iso_march(vs, p, nearest, out, alpha, depth, normals, s);
+}
+
+#endif  // SPR_PART != 1
+
+#if SPR_PART != 0
+// the plain (one ray per thread, global gathers) kernels: latency-bound gathers, which
+// the default scheduler serves better (the max-ILP build made the nearest one 23% slower)
+template <typename T, bool P2>
+hipError_t TypedLaunch<T, P2>::plain_max_project(const VolSlot& vs, const RayParams& p,
+                                                 bool nearest, float* out, float* alpha,
+                                                 hipStream_t s) {
+    const dim3 grid(tile_grid(p.tile_map, p.ntx, p.nty)), block(256);
+    return visit_layout<T, P2>(vs, [&](auto v) {
+        using V = decltype(v);
+        if (nearest)
+            hipLaunchKernelGGL((k_max_project<V, 1>), grid, block, 0, s, v, p, out, alpha);
+        else
+            hipLaunchKernelGGL((k_max_project<V, 0>), grid, block, 0, s, v, p, out, alpha);
+        return hipGetLastError();
+    });
+}
+
+template <typename T, bool P2>
+hipError_t TypedLaunch<T, P2>::plain_iso_march(const VolSlot& vs, const RayParams& p,
+                                               bool nearest, float* out, float* alpha,
+                                               float* depth, float* normals, hipStream_t s) {
+    const dim3 grid(tile_grid(p.tile_map, p.ntx, p.nty)), block(256);
     return visit_layout<T, P2>(vs, [&](auto v) {
         using V = decltype(v);
         if (nearest)
@@ -404,9 +430,6 @@ hipError_t TypedLaunch<T, P2>::iso_march(const LaunchTuning& t, const VolSlot& v
     });
 }
 
-#endif  // SPR_PART != 1
-
-#if SPR_PART != 0
 template <typename T, bool P2>
 hipError_t TypedLaunch<T, P2>::batch(const LaunchTuning& t, int mode, int ch, const BatchJob* d_jobs,
                                      const RayParams& p, const BatchQueues& q, uint32_t* counters,
@@ -415,16 +438,16 @@ hipError_t TypedLaunch<T, P2>::batch(const LaunchTuning& t, int mode, int ch, co
     if (mode == 1) return launch_batch_kernel<T, P2, 16, 1, 16, false>(cus, d_jobs, p, q, counters, s);
     if (mode == 2) return launch_batch_kernel<T, P2, 16, 2, 16, false>(cus, d_jobs, p, q, counters, s);
     if (mode == 3) {
-        if (ch == 16) return launch_batch_kernel<T, P2, 19456, 3, 16, false>(cus, d_jobs, p, q, counters, s);
-        if (ch == 8) return launch_batch_kernel<T, P2, 19456, 3, 8, false>(cus, d_jobs, p, q, counters, s);
-        return launch_batch_kernel<T, P2, 19456, 3, 4, false>(cus, d_jobs, p, q, counters, s);
+        if (ch == 16) return launch_batch_kernel<T, P2, kQuadSlab19, 3, 16, false>(cus, d_jobs, p, q, counters, s);
+        if (ch == 8) return launch_batch_kernel<T, P2, kQuadSlab19, 3, 8, false>(cus, d_jobs, p, q, counters, s);
+        return launch_batch_kernel<T, P2, kQuadSlab19, 3, 4, false>(cus, d_jobs, p, q, counters, s);
     }
     const bool sh = t.staging == 1;
     if (t.slab_kb == 8) return launch_batch_slab<T, P2, 8192>(sh, ch, cus, d_jobs, p, q, counters, s);
     if (t.slab_kb == 16 || (sh && t.slab_kb == 19))
         return launch_batch_slab<T, P2, 16384>(sh, ch, cus, d_jobs, p, q, counters, s);
     if (t.slab_kb == 19)
-        return launch_batch_staged<T, P2, 19456, false>(ch, cus, d_jobs, p, q, counters, s);
+        return launch_batch_staged<T, P2, kQuadSlab19, false>(ch, cus, d_jobs, p, q, counters, s);
     return launch_batch_slab<T, P2, 12288>(sh, ch, cus, d_jobs, p, q, counters, s);
 }
 

@@ -120,6 +120,13 @@ struct TypedLaunch {
     static hipError_t iso_march(const LaunchTuning& t, const VolSlot& vs, RayParams p, bool nearest,
                                 float* out, float* alpha, float* depth, float* normals,
                                 hipStream_t s);
+    // the plain kernels (one ray per thread, every sample gathered from global memory) of
+    // the slot's layout: linear-layout slots, staging switched off (slab_kb 0)
+    static hipError_t plain_max_project(const VolSlot& vs, const RayParams& p, bool nearest,
+                                        float* out, float* alpha, hipStream_t s);
+    static hipError_t plain_iso_march(const VolSlot& vs, const RayParams& p, bool nearest,
+                                      float* out, float* alpha, float* depth, float* normals,
+                                      hipStream_t s);
     // the persistent batch kernel (mode: see dispatch in spimrender.hip)
     static hipError_t batch(const LaunchTuning& t, int mode, int ch, const BatchJob* d_jobs,
                             const RayParams& p, const BatchQueues& q, uint32_t* counters,
