@@ -96,8 +96,7 @@ extern "C" {
    SPR_TUNE_WG_WAVES  waves per workgroup of the per-wave staged max projection (slab 16
                       or 19, and volumes above 4 GiB) and opacity mode: 1 (an 8x8 quarter
                       of a tile: a CU starts new work as soon as any of its waves ends),
-                      4 (a 16x16 tile), 0 = by frame size (default: 1 for frames of at
-                      most 24 tiles per CU, e.g. 1024^2 on 256 CUs, else 4)
+                      4 (a 16x16 tile), 0 = default (1: faster at 1024^2 and 2048^2)
    SPR_TUNE_SKIP      1 = empty-space skipping and early ray termination by the volume's
                       brick bounds (default), 0 = off.  Every padded one-channel volume
                       keeps the minimum and maximum of each 8x8x8 brick (built on upload,

@@ -7,7 +7,7 @@ The kernels' launch knobs (spr_set_tuning) change speed, never results:
   staging    0 = each wave stages its own box, 1 = a workgroup's four waves stage one box
   tile_map   single frames: how 16x16 tiles are dealt to the XCDs (0 interleaved rows,
              3 = the same in descending estimated cost)
-  wg_waves   single frames: waves per workgroup (0 = by frame size, 1, 4)
+  wg_waves   single frames: waves per workgroup (0 = default: 1; 4)
   skip       empty-space skipping / early termination by brick bounds (0, 1)
 The best values depend on voxel type, volume and frame size, step count, render mode and
 the poses (how far the rays' footprints spread across a slab), so the tuner times every
@@ -31,7 +31,7 @@ BATCH_CANDIDATES = tuple(
 # (skipping runs with the default 19 KiB slabs only)
 FRAME_CANDIDATES = tuple(
     dict(tile_map=tm, wg_waves=ww, slab_kb=s, chunk=c, skip=sk)
-    for tm, ww, (s, sk), c in itertools.product((0, 3), (0, 1, 4), ((16, 0), (19, 0), (19, 1)),
+    for tm, ww, (s, sk), c in itertools.product((0, 3), (0, 4), ((16, 0), (19, 0), (19, 1)),
                                                  (0, 16)))
 CANDIDATES = BATCH_CANDIDATES  # the batch tuner's default
 

@@ -126,17 +126,15 @@ void launch_wave_nearest(bool opac, bool skip, int ch, dim3 grid, hipStream_t s,
              : launch_wave_nearest_sk<T, P2, false, false>(ch, grid, s, args...);
 }
 
-// One wave per workgroup for the per-wave staged kernels?  wg_waves 0 (default) decides by
-// frame size: one wave while the frame has at most kOneWaveTilesPerCu 16x16 tiles per CU.
-// A frame of few tiles per CU ends on its slowest waves, and one-wave workgroups refill a
-// CU as soon as any wave ends (1024^2 frames, 16 tiles per CU: 7% faster); a frame of many
-// tiles has no such tail, and four waves of a tile side by side share the CU's L1
-// (2048^2 frames, 64 tiles per CU: four waves 10% faster).
-constexpr uint32_t kOneWaveTilesPerCu = 24;
-
-bool one_wave_for(const LaunchTuning& t, const RayParams& p) {
+// One wave per workgroup for the per-wave staged kernels?  wg_waves 0 (default): yes.  A
+// CU takes a new wave's work as soon as any of its waves ends, instead of when a
+// workgroup's slowest of four waves does: 1024^2 frames 7% faster; 2048^2 frames of the
+// 1024x1024x512 volume 10.6% (max projection), 7% (opacity mode) and 8.5% (iso march)
+// faster, of the 2048x2048x512 one 9.7% and 3.5%.  (Before the max-ILP build and the
+// 20,464-byte slabs four waves were 10% faster on 2048^2 frames.)
+bool one_wave_for(const LaunchTuning& t, const RayParams&) {
     if (t.wg_waves != 0) return t.wg_waves == 1;
-    return p.ntiles <= kOneWaveTilesPerCu * (uint32_t)std::max(t.num_cus, 1);
+    return true;
 }
 
 // a per-wave slab's bytes for the slab_kb tuning value

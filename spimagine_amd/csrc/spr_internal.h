@@ -44,7 +44,7 @@ struct LaunchTuning {
     int staging = 0;   // 0 per-wave boxes, 1 one box per workgroup
     int chunk = 0;     // samples per staged box, 0 = from the ray step (32 when it allows 16)
     int row_pad = 0;   // pad staged slab rows to an odd number of 16-byte units
-    int wg_waves = 0;  // per-wave staged kernels: waves per workgroup (1, 4; 0 = by frame size)
+    int wg_waves = 0;  // per-wave staged kernels: waves per workgroup (1, 4; 0 = default, 1)
     int skip = 1;      // skip boxes / stop rays by the volume's brick bounds (SPR_TUNE_SKIP)
     int num_cus = 0;
 };
