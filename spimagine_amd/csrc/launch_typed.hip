@@ -167,13 +167,18 @@ void launch_staged_wide(const VolSlot& vs, int ch, bool one_wave, dim3 grid, hip
         launch_staged_wide_ch<T, P2, OPAC, SKIP, 4>(v, one_wave, grid, s, p, out, alpha);
 }
 
-// the iso-surface march of a volume above 4 GiB staged (16 KiB slabs: 20,448 B measured
-// 0.7% slower on 2048^2 frames)
+// the iso-surface march of a volume above 4 GiB staged: one wave per workgroup (16-sample
+// boxes) in 20,464-byte slabs, or four waves in 16 KiB slabs (20,448 B measured 0.7%
+// slower on 2048^2 frames)
 template <typename T, bool P2, bool SKIP>
-void launch_iso_wide(const VolSlot& vs, int ch, dim3 grid, hipStream_t s, const RayParams& p,
-                     float* out, float* alpha, float* depth, float* normals) {
+void launch_iso_wide(const VolSlot& vs, int ch, bool one_wave, dim3 grid, hipStream_t s,
+                     const RayParams& p, float* out, float* alpha, float* depth, float* normals) {
     const PaddedVolume<T, P2, true> v = padded_volume<T, P2, true>(vs);
-    if (ch == 16)
+    if (one_wave && ch == 16)
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 16, true, SKIP, 0, true>),
+                           dim3(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty)), dim3(64), 0, s,
+                           v, p, out, alpha, depth, normals);
+    else if (ch == 16)
         hipLaunchKernelGGL((k_iso_staged<T, P2, 16384, 16, true, SKIP>), grid, dim3(256), 0, s, v, p, out, alpha, depth, normals);
     else if (ch == 8)
         hipLaunchKernelGGL((k_iso_staged<T, P2, 16384, 8, true, SKIP>), grid, dim3(256), 0, s, v, p, out, alpha, depth, normals);
@@ -370,13 +375,13 @@ hipError_t TypedLaunch<T, P2>::iso_march(const LaunchTuning& t, const VolSlot& v
     const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
     const int ch = chunk_for(t, p.max_steps / kLoopUnroll, nmax);
     const bool skip = t.skip != 0 && vs.d_bricks != nullptr;
-    if (staged_iso && vs.pad_wide) {
-        skip ? launch_iso_wide<T, P2, true>(vs, ch, grid, s, p, out, alpha, depth, normals)
-             : launch_iso_wide<T, P2, false>(vs, ch, grid, s, p, out, alpha, depth, normals);
-        return hipGetLastError();
-    }
     // one wave per workgroup as for the max projection (interleaved tile maps)
     const bool one_wave = one_wave_for(t, p) && p.tile_map == kTileMapInterleaved;
+    if (staged_iso && vs.pad_wide) {
+        skip ? launch_iso_wide<T, P2, true>(vs, ch, one_wave, grid, s, p, out, alpha, depth, normals)
+             : launch_iso_wide<T, P2, false>(vs, ch, one_wave, grid, s, p, out, alpha, depth, normals);
+        return hipGetLastError();
+    }
     if (staged_iso && vs.layout == SPR_LAYOUT_PADDED) {
         const PaddedVolume<T, P2> v = padded_volume<T, P2>(vs);
         skip ? launch_iso_staged<T, P2, true>(ch, one_wave, grid, s, p, v, p, out, alpha, depth, normals)

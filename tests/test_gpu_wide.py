@@ -93,6 +93,29 @@ def test_wide_padded_volume_matches_linear(torch):
         for a, b in zip(r, res[2]):
             assert torch.equal(a, b)
     assert torch.isfinite(res[0][2]).sum().item() > 200
+    # enough steps for 16-sample boxes (a step of at most ~0.8 voxel): the one-wave staged
+    # max projection and iso march on the wide copy, against four waves and the linear layout
+    for method in (_lib.METHOD_MAX_PROJECT, _lib.METHOD_ISO_SURFACE):
+        p = _lib.make_params(iP, iM, (-1, 1, -1, 1, -1, 1), 0.,
+                             3300. if method == _lib.METHOD_MAX_PROJECT else 3000., 1., 0., .1, 21,
+                             30, 2688, _lib.FILTER_LINEAR)
+        res = []
+        for ctx, slot, waves in runs:
+            ctx.set_tuning("wg_waves", waves)
+            bufs = [torch.empty((H, W), device="cuda") for _ in range(4)]
+            nrm = torch.empty((H, W, 3), device="cuda")
+            ctx.render_slot(slot, p, method, W, H, bufs[0].data_ptr(), bufs[1].data_ptr(),
+                            bufs[2].data_ptr(), nrm.data_ptr(), bufs[3].data_ptr(),
+                            stream=s.cuda_stream)
+            res.append(bufs[:2] if method == _lib.METHOD_MAX_PROJECT else bufs + [nrm])
+        s.synchronize()
+        for r in res[:2]:
+            for a, b in zip(r, res[2]):
+                assert torch.equal(a, b), method
+        if method == _lib.METHOD_MAX_PROJECT:
+            assert (res[0][0] > 0).sum().item() > 5000
+        else:
+            assert torch.isfinite(res[0][2]).sum().item() > 200
     # a wide slot is not batchable (the batch kernels address 32-bit offsets)
     o = torch.empty((H, W), device="cuda")
     with pytest.raises(_lib.SprError):
