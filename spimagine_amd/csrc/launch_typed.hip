@@ -167,17 +167,20 @@ void launch_staged_wide(const VolSlot& vs, int ch, bool one_wave, dim3 grid, hip
         launch_staged_wide_ch<T, P2, OPAC, SKIP, 4>(v, one_wave, grid, s, p, out, alpha);
 }
 
-// the iso-surface march of a volume above 4 GiB staged: one wave per workgroup (16-sample
-// boxes) in 20,464-byte slabs, or four waves in 16 KiB slabs (20,448 B measured 0.7%
-// slower on 2048^2 frames)
+// the iso-surface march of a volume above 4 GiB staged: one wave per workgroup in
+// 20,464-byte slabs, or four waves in 16 KiB slabs (20,448 B measured 0.7% slower on
+// 2048^2 frames)
 template <typename T, bool P2, bool SKIP>
 void launch_iso_wide(const VolSlot& vs, int ch, bool one_wave, dim3 grid, hipStream_t s,
                      const RayParams& p, float* out, float* alpha, float* depth, float* normals) {
     const PaddedVolume<T, P2, true> v = padded_volume<T, P2, true>(vs);
+    const dim3 g1(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty));
     if (one_wave && ch == 16)
-        hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 16, true, SKIP, 0, true>),
-                           dim3(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty)), dim3(64), 0, s,
-                           v, p, out, alpha, depth, normals);
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 16, true, SKIP, 0, true>), g1, dim3(64), 0, s, v, p, out, alpha, depth, normals);
+    else if (one_wave && ch == 8)
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 8, true, SKIP, 0, true>), g1, dim3(64), 0, s, v, p, out, alpha, depth, normals);
+    else if (one_wave)
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 4, true, SKIP, 0, true>), g1, dim3(64), 0, s, v, p, out, alpha, depth, normals);
     else if (ch == 16)
         hipLaunchKernelGGL((k_iso_staged<T, P2, 16384, 16, true, SKIP>), grid, dim3(256), 0, s, v, p, out, alpha, depth, normals);
     else if (ch == 8)
@@ -220,15 +223,18 @@ void launch_staged_opac(int ch, dim3 grid, hipStream_t s, Args... args) {
         hipLaunchKernelGGL((k_max_project_staged<T, P2, kQuadSlab19, 4, false, false, true, SKIP>), grid, dim3(256), 0, s, args...);
 }
 
-// the iso-surface march staged: per-wave 19 KiB slabs; one_wave (16-sample boxes): one
-// wave per workgroup, grid 4 x the interleaved tile grid
+// the iso-surface march staged: per-wave slabs; one_wave: one wave per workgroup, grid 4 x
+// the interleaved tile grid
 template <typename T, bool P2, bool SKIP, int FILT = 0, typename... Args>
 void launch_iso_staged(int ch, bool one_wave, dim3 grid, hipStream_t s, const RayParams& p,
                        Args... args) {
+    const dim3 g1(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty));
     if (one_wave && ch == 16)
-        hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 16, false, SKIP, FILT, true>),
-                           dim3(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty)), dim3(64), 0, s,
-                           args...);
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 16, false, SKIP, FILT, true>), g1, dim3(64), 0, s, args...);
+    else if (one_wave && ch == 8)
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 8, false, SKIP, FILT, true>), g1, dim3(64), 0, s, args...);
+    else if (one_wave)
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 4, false, SKIP, FILT, true>), g1, dim3(64), 0, s, args...);
     else if (ch == 16)
         hipLaunchKernelGGL((k_iso_staged<T, P2, kQuadSlab19, 16, false, SKIP, FILT>), grid, dim3(256), 0, s, args...);
     else if (ch == 8)

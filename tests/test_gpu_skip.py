@@ -148,20 +148,22 @@ def test_skip_iso(VR, dtype, pose):
 @pytest.mark.parametrize("dtype", [np.uint16, np.float32])
 @pytest.mark.parametrize("interp", ["linear", "nearest"])
 def test_iso_one_and_four_waves(VR, dtype, interp):
-    """The iso march with one wave per workgroup (the default for frames of few tiles per
-    CU) and with four: all five outputs bit for bit, skipping on and off, three poses."""
+    """The iso march with one wave per workgroup (the default) and with four: all five
+    outputs bit for bit, skipping on and off, three poses; 16-, 8- and 4-sample boxes (180,
+    48 and 16 steps through the volume)."""
     vol = volume(dtype, (50, 66, 58), seed=41)
     lo, hi = float(vol.min()), float(vol.max())
     iso_max = 2 * (lo + .3 * (hi - lo))
     W, H = 136, 104  # ragged: 9 x 7 tiles, partial quarters
     for k, pose in enumerate(POSES):
+        steps = (180, 48, 16)[k]
         outs = {}
         for waves in (1, 4):
             for skip in (0, 1):
                 rend = VR((W, H), interpolation=interp)
                 rend._ctx.set_tuning("wg_waves", waves)
                 rend._ctx.set_tuning("skip", skip)
-                rend.set_max_steps(180)
+                rend.set_max_steps(steps)
                 rend.set_data(vol)
                 rend.set_projection(mat4_perspective(45, W / float(H), .1, 10))
                 rend.set_modelView(pose)
@@ -169,14 +171,14 @@ def test_iso_one_and_four_waves(VR, dtype, interp):
                 outs[waves, skip] = [rend.output.copy(), rend.output_alpha.copy(),
                                      rend.output_depth.copy(), rend.output_normals.copy(),
                                      rend.output_occlusion.copy()]
-        assert np.isfinite(outs[4, 0][2]).sum() > 300
+        assert np.isfinite(outs[4, 0][2]).sum() > 300, (k, steps)
         for key in ((1, 0), (1, 1), (4, 1)):
             for a, b in zip(outs[key], outs[4, 0]):
                 np.testing.assert_array_equal(a, b, err_msg="pose %d waves/skip %s" % (k, key))
 
 
 def test_skip_four_waves_multipass_and_nan(VR):
-    """Four waves per workgroup (the 2048^2-frame launch shape), multipass parts, and a
+    """Four waves per workgroup (SPR_TUNE_WG_WAVES 4), multipass parts, and a
     float volume holding a NaN voxel (its brick never skips): skipping changes nothing."""
     vol = volume(np.uint16, (40, 56, 50), seed=31)
     lo, hi = float(vol.min()), float(vol.max())
