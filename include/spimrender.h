@@ -86,8 +86,8 @@ extern "C" {
                       chunk from global memory
    SPR_TUNE_STAGING   0 = each wave stages its own box (default), 1 = the four waves of a
                       workgroup stage one union box into a slab of 4 x SLAB_KB
-   SPR_TUNE_CHUNK     samples per staged box: 0 = from the ray step (default: 4 or 8
-                      samples for steps of several voxels, else boxes of two 16-sample
+   SPR_TUNE_CHUNK     samples per staged box: 0 = from the ray step (default: 8 samples
+                      when 15 steps span more than 40 voxels, else boxes of two 16-sample
                       groups), 4, 8, 16, 32 or 64 (boxes of 2 or 4 groups of 16 samples).
                       A box of several groups that does not fit the slab is staged one
                       group at a time.
