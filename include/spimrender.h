@@ -100,8 +100,8 @@ extern "C" {
    SPR_TUNE_SKIP      1 = empty-space skipping and early ray termination by the volume's
                       brick bounds (default), 0 = off.  Every padded one-channel volume
                       keeps the minimum and maximum of each 8x8x8 brick (built on upload,
-                      8 bytes a brick).  With per-wave 19 KiB slabs (the default; 16 KiB
-                      for volumes above 4 GiB) the max projection skips a 16-sample box
+                      8 bytes a brick).  With the default per-wave slabs (slab_kb 19) the
+                      max projection skips a 16-sample box
                       whose bricks' maximum cannot exceed any of the wave's running maxima,
                       the opacity mode stops a wave once no later sample can raise a ray's
                       colour (cumsum x the volume's normalised maximum <= colour, with a

@@ -306,7 +306,7 @@ hipError_t TypedLaunch<T, P2>::max_project(const LaunchTuning& t, const VolSlot&
     const dim3 grid(tile_grid(p.tile_map, p.ntx, p.nty)), block(256);
     const bool opac = p.alpha_pow != 0.0f;
     const bool staged = !nearest && !opac && t.slab_kb > 0;
-    // the opacity mode: per-wave staging (19 KiB slabs; 16 KiB for wide volumes)
+    // the opacity mode: per-wave staging (the slab_kb 19 slabs, volumes above 4 GiB too)
     const bool staged_opac = !nearest && opac && t.slab_kb > 0 && t.staging == 0;
     const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
     // skipping by brick bounds: per-wave slabs of the default size, a volume with bounds
