@@ -453,12 +453,13 @@ def bench_iso(args):
     stream = torch.cuda.Stream()
     sptr = stream.cuda_stream
 
-    def set_pose(k, mode):
+    def set_pose(k, mode, steps=S):
         iP, iM = mats[k]
         if mode == "iso":
-            ctx.set_params(iP, iM, BOX, lo, iso_max, 1., 0., .1, 21, 30, S, _lib.FILTER_LINEAR)
+            ctx.set_params(iP, iM, BOX, lo, iso_max, 1., 0., .1, 21, 30, steps, _lib.FILTER_LINEAR)
         else:
-            ctx.set_params(iP, iM, BOX, lo, hi, 1., .6, .1, 21, 30, S, _lib.FILTER_LINEAR)
+            ctx.set_params(iP, iM, BOX, lo, hi, 1., .6 if mode == "alpha" else 0., .1, 21, 30,
+                           steps, _lib.FILTER_LINEAR)
 
     def frame(mode):
         if mode == "iso":
@@ -500,6 +501,23 @@ def bench_iso(args):
             table.setdefault(mode, {})["on" if skip else "off"] = {
                 "poses_ms": [round(t, 4) for t in ms], "mean_ms": round(float(np.mean(ms)), 4)}
     ctx.set_tuning("skip", keep)
+    # the same poses at the drop-in renderer's default step count (max_steps 200, the
+    # reference's config default): max projection, opacity mode and iso frame
+    default_steps = {}
+    for mode in ("mip", "alpha", "iso"):
+        ms = []
+        for k in range(8):
+            set_pose(k, mode, 200)
+            frame(mode)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(max(args.skip_frames, 1)):
+                frame(mode)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1) / max(args.skip_frames, 1))
+        default_steps[mode] = {"mean_ms": round(float(np.mean(ms)), 4),
+                               "poses_ms": [round(t, 4) for t in ms]}
     log("bench: timing %d iso frames (8 poses, skipping %s)" % (args.steps, "on" if keep else "off"))
     for k in range(8):
         set_pose(k, "iso")
@@ -531,13 +549,15 @@ def bench_iso(args):
         "config": {"workload": "iso-surface (Phong + normal blur + occlusion + shading) of a "
                                "512^3 uint16 volume (2000 seeded blobs, Poisson + read noise) "
                                "-> 1024x1024, iso at 0.3 of the data range, max_steps 896, "
-                               "8 seeded poses; opacity mode at alpha_pow 0.6 timed beside it",
+                               "8 seeded poses; opacity mode at alpha_pow 0.6 timed beside it, "
+                               "and all three modes at max_steps 200 (max_steps_200_ms)",
                    "voxel": "u16", "frame": [W, H], "max_steps": S,
                    "tuning": {k: ctx.get_tuning(k) for k in _lib.TUNING_KEYS}},
         "frames_per_s": round(1e3 / ms_per_step, 2),
         "iso_reference_samples_per_frame": [int(v) for v in iso_samples],
         "gsamples_per_s": round(samp / (kernel_ms * 1e6), 2),
         "skipping_ms": table,
+        "max_steps_200_ms": default_steps,
         "roofline": dict(roofline(b_alg, kernel_ms, None),
                          kernel="iso pipeline per frame (march k_iso_staged + 6 post-pass "
                                 "kernels; the march dominates, see profiles/)",
