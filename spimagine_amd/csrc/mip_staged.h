@@ -223,6 +223,11 @@ __device__ __forceinline__ bool staged_ray_begin(const RayParams& p, uint32_t x,
     return false;
 }
 
+// A staged box's first voxel along x: aligned to 4 bytes (its 16-byte row loads need only
+// dword alignment), so a row carries at most 3 bytes before the box.
+template <typename T>
+constexpr uint32_t kBoxAlign = sizeof(T) >= 4 ? 1u : 4u / (uint32_t)sizeof(T);
+
 // Padded-voxel box of samples first..last of a ray whose sample 0 is at (px, py, pz),
 // packed for a min-reduction: b0 = lo.x | lo.y << 16, b1 = lo.z | (0xFFFF - hi.x) << 16,
 // b2 = (0xFFFF - hi.y) | (0xFFFF - hi.z) << 16 (hi exclusive).  Inactive lanes give all
@@ -711,7 +716,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
         bx.LZ = r1 & 0xFFFFu;
         const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
                        HZ = 0xFFFFu - (r2 >> 16);
-        bx.lox = LX & ~(E - 1u);
+        bx.lox = LX & ~(kBoxAlign<T> - 1u);
         bx.hx = HX;
         const uint32_t SX = (HX + 1u - bx.lox + E - 1u) & ~(E - 1u);  // voxels, multiple of E
         bx.SY = HY - bx.LY + 1u;
@@ -1158,7 +1163,7 @@ __global__ __launch_bounds__(256) void k_tile_cost(PaddedVolume<T, POW2, WIDE> v
             const uint32_t LX = b0 & 0xFFFFu, LY = b0 >> 16, LZ = b1 & 0xFFFFu;
             const uint32_t HX = 0xFFFFu - (b1 >> 16), HY = 0xFFFFu - (b2 & 0xFFFFu),
                            HZ = 0xFFFFu - (b2 >> 16);
-            const uint32_t lox = LX & ~(E - 1u);
+            const uint32_t lox = LX & ~(kBoxAlign<T> - 1u);
             const uint32_t nu = ((HX + 1u - lox + E - 1u) & ~(E - 1u)) / E;
             const uint64_t bytes = (uint64_t)nu * 16u * (HY - LY + 1u) * (HZ - LZ + 1u);
             c += nu <= 64u && bytes <= (uint64_t)slab_bytes ? 0u : 1u;
@@ -1394,7 +1399,7 @@ __global__ __launch_bounds__(WAVE ? 64 : 256) void k_iso_staged(PaddedVolume<T, 
         const uint32_t LX = r0 & 0xFFFFu, LY = r0 >> 16, LZ = r1 & 0xFFFFu;
         const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
                        HZ = 0xFFFFu - (r2 >> 16);
-        const uint32_t lox = LX & ~(E - 1u);
+        const uint32_t lox = LX & ~(kBoxAlign<T> - 1u);
         const uint32_t SX = (HX + 1u - lox + E - 1u) & ~(E - 1u);
         const uint32_t SY = HY - LY + 1u, SZ = HZ - LZ + 1u;
         const uint32_t nu = SX / E;
@@ -1693,7 +1698,7 @@ __global__ __launch_bounds__(256) void k_coarse_mip_pair(PaddedVolume<T, POW2> v
             const uint32_t LX = r0 & 0xFFFFu, LY = r0 >> 16, LZ = r1 & 0xFFFFu;
             const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
                            HZ = 0xFFFFu - (r2 >> 16);
-            const uint32_t lox = LX & ~(E - 1u);
+            const uint32_t lox = LX & ~(kBoxAlign<T> - 1u);
             const uint32_t SX = (HX + 1u - lox + E - 1u) & ~(E - 1u);
             const uint32_t SY = HY - LY + 1u, SZ = HZ - LZ + 1u;
             const uint32_t nu = SX / E;

@@ -370,6 +370,14 @@ class LearnedMIP(object):
         # a second judge stage on the stride-(S/2) grid for the frames the first stage does
         # not accept (split rays only)
         self.judge_refine = True
+        # render_many: the coarse frames of a group on this many side streams, round robin,
+        # so that one frame's launch fills the GPU while another's last waves finish (a
+        # 512^2 coarse frame is two rounds of one-wave workgroups over the CUs, and its tail
+        # left the GPU half idle): 0.2241 against 0.2857 ms a learned frame in groups of 8,
+        # judged 0.328 against 0.391 ms (4 streams: the same as 2).  0: on the caller's
+        # stream one after another
+        self.coarse_streams = 2
+        self._coarse_side = []
 
     def attach(self, slot):
         """The slots whose coarse projections feed the network (created once): for pooled
@@ -392,12 +400,13 @@ class LearnedMIP(object):
             self._bufs[key] = self.torch.empty(shape, device="cuda")
         return self._bufs[key]
 
-    def coarse(self, slot, p, w, h, planes, k=0, stream=None):
+    def coarse(self, slot, p, w, h, planes, k=0, stream=None, alpha_key=None):
         """Renders the two coarse planes of frame k of `planes` (n, 2, h/2, w/2) in one
         launch (spr_render_pair on a channel-pair slot, else spr_render_coarse_pair; the
-        coarse alpha lands in the "alpha" buffer)."""
+        coarse alpha lands in the "alpha" buffer, or in alpha buffer alpha_key)."""
         a, b = self.attach(slot)
-        scratch = self._buffer(("alpha", w, h), (h // 2, w // 2))
+        key = ("alpha", w, h) if alpha_key is None else ("alpha", w, h, alpha_key)
+        scratch = self._buffer(key, (h // 2, w // 2))
         if a == b:
             self.ctx.render_pair(a, coarse_params(p), w // 2, h // 2, planes[k].data_ptr(),
                                  scratch.data_ptr(), stream=stream)
@@ -430,13 +439,33 @@ class LearnedMIP(object):
         base, fb = outs[0].data_ptr(), 4 * h * w
         inplace = all(o.is_contiguous() and o.data_ptr() == base + k * fb
                       for k, o in enumerate(outs))
+        ns = self.coarse_streams
+        while len(self._coarse_side) < ns:
+            self._coarse_side.append(torch.cuda.Stream())
         for g0 in range(0, n, G):
             m = min(G, n - g0)
-            for k in range(m):
-                self.coarse(slot, plist[g0 + k], w, h, planes, k, sh)
-                if alphas is not None:
-                    self.net.upsample2(coarse_alpha.data_ptr(), w // 2, h // 2,
-                                       alphas[g0 + k].data_ptr(), sh)
+            if ns:
+                # each side stream after the caller's earlier work (the previous group's
+                # network pass reads the planes), the network after every side stream; a
+                # coarse alpha scratch plane per side stream
+                for st in self._coarse_side[:ns]:
+                    st.wait_stream(stream)
+                for k in range(m):
+                    st = self._coarse_side[k % ns]
+                    self.coarse(slot, plist[g0 + k], w, h, planes, k, st.cuda_stream,
+                                alpha_key=k % ns)
+                    if alphas is not None:
+                        a = self._buffer(("alpha", w, h, k % ns), (h // 2, w // 2))
+                        self.net.upsample2(a.data_ptr(), w // 2, h // 2,
+                                           alphas[g0 + k].data_ptr(), st.cuda_stream)
+                for st in self._coarse_side[:ns]:
+                    stream.wait_stream(st)
+            else:
+                for k in range(m):
+                    self.coarse(slot, plist[g0 + k], w, h, planes, k, sh)
+                    if alphas is not None:
+                        self.net.upsample2(coarse_alpha.data_ptr(), w // 2, h // 2,
+                                           alphas[g0 + k].data_ptr(), sh)
             if inplace:
                 self.net.forward(planes.data_ptr(), m, h // 2, w // 2, base + g0 * fb, sh)
             else:
