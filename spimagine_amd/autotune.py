@@ -12,7 +12,10 @@ The kernels' launch knobs (spr_set_tuning) change speed, never results:
 The best values depend on voxel type, volume and frame size, step count, render mode and
 the poses (how far the rays' footprints spread across a slab), so the tuner times every
 candidate on a sample of the actual work and keeps the fastest per workload key, in memory
-and optionally in a JSON file.  The reference has no counterpart: its OpenCL launch is a
+and optionally in a JSON file.  The context's tuning before the search (the library's
+defaults unless the caller set others) stays unless a candidate beats it by more than
+MARGIN: candidates within the timing noise of each other would otherwise trade places from
+run to run, and a sample's near-tie can be a clear loss on one of its poses.  The reference has no counterpart: its OpenCL launch is a
 fixed global size of one work item per pixel (spimagine/volumerender/volumerender.py:366-386).
 """
 import itertools
@@ -34,6 +37,9 @@ FRAME_CANDIDATES = tuple(
     for tm, ww, (s, sk), c in itertools.product((0, 3), (0, 4), ((16, 0), (19, 0), (19, 1)),
                                                  (0, 16)))
 CANDIDATES = BATCH_CANDIDATES  # the batch tuner's default
+# a candidate replaces the incumbent tuning only when it is faster by more than this
+# fraction (the spread of repeated timings of one candidate is ~0.5%)
+MARGIN = 0.01
 
 _cache = {}
 
@@ -63,9 +69,10 @@ def _store(key, best, table, cache_file):
             json.dump(_cache, f, indent=1)
 
 
-def _search(ctx, key, candidates, timer, cache_file, force):
+def _search(ctx, key, candidates, timer, cache_file, force, margin=MARGIN):
     """Times timer() under each candidate tuning; sets the fastest on ctx and returns
-    (best, {candidate: ms})."""
+    (best, {candidate: ms}).  The candidate equal to ctx's tuning at the call (if one is)
+    stays the best unless another is faster by more than `margin`."""
     _load_cache(cache_file)
     if key in _cache and not force:
         best = _cache[key]["best"]
@@ -84,7 +91,12 @@ def _search(ctx, key, candidates, timer, cache_file, force):
     finally:
         for k, v in before.items():
             ctx.set_tuning(k, v)
-    best = json.loads(min(table, key=table.get))
+    best = min(table, key=table.get)
+    incumbent = next((c for c in table
+                      if all(before.get(k) == v for k, v in json.loads(c).items())), None)
+    if incumbent is not None and table[best] >= table[incumbent] * (1. - margin):
+        best = incumbent
+    best = json.loads(best)
     for k, v in best.items():
         ctx.set_tuning(k, v)
     _store(key, best, table, cache_file)

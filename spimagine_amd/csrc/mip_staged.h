@@ -227,6 +227,12 @@ __device__ __forceinline__ bool staged_ray_begin(const RayParams& p, uint32_t x,
 // dword alignment), so a row carries at most 3 bytes before the box.
 template <typename T>
 constexpr uint32_t kBoxAlign = sizeof(T) >= 4 ? 1u : 4u / (uint32_t)sizeof(T);
+// The max projection's nearest filter keeps 16-byte aligned boxes: a nearest sample is one
+// LDS read, the copy is most of a box's time, and with 4-byte aligned rows the frames took
+// ~6% longer (1024^2 frames of 1024x1024x512 uint16, random poses: 1.50 against 1.41 ms),
+// while the linear filter's took ~5% less.
+template <typename T, int FILT>
+constexpr uint32_t kBoxAlignMip = FILT == 1 ? 16u / (uint32_t)sizeof(T) : kBoxAlign<T>;
 
 // Padded-voxel box of samples first..last of a ray whose sample 0 is at (px, py, pz),
 // packed for a min-reduction: b0 = lo.x | lo.y << 16, b1 = lo.z | (0xFFFF - hi.x) << 16,
@@ -716,7 +722,7 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
         bx.LZ = r1 & 0xFFFFu;
         const uint32_t HX = 0xFFFFu - (r1 >> 16), HY = 0xFFFFu - (r2 & 0xFFFFu),
                        HZ = 0xFFFFu - (r2 >> 16);
-        bx.lox = LX & ~(kBoxAlign<T> - 1u);
+        bx.lox = LX & ~(kBoxAlignMip<T, FILT> - 1u);
         bx.hx = HX;
         const uint32_t SX = (HX + 1u - bx.lox + E - 1u) & ~(E - 1u);  // voxels, multiple of E
         bx.SY = HY - bx.LY + 1u;

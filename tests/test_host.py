@@ -150,3 +150,49 @@ def test_sparse_frame_table_matches_single_frames():
             i = j * len(plist) + k
             f = _lib.make_frame(3, q, w // S, h // S, 1000 + 4 * i * n, 5000 + 4 * i * n)
             assert bytes(f) == F[i].tobytes(), (j, k)
+
+
+class _FakeTuning:
+    """Stands in for a Context in the autotuner's search: holds tuning values only."""
+
+    def __init__(self):
+        from spimagine_amd import _lib
+        self.t = {k: 0 for k in _lib.TUNING_KEYS}
+        self.t.update(slab_kb=19, skip=1)
+
+    def get_tuning(self, k):
+        return self.t[k]
+
+    def set_tuning(self, k, v):
+        self.t[k] = int(v)
+
+
+def test_autotune_keeps_incumbent_within_margin():
+    from spimagine_amd import autotune
+    cands = autotune.FRAME_CANDIDATES
+    default = dict(tile_map=0, wg_waves=0, slab_kb=19, chunk=0, skip=1)
+    assert default in cands
+
+    def run(times):
+        ctx = _FakeTuning()
+        before = dict(ctx.t)
+
+        def timer():
+            key = tuple(sorted((k, ctx.t[k]) for k in default))
+            return times.get(key, 3.0)
+        best, table = autotune._search(ctx, "test:%d" % len(autotune._cache), cands, timer,
+                                       None, True)
+        assert len(table) == len(cands)
+        # the chosen tuning is left set, every other key as it was
+        assert all(ctx.t[k] == v for k, v in best.items())
+        assert all(ctx.t[k] == before[k] for k in ctx.t if k not in best)
+        return best
+
+    k_def = tuple(sorted(default.items()))
+    k_tm3 = tuple(sorted(dict(default, tile_map=3).items()))
+    # a near-tie (0.5% faster): the incumbent stays
+    assert run({k_def: 2.0, k_tm3: 1.99}) == default
+    # faster by 5%: the candidate replaces it
+    assert run({k_def: 2.0, k_tm3: 1.9}) == dict(default, tile_map=3)
+    # the incumbent fastest: kept
+    assert run({k_def: 1.5, k_tm3: 1.9}) == default
