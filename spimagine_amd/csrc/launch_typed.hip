@@ -175,7 +175,9 @@ void launch_iso_wide(const VolSlot& vs, int ch, bool one_wave, dim3 grid, hipStr
                      const RayParams& p, float* out, float* alpha, float* depth, float* normals) {
     const PaddedVolume<T, P2, true> v = padded_volume<T, P2, true>(vs);
     const dim3 g1(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty));
-    if (one_wave && ch == 16)
+    if (one_wave && ch == 16 && SKIP && p.skip16 > 1)
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 16, true, SKIP, 0, true, true>), g1, dim3(64), 0, s, v, p, out, alpha, depth, normals);
+    else if (one_wave && ch == 16)
         hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 16, true, SKIP, 0, true>), g1, dim3(64), 0, s, v, p, out, alpha, depth, normals);
     else if (one_wave && ch == 8)
         hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 8, true, SKIP, 0, true>), g1, dim3(64), 0, s, v, p, out, alpha, depth, normals);
@@ -224,12 +226,14 @@ void launch_staged_opac(int ch, dim3 grid, hipStream_t s, Args... args) {
 }
 
 // the iso-surface march staged: per-wave slabs; one_wave: one wave per workgroup, grid 4 x
-// the interleaved tile grid
+// the interleaved tile grid (with coarse skip tests when p.skip16 asks for them)
 template <typename T, bool P2, bool SKIP, int FILT = 0, typename... Args>
 void launch_iso_staged(int ch, bool one_wave, dim3 grid, hipStream_t s, const RayParams& p,
                        Args... args) {
     const dim3 g1(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty));
-    if (one_wave && ch == 16)
+    if (one_wave && ch == 16 && SKIP && p.skip16 > 1)
+        hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 16, false, SKIP, FILT, true, true>), g1, dim3(64), 0, s, args...);
+    else if (one_wave && ch == 16)
         hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 16, false, SKIP, FILT, true>), g1, dim3(64), 0, s, args...);
     else if (one_wave && ch == 8)
         hipLaunchKernelGGL((k_iso_staged<T, P2, kWaveSlab19, 8, false, SKIP, FILT, true>), g1, dim3(64), 0, s, args...);
@@ -381,6 +385,7 @@ hipError_t TypedLaunch<T, P2>::iso_march(const LaunchTuning& t, const VolSlot& v
     const int nmax = std::max(vs.nx, std::max(vs.ny, vs.nz));
     const int ch = chunk_for(t, p.max_steps / kLoopUnroll, nmax);
     const bool skip = t.skip != 0 && vs.d_bricks != nullptr;
+    p.skip16 = iso_skip_chunks(ch, p.max_steps, nmax);
     // one wave per workgroup as for the max projection (interleaved tile maps)
     const bool one_wave = one_wave_for(t, p) && p.tile_map == kTileMapInterleaved;
     if (staged_iso && vs.pad_wide) {

@@ -27,6 +27,8 @@ struct RayParams {
     uint32_t tile_map;          // kTileMap* below
     uint32_t row_pad;           // staged kernels: pad LDS slab rows to an odd multiple of 16 B
     uint32_t box16;             // staged kernels: 16-sample groups per staged box (1, 2, 4)
+    uint32_t skip16;            // staged iso march with skipping: 16-sample chunks per
+                                // coarse skip test (1: none)
     const uint32_t* order;      // kTileMapCostXcd: workgroup -> work item (k_tile_order)
     const uint32_t* gate;       // max projections: render only if *gate != 0 (nullptr: always)
 };

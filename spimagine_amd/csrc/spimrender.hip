@@ -232,6 +232,7 @@ RayParams make_params(const spr_params& prm, int tile_map, int w, int h) {
     p.tile_map = tile_map == (int)kTileMapCostXcd ? kTileMapInterleaved : (uint32_t)tile_map;
     p.row_pad = 0;
     p.box16 = 1;
+    p.skip16 = 1;
     p.order = nullptr;
     p.gate = nullptr;
     return p;
