@@ -316,11 +316,13 @@ hipError_t TypedLaunch<T, P2>::max_project(const LaunchTuning& t, const VolSlot&
     // skipping by brick bounds: per-wave slabs of the default size, a volume with bounds
     const bool skip = t.skip != 0 && vs.d_bricks != nullptr && t.staging == 0 &&
                       (t.slab_kb == 19 || vs.pad_wide);
+    // the plain max projection's box lengths (chunk 0: from the ray step)
+    if (t.chunk == 0 && !opac) p.box16 = staged_groups_mip(p.nouter, nmax);
     // the nearest filter (plain or opacity mode) staged: padded slots below 4 GiB, per-wave
     // 19 KiB slabs, one wave per workgroup
     if (nearest && t.slab_kb > 0 && t.staging == 0 && vs.layout == SPR_LAYOUT_PADDED) {
         const PaddedVolume<T, P2> v = padded_volume<T, P2>(vs);
-        const int ch = chunk_for(t, p.nouter, nmax);
+        const int ch = chunk_for(t, p.nouter, nmax, !opac);
         dim3 g(4 * tile_grid(kTileMapInterleaved, p.ntx, p.nty));
         void* obuf = nullptr;
         hipError_t e = order_tiles(t.tile_map, true, v, kWaveSlab19, p, s, &obuf, &g);
@@ -334,7 +336,7 @@ hipError_t TypedLaunch<T, P2>::max_project(const LaunchTuning& t, const VolSlot&
     }
     if ((staged || staged_opac) && vs.pad_wide) {
         const bool one_wave = one_wave_for(t, p);
-        const int ch = chunk_for(t, p.nouter, nmax);
+        const int ch = chunk_for(t, p.nouter, nmax, !opac);
         if (staged)
             skip ? launch_staged_wide<T, P2, false, true>(vs, ch, one_wave, grid, s, p, out, alpha)
                  : launch_staged_wide<T, P2, false, false>(vs, ch, one_wave, grid, s, p, out, alpha);
@@ -345,7 +347,7 @@ hipError_t TypedLaunch<T, P2>::max_project(const LaunchTuning& t, const VolSlot&
     }
     if ((staged || staged_opac) && vs.layout == SPR_LAYOUT_PADDED) {
         const PaddedVolume<T, P2> v = padded_volume<T, P2>(vs);
-        const int ch = chunk_for(t, p.nouter, nmax);
+        const int ch = chunk_for(t, p.nouter, nmax, !opac);
         // one wave per workgroup: per-wave slabs of 16 or 19 KiB
         const bool one_wave = one_wave_for(t, p) && (staged_opac || (t.staging == 0 && t.slab_kb >= 16));
         // (one wave: tile rows interleaved over the XCDs unless cost-ordered)

@@ -95,9 +95,10 @@ PaddedVolume<T, POW2, WIDE> padded_volume(const VolSlot& s) {
 }
 
 // samples per unrolled group of the staged kernels: the tuning value (32 and 64 are boxes
-// of 2 and 4 groups of 16), else from the ray step
-inline int chunk_for(const LaunchTuning& t, int nouter, int nmax) {
-    return t.chunk ? std::min(t.chunk, 16) : staged_chunk(nouter, nmax);
+// of 2 and 4 groups of 16), else from the ray step (mip: the plain max projection's rule)
+inline int chunk_for(const LaunchTuning& t, int nouter, int nmax, bool mip = false) {
+    return t.chunk ? std::min(t.chunk, 16)
+                   : mip ? staged_chunk_mip(nouter, nmax) : staged_chunk(nouter, nmax);
 }
 
 // the staged kernels' launch options carried in RayParams
