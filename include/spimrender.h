@@ -88,9 +88,11 @@ extern "C" {
                       workgroup stage one union box into a slab of 4 x SLAB_KB
    SPR_TUNE_CHUNK     samples per staged box: 0 = from the ray step (default: 8 samples
                       when 15 steps span more than 40 voxels, else boxes of two 16-sample
-                      groups), 4, 8, 16, 32 or 64 (boxes of 2 or 4 groups of 16 samples).
-                      A box of several groups that does not fit the slab is staged one
-                      group at a time.
+                      groups; the plain max projection: 8 samples beyond 36 voxels, two
+                      groups only while 31 steps span at most 40 voxels, else one), 4, 8,
+                      16, 32 or 64 (boxes of 2 or 4 groups of 16 samples).  A box of
+                      several groups that does not fit the slab is staged one group at a
+                      time.
    SPR_TUNE_ROW_PAD   1 = pad staged rows to an odd number of 16-byte units (spreads the
                       LDS banks rows a few apart fall on), 0 = no padding (default)
    SPR_TUNE_WG_WAVES  waves per workgroup of the per-wave staged max projection (slab 16
@@ -107,8 +109,10 @@ extern "C" {
                       colour (cumsum x the volume's normalised maximum <= colour, with a
                       transparency factor provably in [0, 1]), and the iso march skips a
                       box whose bricks lie wholly on each marching ray's starting side of
-                      the iso value.  Only skipped samples' positions are still advanced,
-                      by the same additions: results are bit-identical either way. */
+                      the iso value (after a skipped box, the next four tested as one
+                      while 63 steps span at most 40 voxels).  Only skipped samples'
+                      positions are still advanced, by the same additions: results are
+                      bit-identical either way. */
 #define SPR_TUNE_TILE_MAP 0
 #define SPR_TUNE_SLAB_KB 1
 #define SPR_TUNE_STAGING 2
