@@ -85,7 +85,17 @@ constexpr uint32_t kWaveSlab19 = 20464;
 // one wave per workgroup: the plain max projection (slab 16 or 19 KiB) or the opacity mode;
 // skip: the variant that skips boxes by their brick bounds (19 KiB slabs)
 template <typename T, bool P2, bool OPAC, int CH, typename... Args>
-void launch_wave_ch(int slab_kb, bool skip, dim3 grid, hipStream_t s, Args... args) {
+void launch_wave_ch(int slab_kb, bool skip, bool fastdiv, dim3 grid, hipStream_t s, Args... args) {
+    // the opacity mode of integer voxels with its normalisation free of divisions
+    if constexpr (OPAC && !std::is_same<T, float>::value) {
+        if (fastdiv) {
+            if (skip)
+                hipLaunchKernelGGL((k_max_project_wave<T, P2, kWaveSlab19, CH, false, true, true, 0, true>), grid, dim3(64), 0, s, args...);
+            else
+                hipLaunchKernelGGL((k_max_project_wave<T, P2, kWaveSlab19, CH, false, true, false, 0, true>), grid, dim3(64), 0, s, args...);
+            return;
+        }
+    }
     if (slab_kb == 16 && !OPAC)
         hipLaunchKernelGGL((k_max_project_wave<T, P2, 16384, CH, false, OPAC>), grid, dim3(64), 0, s, args...);
     else if (skip)
@@ -95,13 +105,14 @@ void launch_wave_ch(int slab_kb, bool skip, dim3 grid, hipStream_t s, Args... ar
 }
 
 template <typename T, bool P2, bool OPAC, typename... Args>
-void launch_wave(int slab_kb, bool skip, int ch, dim3 grid, hipStream_t s, Args... args) {
+void launch_wave(int slab_kb, bool skip, bool fastdiv, int ch, dim3 grid, hipStream_t s,
+                 Args... args) {
     if (ch == 16)
-        launch_wave_ch<T, P2, OPAC, 16>(slab_kb, skip, grid, s, args...);
+        launch_wave_ch<T, P2, OPAC, 16>(slab_kb, skip, fastdiv, grid, s, args...);
     else if (ch == 8)
-        launch_wave_ch<T, P2, OPAC, 8>(slab_kb, skip, grid, s, args...);
+        launch_wave_ch<T, P2, OPAC, 8>(slab_kb, skip, fastdiv, grid, s, args...);
     else
-        launch_wave_ch<T, P2, OPAC, 4>(slab_kb, skip, grid, s, args...);
+        launch_wave_ch<T, P2, OPAC, 4>(slab_kb, skip, fastdiv, grid, s, args...);
 }
 
 // the nearest filter staged: one wave per workgroup, 19 KiB slabs; OPAC: the opacity mode;
@@ -361,9 +372,10 @@ hipError_t TypedLaunch<T, P2>::max_project(const LaunchTuning& t, const VolSlot&
                             p, s, &obuf, &g);
         if (e == hipSuccess) {
             if (one_wave && staged_opac)
-                launch_wave<T, P2, true>(19, skip, ch, g, s, v, p, out, alpha);
+                launch_wave<T, P2, true>(19, skip, fast_norm_ok(p.min_val, p.max_val), ch, g, s,
+                                         v, p, out, alpha);
             else if (one_wave)
-                launch_wave<T, P2, false>(t.slab_kb, skip, ch, g, s, v, p, out, alpha);
+                launch_wave<T, P2, false>(t.slab_kb, skip, false, ch, g, s, v, p, out, alpha);
             else if (staged_opac)
                 skip ? launch_staged_opac<T, P2, true>(ch, g, s, v, p, out, alpha)
                      : launch_staged_opac<T, P2, false>(ch, g, s, v, p, out, alpha);

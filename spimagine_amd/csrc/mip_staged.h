@@ -421,7 +421,36 @@ struct Opacity {
     bool norm;       // max_val != 0
     float mn, den;   // min_val, max_val - min_val
     float a;         // 0.1 alpha_pow^2 (integer voxels) or alpha_pow^2 (float voxels)
+    float rcp;       // 1 / den, correctly rounded
+    static constexpr bool kFast = false;
 };
+// The same state for launches whose normalisation (v - mn) / den runs without a division
+// (norm_quotient): integer voxels, max_val != 0, |den| in [2^-32, 2^32], |mn| in
+// [2^-20, 2^39] (fast_norm_ok).
+struct OpacityFast : Opacity {
+    static constexpr bool kFast = true;
+};
+
+// x / den correctly rounded without a division: q0 = x r with r = RN(1/den), then two fma
+// residual corrections q' = q + (x - den q) r.  After the first q is a faithful quotient,
+// and with r the correctly rounded reciprocal the second gives RN(x / den) (Markstein's
+// theorem, P. Markstein, IBM J. Res. Dev. 34(1), 1990) while nothing overflows or
+// underflows: |den| in [2^-32, 2^32] and x = 0 or |x| in [2^-60, 2^40].  With integer
+// voxels v in [0, 65535] and |mn| in [2^-20, 2^39], x = v - mn is 0 (v == mn) or at
+// least 2^-44 in magnitude (Sterbenz: v - mn is exact when v and mn are within a factor
+// of two, and at least mn / 2 otherwise), and at most 2^40.  Checked against IEEE
+// division on the CPU over 5.4e9 pairs (random exponents, integer and lerp-like x,
+// quotients next to rounding midpoints, divisors with significands near 1 and 2).
+__device__ __forceinline__ float norm_quotient(float x, float den, float rcp) {
+    const float q0 = x * rcp;
+    const float q1 = fmaf(fmaf(-den, q0, x), rcp, q0);
+    return fmaf(fmaf(-den, q1, x), rcp, q1);
+}
+__host__ inline bool fast_norm_ok(float min_val, float max_val) {
+    const float den = max_val - min_val, amn = std::fabs(min_val), aden = std::fabs(den);
+    return max_val != 0.0f && aden >= 0x1p-32f && aden <= 0x1p32f && amn >= 0x1p-20f &&
+           amn <= 0x1p39f;
+}
 
 // Whether a ray of the opacity mode may stop before its last sample (uniform).  With every
 // sample of the volume in [vmin, vmax] (brick bounds widened for float voxels), normalised
@@ -452,11 +481,14 @@ __device__ __forceinline__ bool opacity_spent(const Opacity& o, float col, float
     return o.cumsum * hi <= col;
 }
 
-template <bool kShort>
-__device__ __forceinline__ void opacity_step(Opacity& o, float v, float& col, float& px,
+template <bool kShort, typename OP>
+__device__ __forceinline__ void opacity_step(OP& o, float v, float& col, float& px,
                                              float& py, float& pz, float dx, float dy,
                                              float dz) {
-    v = o.norm ? (v - o.mn) / o.den : v;
+    if constexpr (OP::kFast)
+        v = norm_quotient(v - o.mn, o.den, o.rcp);
+    else
+        v = o.norm ? (v - o.mn) / o.den : v;
     const float c = fmaxf(col, o.cumsum * v);
     const float cs = kShort ? o.cumsum * (1.0f - o.a * v)
                             : o.cumsum * (1.0f - o.a * fminf(fmaxf(v, 0.0f), 1.0f));
@@ -474,12 +506,12 @@ __device__ __forceinline__ void opacity_step(Opacity& o, float v, float& col, fl
 // slab offset of corner (c+1) relative to the box computed exactly in float (every term is
 // an integer below 2^24).  FILT 1: the nearest filter, one voxel per sample (nearest's
 // voxel is one of the linear sample's 8 corners, so it lies in the box staged for them).
-template <typename V, int CH, bool OPAC = false, int FILT = 0>
+template <typename V, int CH, bool OPAC = false, int FILT = 0, typename OP = Opacity>
 __device__ __forceinline__ void staged_sample_slab(const V& vol,
                                                    const uint8_t* slab, uint32_t lox, uint32_t LY,
                                                    uint32_t LZ, uint32_t rowb, uint32_t SY,
                                                    float& px, float& py, float& pz, float dx,
-                                                   float dy, float dz, float& col, Opacity& op) {
+                                                   float dy, float dz, float& col, OP& op) {
     using T = typename V::value_type;
     constexpr uint32_t TS = sizeof(T);
     const uint32_t sy = rowb, sz = rowb * SY;
@@ -531,10 +563,10 @@ __device__ __forceinline__ void staged_sample_slab(const V& vol,
     }
 }
 
-template <typename V, int CH, bool OPAC = false, int FILT = 0>
+template <typename V, int CH, bool OPAC = false, int FILT = 0, typename OP = Opacity>
 __device__ __forceinline__ void staged_sample_global(const V& vol, float& px,
                                                      float& py, float& pz, float dx, float dy,
-                                                     float dz, float& col, Opacity& op) {
+                                                     float dz, float& col, OP& op) {
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
         const float v = FILT == 1 ? vol.nearest(px, py, pz) : vol.linear(px, py, pz);
@@ -613,12 +645,12 @@ __device__ __forceinline__ void staged_sample_global_pair(const V& vol, float& p
 }
 
 // the sampler matching the voxel type (col2: the second channel of a U16Pair volume)
-template <typename V, int N, bool OPAC, int FILT = 0>
+template <typename V, int N, bool OPAC, int FILT = 0, typename OP = Opacity>
 __device__ __forceinline__ void sample_slab_any(const V& vol, const uint8_t* slab, uint32_t lox,
                                                 uint32_t LY, uint32_t LZ, uint32_t rowb,
                                                 uint32_t SY, float& px, float& py, float& pz,
                                                 float dx, float dy, float dz, float& col,
-                                                float& col2, Opacity& op) {
+                                                float& col2, OP& op) {
     if constexpr (std::is_same<typename V::value_type, U16Pair>::value)
         staged_sample_slab_pair<V, N>(vol, slab, lox, LY, LZ, rowb, SY, px, py, pz, dx, dy, dz, col,
                                       col2);
@@ -627,10 +659,10 @@ __device__ __forceinline__ void sample_slab_any(const V& vol, const uint8_t* sla
                                              dz, col, op);
 }
 
-template <typename V, int N, bool OPAC, int FILT = 0>
+template <typename V, int N, bool OPAC, int FILT = 0, typename OP = Opacity>
 __device__ __forceinline__ void sample_global_any(const V& vol, float& px, float& py, float& pz,
                                                   float dx, float dy, float dz, float& col,
-                                                  float& col2, Opacity& op) {
+                                                  float& col2, OP& op) {
     if constexpr (std::is_same<typename V::value_type, U16Pair>::value)
         staged_sample_global_pair<V, N>(vol, px, py, pz, dx, dy, dz, col, col2);
     else
@@ -690,7 +722,7 @@ __device__ __forceinline__ void shadow_first_active(uint64_t act, bool active, f
 // additions) and stops once every ray holds the volume's maximum; the opacity mode stops
 // once no later sample can raise any ray's colour (opacity_spent).
 template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE, bool OPAC,
-          bool HALVES = true, bool SKIP = false, int FILT = 0>
+          bool HALVES = true, bool SKIP = false, int FILT = 0, bool FASTDIV = false>
 __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WIDE>& vol,
                                                  const RayParams& p, bool active, bool live,
                                                  uint8_t* slab, uint32_t* s_box, float& px,
@@ -840,12 +872,14 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
     static_assert(!(OPAC && kPair), "pair renders are plain max projections");
     static_assert(FILT == 0 || (!kPair && !WIDE), "nearest: one-channel volumes below 4 GiB");
     const int G = CH == 16 ? (int)p.box16 : 1;  // groups per box, uniform
-    Opacity op;
+    static_assert(!FASTDIV || (OPAC && kShort), "division-free normalisation: integer voxels");
+    std::conditional_t<FASTDIV, OpacityFast, Opacity> op;
     op.cumsum = 1.0f;
     op.brk = false;
     op.norm = p.max_val != 0.0f;
     op.mn = p.min_val;
     op.den = p.max_val - p.min_val;
+    op.rcp = 1.0f / op.den;
     op.a = kShort ? 0.1f * p.alpha_pow * p.alpha_pow : p.alpha_pow * p.alpha_pow;
     int done = 0;  // samples of the current ray position sequence taken so far (uniform)
     // before n samples: a new reference 16-sample group starts the break state afresh
@@ -1070,7 +1104,8 @@ __device__ __forceinline__ void mip_staged_march(const PaddedVolume<T, POW2, WID
 // OPAC: the opacity mode (alpha_pow > 0, Opacity); boxes are not loaded ahead (a spent
 // lane's position does not advance with the group).
 template <typename T, bool POW2, int SLAB_BYTES, int CH, bool SHARED, bool WIDE = false,
-          bool OPAC = false, bool HALVES = true, bool SKIP = false, int FILT = 0>
+          bool OPAC = false, bool HALVES = true, bool SKIP = false, int FILT = 0,
+          bool FASTDIV = false>
 __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE>& vol, const RayParams& p,
                                                 uint32_t x, uint32_t y, bool inside, uint8_t* slab,
                                                 uint32_t* s_box, float* __restrict__ out,
@@ -1093,7 +1128,7 @@ __device__ __forceinline__ void mip_staged_rays(const PaddedVolume<T, POW2, WIDE
     const bool live = act != 0;  // uniform over the wave
     if (live) shadow_first_active(act, active, px, py, pz, dx, dy, dz);
     float col = 0.0f, col2 = 0.0f;
-    mip_staged_march<T, POW2, SLAB_BYTES, CH, SHARED, WIDE, OPAC, HALVES, SKIP, FILT>(
+    mip_staged_march<T, POW2, SLAB_BYTES, CH, SHARED, WIDE, OPAC, HALVES, SKIP, FILT, FASTDIV>(
         vol, p, active, live, slab, s_box, px, py, pz, dx, dy, dz, 0,
         (p.nouter + 1) * (kLoopUnroll / CH), col, col2);
     if (active) {
@@ -1264,14 +1299,14 @@ __device__ __forceinline__ bool wave_tile_pixel(const RayParams& p, uint32_t* x,
 // new wave's work as soon as any of its waves ends, instead of when a workgroup's slowest
 // of four waves does.  Grid: 4 x tile_grid (interleaved or cost-ordered maps).
 template <typename T, bool POW2, int SLAB_BYTES, int CH, bool WIDE = false, bool OPAC = false,
-          bool SKIP = false, int FILT = 0>
+          bool SKIP = false, int FILT = 0, bool FASTDIV = false>
 __global__ __launch_bounds__(64) void k_max_project_wave(PaddedVolume<T, POW2, WIDE> vol,
                                                          RayParams p, float* __restrict__ out,
                                                          float* __restrict__ alpha) {
     __shared__ Unit16 lds[SLAB_BYTES / 16 + 1];  // +16 B: lds_pair reads past a pair
     uint32_t x = 0, y = 0;
     const bool inside = wave_tile_pixel(p, &x, &y);
-    mip_staged_rays<T, POW2, SLAB_BYTES, CH, false, WIDE, OPAC, true, SKIP, FILT>(
+    mip_staged_rays<T, POW2, SLAB_BYTES, CH, false, WIDE, OPAC, true, SKIP, FILT, FASTDIV>(
         vol, p, x, y, inside, reinterpret_cast<uint8_t*>(lds), nullptr, out, alpha);
 }
 

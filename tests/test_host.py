@@ -196,3 +196,46 @@ def test_autotune_keeps_incumbent_within_margin():
     assert run({k_def: 2.0, k_tm3: 1.9}) == dict(default, tile_map=3)
     # the incumbent fastest: kept
     assert run({k_def: 1.5, k_tm3: 1.9}) == default
+
+
+def _fma32(a, b, c):
+    """fmaf on float32 arrays, emulated in float64: the product of two float32 values is
+    exact there, and so is the residual x - den q (it cancels); the correction q + r rcp is
+    rounded to float64 first, which differs from one rounding only when that lands on a
+    float32 rounding midpoint."""
+    return (a.astype(np.float64) * b.astype(np.float64) + c.astype(np.float64)).astype(np.float32)
+
+
+def test_opacity_normalisation_without_division():
+    """The staged opacity mode of integer voxels normalises (v - mn) / den as q0 = x r with
+    r = RN(1/den) and two fma residual corrections; for x = 0 or |x| in [2^-60, 2^40] and
+    |den| in [2^-32, 2^32] that must equal IEEE x / den bit for bit.  Random and hard cases:
+    quotients next to rounding midpoints, divisors with significands near 1 and 2."""
+    rng = np.random.default_rng(7)
+    n = 300000
+
+    def rand_exp(lo, hi, size):
+        m = rng.integers(0, 1 << 23, size, dtype=np.uint32)
+        e = (rng.integers(lo, hi, size) + 127).astype(np.uint32)
+        f = ((e << np.uint32(23)) | m).view(np.float32)
+        return np.where(rng.random(size) < .5, -f, f).astype(np.float32)
+
+    def sig(e, mant):
+        return ((np.uint32(127 + e) << np.uint32(23)) | mant.astype(np.uint32)).view(np.float32)
+
+    den = np.concatenate([rand_exp(-32, 32, n), rng.integers(1, 65536, n).astype(np.float32),
+                          sig(3, 0x7FFFFF - rng.integers(0, 64, n)), sig(-5, rng.integers(0, 64, n))])
+    m = den.size
+    q = rand_exp(-40, 40, m)
+    mid = q.astype(np.float64) + (np.nextafter(q, np.float32(np.inf)) - q).astype(np.float64) / 2
+    plain = np.concatenate([rand_exp(-60, 40, m // 2),
+                            (rng.integers(0, 65536, m - m // 2) + rng.random(m - m // 2)).astype(np.float32)])
+    x = np.where(rng.random(m) < .5, plain, (mid * den).astype(np.float32))
+    ok = (x == 0) | ((np.abs(x) >= 2.0 ** -60) & (np.abs(x) <= 2.0 ** 40))
+    x, den = x[ok], den[ok]
+    assert x.size > 3 * n
+    rcp = (np.float32(1) / den).astype(np.float32)
+    q0 = (x * rcp).astype(np.float32)
+    q1 = _fma32(_fma32(-den, q0, x), rcp, q0)
+    q2 = _fma32(_fma32(-den, q1, x), rcp, q1)
+    np.testing.assert_array_equal(q2.view(np.uint32), (x / den).astype(np.float32).view(np.uint32))
